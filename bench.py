@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X particle-mechanics path.
+
+Default workload: 1,000,000 FP64 spheres, radii uniform in [0.5e-4, 0.7e-4]
+cm, random sequential insertion at volume fraction 0.3 in a 2:2:1 box
+periodic in x/y with the growth surface at the bottom, cubic contact law
+with adhesion band 0.1 * mean radius, dt = 0.25, neighbor list rebuilt every
+10 steps.  A step is one pass of the hot path (binning + list build when due,
+fused force + overdamped update) over all particles, with the particles
+already resident in HBM.
+
+Prints one JSON line with the throughput (particle-steps/s), the roofline of
+the dominant kernel (fused force + update, HIP events on the library's own
+stream over the timed region) and the CPU baseline: the oracle (the C
+restatement of the reference's CPU path, 448-byte AoS records, stencil list
+rebuilt every step) timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "particle-steps/s at 1M FP64 spheres; batched ensemble speedup; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_particle(mean_row: float) -> float:
+    """Compulsory HBM bytes per particle per force launch: read x,y,z,r
+    (32 B), the row length (4 B) and mean_row 4-byte neighbor indices; write
+    x,y,z,r (32 B).  Neighbor coordinates are gathered from cache."""
+    return 32.0 + 4.0 + 4.0 * mean_row + 32.0
+
+
+def cpu_baseline(col, budget_s: float = 12.0):
+    """The oracle in the reference's configuration (AoS records, per-step
+    stencil list over mesh-cell bins, per-particle loop), timed on the host:
+    as many whole steps of the workload as fit in budget_s, one thread and
+    all threads."""
+    import numpy as np
+    import oracle_bmx as orc
+
+    L = col.prob_hi - col.prob_lo
+    dx = 2.0e-4  # mesh cell no smaller than the largest r_lm (1.46e-4 cm)
+    n_cell = [max(4, int(L[d] // dx)) for d in range(3)]
+    hood = 1.1 * (2.0 * 7e-5 + 1e-2)  # DEM::neighborhood (bmx_chem.cpp:124)
+    results = {}
+    lib = orc.lib()
+    threads_all = lib.orc_num_threads()
+    for label, nt in (("1", 1), ("all", threads_all)):
+        lib.orc_set_num_threads(nt)
+        recs = col.recs.copy()
+        g = orc.make_geom(col.prob_lo, col.prob_hi, col.periodic, n_cell)
+        st = orc.State(recs, g, col.fpar, orc.NBR_STENCIL, hood, 1)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            st.evolve(col.dt, 1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 2 or steps >= 50:
+                break
+        results[label] = (len(recs) * steps / el, steps, el, nt)
+        st.close()
+    lib.orc_set_num_threads(threads_all)
+    v1 = results["1"]
+    va = results["all"]
+    return {
+        "value": va[0], "unit": "particle-steps/s", "cores": va[3], "kind": "port",
+        "sample": f"{va[1]} steps of the {len(col.recs)}-sphere workload in the "
+                  f"reference configuration (stencil list every step, AoS records) on "
+                  f"{va[3]} OpenMP threads; 1 thread: {v1[0]:.4g} particle-steps/s "
+                  f"over {v1[1]} steps",
+        "value_1core": v1[0],
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--remap", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the workload would shard)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import boltzmannmfx_amd as bmx
+    from boltzmannmfx_amd import workloads as W
+
+    col = W.c2_large(args.n)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin,
+                            max_neighbors=64)
+    ctx.set_launch_params(args.block, args.remap)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    ctx.evolve(col.dt, args.warmup)
+
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.evolve(col.dt, args.steps)  # synchronizes and checks skin/overflow/NaN
+    t1 = time.perf_counter()
+    tm = ctx.timing()
+    ctx.set_timing(False)
+
+    n = len(recs)
+    secs = t1 - t0
+    value = n * args.steps / secs
+    mean_row = tm.neighbor_entries / max(n, 1)
+    per_launch_ms = tm.force_ms / max(tm.force_launches, 1)
+    alg_bytes = algorithmic_bytes_per_particle(mean_row) * n
+    achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "force_sphere_1M_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "1M FP64 spheres, r~U[0.5e-4,0.7e-4] cm, phi=0.3, 2:2:1 box "
+                        "periodic x/y with growth surface, cubic law W_A=6e-6, dt=0.25, "
+                        "Verlet list (skin 2e-5) rebuilt every 10 steps",
+            "n": n,
+            "mean_row": mean_row,
+            "rebuilds": tm.rebuilds,
+            "block": args.block,
+            "xcd_remap": args.remap,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_force_sphere",
+            "kernel_avg_ms": per_launch_ms,
+            "kernel_share_of_step": tm.force_ms / (secs * 1e3),
+            "rebuild_avg_ms": tm.rebuild_ms / max(tm.rebuilds, 1),
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(col, args.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

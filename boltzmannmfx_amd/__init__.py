@@ -1,0 +1,401 @@
+"""MI355X particle-mechanics path of BMX (BoltzmannMFX), Python bindings.
+
+The product is the C-ABI library ``libbmx_mi355x.so`` (gfx950 HIP kernels
+plus a C++ host layer, declared in ``include/bmx_mi355x.h``).  This module is
+a thin ctypes layer over it for tests, benchmarks and scripting; it never
+falls back to a CPU implementation: if the library or a GPU is missing the
+calls raise.
+
+Particle records use the reference's AMReX ``Particle<34,35>`` layout
+(src/des/bmx_pc.H:86-96): ``RECORD_DTYPE`` below.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbmx_mi355x.so")
+
+# --- record layout (src/chemistry/bmx_chem.H:38-118) -----------------------
+NREAL = 34
+NINT = 35
+RECORD_DTYPE = np.dtype(
+    {
+        "names": ["pos", "rdata", "id", "cpu", "idata"],
+        "formats": [("<f8", 3), ("<f8", NREAL), "<i4", "<i4", ("<i4", NINT)],
+        "offsets": [0, 24, 296, 300, 304],
+        "itemsize": 448,
+    }
+)
+
+
+class R:  # realIdx
+    radius, c_length, theta, phi, area, vol = 0, 1, 2, 3, 4, 5
+    velx, vely, velz, wx, wy, wz = 6, 7, 8, 9, 10, 11
+    fx, fy, fz, taux, tauy, tauz, gx, gy, gz, dadt, dvdt = range(12, 23)
+    tau_split, bond_scale, first_data = 23, 24, 25
+
+
+class I:  # intIdx
+    cell_type, n_bnds = 5, 6
+    seg1_id1, seg1_id2, site1 = 7, 11, 15
+    position, fuse_flag, split_flag, new_flag = 19, 20, 21, 22
+    fuse_id, fuse_cpu, fuse_tip = 23, 24, 25
+    dltd_site1, dltd_cpu2, fix_site, id, cpu = 26, 31, 32, 33, 34
+
+
+YEAST, FUNGI = 0, 1
+TIP, SECOND_1, SECOND_2, INTERIOR = 0, 1, 2, 3
+
+# fpar order (src/chemistry/bmx_cell_interaction.cpp:84-93)
+FPAR_NAMES = (
+    "boundary_width", "stiffness", "z_wall", "wall_boundary_width",
+    "wall_stiffness", "gravity", "bond_strength", "bond_cutoff", "viscous_drag",
+)
+
+NBR_VERLET, NBR_STENCIL = 0, 1
+
+STATUS = {
+    0: "OK", 1: "BAD_ARG", 2: "OOM", 3: "HIP", 4: "NAN", 5: "NBR_OVERFLOW",
+    6: "SKIN", 7: "STATE", 8: "IO", 9: "BOND_OVERFLOW", 10: "NO_DEVICE",
+}
+
+
+class BMXError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+# --- C structs ---------------------------------------------------------------
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "stride_bytes", "pos_offset", "rdata_offset", "id_offset",
+        "cpu_offset", "idata_offset", "n_real", "n_int")]
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("prob_lo", C.c_double * 3), ("prob_hi", C.c_double * 3),
+        ("periodic", C.c_int32 * 3), ("n_cell", C.c_int32 * 3),
+    ]
+
+
+class NeighborParams(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("rebuild_every", C.c_int32),
+        ("max_neighbors", C.c_int32), ("bin_factor_pct", C.c_int32),
+        ("skin", C.c_double), ("neighborhood", C.c_double),
+    ]
+
+
+class LaunchParams(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("xcd_remap", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("force_ms", C.c_double), ("force_launches", C.c_int64),
+        ("rebuild_ms", C.c_double), ("rebuilds", C.c_int64),
+        ("neighbor_entries", C.c_int64), ("max_row", C.c_int32),
+        ("nbins", C.c_int32),
+    ]
+
+
+class AdjointParams(C.Structure):
+    _fields_ = [
+        ("checkpoint_every", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Load libbmx_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing; build it with `python -m boltzmannmfx_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    P, vp = C.POINTER, C.c_void_p
+    sig = {
+        "bmx_abi_version": (C.c_int32, []),
+        "bmx_last_error": (C.c_char_p, []),
+        "bmx_ctx_create": (C.c_int, [P(vp), C.c_int32]),
+        "bmx_ctx_destroy": (None, [vp]),
+        "bmx_set_force_params": (C.c_int, [vp, P(C.c_double)]),
+        "bmx_set_mobility": (C.c_int, [vp, C.c_double]),
+        "bmx_set_geometry": (C.c_int, [vp, P(Geometry)]),
+        "bmx_set_neighbor_params": (C.c_int, [vp, P(NeighborParams)]),
+        "bmx_set_launch_params": (C.c_int, [vp, P(LaunchParams)]),
+        "bmx_default_layout": (Layout, []),
+        "bmx_bind_particles_aos": (C.c_int, [vp, vp, C.c_int64, P(Layout), C.c_int32]),
+        "bmx_bind_colonies_aos": (C.c_int, [vp, vp, P(C.c_int64), C.c_int32,
+                                            P(Geometry), P(Layout), C.c_int32]),
+        "bmx_sync_to_aos": (C.c_int, [vp, vp]),
+        "bmx_num_particles": (C.c_int64, [vp]),
+        "bmx_init_bonds": (C.c_int, [vp]),
+        "bmx_evolve": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "bmx_evolve_graph": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "bmx_get_fc": (C.c_int, [vp, P(C.c_double)]),
+        "bmx_get_positions": (C.c_int, [vp, P(C.c_double)]),
+        "bmx_get_orientations": (C.c_int, [vp, P(C.c_double)]),
+        "bmx_get_int_fields": (C.c_int, [vp, P(C.c_int32)]),
+        "bmx_get_neighbors": (C.c_int, [vp, P(C.c_int64), P(C.c_int32), C.c_int64,
+                                        P(C.c_int64)]),
+        "bmx_get_bins": (C.c_int, [vp, P(C.c_int32)]),
+        "bmx_get_bin_dims": (C.c_int, [vp, C.c_int32, P(C.c_int32)]),
+        "bmx_set_timing": (C.c_int, [vp, C.c_int32]),
+        "bmx_get_timing": (C.c_int, [vp, P(Timing)]),
+        "bmx_reset_timing": (C.c_int, [vp]),
+        "bmx_synchronize": (C.c_int, [vp]),
+        "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
+                                               P(C.c_int64), P(Layout)]),
+        "bmx_write_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout)]),
+        "bmx_mean_sphere_volume": (C.c_double, [C.c_double, C.c_double]),
+        "bmx_generate_spheres": (C.c_int, [vp, C.c_int64, C.c_double, C.c_double,
+                                           C.c_double, C.c_uint64, P(C.c_double),
+                                           P(C.c_int32), C.c_int32, P(C.c_double),
+                                           P(C.c_double), P(Layout)]),
+        "bmx_generate_hyphae": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_double,
+                                          C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_uint64, P(C.c_double),
+                                          P(C.c_double), C.c_double, P(Layout)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(code: int) -> None:
+    if code != 0:
+        msg = load_library().bmx_last_error().decode(errors="replace")
+        raise BMXError(code, msg)
+
+
+def make_geometry(prob_lo, prob_hi, periodic, n_cell=(1, 1, 1)) -> Geometry:
+    g = Geometry()
+    for d in range(3):
+        g.prob_lo[d] = float(prob_lo[d])
+        g.prob_hi[d] = float(prob_hi[d])
+        g.periodic[d] = int(bool(periodic[d]))
+        g.n_cell[d] = int(n_cell[d])
+    return g
+
+
+class Context:
+    """One bmx_ctx: device buffers, stream and the bound particles."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.bmx_ctx_create(C.byref(h), int(device)))
+        self.h = h
+        self.n = 0
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.bmx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # configuration
+    def set_force_params(self, fpar: Sequence[float]) -> None:
+        a = np.ascontiguousarray(fpar, dtype=np.float64)
+        assert a.shape == (9,)
+        _check(self.lib.bmx_set_force_params(self.h, _dptr(a)))
+
+    def set_mobility(self, m: float) -> None:
+        _check(self.lib.bmx_set_mobility(self.h, float(m)))
+
+    def set_geometry(self, prob_lo, prob_hi, periodic, n_cell=(1, 1, 1)) -> None:
+        g = make_geometry(prob_lo, prob_hi, periodic, n_cell)
+        _check(self.lib.bmx_set_geometry(self.h, C.byref(g)))
+
+    def set_neighbor_params(self, mode=NBR_VERLET, rebuild_every=10, skin=0.0,
+                            max_neighbors=0, bin_factor_pct=100,
+                            neighborhood=1.0) -> None:
+        p = NeighborParams(int(mode), int(rebuild_every), int(max_neighbors),
+                           int(bin_factor_pct), float(skin), float(neighborhood))
+        _check(self.lib.bmx_set_neighbor_params(self.h, C.byref(p)))
+
+    def set_launch_params(self, block_size=256, xcd_remap=1) -> None:
+        p = LaunchParams(int(block_size), int(xcd_remap))
+        _check(self.lib.bmx_set_launch_params(self.h, C.byref(p)))
+
+    # particles
+    def bind(self, recs: np.ndarray) -> None:
+        assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+        self._recs = recs
+        lay = self.lib.bmx_default_layout()
+        _check(self.lib.bmx_bind_particles_aos(
+            self.h, recs.ctypes.data_as(C.c_void_p), len(recs), C.byref(lay), 0))
+        self.n = len(recs)
+
+    def bind_colonies(self, recs: np.ndarray, colony_start: Sequence[int],
+                      geoms: Sequence[Geometry]) -> None:
+        assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+        self._recs = recs
+        cs = np.ascontiguousarray(colony_start, dtype=np.int64)
+        ga = (Geometry * len(geoms))(*geoms)
+        lay = self.lib.bmx_default_layout()
+        _check(self.lib.bmx_bind_colonies_aos(
+            self.h, recs.ctypes.data_as(C.c_void_p),
+            cs.ctypes.data_as(C.POINTER(C.c_int64)), len(geoms), ga,
+            C.byref(lay), 0))
+        self.n = len(recs)
+
+    def sync(self, recs: Optional[np.ndarray] = None) -> None:
+        ptr = recs.ctypes.data_as(C.c_void_p) if recs is not None else None
+        _check(self.lib.bmx_sync_to_aos(self.h, ptr))
+
+    # hot path
+    def init_bonds(self) -> None:
+        _check(self.lib.bmx_init_bonds(self.h))
+
+    def evolve(self, dt: float, nsteps: int) -> None:
+        _check(self.lib.bmx_evolve(self.h, float(dt), int(nsteps)))
+
+    def evolve_graph(self, dt: float, nsteps: int) -> None:
+        _check(self.lib.bmx_evolve_graph(self.h, float(dt), int(nsteps)))
+
+    # inspection
+    def fc(self) -> np.ndarray:
+        out = np.empty(6 * self.n, dtype=np.float64)
+        _check(self.lib.bmx_get_fc(self.h, _dptr(out)))
+        return out.reshape(6, self.n)
+
+    def positions(self) -> np.ndarray:
+        out = np.empty(3 * self.n, dtype=np.float64)
+        _check(self.lib.bmx_get_positions(self.h, _dptr(out)))
+        return out.reshape(self.n, 3)
+
+    def orientations(self) -> np.ndarray:
+        out = np.empty(2 * self.n, dtype=np.float64)
+        _check(self.lib.bmx_get_orientations(self.h, _dptr(out)))
+        return out.reshape(self.n, 2)
+
+    def int_fields(self) -> np.ndarray:
+        out = np.empty(35 * self.n, dtype=np.int32)
+        _check(self.lib.bmx_get_int_fields(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out.reshape(self.n, 35)
+
+    def neighbors(self):
+        off = np.empty(self.n + 1, dtype=np.int64)
+        tot = C.c_int64()
+        _check(self.lib.bmx_get_neighbors(
+            self.h, off.ctypes.data_as(C.POINTER(C.c_int64)), None, 0, C.byref(tot)))
+        idx = np.empty(max(tot.value, 1), dtype=np.int32)
+        _check(self.lib.bmx_get_neighbors(
+            self.h, off.ctypes.data_as(C.POINTER(C.c_int64)),
+            idx.ctypes.data_as(C.POINTER(C.c_int32)), len(idx), C.byref(tot)))
+        return off, idx[: tot.value]
+
+    def bins(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(self.lib.bmx_get_bins(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def bin_dims(self, colony: int = 0):
+        out = (C.c_int32 * 3)()
+        _check(self.lib.bmx_get_bin_dims(self.h, int(colony), out))
+        return tuple(int(v) for v in out)
+
+    def set_timing(self, on: bool) -> None:
+        _check(self.lib.bmx_set_timing(self.h, int(bool(on))))
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(self.lib.bmx_get_timing(self.h, C.byref(t)))
+        return t
+
+    def reset_timing(self) -> None:
+        _check(self.lib.bmx_reset_timing(self.h))
+
+    def synchronize(self) -> None:
+        _check(self.lib.bmx_synchronize(self.h))
+
+
+# --- particle files and synthetic inputs ------------------------------------
+def read_particles_ascii(path: str) -> np.ndarray:
+    """InitParticlesAscii's 27-field format -> records (src/des/bmx_pc_init.cpp:17-132)."""
+    lib = load_library()
+    n = C.c_int64()
+    _check(lib.bmx_read_particles_ascii(path.encode(), None, 0, C.byref(n), None))
+    recs = np.zeros(n.value, dtype=RECORD_DTYPE)
+    _check(lib.bmx_read_particles_ascii(path.encode(), recs.ctypes.data_as(C.c_void_p),
+                                        n.value, C.byref(n), None))
+    return recs
+
+
+def write_particles_ascii(path: str, recs: np.ndarray) -> None:
+    lib = load_library()
+    recs = np.ascontiguousarray(recs, dtype=RECORD_DTYPE)
+    _check(lib.bmx_write_particles_ascii(path.encode(), recs.ctypes.data_as(C.c_void_p),
+                                         len(recs), None))
+
+
+def generate_spheres(n: int, rmin: float, rmax: float, phi: float, seed: int,
+                     aspect=(2.0, 2.0, 1.0), periodic=(1, 1, 0),
+                     overlap_free: bool = True):
+    """Seeded sphere colony; returns (records, prob_lo, prob_hi)."""
+    lib = load_library()
+    recs = np.zeros(n, dtype=RECORD_DTYPE)
+    asp = np.asarray(aspect, dtype=np.float64)
+    per = np.asarray(periodic, dtype=np.int32)
+    lo = np.zeros(3)
+    hi = np.zeros(3)
+    _check(lib.bmx_generate_spheres(
+        recs.ctypes.data_as(C.c_void_p), n, rmin, rmax, phi, seed & (2**64 - 1),
+        _dptr(asp), per.ctypes.data_as(C.POINTER(C.c_int32)), int(overlap_free),
+        _dptr(lo), _dptr(hi), None))
+    return recs, lo, hi
+
+
+def generate_hyphae(n: int, nroots: int, seg_len: float, seg_rad: float,
+                    branch_prob: float, angle_min: float, angle_max: float,
+                    seed: int, box_lo, box_hi, z_surface: float) -> np.ndarray:
+    """Seeded branching segment chains lying above z_surface."""
+    lib = load_library()
+    recs = np.zeros(n, dtype=RECORD_DTYPE)
+    lo = np.asarray(box_lo, dtype=np.float64)
+    hi = np.asarray(box_hi, dtype=np.float64)
+    _check(lib.bmx_generate_hyphae(
+        recs.ctypes.data_as(C.c_void_p), n, nroots, seg_len, seg_rad, branch_prob,
+        angle_min, angle_max, seed & (2**64 - 1), _dptr(lo), _dptr(hi),
+        float(z_surface), None))
+    return recs
+
+
+def mean_sphere_volume(rmin: float, rmax: float) -> float:
+    return load_library().bmx_mean_sphere_volume(rmin, rmax)

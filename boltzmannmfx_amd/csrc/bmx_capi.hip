@@ -1,0 +1,1048 @@
+// bmx_capi.hip -- the C ABI (include/bmx_mi355x.h) over the gfx950 kernels.
+//
+// One context = one HIP stream, one set of device buffers, one or more
+// periodic boxes.  Per step the host enqueues (every rebuild_every steps) the
+// binning and list build, then one fused force+update launch; nothing is
+// copied back until the caller asks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bmx_mi355x.h"
+#include "bmx_kernels.h"
+
+using namespace bmx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+bmx_status fail(bmx_status st, const std::string &msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define HIPCHK(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(BMX_ERR_HIP, std::string(#expr) + ": " +                  \
+                                   hipGetErrorString(e_));                  \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct bmx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  double fpar[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double mobility = 1.0;
+  bool have_fpar = false;
+  bmx_geometry geom{};
+  bool have_geom = false;
+  bmx_neighbor_params nbp{};
+  bmx_launch_params lp{256, 1};
+
+  // particles
+  int64_t n = 0, ncap = 0;
+  int32_t ncol = 1;
+  std::vector<bmx_geometry> colony_geom;
+  std::vector<int64_t> colony_start;
+  std::vector<BoxGeom> boxes;
+  DevBuf<BoxGeom> d_boxes;
+  DevBuf<int32_t> d_col_in;
+  bool bound = false;
+  bool any_fungi = false, any_yeast = false;
+  double rmax = 0.0, lmax = 0.0;
+
+  // sorted arrays
+  DevBuf<int32_t> oid[2], type[2], col[2], key[2];
+  DevBuf<double> bscale[2];
+  DevBuf<double4> pos[2], rod[2], trig[2];
+  DevBuf<Topo> topo[2];
+  int cs = 0;  // current static buffer
+  int cd = 0;  // current dynamic buffer
+
+  // record array
+  DevBuf<uint8_t> d_aos;
+  void *user_aos = nullptr;
+  bool aos_on_device = false;
+  AosLayout lay{};
+
+  // binning and rows
+  int32_t nbins = 0;
+  double crit = 0.0;
+  double verlet_radius = 0.0;
+  DevBuf<int32_t> d_keytmp, d_rank, d_count, d_start, d_scan;
+  DevBuf<unsigned long long> d_permk;
+  int32_t maxn = 96;
+  DevBuf<int32_t> d_nbr, d_ncnt;
+  bool have_list = false;
+  int64_t step = 0, last_build = -1;
+
+  // outputs and flags
+  DevBuf<double> d_fc;
+  bool wrote_step = false;
+  DevBuf<int32_t> d_err, d_stats;
+  DevBuf<unsigned long long> d_total, d_vmax, d_ustats;
+  int32_t vmax_slots = 1;
+  int32_t *h_flags = nullptr;  // pinned: err, max row, pad, pad, total(2)
+
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_force, ev_rebuild;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double force_ms = 0.0, rebuild_ms = 0.0;
+  int64_t force_launches = 0, rebuilds = 0;
+  int64_t last_entries = 0;
+  int32_t last_max_row = 0;
+
+  // captured step sequences
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  int32_t graph_nsteps = 0;
+  int64_t graph_step0 = -1;
+  int graph_cs = -1, graph_cd = -1;
+  bool graph_have_list = false;
+};
+
+namespace {
+
+LaunchShape shape(const bmx_ctx *c) {
+  LaunchShape s;
+  s.block = c->lp.block_size;
+  s.remap = c->lp.xcd_remap;
+  return s;
+}
+
+StaticArrays static_arrays(bmx_ctx *c, int k) {
+  StaticArrays s;
+  s.oid = c->oid[k].p;
+  s.type = c->type[k].p;
+  s.col = c->ncol > 1 ? c->col[k].p : nullptr;
+  s.bscale = c->bscale[k].p;
+  s.key = c->key[k].p;
+  return s;
+}
+DynArrays dyn_arrays(bmx_ctx *c, int k) {
+  DynArrays d;
+  d.pos = c->pos[k].p;
+  d.rod = c->rod[k].p;
+  d.trig = c->trig[k].p;
+  d.topo = c->topo[k].p;
+  return d;
+}
+
+ForceParams force_params(const bmx_ctx *c, double dt) {
+  ForceParams fp;
+  fp.WA = c->fpar[0];
+  fp.mu = c->fpar[1];
+  fp.zw = c->fpar[2];
+  fp.WZ = c->fpar[3];
+  fp.xi = c->fpar[4];
+  fp.grav = c->fpar[5];
+  fp.kb = c->fpar[6];
+  fp.bcut = c->fpar[7];
+  fp.visc = c->fpar[8];
+  fp.mob = c->mobility;
+  fp.dt = dt;
+  return fp;
+}
+
+hipEvent_t next_event(bmx_ctx *c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+
+// Collect finished event pairs into the running totals.
+bmx_status drain_timing(bmx_ctx *c) {
+  for (auto &pr : c->ev_force) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->force_ms += ms;
+    c->force_launches++;
+  }
+  for (auto &pr : c->ev_rebuild) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    c->rebuild_ms += ms;
+    c->rebuilds++;
+  }
+  c->ev_force.clear();
+  c->ev_rebuild.clear();
+  c->ev_used = 0;
+  return BMX_OK;
+}
+
+// Box geometry with bins chosen for the neighbor policy.
+bmx_status make_box(const bmx_ctx *c, const bmx_geometry &gm, BoxGeom &b,
+                    int32_t bin_offset) {
+  for (int d = 0; d < 3; d++) {
+    if (!(gm.prob_hi[d] > gm.prob_lo[d]))
+      return fail(BMX_ERR_BAD_ARG, "prob_hi must exceed prob_lo");
+    b.plo[d] = gm.prob_lo[d];
+    b.phi[d] = gm.prob_hi[d];
+    b.L[d] = gm.prob_hi[d] - gm.prob_lo[d];
+    b.halfL[d] = 0.5 * b.L[d];
+    b.periodic[d] = gm.periodic[d] ? 1 : 0;
+    int nb;
+    if (c->nbp.mode == BMX_NBR_STENCIL) {
+      nb = gm.n_cell[d];
+      if (nb < 1) return fail(BMX_ERR_BAD_ARG, "n_cell must be >= 1");
+      if (b.periodic[d] && nb < 4)
+        return fail(BMX_ERR_BAD_ARG,
+                    "stencil lists need >= 4 cells along a periodic axis");
+    } else {
+      double pct = c->nbp.bin_factor_pct > 0 ? c->nbp.bin_factor_pct : 100;
+      double edge = c->verlet_radius * pct / 100.0;
+      double f = std::floor(b.L[d] / edge);
+      nb = f < 1.0 ? 1 : (f > 1.0e6 ? 1000000 : (int)f);
+      if (b.periodic[d] && nb < 3)
+        return fail(BMX_ERR_BAD_ARG,
+                    "periodic box shorter than 3 interaction radii");
+    }
+    b.nbin[d] = nb;
+    double dx = b.L[d] / (double)nb;
+    b.dxi[d] = 1.0 / dx;
+  }
+  b.bin_offset = bin_offset;
+  b.pad = 0;
+  int64_t tot = (int64_t)b.nbin[0] * b.nbin[1] * b.nbin[2];
+  if (tot > (int64_t)1 << 30) return fail(BMX_ERR_BAD_ARG, "too many bins");
+  return BMX_OK;
+}
+
+bmx_status setup_boxes(bmx_ctx *c) {
+  // Verlet radius: the largest r_lm any pair can have, plus the skin
+  // (K.H:23-40).
+  double rl = 0.0;
+  if (c->any_yeast) rl = std::max(rl, 2.0 * c->rmax + c->fpar[0]);
+  if (c->any_fungi) rl = std::max(rl, 1.5 * (2.0 * c->lmax) + c->fpar[0]);
+  c->verlet_radius = rl + c->nbp.skin;
+  if (c->nbp.mode == BMX_NBR_VERLET) {
+    if (!(c->verlet_radius > 0.0))
+      return fail(BMX_ERR_BAD_ARG, "interaction radius must be positive");
+    c->crit = c->verlet_radius * c->verlet_radius;
+  } else {
+    c->crit = c->nbp.neighborhood;
+  }
+  c->boxes.resize(c->ncol);
+  int64_t off = 0;
+  for (int k = 0; k < c->ncol; k++) {
+    bmx_status st = make_box(c, c->colony_geom[k], c->boxes[k], (int32_t)off);
+    if (st != BMX_OK) return st;
+    off += (int64_t)c->boxes[k].nbin[0] * c->boxes[k].nbin[1] *
+           c->boxes[k].nbin[2];
+    if (off > (int64_t)1 << 30) return fail(BMX_ERR_BAD_ARG, "too many bins");
+  }
+  c->nbins = (int32_t)off;
+  HIPCHK(c->d_boxes.alloc(c->ncol));
+  HIPCHK(hipMemcpyAsync(c->d_boxes.p, c->boxes.data(),
+                        sizeof(BoxGeom) * c->ncol, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(c->d_count.alloc(c->nbins + 1));
+  HIPCHK(c->d_start.alloc(c->nbins + 1));
+  HIPCHK(c->d_scan.alloc(scan_scratch_elems(c->nbins) + 1));
+  c->have_list = false;
+  c->graph_step0 = -1;
+  return BMX_OK;
+}
+
+bmx_status ensure_rows(bmx_ctx *c) {
+  size_t need = (size_t)c->maxn * (size_t)c->ncap;
+  HIPCHK(c->d_nbr.alloc(need));
+  HIPCHK(c->d_ncnt.alloc(c->ncap));
+  return BMX_OK;
+}
+
+// Binning + gather into the other buffers + list build.
+bmx_status rebuild(bmx_ctx *c) {
+  hipStream_t s = c->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    e0 = next_event(c);
+    e1 = next_event(c);
+    if (!e0 || !e1) return fail(BMX_ERR_HIP, "event allocation failed");
+    HIPCHK(hipEventRecord(e0, s));
+  }
+  int multi = c->ncol > 1;
+  const BoxGeom g0 = c->boxes[0];
+  StaticArrays ss = static_arrays(c, c->cs), ds = static_arrays(c, 1 - c->cs);
+  DynArrays sd = dyn_arrays(c, c->cd), dd = dyn_arrays(c, 1 - c->cd);
+  HIPCHK(launch_fill_i32(c->d_count.p, c->nbins + 1, 0, s));
+  HIPCHK(launch_bin_count(sd.pos, ss.col, c->d_boxes.p, g0, multi, c->n,
+                          c->d_keytmp.p, c->d_rank.p, c->d_count.p, s));
+  HIPCHK(launch_scan(c->d_count.p, c->d_start.p, c->nbins, c->d_scan.p, s));
+  HIPCHK(launch_bin_scatter(c->d_keytmp.p, c->d_rank.p, c->d_start.p, ss.oid,
+                            c->n, c->d_permk.p, s));
+  HIPCHK(launch_bin_sort(c->d_start.p, c->nbins, c->d_permk.p, c->d_err.p, s));
+  HIPCHK(launch_gather(c->d_permk.p, c->n, ss, ds, sd, dd, c->d_keytmp.p,
+                       multi, c->any_fungi ? 1 : 0, s));
+  c->cs = 1 - c->cs;
+  c->cd = 1 - c->cd;
+  StaticArrays st = static_arrays(c, c->cs);
+  DynArrays dy = dyn_arrays(c, c->cd);
+  HIPCHK(launch_fill_i32(c->d_stats.p, 1, 0, s));
+  HIPCHK(launch_fill_u64(c->d_total.p, 1, 0ull, s));
+  HIPCHK(launch_build_nbr(dy.pos, st.col, c->d_boxes.p, g0, multi,
+                          c->d_start.p, st.key, c->n, c->crit, c->d_nbr.p,
+                          c->d_ncnt.p, c->ncap, c->maxn, c->d_err.p,
+                          c->d_stats.p, c->d_total.p, shape(c), s));
+  if (c->timing) {
+    HIPCHK(hipEventRecord(e1, s));
+    c->ev_rebuild.push_back({e0, e1});
+  }
+  c->have_list = true;
+  c->last_build = c->step;
+  return BMX_OK;
+}
+
+bool verlet_checked(const bmx_ctx *c) {
+  return c->nbp.mode == BMX_NBR_VERLET && c->nbp.rebuild_every > 1;
+}
+
+bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
+  hipStream_t s = c->stream;
+  bool need = !c->have_list ||
+              (c->step - c->last_build) >= (int64_t)c->nbp.rebuild_every;
+  if (need) {
+    if (c->have_list && verlet_checked(c))
+      HIPCHK(launch_skin_check(c->d_vmax.p, c->vmax_slots,
+                               0.5 * c->nbp.skin, c->d_err.p, s));
+    bmx_status st = rebuild(c);
+    if (st != BMX_OK) return st;
+  }
+  int slot = (int)(c->step - c->last_build);
+  unsigned long long *vm =
+      verlet_checked(c) ? c->d_vmax.p + (slot % c->vmax_slots) : nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    e0 = next_event(c);
+    e1 = next_event(c);
+    if (!e0 || !e1) return fail(BMX_ERR_HIP, "event allocation failed");
+    HIPCHK(hipEventRecord(e0, s));
+  }
+  ForceParams fp = force_params(c, dt);
+  int multi = c->ncol > 1;
+  double *fc = write_fc ? c->d_fc.p : nullptr;
+  if (!c->any_fungi) {
+    HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p,
+                               c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
+                               multi ? c->col[c->cs].p : nullptr,
+                               c->d_boxes.p, c->boxes[0], multi, fp, fc, vm,
+                               c->d_err.p, shape(c), s));
+  } else {
+    HIPCHK(launch_force_general(dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd),
+                                static_arrays(c, c->cs), c->d_nbr.p,
+                                c->d_ncnt.p, c->ncap, c->n, c->d_boxes.p,
+                                c->boxes[0], multi, fp, fc, vm, c->d_err.p,
+                                shape(c), s));
+  }
+  if (c->timing) {
+    HIPCHK(hipEventRecord(e1, s));
+    c->ev_force.push_back({e0, e1});
+  }
+  c->cd = 1 - c->cd;
+  c->step++;
+  if (write_fc) c->wrote_step = true;
+  return BMX_OK;
+}
+
+// Reads the error word and list statistics; clears the error word.
+bmx_status check_flags(bmx_ctx *c) {
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->h_flags, c->d_err.p, sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->h_flags + 1, c->d_stats.p, sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->d_total.p, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (c->timing) {
+    bmx_status st = drain_timing(c);
+    if (st != BMX_OK) return st;
+  }
+  int32_t err = c->h_flags[0];
+  c->last_max_row = c->h_flags[1];
+  unsigned long long tot;
+  std::memcpy(&tot, c->h_flags + 2, sizeof(tot));
+  c->last_entries = (int64_t)tot;
+  if (err & ERRBIT_OVERFLOW) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf,
+                  "neighbor row of %d entries exceeds max_neighbors=%d",
+                  c->last_max_row, c->maxn);
+    c->have_list = false;
+    return fail(BMX_ERR_NBR_OVERFLOW, buf);
+  }
+  if (err & ERRBIT_BOND)
+    return fail(BMX_ERR_BOND_OVERFLOW, "a segment would get a fifth bond");
+  if (err & ERRBIT_SKIN)
+    return fail(BMX_ERR_SKIN,
+                "displacement since the last rebuild exceeded skin/2");
+  if (err & ERRBIT_NAN)
+    return fail(BMX_ERR_NAN, "non-finite force (coincident particles?)");
+  return BMX_OK;
+}
+
+bmx_status require_ready(const bmx_ctx *c) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  if (!c->bound) return fail(BMX_ERR_STATE, "no particles bound");
+  return BMX_OK;
+}
+
+void reset_graph(bmx_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+  c->graph_step0 = -1;
+}
+
+// Writes the state the force pass changes into the device record copy.
+bmx_status pack_records(bmx_ctx *c) {
+  HIPCHK(launch_aos_pack(c->d_aos.p, c->lay, c->n, static_arrays(c, c->cs),
+                         dyn_arrays(c, c->cd), c->d_fc.p, c->ncap,
+                         c->mobility, c->wrote_step ? 1 : 0,
+                         c->any_fungi ? 1 : 0, c->stream));
+  return BMX_OK;
+}
+
+bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
+                       int32_t ncol, const bmx_geometry *geoms,
+                       const bmx_layout *layout, int32_t on_device) {
+  if (!aos && cstart[ncol] > 0)
+    return fail(BMX_ERR_BAD_ARG, "null record array");
+  if (!c->have_fpar) return fail(BMX_ERR_STATE, "force params not set");
+  bmx_layout L = layout ? *layout : bmx_default_layout();
+  if (L.stride_bytes <= 0 || L.pos_offset % 8 || L.rdata_offset % 8 ||
+      L.idata_offset % 4 || L.stride_bytes % 8 || L.n_real < 26 ||
+      L.n_int < 35 || L.pos_offset + 24 > L.stride_bytes ||
+      L.rdata_offset + 8 * L.n_real > L.stride_bytes ||
+      L.idata_offset + 4 * L.n_int > L.stride_bytes)
+    return fail(BMX_ERR_BAD_ARG, "invalid record layout");
+  int64_t n = cstart[ncol];
+  if (n < 0 || n > (int64_t)INT32_MAX / 2)
+    return fail(BMX_ERR_BAD_ARG, "particle count out of range");
+  for (int k = 0; k < ncol; k++)
+    if (cstart[k + 1] < cstart[k])
+      return fail(BMX_ERR_BAD_ARG, "colony offsets must be non-decreasing");
+  reset_graph(c);
+  hipStream_t s = c->stream;
+  c->n = n;
+  c->ncap = std::max<int64_t>(((n + 255) / 256) * 256, 256);
+  c->ncol = ncol;
+  c->colony_geom.assign(geoms, geoms + ncol);
+  c->colony_start.assign(cstart, cstart + ncol + 1);
+  c->lay.stride = L.stride_bytes;
+  c->lay.pos = L.pos_offset;
+  c->lay.rdata = L.rdata_offset;
+  c->lay.id = L.id_offset;
+  c->lay.cpu = L.cpu_offset;
+  c->lay.idata = L.idata_offset;
+  size_t cap = (size_t)c->ncap;
+  for (int k = 0; k < 2; k++) {
+    HIPCHK(c->oid[k].alloc(cap));
+    HIPCHK(c->type[k].alloc(cap));
+    HIPCHK(c->key[k].alloc(cap));
+    HIPCHK(c->bscale[k].alloc(cap));
+    HIPCHK(c->pos[k].alloc(cap));
+    HIPCHK(c->rod[k].alloc(cap));
+    HIPCHK(c->trig[k].alloc(cap));
+    HIPCHK(c->topo[k].alloc(cap));
+    if (ncol > 1) HIPCHK(c->col[k].alloc(cap));
+  }
+  HIPCHK(c->d_keytmp.alloc(cap));
+  HIPCHK(c->d_rank.alloc(cap));
+  HIPCHK(c->d_permk.alloc(cap));
+  HIPCHK(c->d_fc.alloc(6 * cap));
+  HIPCHK(c->d_ustats.alloc(8));
+  size_t bytes = (size_t)n * (size_t)L.stride_bytes;
+  c->aos_on_device = on_device != 0;
+  c->user_aos = const_cast<void *>(aos);
+  HIPCHK(c->d_aos.alloc(bytes));
+  if (bytes)
+    HIPCHK(hipMemcpyAsync(c->d_aos.p, aos, bytes,
+                          on_device ? hipMemcpyDeviceToDevice
+                                    : hipMemcpyHostToDevice,
+                          s));
+  // Boxes before unpack (wrap needs them); bins are set after the stats.
+  c->boxes.resize(ncol);
+  for (int k = 0; k < ncol; k++) {
+    BoxGeom &b = c->boxes[k];
+    std::memset(&b, 0, sizeof b);
+    for (int d = 0; d < 3; d++) {
+      b.plo[d] = geoms[k].prob_lo[d];
+      b.phi[d] = geoms[k].prob_hi[d];
+      b.L[d] = b.phi[d] - b.plo[d];
+      b.halfL[d] = 0.5 * b.L[d];
+      b.periodic[d] = geoms[k].periodic[d] ? 1 : 0;
+      b.nbin[d] = 1;
+      b.dxi[d] = 1.0;
+      if (!(b.L[d] > 0.0))
+        return fail(BMX_ERR_BAD_ARG, "prob_hi must exceed prob_lo");
+    }
+  }
+  HIPCHK(c->d_boxes.alloc(ncol));
+  HIPCHK(hipMemcpyAsync(c->d_boxes.p, c->boxes.data(), sizeof(BoxGeom) * ncol,
+                        hipMemcpyHostToDevice, s));
+  const int32_t *dcol = nullptr;
+  if (ncol > 1) {
+    std::vector<int32_t> hc((size_t)n);
+    for (int k = 0; k < ncol; k++)
+      for (int64_t i = cstart[k]; i < cstart[k + 1]; i++) hc[i] = k;
+    HIPCHK(c->d_col_in.alloc(cap));
+    HIPCHK(hipMemcpyAsync(c->d_col_in.p, hc.data(), sizeof(int32_t) * n,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    dcol = c->d_col_in.p;
+  }
+  c->cs = 0;
+  c->cd = 0;
+  HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
+  StaticArrays st = static_arrays(c, 0);
+  DynArrays dy = dyn_arrays(c, 0);
+  HIPCHK(launch_aos_unpack(c->d_aos.p, c->lay, n, dcol, c->d_boxes.p, st, dy,
+                           c->d_ustats.p, s));
+  unsigned long long hs[8];
+  HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hs[4] != 0)
+    return fail(BMX_ERR_BAD_ARG,
+                "particle records with negative radius, non-finite position, "
+                "or (segments) n_bnds outside 0..4 or c_length <= 0");
+  std::memcpy(&c->rmax, &hs[0], sizeof(double));
+  std::memcpy(&c->lmax, &hs[1], sizeof(double));
+  c->any_yeast = hs[2] != 0;
+  c->any_fungi = hs[3] != 0;
+  bmx_status bs = setup_boxes(c);
+  if (bs != BMX_OK) return bs;
+  bs = ensure_rows(c);
+  if (bs != BMX_OK) return bs;
+  HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
+  c->vmax_slots = std::max(1, c->nbp.rebuild_every);
+  HIPCHK(c->d_vmax.alloc(c->vmax_slots));
+  HIPCHK(launch_fill_u64(c->d_vmax.p, c->vmax_slots, 0ull, s));
+  HIPCHK(hipStreamSynchronize(s));
+  c->step = 0;
+  c->last_build = -1;
+  c->have_list = false;
+  c->wrote_step = false;
+  c->bound = true;
+  return BMX_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ===
+
+extern "C" {
+
+int32_t bmx_abi_version(void) { return BMX_ABI_VERSION; }
+
+void bmx_io_set_error(const char *msg) { g_last_error = msg ? msg : ""; }
+
+const char *bmx_last_error(void) { return g_last_error.c_str(); }
+
+bmx_layout bmx_default_layout(void) {
+  bmx_layout L;
+  L.stride_bytes = 448;
+  L.pos_offset = 0;
+  L.rdata_offset = 24;
+  L.id_offset = 296;
+  L.cpu_offset = 300;
+  L.idata_offset = 304;
+  L.n_real = 34;
+  L.n_int = 35;
+  return L;
+}
+
+bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
+  if (!out) return fail(BMX_ERR_BAD_ARG, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(BMX_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count)
+    return fail(BMX_ERR_BAD_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(device));
+  bmx_ctx *c = new (std::nothrow) bmx_ctx();
+  if (!c) return fail(BMX_ERR_OOM, "context allocation failed");
+  c->device = device;
+  c->nbp.mode = BMX_NBR_VERLET;
+  c->nbp.rebuild_every = 10;
+  c->nbp.max_neighbors = 0;
+  c->nbp.bin_factor_pct = 100;
+  c->nbp.skin = 0.0;
+  c->nbp.neighborhood = 1.0;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_flags, 64);
+  if (e == hipSuccess) e = c->d_err.alloc(1) ;
+  if (e == hipSuccess) e = c->d_stats.alloc(4);
+  if (e == hipSuccess) e = c->d_total.alloc(1);
+  if (e != hipSuccess) {
+    bmx_ctx_destroy(c);
+    return fail(BMX_ERR_HIP, std::string("context setup: ") +
+                                 hipGetErrorString(e));
+  }
+  std::memset(c->h_flags, 0, 64);
+  *out = c;
+  return BMX_OK;
+}
+
+void bmx_ctx_destroy(bmx_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  reset_graph(c);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; k++) {
+    c->oid[k].release();
+    c->type[k].release();
+    c->col[k].release();
+    c->key[k].release();
+    c->bscale[k].release();
+    c->pos[k].release();
+    c->rod[k].release();
+    c->trig[k].release();
+    c->topo[k].release();
+  }
+  c->d_boxes.release();
+  c->d_col_in.release();
+  c->d_aos.release();
+  c->d_keytmp.release();
+  c->d_rank.release();
+  c->d_count.release();
+  c->d_start.release();
+  c->d_scan.release();
+  c->d_permk.release();
+  c->d_nbr.release();
+  c->d_ncnt.release();
+  c->d_fc.release();
+  c->d_err.release();
+  c->d_stats.release();
+  c->d_total.release();
+  c->d_vmax.release();
+  c->d_ustats.release();
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+bmx_status bmx_set_force_params(bmx_ctx *c, const double fpar[9]) {
+  if (!c || !fpar) return fail(BMX_ERR_BAD_ARG, "null argument");
+  for (int k = 0; k < 9; k++)
+    if (!std::isfinite(fpar[k]))
+      return fail(BMX_ERR_BAD_ARG, "force parameters must be finite");
+  std::memcpy(c->fpar, fpar, sizeof c->fpar);
+  c->have_fpar = true;
+  reset_graph(c);
+  if (c->bound) {
+    (void)hipSetDevice(c->device);
+    return setup_boxes(c);  // interaction radius depends on W_A
+  }
+  return BMX_OK;
+}
+
+bmx_status bmx_set_mobility(bmx_ctx *c, double m) {
+  if (!c || !std::isfinite(m)) return fail(BMX_ERR_BAD_ARG, "bad mobility");
+  c->mobility = m;
+  reset_graph(c);
+  return BMX_OK;
+}
+
+bmx_status bmx_set_geometry(bmx_ctx *c, const bmx_geometry *g) {
+  if (!c || !g) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (c->bound) return fail(BMX_ERR_STATE, "set geometry before binding");
+  c->geom = *g;
+  c->have_geom = true;
+  return BMX_OK;
+}
+
+bmx_status bmx_set_neighbor_params(bmx_ctx *c, const bmx_neighbor_params *p) {
+  if (!c || !p) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (p->mode != BMX_NBR_VERLET && p->mode != BMX_NBR_STENCIL)
+    return fail(BMX_ERR_BAD_ARG, "unknown neighbor mode");
+  if (p->rebuild_every < 1 || p->max_neighbors < 0 || !(p->skin >= 0.0) ||
+      p->bin_factor_pct < 0)
+    return fail(BMX_ERR_BAD_ARG, "bad neighbor parameters");
+  if (p->mode == BMX_NBR_STENCIL && p->rebuild_every != 1)
+    return fail(BMX_ERR_BAD_ARG, "stencil lists are rebuilt every step");
+  c->nbp = *p;
+  if (c->nbp.bin_factor_pct == 0) c->nbp.bin_factor_pct = 100;
+  c->maxn = p->max_neighbors > 0 ? p->max_neighbors : 96;
+  reset_graph(c);
+  if (c->bound) {
+    (void)hipSetDevice(c->device);
+    bmx_status st = setup_boxes(c);
+    if (st != BMX_OK) return st;
+    st = ensure_rows(c);
+    if (st != BMX_OK) return st;
+    c->vmax_slots = std::max(1, c->nbp.rebuild_every);
+    HIPCHK(c->d_vmax.alloc(c->vmax_slots));
+    HIPCHK(launch_fill_u64(c->d_vmax.p, c->vmax_slots, 0ull, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return BMX_OK;
+}
+
+bmx_status bmx_set_launch_params(bmx_ctx *c, const bmx_launch_params *p) {
+  if (!c || !p) return fail(BMX_ERR_BAD_ARG, "null argument");
+  int b = p->block_size;
+  if (b != 64 && b != 128 && b != 256 && b != 512)
+    return fail(BMX_ERR_BAD_ARG, "block_size must be 64, 128, 256 or 512");
+  c->lp = *p;
+  reset_graph(c);
+  return BMX_OK;
+}
+
+bmx_status bmx_bind_particles_aos(bmx_ctx *c, const void *aos, int64_t n,
+                                  const bmx_layout *layout,
+                                  int32_t on_device) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  if (!c->have_geom) return fail(BMX_ERR_STATE, "geometry not set");
+  (void)hipSetDevice(c->device);
+  int64_t cs[2] = {0, n};
+  return bind_common(c, aos, cs, 1, &c->geom, layout, on_device);
+}
+
+bmx_status bmx_bind_colonies_aos(bmx_ctx *c, const void *aos,
+                                 const int64_t *colony_start, int32_t ncol,
+                                 const bmx_geometry *geoms,
+                                 const bmx_layout *layout,
+                                 int32_t on_device) {
+  if (!c || !colony_start || !geoms || ncol < 1)
+    return fail(BMX_ERR_BAD_ARG, "bad colony arguments");
+  if (colony_start[0] != 0)
+    return fail(BMX_ERR_BAD_ARG, "colony_start[0] must be 0");
+  (void)hipSetDevice(c->device);
+  c->geom = geoms[0];
+  c->have_geom = true;
+  return bind_common(c, aos, colony_start, ncol, geoms, layout, on_device);
+}
+
+int64_t bmx_num_particles(const bmx_ctx *c) { return c ? c->n : 0; }
+
+bmx_status bmx_sync_to_aos(bmx_ctx *c, void *aos) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  (void)hipSetDevice(c->device);
+  st = pack_records(c);
+  if (st != BMX_OK) return st;
+  void *dst = aos ? aos : c->user_aos;
+  size_t bytes = (size_t)c->n * (size_t)c->lay.stride;
+  if (dst && bytes)
+    HIPCHK(hipMemcpyAsync(dst, c->d_aos.p, bytes,
+                          c->aos_on_device ? hipMemcpyDeviceToDevice
+                                           : hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return BMX_OK;
+}
+
+bmx_status bmx_init_bonds(bmx_ctx *c) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  (void)hipSetDevice(c->device);
+  reset_graph(c);
+  st = rebuild(c);
+  if (st != BMX_OK) return st;
+  if (c->any_fungi)
+    HIPCHK(launch_init_bonds(dyn_arrays(c, c->cd), static_arrays(c, c->cs),
+                             c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
+                             c->d_boxes.p, c->boxes[0], c->ncol > 1,
+                             force_params(c, 0.0), c->d_err.p, c->stream));
+  // the first evolve step rebuilds, as EvolveParticles does after InitBonds
+  c->have_list = false;
+  return check_flags(c);
+}
+
+bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (nsteps < 0 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
+  (void)hipSetDevice(c->device);
+  for (int32_t k = 0; k < nsteps; k++) {
+    st = enqueue_step(c, dt, k == nsteps - 1);
+    if (st != BMX_OK) return st;
+  }
+  if (verlet_checked(c) && nsteps > 0 && c->have_list)
+    HIPCHK(launch_skin_check(c->d_vmax.p, c->vmax_slots, 0.5 * c->nbp.skin,
+                             c->d_err.p, c->stream));
+  return check_flags(c);
+}
+
+bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (nsteps <= 0 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
+  if (c->timing)
+    return fail(BMX_ERR_STATE, "disable event timing for graph replay");
+  (void)hipSetDevice(c->device);
+  // A captured sequence is valid for the same dt, step count, list phase and
+  // buffer parity; rebuild points are part of the graph.
+  int64_t phase = c->have_list ? (c->step - c->last_build) : -1;
+  bool reuse = c->graph_exec && c->graph_dt == dt &&
+               c->graph_nsteps == nsteps && c->graph_cs == c->cs &&
+               c->graph_cd == c->cd && c->graph_step0 == phase &&
+               c->graph_have_list == c->have_list;
+  int cs0 = c->cs, cd0 = c->cd;
+  bool hl0 = c->have_list;
+  if (!reuse) {
+    reset_graph(c);
+    int64_t save_step = c->step, save_last = c->last_build;
+    bool save_wrote = c->wrote_step;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bmx_status cst = BMX_OK;
+    for (int32_t k = 0; k < nsteps && cst == BMX_OK; k++)
+      cst = enqueue_step(c, dt, k == nsteps - 1);
+    if (cst == BMX_OK && verlet_checked(c))
+      (void)launch_skin_check(c->d_vmax.p, c->vmax_slots, 0.5 * c->nbp.skin,
+                              c->d_err.p, c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (cst != BMX_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return cst;
+    }
+    if (e != hipSuccess)
+      return fail(BMX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return fail(BMX_ERR_HIP,
+                  std::string("instantiate: ") + hipGetErrorString(e));
+    c->graph_dt = dt;
+    c->graph_nsteps = nsteps;
+    c->graph_cs = cs0;
+    c->graph_cd = cd0;
+    c->graph_step0 = phase;
+    c->graph_have_list = hl0;
+    // capture recorded the launches without running them: restore the
+    // bookkeeping to the pre-capture state, then replay below
+    c->step = save_step;
+    c->last_build = save_last;
+    c->wrote_step = save_wrote;
+    c->cs = cs0;
+    c->cd = cd0;
+    c->have_list = hl0;
+  }
+  HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+  // advance bookkeeping exactly as the eager loop would
+  for (int32_t k = 0; k < nsteps; k++) {
+    bool need = !c->have_list ||
+                (c->step - c->last_build) >= (int64_t)c->nbp.rebuild_every;
+    if (need) {
+      c->cs = 1 - c->cs;
+      c->cd = 1 - c->cd;
+      c->have_list = true;
+      c->last_build = c->step;
+    }
+    c->cd = 1 - c->cd;
+    c->step++;
+  }
+  c->wrote_step = true;
+  return check_flags(c);
+}
+
+bmx_status bmx_get_fc(bmx_ctx *c, double *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out && c->n > 0) return fail(BMX_ERR_BAD_ARG, "null output");
+  if (!c->wrote_step) return fail(BMX_ERR_STATE, "no step has run");
+  (void)hipSetDevice(c->device);
+  DevBuf<double> tmp;
+  HIPCHK(tmp.alloc(6 * (size_t)std::max<int64_t>(c->n, 1)));
+  HIPCHK(launch_export_fc(c->d_fc.p, c->ncap, c->oid[c->cs].p, c->n, tmp.p,
+                          c->stream));
+  if (c->n)
+    HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * 6 * c->n,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  tmp.release();
+  return BMX_OK;
+}
+
+bmx_status bmx_get_positions(bmx_ctx *c, double *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out && c->n > 0) return fail(BMX_ERR_BAD_ARG, "null output");
+  (void)hipSetDevice(c->device);
+  DevBuf<double> tmp;
+  HIPCHK(tmp.alloc(3 * (size_t)std::max<int64_t>(c->n, 1)));
+  HIPCHK(launch_export_pos(c->pos[c->cd].p, c->oid[c->cs].p, c->n, tmp.p,
+                           c->stream));
+  if (c->n)
+    HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * 3 * c->n,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  tmp.release();
+  return BMX_OK;
+}
+
+bmx_status bmx_get_orientations(bmx_ctx *c, double *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out && c->n > 0) return fail(BMX_ERR_BAD_ARG, "null output");
+  (void)hipSetDevice(c->device);
+  DevBuf<double> tmp;
+  HIPCHK(tmp.alloc(2 * (size_t)std::max<int64_t>(c->n, 1)));
+  HIPCHK(launch_export_orient(c->rod[c->cd].p, c->oid[c->cs].p, c->n, tmp.p,
+                              c->stream));
+  if (c->n)
+    HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * 2 * c->n,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  tmp.release();
+  return BMX_OK;
+}
+
+bmx_status bmx_get_int_fields(bmx_ctx *c, int32_t *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out && c->n > 0) return fail(BMX_ERR_BAD_ARG, "null output");
+  (void)hipSetDevice(c->device);
+  st = pack_records(c);
+  if (st != BMX_OK) return st;
+  std::vector<uint8_t> host((size_t)c->n * c->lay.stride);
+  if (!host.empty())
+    HIPCHK(hipMemcpyAsync(host.data(), c->d_aos.p, host.size(),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < c->n; i++)
+    std::memcpy(out + 35 * i, host.data() + i * c->lay.stride + c->lay.idata,
+                35 * sizeof(int32_t));
+  return BMX_OK;
+}
+
+bmx_status bmx_get_neighbors(bmx_ctx *c, int64_t *offsets, int32_t *indices,
+                             int64_t capacity, int64_t *total) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!offsets || !total) return fail(BMX_ERR_BAD_ARG, "null output");
+  if (!c->have_list) return fail(BMX_ERR_STATE, "no current neighbor list");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  DevBuf<int64_t> cnt;
+  HIPCHK(cnt.alloc((size_t)c->n + 1));
+  HIPCHK(launch_export_counts(c->d_ncnt.p, c->oid[c->cs].p, c->n, cnt.p, s));
+  std::vector<int64_t> hc((size_t)c->n + 1, 0);
+  if (c->n)
+    HIPCHK(hipMemcpyAsync(hc.data(), cnt.p, sizeof(int64_t) * c->n,
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  offsets[0] = 0;
+  for (int64_t i = 0; i < c->n; i++) offsets[i + 1] = offsets[i] + hc[i];
+  *total = offsets[c->n];
+  if (indices) {
+    if (capacity < *total) return fail(BMX_ERR_BAD_ARG, "capacity too small");
+    DevBuf<int64_t> off;
+    DevBuf<int32_t> out;
+    HIPCHK(off.alloc((size_t)c->n + 1));
+    HIPCHK(out.alloc((size_t)std::max<int64_t>(*total, 1)));
+    HIPCHK(hipMemcpyAsync(off.p, offsets, sizeof(int64_t) * (c->n + 1),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(launch_export_nbr(c->d_nbr.p, c->d_ncnt.p, c->ncap,
+                             c->oid[c->cs].p, c->n, off.p, out.p, s));
+    if (*total)
+      HIPCHK(hipMemcpyAsync(indices, out.p, sizeof(int32_t) * (*total),
+                            hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    off.release();
+    out.release();
+  }
+  cnt.release();
+  return BMX_OK;
+}
+
+bmx_status bmx_get_bins(bmx_ctx *c, int32_t *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out && c->n > 0) return fail(BMX_ERR_BAD_ARG, "null output");
+  if (!c->have_list) return fail(BMX_ERR_STATE, "no current binning");
+  (void)hipSetDevice(c->device);
+  DevBuf<int32_t> tmp;
+  HIPCHK(tmp.alloc((size_t)std::max<int64_t>(c->n, 1)));
+  HIPCHK(launch_export_i32(c->key[c->cs].p, c->oid[c->cs].p, c->n, tmp.p,
+                           c->stream));
+  if (c->n)
+    HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(int32_t) * c->n,
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  tmp.release();
+  return BMX_OK;
+}
+
+bmx_status bmx_get_bin_dims(bmx_ctx *c, int32_t colony, int32_t out3[3]) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out3 || colony < 0 || colony >= c->ncol)
+    return fail(BMX_ERR_BAD_ARG, "bad colony");
+  for (int d = 0; d < 3; d++) out3[d] = c->boxes[colony].nbin[d];
+  return BMX_OK;
+}
+
+bmx_status bmx_set_timing(bmx_ctx *c, int32_t enable) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  c->timing = enable != 0;
+  return BMX_OK;
+}
+
+bmx_status bmx_get_timing(bmx_ctx *c, bmx_timing *out) {
+  if (!c || !out) return fail(BMX_ERR_BAD_ARG, "null argument");
+  out->force_ms = c->force_ms;
+  out->force_launches = c->force_launches;
+  out->rebuild_ms = c->rebuild_ms;
+  out->rebuilds = c->rebuilds;
+  out->neighbor_entries = c->last_entries;
+  out->max_row = c->last_max_row;
+  out->nbins = c->nbins;
+  return BMX_OK;
+}
+
+bmx_status bmx_reset_timing(bmx_ctx *c) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  c->force_ms = c->rebuild_ms = 0.0;
+  c->force_launches = c->rebuilds = 0;
+  return BMX_OK;
+}
+
+bmx_status bmx_synchronize(bmx_ctx *c) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  (void)hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return BMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// bmx_kernels.h -- host-side launchers for the gfx950 kernels.
+#pragma once
+
+#include "bmx_internal.h"
+
+namespace bmx {
+
+struct LaunchShape {
+  int block;  // threads per block
+  int remap;  // XCD-contiguous tile mapping
+};
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n,
+                           unsigned long long v, hipStream_t s);
+
+// AoS -> sorted SoA (identity order), periodic wrap, per-box bounds stats.
+// stats: [0] max radius bits, [1] max c_length bits, [2] any yeast,
+// [3] any fungi, [4] bad record count
+hipError_t launch_aos_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
+                             const int32_t *col, const BoxGeom *boxes,
+                             StaticArrays st, DynArrays dy,
+                             unsigned long long *stats, hipStream_t s);
+// sorted SoA -> AoS at original positions (fields the force pass writes)
+hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
+                           StaticArrays st, DynArrays dy, const double *fc,
+                           int64_t ncap, double mob, int wrote_step,
+                           int any_fungi, hipStream_t s);
+
+// ---- binning: key/rank, scan, scatter, per-bin order, gather
+hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
+                            const BoxGeom *boxes, BoxGeom g0, int multi,
+                            int64_t n, int32_t *key, int32_t *rank,
+                            int32_t *count, hipStream_t s);
+// exclusive scan of count[0..nb) into start[0..nb]; scratch >= scan_scratch(nb)
+int64_t scan_scratch_elems(int64_t nb);
+hipError_t launch_scan(const int32_t *count, int32_t *start, int64_t nb,
+                       int32_t *scratch, hipStream_t s);
+hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
+                              const int32_t *start, const int32_t *oid,
+                              int64_t n, unsigned long long *permk,
+                              hipStream_t s);
+hipError_t launch_bin_sort(const int32_t *start, int64_t nb,
+                           unsigned long long *permk, int32_t *err,
+                           hipStream_t s);
+hipError_t launch_gather(const unsigned long long *permk, int64_t n,
+                         StaticArrays src_s, StaticArrays dst_s,
+                         DynArrays src_d, DynArrays dst_d,
+                         const int32_t *key, int multi, int rods,
+                         hipStream_t s);
+
+// ---- neighbor list (column-major rows: nbr[k*ncap + i])
+// mode 0: r2 <= crit (crit = radius^2); mode 1: r2 <= crit (neighborhood)
+// stats: [0] max row, total_entries accumulates row lengths
+hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
+                            const BoxGeom *boxes, BoxGeom g0, int multi,
+                            const int32_t *start, const int32_t *key,
+                            int64_t n, double crit, int32_t *nbr,
+                            int32_t *ncnt, int64_t ncap, int32_t maxn,
+                            int32_t *err, int32_t *stats,
+                            unsigned long long *total, LaunchShape ls,
+                            hipStream_t s);
+
+// ---- force + overdamped update, fused.  fc (6*ncap) is written when
+// non-null; vmax receives max(dt*|vel|) as double bits when non-null.
+hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
+                               const int32_t *nbr, const int32_t *ncnt,
+                               int64_t ncap, int64_t n, const int32_t *col,
+                               const BoxGeom *boxes, BoxGeom g0, int multi,
+                               ForceParams fp, double *fc,
+                               unsigned long long *vmax, int32_t *err,
+                               LaunchShape ls, hipStream_t s);
+hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
+                                const int32_t *nbr, const int32_t *ncnt,
+                                int64_t ncap, int64_t n, const BoxGeom *boxes,
+                                BoxGeom g0, int multi, ForceParams fp,
+                                double *fc, unsigned long long *vmax,
+                                int32_t *err, LaunchShape ls, hipStream_t s);
+hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
+                             const int32_t *ncnt, int64_t ncap, int64_t n,
+                             const BoxGeom *boxes, BoxGeom g0, int multi,
+                             ForceParams fp, int32_t *err, hipStream_t s);
+// sum of vmax[0..nslots) > limit -> err |= ERRBIT_SKIN; then clear slots
+hipError_t launch_skin_check(unsigned long long *vmax, int32_t nslots,
+                             double limit, int32_t *err, hipStream_t s);
+
+// ---- exports to original order
+hipError_t launch_export_fc(const double *fc, int64_t ncap, const int32_t *oid,
+                            int64_t n, double *out, hipStream_t s);
+hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,
+                             double *out, hipStream_t s);
+hipError_t launch_export_orient(const double4 *rod, const int32_t *oid,
+                                int64_t n, double *out, hipStream_t s);
+hipError_t launch_export_i32(const int32_t *v, const int32_t *oid, int64_t n,
+                             int32_t *out, hipStream_t s);
+hipError_t launch_export_nbr(const int32_t *nbr, const int32_t *ncnt,
+                             int64_t ncap, const int32_t *oid, int64_t n,
+                             const int64_t *row_off /* by original index */,
+                             int32_t *out, hipStream_t s);
+hipError_t launch_export_counts(const int32_t *ncnt, const int32_t *oid,
+                                int64_t n, int64_t *out_counts,
+                                hipStream_t s);
+
+}  // namespace bmx

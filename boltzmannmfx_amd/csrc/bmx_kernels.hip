@@ -1,0 +1,948 @@
+// bmx_kernels.hip -- gfx950 kernels of the particle-mechanics path.
+//
+// Data live in bin-sorted order.  A rebuild bins every particle
+// (floor((x - plo) * dxi), src/interpolation/bmx_interp_K.H:55-56), counts
+// bins with atomics, scans the counts, scatters (original index, slot)
+// pairs, orders each bin by original index, gathers every per-particle array
+// into the new order and builds column-major neighbor rows over the +-1 bin
+// stencil.  Rows are visited in a fixed order (stencil offsets dz, dy, dx
+// lexicographic, ascending original index inside a bin), so the force pass
+// sums each particle's pair terms in an order that does not depend on
+// thread scheduling.  The force kernel is a gather: particle i reads its row
+// and adds only into its own accumulators, as the reference's force lambda
+// does (src/des/bmx_pc_interaction.cpp:170-279), then applies the overdamped
+// update (src/des/bmx_pc_interaction.cpp:311-449) in the same launch,
+// writing the next state into the other half of a ping-pong pair.
+#include "bmx_device.h"
+#include "bmx_kernels.h"
+
+namespace bmx {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+// Hardware dispatches workgroup b to XCD b % 8.  With remap, XCD x instead
+// works on a contiguous range of tiles, so neighbor gathers of adjacent
+// tiles share that XCD's L2.
+__device__ __forceinline__ int64_t tile_of(int remap) {
+  int64_t b = blockIdx.x, nb = gridDim.x;
+  if (!remap) return b;
+  int64_t x = b & 7, r = b >> 3, per = nb >> 3, extra = nb & 7;
+  return x * per + (x < extra ? x : extra) + r;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+int grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+int grid_capped(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+// ------------------------------------------------------------- AoS I/O ---
+
+__global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
+                             int64_t n, const int32_t *__restrict__ col,
+                             const BoxGeom *__restrict__ boxes, StaticArrays st,
+                             DynArrays dy, unsigned long long *stats) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint8_t *r = aos + t * (int64_t)lay.stride;
+  const double *pos = reinterpret_cast<const double *>(r + lay.pos);
+  const double *rd = reinterpret_cast<const double *>(r + lay.rdata);
+  const int32_t *id = reinterpret_cast<const int32_t *>(r + lay.idata);
+  const BoxGeom &g = boxes[col ? col[t] : 0];
+  double x = wrap_axis(pos[0], g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+  double y = wrap_axis(pos[1], g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+  double z = wrap_axis(pos[2], g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+  double rad = rd[rec::radius];
+  dy.pos[t] = make_double4(x, y, z, rad);
+  st.oid[t] = (int32_t)t;
+  int ty = id[rec::cell_type];
+  st.type[t] = ty;
+  if (st.col) st.col[t] = col[t];
+  bool bad = !(rad >= 0.0) || !(x == x) || !(y == y) || !(z == z);
+  if (ty == rec::YEAST) atomicOr(&stats[2], 1ull);
+  if (ty == rec::FUNGI) atomicOr(&stats[3], 1ull);
+  if (rad >= 0.0) atomicMax(&stats[0], dbits(rad));
+  if (dy.rod) {
+    double L = rd[rec::c_length], th = rd[rec::theta], ph = rd[rec::phi];
+    dy.rod[t] = make_double4(L, th, ph, rd[rec::area]);
+    dy.trig[t] = make_double4(cos(th), sin(th), cos(ph), sin(ph));
+    st.bscale[t] = rd[rec::bond_scale];
+    Topo tp;
+    tp.n_bnds = id[rec::n_bnds];
+    for (int b = 0; b < 4; b++) {
+      tp.seg_id[b] = id[rec::seg1_id1 + b];
+      tp.seg_cpu[b] = id[rec::seg1_id2 + b];
+      tp.site[b] = id[rec::site1 + b];
+    }
+    tp.position = id[rec::position];
+    tp.fuse_tip = id[rec::fuse_tip];
+    tp.fix_site = id[rec::fix_site];
+    tp.self_id = id[rec::id];
+    tp.self_cpu = id[rec::cpu];
+    tp.dltd = 0;
+    tp.pad = 0;
+    dy.topo[t] = tp;
+    if (ty == rec::FUNGI) {
+      if (L >= 0.0) atomicMax(&stats[1], dbits(L));
+      if (tp.n_bnds < 0 || tp.n_bnds > 4 || !(L > 0.0)) bad = true;
+    }
+  }
+  if (bad) atomicAdd(&stats[4], 1ull);
+}
+
+__global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
+                           StaticArrays st, DynArrays dy,
+                           const double *__restrict__ fc, int64_t ncap,
+                           double mob, int wrote_step, int any_fungi) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t o = st.oid[t];
+  uint8_t *r = aos + o * (int64_t)lay.stride;
+  double *pos = reinterpret_cast<double *>(r + lay.pos);
+  double *rd = reinterpret_cast<double *>(r + lay.rdata);
+  int32_t *id = reinterpret_cast<int32_t *>(r + lay.idata);
+  double4 p = dy.pos[t];
+  pos[0] = p.x;
+  pos[1] = p.y;
+  pos[2] = p.z;
+  if (wrote_step) {
+    rd[rec::velx] = mob * fc[t];
+    rd[rec::vely] = mob * fc[t + ncap];
+    rd[rec::velz] = mob * fc[t + 2 * ncap];
+    rd[rec::wx] = fc[t + 3 * ncap];
+    rd[rec::wy] = fc[t + 4 * ncap];
+    rd[rec::wz] = fc[t + 5 * ncap];
+    id[rec::fuse_flag] = 0;
+    id[rec::split_flag] = 0;
+    id[rec::new_flag] = 0;
+  }
+  if (any_fungi && st.type[t] == rec::FUNGI) {
+    double4 rr = dy.rod[t];
+    rd[rec::theta] = rr.y;
+    rd[rec::phi] = rr.z;
+    Topo tp = dy.topo[t];
+    id[rec::n_bnds] = tp.n_bnds;
+    for (int b = 0; b < 4; b++) {
+      id[rec::seg1_id1 + b] = tp.seg_id[b];
+      id[rec::seg1_id2 + b] = tp.seg_cpu[b];
+      id[rec::site1 + b] = tp.site[b];
+    }
+    id[rec::position] = tp.position;
+    id[rec::fix_site] = tp.fix_site;
+    if (tp.dltd)
+      for (int k = rec::dltd_site1; k <= rec::dltd_cpu2; k++) id[k] = -1;
+  }
+}
+
+// ------------------------------------------------------------- binning ---
+
+template <bool MULTI>
+__global__ void k_bin_count(const double4 *__restrict__ pos,
+                            const int32_t *__restrict__ col,
+                            const BoxGeom *__restrict__ boxes, BoxGeom g0,
+                            int64_t n, int32_t *__restrict__ key,
+                            int32_t *__restrict__ rank,
+                            int32_t *__restrict__ count) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const BoxGeom &g = MULTI ? boxes[col[s]] : g0;
+  int b = bin_of(pos[s], g);
+  key[s] = b;
+  rank[s] = atomicAdd(&count[b], 1);
+}
+
+// Exclusive scan: 256 threads x 8 items per block, wave prefix by shuffles.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t s, int32_t *wsum,
+                                                   int32_t &total) {
+  int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  int32_t incl = s;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int32_t y = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += y;
+  }
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  int32_t pre = 0, tot = 0;
+  int nw = blockDim.x / kWave;
+  for (int w = 0; w < nw; w++) {
+    if (w < wid) pre += wsum[w];
+    tot += wsum[w];
+  }
+  total = tot;
+  return pre + incl - s;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(
+    const int32_t *__restrict__ in, int32_t *__restrict__ out,
+    int32_t *__restrict__ bsum, int64_t n) {
+  __shared__ int32_t wsum[kScanThreads / kWave];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    int32_t x = (base + i < n) ? in[base + i] : 0;
+    v[i] = s;
+    s += x;
+  }
+  int32_t total;
+  int32_t ex = block_excl_scan(s, wsum, total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++)
+    if (base + i < n) out[base + i] = v[i] + ex;
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Single block scans the per-tile sums in place and writes the grand total
+// to out_total.
+__global__ __launch_bounds__(1024) void k_scan_tiles(int32_t *__restrict__ bsum,
+                                                     int64_t nt,
+                                                     int32_t *__restrict__ out_total) {
+  __shared__ int32_t wsum[1024 / kWave];
+  int64_t per = (nt + blockDim.x - 1) / blockDim.x;
+  int64_t base = threadIdx.x * per;
+  int32_t s = 0;
+  for (int64_t i = 0; i < per; i++)
+    if (base + i < nt) s += bsum[base + i];
+  int32_t total;
+  int32_t ex = block_excl_scan(s, wsum, total);
+  int32_t run = ex;
+  for (int64_t i = 0; i < per; i++)
+    if (base + i < nt) {
+      int32_t x = bsum[base + i];
+      bsum[base + i] = run;
+      run += x;
+    }
+  if (threadIdx.x == 0) *out_total = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(
+    int32_t *__restrict__ out, const int32_t *__restrict__ bsum, int64_t n) {
+  int32_t add = bsum[blockIdx.x];
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += kScanThreads)
+    if (base + i < n) out[base + i] += add;
+}
+
+__global__ void k_bin_scatter(const int32_t *__restrict__ key,
+                              const int32_t *__restrict__ rank,
+                              const int32_t *__restrict__ start,
+                              const int32_t *__restrict__ oid, int64_t n,
+                              unsigned long long *__restrict__ permk) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int64_t dst = (int64_t)start[key[s]] + rank[s];
+  permk[dst] = ((unsigned long long)(uint32_t)oid[s] << 32) |
+               (unsigned long long)(uint32_t)s;
+}
+
+// Orders each bin by original index (the high word of permk).  A bin of more
+// than kMaxBinSort particles is left unsorted and reported as an overflow
+// (its rows could not fit anyway), which bounds the work of every thread;
+// after a non-finite force nothing is sorted.
+constexpr int64_t kMaxBinSort = 1024;
+__global__ void k_bin_sort(const int32_t *__restrict__ start, int64_t nb,
+                           unsigned long long *__restrict__ permk,
+                           int32_t *__restrict__ err) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (*err & ERRBIT_NAN) return;
+  int64_t a = start[b], e = start[b + 1];
+  if (e - a > kMaxBinSort) {
+    atomicOr(err, ERRBIT_OVERFLOW);
+    return;
+  }
+  for (int64_t i = a + 1; i < e; i++) {
+    unsigned long long v = permk[i];
+    int64_t j = i - 1;
+    while (j >= a && permk[j] > v) {
+      permk[j + 1] = permk[j];
+      j--;
+    }
+    permk[j + 1] = v;
+  }
+}
+
+__global__ void k_gather(const unsigned long long *__restrict__ permk,
+                         int64_t n, StaticArrays ss, StaticArrays ds,
+                         DynArrays sd, DynArrays dd,
+                         const int32_t *__restrict__ key, int multi,
+                         int rods) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t s = (int64_t)(uint32_t)(permk[t] & 0xffffffffull);
+  dd.pos[t] = sd.pos[s];
+  ds.oid[t] = ss.oid[s];
+  ds.type[t] = ss.type[s];
+  ds.key[t] = key[s];
+  if (multi) ds.col[t] = ss.col[s];
+  if (rods) {
+    ds.bscale[t] = ss.bscale[s];
+    dd.rod[t] = sd.rod[s];
+    dd.trig[t] = sd.trig[s];
+    dd.topo[t] = sd.topo[s];
+  }
+}
+
+// -------------------------------------------------------- neighbor rows ---
+
+template <bool MULTI>
+__global__ __launch_bounds__(512) void k_build_nbr(
+    const double4 *__restrict__ pos, const int32_t *__restrict__ col,
+    const BoxGeom *__restrict__ boxes, BoxGeom g0,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ key,
+    int64_t n, double crit, int32_t *__restrict__ nbr,
+    int32_t *__restrict__ ncnt, int64_t ncap, int32_t maxn,
+    int32_t *__restrict__ err, int32_t *__restrict__ stats,
+    unsigned long long *__restrict__ total, int remap) {
+  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  int cnt = 0;
+  // Candidates visited per particle are capped (a pile of particles in a few
+  // bins would otherwise make this quadratic); hitting the cap is reported
+  // as an overflow.  After a non-finite force no rows are built.
+  const int64_t scan_cap = 64 * (int64_t)maxn + 4096;
+  int64_t scanned = 0;
+  if (t < n && !(*err & ERRBIT_NAN)) {
+    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
+    const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
+    double4 p = pos[t];
+    int lb = key[t] - g.bin_offset;
+    int bx = lb % nbx, rr = lb / nbx, by = rr % nby, bz = rr / nby;
+    for (int oz = -1; oz <= 1; oz++) {
+      int cz = bz + oz;
+      if (g.periodic[2]) {
+        cz = (cz + nbz) % nbz;
+      } else if (cz < 0 || cz >= nbz) {
+        continue;
+      }
+      for (int oy = -1; oy <= 1; oy++) {
+        int cy = by + oy;
+        if (g.periodic[1]) {
+          cy = (cy + nby) % nby;
+        } else if (cy < 0 || cy >= nby) {
+          continue;
+        }
+        int row = g.bin_offset + (cz * nby + cy) * nbx;
+        // up to three x-ranges; one contiguous range when no x wrap
+        int ranges[3][2];
+        int nr = 0;
+        if (bx >= 1 && bx + 1 < nbx) {
+          ranges[0][0] = start[row + bx - 1];
+          ranges[0][1] = start[row + bx + 2];
+          nr = 1;
+        } else {
+          for (int ox = -1; ox <= 1; ox++) {
+            int cx = bx + ox;
+            if (g.periodic[0]) {
+              cx = (cx + nbx) % nbx;
+            } else if (cx < 0 || cx >= nbx) {
+              continue;
+            }
+            ranges[nr][0] = start[row + cx];
+            ranges[nr][1] = start[row + cx + 1];
+            nr++;
+          }
+        }
+        for (int q = 0; q < nr; q++) {
+          int64_t len = ranges[q][1] - ranges[q][0];
+          if (scanned + len > scan_cap) {
+            cnt = maxn + 1;  // reported as overflow below
+            ranges[q][1] = ranges[q][0];
+          }
+          scanned += len;
+          for (int u = ranges[q][0]; u < ranges[q][1]; u++) {
+            if (u == t) continue;
+            double4 o = pos[u];
+            double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+            double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+            double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+            double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            if (r2 <= crit) {
+              if (cnt < maxn) nbr[(int64_t)cnt * ncap + t] = u;
+              cnt++;
+            }
+          }
+        }
+      }
+    }
+    ncnt[t] = cnt < maxn ? cnt : maxn;
+    if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
+  } else if (t < n) {
+    ncnt[t] = 0;
+  }
+  int m = wave_max_i32(cnt);
+  long long sm = wave_sum_i64(cnt);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(&stats[0], m);
+    atomicAdd(total, (unsigned long long)sm);
+  }
+}
+
+// ------------------------------------------------------- force + update ---
+
+__device__ __forceinline__ void block_vmax(double v2, double dt,
+                                           unsigned long long *vmax) {
+  double m = wave_max_f64(v2);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0)
+    atomicMax(vmax, dbits(dt * sqrt(m)));
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(512) void k_force_sphere(
+    const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
+    int64_t ncap, int64_t n, const int32_t *__restrict__ col,
+    const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
+    double *__restrict__ fc, unsigned long long *__restrict__ vmax,
+    int32_t *__restrict__ err, int remap) {
+  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  double v2 = 0.0;
+  if (t < n) {
+    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
+    double4 p = pos_in[t];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    int c = ncnt[t];
+    const int32_t *row = nbr + t;
+    for (int k = 0; k < c; k++) {
+      int u = row[(int64_t)k * ncap];
+      double4 o = pos_in[u];
+      double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+      double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+      double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      double rS = p.w + o.w;
+      double lim = rS + fp.WA - 1.0e-15;  // r_lm - small_number (PCI:212)
+      if (r2 <= lim * lim) sphere_pair(d0, d1, d2, r2, rS, fp, a0, a1, a2);
+    }
+    // growth surface (K.H:1009-1030); vcom x/y are zero
+    a0 += 0.0;
+    a1 += 0.0;
+    a2 += wall_law(p.z, p.w, fp);
+    double vx = fp.mob * a0, vy = fp.mob * a1, vz = fp.mob * a2;
+    double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
+    x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+    y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+    z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+    pos_out[t] = make_double4(x, y, z, p.w);
+    if (fc) {
+      fc[t] = a0;
+      fc[t + ncap] = a1;
+      fc[t + 2 * ncap] = a2;
+      fc[t + 3 * ncap] = 0.0;
+      fc[t + 4 * ncap] = 0.0;
+      fc[t + 5 * ncap] = 0.0;
+    }
+    if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
+    v2 = vx * vx + vy * vy + vz * vz;
+  }
+  if (vmax) block_vmax(v2, fp.dt, vmax);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_force_general(
+    DynArrays in, DynArrays out, StaticArrays st,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
+    int64_t ncap, int64_t n, const BoxGeom *__restrict__ boxes, BoxGeom g0,
+    ForceParams fp, double *__restrict__ fc,
+    unsigned long long *__restrict__ vmax, int32_t *__restrict__ err,
+    int remap) {
+  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  double v2 = 0.0;
+  if (t < n) {
+    const BoxGeom &g = MULTI ? boxes[st.col[t]] : g0;
+    const int ty = st.type[t];
+    const double4 p = in.pos[t];
+    const double4 rod_i = in.rod[t];
+    const double4 tr_i = in.trig[t];
+    Topo ti = in.topo[t];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int c = ncnt[t];
+    const int32_t *row = nbr + t;
+    if (ty == rec::YEAST) {
+      for (int k = 0; k < c; k++) {
+        int u = row[(int64_t)k * ncap];
+        if (st.type[u] != rec::YEAST) continue;  // r_lm = 0 (K.H:39)
+        double4 o = in.pos[u];
+        double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+        double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+        double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double rS = p.w + o.w;
+        double lim = rS + fp.WA - 1.0e-15;
+        if (r2 <= lim * lim)
+          sphere_pair(d0, d1, d2, r2, rS, fp, acc[0], acc[1], acc[2]);
+      }
+      acc[0] += 0.0;
+      acc[1] += 0.0;
+      acc[2] += wall_law(p.z, p.w, fp);
+      acc[3] += 0.0;
+      acc[4] += 0.0;
+      acc[5] += 0.0;
+    } else if (ty == rec::FUNGI) {
+      RodSelf S;
+      rod_self_init(S, rod_i, tr_i, p.w, st.bscale[t], fp.visc);
+      for (int k = 0; k < c; k++) {
+        int u = row[(int64_t)k * ncap];
+        if (st.type[u] != rec::FUNGI) continue;
+        double4 o = in.pos[u];
+        double4 rod_j = in.rod[u];
+        double d[3];
+        d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+        d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+        d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double rlm = 1.5 * (rod_i.x + rod_j.x) + fp.WA;  // K.H:33-37
+        double lim = rlm - 1.0e-15;
+        if (r2 <= lim * lim) {
+          Topo tj = in.topo[u];
+          rod_pair_i(d, S, ti, tj, rod_j.x, o.w, st.bscale[u], in.trig[u], fp,
+                     acc);
+        }
+      }
+      double vs[3], ws[3];
+      rod_surface(p, S, fp, vs, ws);
+      acc[0] += vs[0];
+      acc[1] += vs[1];
+      acc[2] += vs[2];
+      acc[3] += ws[0];
+      acc[4] += ws[1];
+      acc[5] += ws[2];
+      ti.fix_site = 0;  // K.H:1060
+    }
+    double vx = fp.mob * acc[0], vy = fp.mob * acc[1], vz = fp.mob * acc[2];
+    double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
+    x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+    y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+    z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+    out.pos[t] = make_double4(x, y, z, p.w);
+    if (ty == rec::FUNGI) {
+      double th = rod_i.y, ph = rod_i.z;
+      double w[3] = {acc[3], acc[4], acc[5]};
+      rod_rotate(th, ph, tr_i, rod_i.x, w, fp.dt);
+      out.rod[t] = make_double4(rod_i.x, th, ph, rod_i.w);
+      out.trig[t] = make_double4(cos(th), sin(th), cos(ph), sin(ph));
+    } else {
+      out.rod[t] = rod_i;
+      out.trig[t] = tr_i;
+    }
+    out.topo[t] = ti;
+    if (fc) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) fc[t + k * ncap] = acc[k];
+    }
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < 6; k++) fin = fin && isfinite(acc[k]);
+    if (!fin) atomicOr(err, ERRBIT_NAN);
+    v2 = vx * vx + vy * vy + vz * vz;
+  }
+  if (vmax) block_vmax(v2, fp.dt, vmax);
+}
+
+// InitBonds: per segment, scan the row in order and append a bond for the
+// first site pair closer than rlim (K.H:1285-1444 with rlim = 1e-7, PCI:508).
+template <bool MULTI>
+__global__ void k_init_bonds(DynArrays dy, StaticArrays st,
+                             const int32_t *__restrict__ nbr,
+                             const int32_t *__restrict__ ncnt, int64_t ncap,
+                             int64_t n, const BoxGeom *__restrict__ boxes,
+                             BoxGeom g0, ForceParams fp,
+                             int32_t *__restrict__ err) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || st.type[t] != rec::FUNGI) return;
+  const BoxGeom &g = MULTI ? boxes[st.col[t]] : g0;
+  const double rlim = 1.0e-7;
+  double4 p = dy.pos[t];
+  double4 rod_i = dy.rod[t];
+  double4 tr = dy.trig[t];
+  double Li = rod_i.x;
+  double nxi = tr.y * tr.z, nyi = tr.y * tr.w, nzi = tr.x;
+  Topo ti = dy.topo[t];
+  int c = ncnt[t];
+  for (int k = 0; k < c; k++) {
+    int u = nbr[(int64_t)k * ncap + t];
+    if (st.type[u] != rec::FUNGI) continue;
+    double4 o = dy.pos[u];
+    double4 rod_j = dy.rod[u];
+    double d[3];
+    d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+    d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+    d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double lim = 1.5 * (Li + rod_j.x) + fp.WA - 1.0e-15;
+    if (!(r2 <= lim * lim)) continue;
+    double4 trj = dy.trig[u];
+    double Lj = rod_j.x;
+    double nxj = trj.y * trj.z, nyj = trj.y * trj.w, nzj = trj.x;
+    double xi1 = d[0] - 0.5 * nxi * Li, yi1 = d[1] - 0.5 * nyi * Li,
+           zi1 = d[2] - 0.5 * nzi * Li;
+    double xi2 = d[0] + 0.5 * nxi * Li, yi2 = d[1] + 0.5 * nyi * Li,
+           zi2 = d[2] + 0.5 * nzi * Li;
+    double xj1 = -0.5 * nxj * Lj, yj1 = -0.5 * nyj * Lj, zj1 = -0.5 * nzj * Lj;
+    double xj2 = 0.5 * nxj * Lj, yj2 = 0.5 * nyj * Lj, zj2 = 0.5 * nzj * Lj;
+    // site pairs in the reference's order 1-1, 2-1, 1-2, 2-2
+    for (int cse = 0; cse < 4; cse++) {
+      bool i2 = (cse & 1) != 0, j2 = (cse & 2) != 0;
+      double rx = (i2 ? xi2 : xi1) - (j2 ? xj2 : xj1);
+      double ry = (i2 ? yi2 : yi1) - (j2 ? yj2 : yj1);
+      double rz = (i2 ? zi2 : zi1) - (j2 ? zj2 : zj1);
+      double r = sqrt(rx * rx + ry * ry + rz * rz);
+      if (r < rlim) {
+        int nb = ti.n_bnds;
+        if (nb < 0 || nb > 3) {
+          atomicOr(err, ERRBIT_BOND);
+        } else {
+          ti.seg_id[nb] = dy.topo[u].self_id;
+          ti.seg_cpu[nb] = dy.topo[u].self_cpu;
+          ti.site[nb] = i2 ? 2 : 1;
+          ti.n_bnds = nb + 1;
+        }
+        break;
+      }
+    }
+  }
+  dy.topo[t].n_bnds = ti.n_bnds;
+  for (int b = 0; b < 4; b++) {
+    dy.topo[t].seg_id[b] = ti.seg_id[b];
+    dy.topo[t].seg_cpu[b] = ti.seg_cpu[b];
+    dy.topo[t].site[b] = ti.site[b];
+  }
+}
+
+__global__ void k_skin_check(unsigned long long *vmax, int32_t nslots,
+                             double limit, int32_t *err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < nslots; i++) {
+    s += __longlong_as_double((long long)vmax[i]);
+    vmax[i] = 0ull;
+  }
+  if (s > limit) atomicOr(err, ERRBIT_SKIN);
+}
+
+// ------------------------------------------------------------- exports ---
+
+__global__ void k_export_fc(const double *__restrict__ fc, int64_t ncap,
+                            const int32_t *__restrict__ oid, int64_t n,
+                            double *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t o = oid[t];
+  for (int c = 0; c < 6; c++) out[c * n + o] = fc[c * ncap + t];
+}
+__global__ void k_export_pos(const double4 *__restrict__ pos,
+                             const int32_t *__restrict__ oid, int64_t n,
+                             double *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t o = oid[t];
+  double4 p = pos[t];
+  out[3 * o] = p.x;
+  out[3 * o + 1] = p.y;
+  out[3 * o + 2] = p.z;
+}
+__global__ void k_export_orient(const double4 *__restrict__ rod,
+                                const int32_t *__restrict__ oid, int64_t n,
+                                double *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t o = oid[t];
+  double4 r = rod[t];
+  out[2 * o] = r.y;
+  out[2 * o + 1] = r.z;
+}
+__global__ void k_export_i32(const int32_t *__restrict__ v,
+                             const int32_t *__restrict__ oid, int64_t n,
+                             int32_t *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[oid[t]] = v[t];
+}
+__global__ void k_export_counts(const int32_t *__restrict__ ncnt,
+                                const int32_t *__restrict__ oid, int64_t n,
+                                int64_t *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[oid[t]] = ncnt[t];
+}
+__global__ void k_export_nbr(const int32_t *__restrict__ nbr,
+                             const int32_t *__restrict__ ncnt, int64_t ncap,
+                             const int32_t *__restrict__ oid, int64_t n,
+                             const int64_t *__restrict__ row_off,
+                             int32_t *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t base = row_off[oid[t]];
+  int c = ncnt[t];
+  for (int k = 0; k < c; k++) out[base + k] = oid[nbr[(int64_t)k * ncap + t]];
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- launchers ---
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_capped(n, 256)), dim3(256), 0,
+                     s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(unsigned long long *p, int64_t n,
+                           unsigned long long v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(grid_capped(n, 256)),
+                     dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
+                             const int32_t *col, const BoxGeom *boxes,
+                             StaticArrays st, DynArrays dy,
+                             unsigned long long *stats, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_unpack, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     aos, lay, n, col, boxes, st, dy, stats);
+  return hipGetLastError();
+}
+hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
+                           StaticArrays st, DynArrays dy, const double *fc,
+                           int64_t ncap, double mob, int wrote_step,
+                           int any_fungi, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_pack, dim3(grid_for(n, 256)), dim3(256), 0, s, aos,
+                     lay, n, st, dy, fc, ncap, mob, wrote_step, any_fungi);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
+                            const BoxGeom *boxes, BoxGeom g0, int multi,
+                            int64_t n, int32_t *key, int32_t *rank,
+                            int32_t *count, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (multi)
+    hipLaunchKernelGGL(k_bin_count<true>, dim3(grid_for(n, 256)), dim3(256), 0,
+                       s, pos, col, boxes, g0, n, key, rank, count);
+  else
+    hipLaunchKernelGGL(k_bin_count<false>, dim3(grid_for(n, 256)), dim3(256),
+                       0, s, pos, col, boxes, g0, n, key, rank, count);
+  return hipGetLastError();
+}
+
+int64_t scan_scratch_elems(int64_t nb) {
+  return (nb + kScanTile - 1) / kScanTile + 1;
+}
+
+hipError_t launch_scan(const int32_t *count, int32_t *start, int64_t nb,
+                       int32_t *scratch, hipStream_t s) {
+  int64_t nt = (nb + kScanTile - 1) / kScanTile;
+  if (nt < 1) nt = 1;
+  if (nt > 1024 * 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nt), dim3(kScanThreads), 0,
+                     s, count, start, scratch, nb);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, scratch, nt,
+                     start + nb);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(kScanThreads), 0, s,
+                     start, scratch, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
+                              const int32_t *start, const int32_t *oid,
+                              int64_t n, unsigned long long *permk,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     key, rank, start, oid, n, permk);
+  return hipGetLastError();
+}
+hipError_t launch_bin_sort(const int32_t *start, int64_t nb,
+                           unsigned long long *permk, int32_t *err,
+                           hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_sort, dim3(grid_for(nb, 256)), dim3(256), 0, s,
+                     start, nb, permk, err);
+  return hipGetLastError();
+}
+hipError_t launch_gather(const unsigned long long *permk, int64_t n,
+                         StaticArrays src_s, StaticArrays dst_s,
+                         DynArrays src_d, DynArrays dst_d, const int32_t *key,
+                         int multi, int rods, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, permk,
+                     n, src_s, dst_s, src_d, dst_d, key, multi, rods);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
+                            const BoxGeom *boxes, BoxGeom g0, int multi,
+                            const int32_t *start, const int32_t *key,
+                            int64_t n, double crit, int32_t *nbr,
+                            int32_t *ncnt, int64_t ncap, int32_t maxn,
+                            int32_t *err, int32_t *stats,
+                            unsigned long long *total, LaunchShape ls,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int b = ls.block;
+  if (multi)
+    hipLaunchKernelGGL(k_build_nbr<true>, dim3(grid_for(n, b)), dim3(b), 0, s,
+                       pos, col, boxes, g0, start, key, n, crit, nbr, ncnt,
+                       ncap, maxn, err, stats, total, ls.remap);
+  else
+    hipLaunchKernelGGL(k_build_nbr<false>, dim3(grid_for(n, b)), dim3(b), 0, s,
+                       pos, col, boxes, g0, start, key, n, crit, nbr, ncnt,
+                       ncap, maxn, err, stats, total, ls.remap);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
+                               const int32_t *nbr, const int32_t *ncnt,
+                               int64_t ncap, int64_t n, const int32_t *col,
+                               const BoxGeom *boxes, BoxGeom g0, int multi,
+                               ForceParams fp, double *fc,
+                               unsigned long long *vmax, int32_t *err,
+                               LaunchShape ls, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int b = ls.block;
+  if (multi)
+    hipLaunchKernelGGL(k_force_sphere<true>, dim3(grid_for(n, b)), dim3(b), 0,
+                       s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
+                       fp, fc, vmax, err, ls.remap);
+  else
+    hipLaunchKernelGGL(k_force_sphere<false>, dim3(grid_for(n, b)), dim3(b), 0,
+                       s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
+                       fp, fc, vmax, err, ls.remap);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
+                                const int32_t *nbr, const int32_t *ncnt,
+                                int64_t ncap, int64_t n, const BoxGeom *boxes,
+                                BoxGeom g0, int multi, ForceParams fp,
+                                double *fc, unsigned long long *vmax,
+                                int32_t *err, LaunchShape ls, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int b = ls.block > 256 ? 256 : ls.block;
+  if (multi)
+    hipLaunchKernelGGL(k_force_general<true>, dim3(grid_for(n, b)), dim3(b), 0,
+                       s, in, out, st, nbr, ncnt, ncap, n, boxes, g0, fp, fc,
+                       vmax, err, ls.remap);
+  else
+    hipLaunchKernelGGL(k_force_general<false>, dim3(grid_for(n, b)), dim3(b), 0,
+                       s, in, out, st, nbr, ncnt, ncap, n, boxes, g0, fp, fc,
+                       vmax, err, ls.remap);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
+                             const int32_t *ncnt, int64_t ncap, int64_t n,
+                             const BoxGeom *boxes, BoxGeom g0, int multi,
+                             ForceParams fp, int32_t *err, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (multi)
+    hipLaunchKernelGGL(k_init_bonds<true>, dim3(grid_for(n, 128)), dim3(128),
+                       0, s, dy, st, nbr, ncnt, ncap, n, boxes, g0, fp, err);
+  else
+    hipLaunchKernelGGL(k_init_bonds<false>, dim3(grid_for(n, 128)), dim3(128),
+                       0, s, dy, st, nbr, ncnt, ncap, n, boxes, g0, fp, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_skin_check(unsigned long long *vmax, int32_t nslots,
+                             double limit, int32_t *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_skin_check, dim3(1), dim3(64), 0, s, vmax, nslots,
+                     limit, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_fc(const double *fc, int64_t ncap, const int32_t *oid,
+                            int64_t n, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_fc, dim3(grid_for(n, 256)), dim3(256), 0, s, fc,
+                     ncap, oid, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,
+                             double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_pos, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     pos, oid, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_export_orient(const double4 *rod, const int32_t *oid,
+                                int64_t n, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_orient, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     rod, oid, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_export_i32(const int32_t *v, const int32_t *oid, int64_t n,
+                             int32_t *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, v,
+                     oid, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_export_counts(const int32_t *ncnt, const int32_t *oid,
+                                int64_t n, int64_t *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_counts, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     ncnt, oid, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_export_nbr(const int32_t *nbr, const int32_t *ncnt,
+                             int64_t ncap, const int32_t *oid, int64_t n,
+                             const int64_t *row_off, int32_t *out,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_nbr, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     nbr, ncnt, ncap, oid, n, row_off, out);
+  return hipGetLastError();
+}
+
+}  // namespace bmx

@@ -1,0 +1,233 @@
+/*
+ * bmx_mi355x.h -- C ABI of the MI355X particle-mechanics library.
+ *
+ * Replaces, for one AMReX level, the particle-mechanics path of BMX's
+ * BMXParticleContainer (src/des/bmx_pc.H:86-174):
+ *   - neighbor binning and list construction done by AMReX's
+ *     NeighborParticleContainer (clearNeighbors / Redistribute /
+ *     fillNeighbors / buildNeighborList, src/des/bmx_pc_interaction.cpp:97-106),
+ *   - the force lambda (src/des/bmx_pc_interaction.cpp:170-279) over the
+ *     force laws of src/chemistry/bmx_cell_interaction_K.H,
+ *   - the overdamped update lambda (src/des/bmx_pc_interaction.cpp:311-449),
+ *   - InitBonds (src/des/bmx_pc_interaction.cpp:500-668).
+ *
+ * Plain C types only.  Every call returns a bmx_status; bmx_last_error()
+ * gives a message for the calling thread's last failure.  A context owns one
+ * HIP stream and all device buffers; calls on one context must not overlap.
+ */
+#ifndef BMX_MI355X_H
+#define BMX_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMX_ABI_VERSION 1
+
+typedef struct bmx_ctx bmx_ctx;
+
+typedef enum {
+  BMX_OK = 0,
+  BMX_ERR_BAD_ARG = 1,       /* invalid argument or geometry */
+  BMX_ERR_OOM = 2,           /* device or host allocation failed */
+  BMX_ERR_HIP = 3,           /* HIP runtime error */
+  BMX_ERR_NAN = 4,           /* non-finite force (coincident particles) */
+  BMX_ERR_NBR_OVERFLOW = 5,  /* a neighbor row exceeded max_neighbors */
+  BMX_ERR_SKIN = 6,          /* displacement bound exceeded skin/2 between
+                                rebuilds: the pair set may be incomplete */
+  BMX_ERR_STATE = 7,         /* call order (e.g. evolve before bind) */
+  BMX_ERR_IO = 8,            /* file parse / write failure */
+  BMX_ERR_BOND_OVERFLOW = 9, /* InitBonds found a fifth bond */
+  BMX_ERR_NO_DEVICE = 10     /* no HIP device / library not usable */
+} bmx_status;
+
+/* Where the fields of one particle record sit (bytes).  The default is
+ * AMReX Particle<34,35> as BMX declares it (src/des/bmx_pc.H:86-96):
+ * pos[3] f64 @0, rdata[34] f64 @24, id i32 @296, cpu i32 @300,
+ * idata[35] i32 @304, stride 448. */
+typedef struct {
+  int32_t stride_bytes;
+  int32_t pos_offset;
+  int32_t rdata_offset;
+  int32_t id_offset;
+  int32_t cpu_offset;
+  int32_t idata_offset;
+  int32_t n_real;
+  int32_t n_int;
+} bmx_layout;
+
+/* Level geometry (Geom(lev).ProbLo/ProbHi, geometry.is_periodic) and the
+ * level mesh n_cell, which sets the bins in BMX_NBR_STENCIL mode. */
+typedef struct {
+  double prob_lo[3];
+  double prob_hi[3];
+  int32_t periodic[3];
+  int32_t n_cell[3];
+} bmx_geometry;
+
+/* Neighbor-list policy.
+ *   BMX_NBR_STENCIL: candidates are all particles in the +-1 bin stencil of
+ *     the level mesh whose squared separation is <= 'neighborhood'
+ *     (BMXCheckPair, src/des/bmx_pc.H:63-81); rebuilt every step as the
+ *     reference does (src/des/bmx_pc_interaction.cpp:97-103).
+ *   BMX_NBR_VERLET: candidates within max interaction distance + skin, bins
+ *     no smaller than that radius, rebuilt every rebuild_every steps.  The set
+ *     of interacting pairs is identical to the stencil list's as long as
+ *     BMX_ERR_SKIN is not raised. */
+enum { BMX_NBR_VERLET = 0, BMX_NBR_STENCIL = 1 };
+
+typedef struct {
+  int32_t mode;           /* BMX_NBR_VERLET or BMX_NBR_STENCIL */
+  int32_t rebuild_every;  /* steps between rebuilds (>= 1) */
+  int32_t max_neighbors;  /* row capacity (0 = default 96) */
+  int32_t bin_factor_pct; /* VERLET: bin edge >= radius*pct/100 (0 = 100) */
+  double skin;            /* VERLET: skin added to the max r_lm */
+  double neighborhood;    /* STENCIL: DEM::neighborhood
+                             (src/chemistry/bmx_chem.cpp:124) */
+} bmx_neighbor_params;
+
+/* Launch shape of the pair kernels (the autotuner's search space). */
+typedef struct {
+  int32_t block_size;     /* threads per block: 64, 128, 256, 512 */
+  int32_t xcd_remap;      /* 1: give each of the 8 XCDs a contiguous range
+                             of particle tiles */
+} bmx_launch_params;
+
+typedef struct {
+  double force_ms;        /* summed duration of force+update launches */
+  int64_t force_launches;
+  double rebuild_ms;      /* summed duration of binning + list builds */
+  int64_t rebuilds;
+  int64_t neighbor_entries; /* total list entries at the last rebuild */
+  int32_t max_row;          /* longest row at the last rebuild */
+  int32_t nbins;            /* bins at the last rebuild */
+} bmx_timing;
+
+/* --- lifetime ---------------------------------------------------------- */
+int32_t bmx_abi_version(void);
+const char *bmx_last_error(void);
+/* device = HIP ordinal */
+bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device);
+void bmx_ctx_destroy(bmx_ctx *ctx);
+
+/* --- configuration ----------------------------------------------------- */
+/* fpar[9] = {boundary_width W_A, stiffness mu, z_wall, wall_boundary_width,
+ *            wall_stiffness, gravity, bond_strength, bond_cutoff,
+ *            viscous_drag}: BMXCellInteraction::getForceParams()
+ *            (src/chemistry/bmx_cell_interaction.cpp:84-104). */
+bmx_status bmx_set_force_params(bmx_ctx *ctx, const double fpar[9]);
+/* Overdamped mobility m: vel = m * fc.  The reference has m = 1
+ * (src/des/bmx_pc_interaction.cpp:321-323), which is the default. */
+bmx_status bmx_set_mobility(bmx_ctx *ctx, double mobility);
+bmx_status bmx_set_geometry(bmx_ctx *ctx, const bmx_geometry *geom);
+bmx_status bmx_set_neighbor_params(bmx_ctx *ctx,
+                                   const bmx_neighbor_params *p);
+bmx_status bmx_set_launch_params(bmx_ctx *ctx, const bmx_launch_params *p);
+bmx_layout bmx_default_layout(void);
+
+/* --- particles --------------------------------------------------------- */
+/* Copies n records (host memory when aos_on_device == 0, device memory
+ * otherwise) into the context and applies the periodic wrap that
+ * Redistribute performs (src/des/bmx_pc_init.cpp:131).  The record array is
+ * remembered for bmx_sync_to_aos. */
+bmx_status bmx_bind_particles_aos(bmx_ctx *ctx, const void *aos, int64_t n,
+                                  const bmx_layout *layout,
+                                  int32_t aos_on_device);
+/* Several independent periodic boxes ("colonies") in one context: particles
+ * [colony_start[c], colony_start[c+1]) live in geoms[c].  Each colony's
+ * neighbor search, bins and wrap are its own. */
+bmx_status bmx_bind_colonies_aos(bmx_ctx *ctx, const void *aos,
+                                 const int64_t *colony_start, int32_t ncol,
+                                 const bmx_geometry *geoms,
+                                 const bmx_layout *layout,
+                                 int32_t aos_on_device);
+/* Writes positions, velocities, angular velocities, orientation angles and
+ * the integer fields the force pass changes back into 'aos' (host or device,
+ * as bound; NULL = the bound array). */
+bmx_status bmx_sync_to_aos(bmx_ctx *ctx, void *aos);
+int64_t bmx_num_particles(const bmx_ctx *ctx);
+
+/* --- the hot path ------------------------------------------------------ */
+/* InitBonds: bond detection between touching segment ends (rlim = 1e-7,
+ * src/des/bmx_pc_interaction.cpp:508). */
+bmx_status bmx_init_bonds(bmx_ctx *ctx);
+/* EvolveParticles: nsteps explicit steps of size dt (nsubsteps == 1 per
+ * step, src/des/bmx_pc_interaction.cpp:56-65).  Synchronizes before
+ * returning and reports list overflow, skin violation or non-finite forces. */
+bmx_status bmx_evolve(bmx_ctx *ctx, double dt, int32_t nsteps);
+/* Same steps, replayed from one captured HIP graph (captured on first use
+ * for this (dt, nsteps) and re-used while the configuration is unchanged). */
+bmx_status bmx_evolve_graph(bmx_ctx *ctx, double dt, int32_t nsteps);
+
+/* --- inspection (original particle order) ------------------------------ */
+/* fc of the last step, 6*n doubles, component-major: the reference's
+ * fc[i + c*ntot] (src/des/bmx_pc_interaction.cpp:244-249). */
+bmx_status bmx_get_fc(bmx_ctx *ctx, double *out6n);
+/* positions, n*3 doubles (x,y,z per particle) */
+bmx_status bmx_get_positions(bmx_ctx *ctx, double *out3n);
+/* theta, phi per particle (2*n doubles, interleaved) */
+bmx_status bmx_get_orientations(bmx_ctx *ctx, double *out2n);
+/* integer record fields idata[0..34] per particle (35*n int32) */
+bmx_status bmx_get_int_fields(bmx_ctx *ctx, int32_t *out35n);
+/* Current list, rows in original order, entries as original indices in the
+ * order the force pass sums them.  offsets has n+1 entries; *total receives
+ * the entry count; indices may be NULL to query the size. */
+bmx_status bmx_get_neighbors(bmx_ctx *ctx, int64_t *offsets,
+                             int32_t *indices, int64_t capacity,
+                             int64_t *total);
+/* Linear bin id (bz*nby + by)*nbx + bx of each particle at the last rebuild,
+ * bins from floor((pos - prob_lo) * dxi) (src/interpolation/bmx_interp_K.H:55-56). */
+bmx_status bmx_get_bins(bmx_ctx *ctx, int32_t *out_n);
+/* bins per dimension in use for colony c */
+bmx_status bmx_get_bin_dims(bmx_ctx *ctx, int32_t colony, int32_t out3[3]);
+
+/* --- measurement ------------------------------------------------------- */
+/* When enabled, every force and rebuild launch is bracketed by HIP events on
+ * the context's stream. */
+bmx_status bmx_set_timing(bmx_ctx *ctx, int32_t enable);
+bmx_status bmx_get_timing(bmx_ctx *ctx, bmx_timing *out);
+bmx_status bmx_reset_timing(bmx_ctx *ctx);
+bmx_status bmx_synchronize(bmx_ctx *ctx);
+
+/* --- particle files and synthetic inputs (host only) ------------------- */
+/* InitParticlesAscii format (src/des/bmx_pc_init.cpp:32-104): the count on
+ * line 1, then per particle x y z radius c_length theta phi area vol velx
+ * vely velz wx wy wz fx fy fz taux tauy tauz gx gy gz dadt dvdt cell_type
+ * (extra trailing columns ignored).  Records are zeroed, then bond_scale = 1,
+ * fuse_tip = 0, id = index + 1, cpu = 0 (mirrored into idata).  records ==
+ * NULL queries the count. */
+bmx_status bmx_read_particles_ascii(const char *path, void *records,
+                                    int64_t capacity, int64_t *n_out,
+                                    const bmx_layout *layout);
+bmx_status bmx_write_particles_ascii(const char *path, const void *records,
+                                     int64_t n, const bmx_layout *layout);
+/* E[4/3 pi r^3] for r uniform in [rmin, rmax] */
+double bmx_mean_sphere_volume(double rmin, double rmax);
+/* n spheres, radius uniform in [rmin, rmax], in a box of the given aspect
+ * sized to volume fraction phi (lo = 0); periodic axes are filled
+ * uniformly, other axes keep spheres inside [r, L - r].  overlap_free = 1
+ * uses random sequential insertion. */
+bmx_status bmx_generate_spheres(void *records, int64_t n, double rmin,
+                                double rmax, double phi, uint64_t seed,
+                                const double aspect[3],
+                                const int32_t periodic[3],
+                                int32_t overlap_free, double out_lo[3],
+                                double out_hi[3], const bmx_layout *layout);
+/* n rod segments grown as branching chains from nroots roots lying in the
+ * plane above z_surface (each tree at its own height in [r, 3r] above it):
+ * every tip extends straight, and with probability branch_prob a segment
+ * also starts a side branch at +-[angle_min, angle_max] degrees.  Chained
+ * ends coincide, so InitBonds bonds them. */
+bmx_status bmx_generate_hyphae(void *records, int64_t n, int32_t nroots,
+                               double seg_len, double seg_rad,
+                               double branch_prob, double angle_min_deg,
+                               double angle_max_deg, uint64_t seed,
+                               const double box_lo[3], const double box_hi[3],
+                               double z_surface, const bmx_layout *layout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

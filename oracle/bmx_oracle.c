@@ -1,0 +1,1021 @@
+/*
+ * bmx_oracle.c -- CPU restatement of the BMX particle-mechanics path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never shipped
+ * with it and never called by it.  Build: oracle/Makefile (-O2
+ * -ffp-contract=off), output oracle/_build/liboracle_bmx.so.
+ *
+ * Each function states the reference file:line it follows.  Floating-point
+ * expressions keep the reference's association order (C evaluates a*b*c as
+ * (a*b)*c, and -s*a*b as ((-s)*a)*b); with contraction off every operation
+ * is rounded exactly as the reference's CPU build rounds it.
+ */
+#include "bmx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846
+
+/* K.H:23-40 */
+double orc_max_interaction_distance(const double *p1, const double *p2,
+                                    const int32_t *i1, const int32_t *i2,
+                                    double bndry_width) {
+  if (i1[ORC_I_CELL_TYPE] == ORC_YEAST && i2[ORC_I_CELL_TYPE] == ORC_YEAST) {
+    return p1[ORC_R_RADIUS] + p2[ORC_R_RADIUS] + bndry_width;
+  } else if (i1[ORC_I_CELL_TYPE] == ORC_FUNGI &&
+             i2[ORC_I_CELL_TYPE] == ORC_FUNGI) {
+    return 1.5 * (p1[ORC_R_CLENGTH] + p2[ORC_R_CLENGTH]) + bndry_width;
+  }
+  return 0.0;
+}
+
+static int in01(double t) { return t >= 0.0 && t <= 1.0; }
+
+/* K.H:51-221: closest points of two segments, tau in [0,1] along each. */
+void orc_min_distance(double xi1, double yi1, double zi1, double xi2,
+                      double yi2, double zi2, double xj1, double yj1,
+                      double zj1, double xj2, double yj2, double zj2,
+                      double *tau_i, double *tau_j) {
+  double ri0 = xi2 - xi1, ri1 = yi2 - yi1, ri2 = zi2 - zi1;
+  double rj0 = xj2 - xj1, rj1 = yj2 - yj1, rj2 = zj2 - zj1;
+  double rirj = 0.0;
+  rirj += ri0 * rj0;
+  rirj += ri1 * rj1;
+  rirj += ri2 * rj2;
+  double Aii = ri0 * ri0 + ri1 * ri1 + ri2 * ri2;
+  double Aij = -rirj;
+  double Aji = Aij;
+  double Ajj = rj0 * rj0 + rj1 * rj1 + rj2 * rj2;
+  double det = Aii * Ajj - Aij * Aji;
+  double Bi = -((xi1 - xj1) * ri0 + (yi1 - yj1) * ri1 + (zi1 - zj1) * ri2);
+  double Bj = (xi1 - xj1) * rj0 + (yi1 - yj1) * rj1 + (zi1 - zj1) * rj2;
+  int search = 0;
+  if (Aij == 0.0) {
+    *tau_i = Bi / Aii;
+    *tau_j = Bj / Ajj;
+  } else if (det == 0.0) {
+    double tj0 = Bj / Ajj;
+    if (in01(tj0)) {
+      *tau_i = 0.0;
+      *tau_j = tj0;
+    } else {
+      double tj1 = (Bj + rirj) / Ajj;
+      if (in01(tj1)) {
+        *tau_i = 1.0;
+        *tau_j = tj1;
+      } else {
+        search = 1;
+      }
+    }
+  } else {
+    *tau_i = (Ajj * Bi - Aij * Bj) / det;
+    *tau_j = (Aii * Bj - Aji * Bi) / det;
+    if (in01(*tau_i)) {
+      if (*tau_j < 0.0 || *tau_j > 1.0) {
+        double ti0 = Bi / Aii;
+        double ti1 = (Bi + rirj) / Aii;
+        if (in01(ti0) && in01(ti1)) {
+          double rx = xi1 + ti0 * ri0, ry = yi1 + ti0 * ri1, rz = zi1 + ti0 * ri2;
+          double sep0 = (rx - xj1) * (rx - xj1) + (ry - yj1) * (ry - yj1) +
+                        (rz - zj1) * (rz - zj1);
+          rx = xi1 + ti1 * ri0;
+          ry = yi1 + ti1 * ri1;
+          rz = zi1 + ti1 * ri2;
+          double sep1 = (rx - xj2) * (rx - xj2) + (ry - yj2) * (ry - yj2) +
+                        (rz - zj2) * (rz - zj2);
+          if (sep0 < sep1) {
+            *tau_j = 0.0;
+            *tau_i = ti0;
+          } else {
+            *tau_j = 1.0;
+            *tau_i = ti1;
+          }
+        } else if (in01(ti0)) {
+          *tau_j = 0.0;
+          *tau_i = ti0;
+        } else if (in01(ti1)) {
+          *tau_j = 1.0;
+          *tau_i = ti1;
+        } else {
+          search = 1;
+        }
+      }
+    } else if (in01(*tau_j)) {
+      double tj0 = Bj / Ajj;
+      double tj1 = (Bj + rirj) / Ajj;
+      if (in01(tj0) && in01(tj1)) {
+        double rx = xj1 + tj0 * rj0, ry = yj1 + tj0 * rj1, rz = zj1 + tj0 * rj2;
+        double sep0 = (rx - xi1) * (rx - xi1) + (ry - yi1) * (ry - yi1) +
+                      (rz - zi1) * (rz - zi1);
+        rx = xj1 + tj1 * rj0;
+        ry = yj1 + tj1 * rj1;
+        rz = zj1 + tj1 * rj2;
+        double sep1 = (rx - xi2) * (rx - xi2) + (ry - yi2) * (ry - yi2) +
+                      (rz - zi2) * (rz - zi2);
+        if (sep0 < sep1) {
+          *tau_i = 0.0;
+          *tau_j = tj0;
+        } else {
+          *tau_i = 1.0;
+          *tau_j = tj1;
+        }
+      } else if (in01(tj0)) {
+        *tau_i = 0.0;
+        *tau_j = tj0;
+      } else if (in01(tj1)) {
+        *tau_i = 1.0;
+        *tau_j = tj1;
+      } else {
+        search = 1;
+      }
+    } else {
+      search = 1;
+    }
+  }
+  if (search) {
+    double m0 = (xi1 - xj1) * (xi1 - xj1) + (yi1 - yj1) * (yi1 - yj1) +
+                (zi1 - zj1) * (zi1 - zj1);
+    double m1 = (xi1 - xj2) * (xi1 - xj2) + (yi1 - yj2) * (yi1 - yj2) +
+                (zi1 - zj2) * (zi1 - zj2);
+    double m2 = (xi2 - xj1) * (xi2 - xj1) + (yi2 - yj1) * (yi2 - yj1) +
+                (zi2 - zj1) * (zi2 - zj1);
+    double m3 = (xi2 - xj2) * (xi2 - xj2) + (yi2 - yj2) * (yi2 - yj2) +
+                (zi2 - zj2) * (zi2 - zj2);
+    int imin = 0;
+    double mn = m0;
+    if (m1 < mn) { imin = 1; mn = m1; }
+    if (m2 < mn) { imin = 2; mn = m2; }
+    if (m3 < mn) { imin = 3; mn = m3; }
+    *tau_i = (imin >= 2) ? 1.0 : 0.0;
+    *tau_j = (imin & 1) ? 1.0 : 0.0;
+  }
+}
+
+/* Rotation taking the body x-axis to the segment axis (K.H:574-583). */
+static void seg_rotation(double ct, double st, double cp, double sp,
+                         double R[3][3]) {
+  R[0][0] = cp * st; R[0][1] = -sp; R[0][2] = -cp * ct;
+  R[1][0] = sp * st; R[1][1] = cp;  R[1][2] = -sp * ct;
+  R[2][0] = ct;      R[2][1] = 0.0; R[2][2] = st;
+}
+
+/* Two-site mobility tensors in the lab frame (K.H:812-882 / 895-959). */
+static void seg_mobility(double area, double c_length, double visc,
+                         double R[3][3], double dii[3][3], double dij[3][3]) {
+  area /= 2.0;
+  double arad = sqrt(area / (4.0 * ORC_PI));
+  double d0[3][3], d1[3][3], tmp[3][3];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      d0[a][b] = (a == b) ? 1.0 / (6.0 * ORC_PI * arad * visc) : 0.0;
+      d1[a][b] = 0.0;
+    }
+  if (c_length > 2.0 * arad) {
+    d1[0][0] = (c_length * c_length - 2.0 * arad * arad / 3.0) /
+               (4.0 * ORC_PI * pow(c_length, 3) * visc);
+    d1[1][1] = (c_length * c_length + 2.0 * arad * arad / 3.0) /
+               (8.0 * ORC_PI * pow(c_length, 3) * visc);
+    d1[2][2] = d1[1][1];
+  } else {
+    d1[0][0] = (1.0 - 3.0 * c_length / (16.0 * arad)) /
+               (6.0 * ORC_PI * arad * visc);
+    d1[1][1] = (1.0 - 9.0 * c_length / (32.0 * arad)) /
+               (6.0 * ORC_PI * arad * visc);
+    d1[2][2] = d1[1][1];
+  }
+  /* D <- R * D * R^T, accumulated in the reference's loop order */
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      tmp[a][b] = 0.0;
+      for (int k = 0; k < 3; k++) tmp[a][b] += d0[a][k] * R[b][k];
+    }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      dii[a][b] = 0.0;
+      for (int k = 0; k < 3; k++) dii[a][b] += R[a][k] * tmp[k][b];
+    }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      tmp[a][b] = 0.0;
+      for (int k = 0; k < 3; k++) tmp[a][b] += d1[a][k] * R[b][k];
+    }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      dij[a][b] = 0.0;
+      for (int k = 0; k < 3; k++) dij[a][b] += R[a][k] * tmp[k][b];
+    }
+}
+
+/* Append a partner into the first free bond slot (K.H:422-458). */
+static void add_bond_slot(int32_t *ip, int nb, int id, int cpu, int site,
+                          int *ids, int *cpus, int *sites) {
+  if (nb < 0 || nb > 3) return;
+  ip[ORC_I_SEG1_ID1 + nb] = id;
+  ip[ORC_I_SEG1_ID2 + nb] = cpu;
+  ip[ORC_I_SITE1 + nb] = site;
+  ids[nb] = id;
+  cpus[nb] = cpu;
+  sites[nb] = site;
+  ip[ORC_I_N_BNDS] = nb + 1;
+}
+
+static void remove_axial(double f1[3], double f2[3], double rx, double ry,
+                         double rz) {
+  double r = sqrt(rx * rx + ry * ry + rz * rz);
+  double nx = rx / r, ny = ry / r, nz = rz / r;
+  double fx = f1[0] - f2[0], fy = f1[1] - f2[1], fz = f1[2] - f2[2];
+  double fn = fx * nx + fy * ny + fz * nz;
+  f1[0] -= 0.5 * fn * nx;
+  f1[1] -= 0.5 * fn * ny;
+  f1[2] -= 0.5 * fn * nz;
+  f2[0] += 0.5 * fn * nx;
+  f2[1] += 0.5 * fn * ny;
+  f2[2] += 0.5 * fn * nz;
+}
+
+static void site_velocities(double dii[3][3], double dij[3][3],
+                            const double f1[3], const double f2[3],
+                            double v1[3], double v2[3]) {
+  for (int a = 0; a < 3; a++) {
+    v1[a] = 0.0;
+    v2[a] = 0.0;
+    for (int b = 0; b < 3; b++) {
+      v1[a] += dii[a][b] * f1[b] + dij[a][b] * f2[b];
+      v2[a] += dii[a][b] * f2[b] + dij[a][b] * f1[b];
+    }
+  }
+}
+
+/* K.H:237-994 */
+void orc_evaluate_force(const double *r12, double *par1, double *par2,
+                        int32_t *ipar1, int32_t *ipar2, double *vel1,
+                        double *vel2, double *rotv1, double *rotv2,
+                        const double *fpar) {
+  if (ipar1[ORC_I_CELL_TYPE] == ORC_YEAST &&
+      ipar2[ORC_I_CELL_TYPE] == ORC_YEAST) {
+    /* K.H:243-276 */
+    double bw = fpar[ORC_F_WA], mu = fpar[ORC_F_MU];
+    double rS = par1[ORC_R_RADIUS] + par2[ORC_R_RADIUS];
+    double rA = rS + bw;
+    double rx = r12[0], ry = r12[1], rz = r12[2];
+    double rn = sqrt(rx * rx + ry * ry + rz * rz);
+    rx /= rn;
+    ry /= rn;
+    rz /= rn;
+    double F = (rn < rA) ? -mu * (rn - rA) * (rn - rA) * (rn - rS) : 0.0;
+    vel1[0] = F * rx;
+    vel1[1] = F * ry;
+    vel1[2] = F * rz;
+    vel2[0] = -F * rx;
+    vel2[1] = -F * ry;
+    vel2[2] = -F * rz;
+    rotv1[0] = rotv1[1] = rotv1[2] = 0.0;
+    rotv2[0] = rotv2[1] = rotv2[2] = 0.0;
+    return;
+  }
+  if (!(ipar1[ORC_I_CELL_TYPE] == ORC_FUNGI &&
+        ipar2[ORC_I_CELL_TYPE] == ORC_FUNGI))
+    return;
+
+  /* K.H:277-993 */
+  double bw = fpar[ORC_F_WA], mu = fpar[ORC_F_MU];
+  double rS = par1[ORC_R_RADIUS] + par2[ORC_R_RADIUS];
+  double rA = rS + bw;
+  int bonded = 0;
+  double scale = par1[ORC_R_BOND_SCALE] * par2[ORC_R_BOND_SCALE];
+  double kb = fpar[ORC_F_KBOND];
+  double visc = fpar[ORC_F_VISC];
+  double f1s1[3] = {0, 0, 0}, f1s2[3] = {0, 0, 0};
+  double f2s1[3] = {0, 0, 0}, f2s2[3] = {0, 0, 0};
+  int i_id[4], i_cpu[4], j_id[4], j_cpu[4], i_site[4], j_site[4];
+  int i_bnds = ipar1[ORC_I_N_BNDS];
+  int j_bnds = ipar2[ORC_I_N_BNDS];
+  for (int b = 0; b < 4; b++) {
+    i_id[b] = ipar1[ORC_I_SEG1_ID1 + b];
+    i_cpu[b] = ipar1[ORC_I_SEG1_ID2 + b];
+    i_site[b] = ipar1[ORC_I_SITE1 + b];
+    j_id[b] = ipar2[ORC_I_SEG1_ID1 + b];
+    j_cpu[b] = ipar2[ORC_I_SEG1_ID2 + b];
+    j_site[b] = ipar2[ORC_I_SITE1 + b];
+  }
+  int iid = ipar1[ORC_I_ID], icpu = ipar1[ORC_I_CPU];
+  int jid = ipar2[ORC_I_ID], jcpu = ipar2[ORC_I_CPU];
+
+  /* K.H:338-362: demote SECOND_x to INTERIOR when the partner on that site
+   * is no longer a tip */
+  if (ipar1[ORC_I_POSITION] == ORC_SECOND_1 ||
+      ipar1[ORC_I_POSITION] == ORC_SECOND_2) {
+    int site = (ipar1[ORC_I_POSITION] == ORC_SECOND_1) ? 1 : 2;
+    int ib;
+    for (ib = 0; ib < i_bnds; ib++)
+      if (i_id[ib] == jid && i_cpu[ib] == jcpu && i_site[ib] == site) break;
+    if (ib < i_bnds && ipar2[ORC_I_POSITION] != ORC_TIP)
+      ipar1[ORC_I_POSITION] = ORC_INTERIOR;
+  }
+  /* K.H:364-369 */
+  ipar1[ORC_I_DLTD_SITE1] = -1;
+  ipar1[ORC_I_DLTD_SITE2] = -1;
+  ipar1[ORC_I_DLTD_ID1] = -1;
+  ipar1[ORC_I_DLTD_ID2] = -1;
+  ipar1[ORC_I_DLTD_CPU1] = -1;
+  ipar1[ORC_I_DLTD_CPU2] = -1;
+  /* K.H:372-545: repair a bond registered on one side only */
+  int i_to_j = 0, j_to_i = 0;
+  for (int b = 0; b < i_bnds; b++)
+    if (i_id[b] == jid && i_cpu[b] == jcpu) { i_to_j = 1; break; }
+  for (int b = 0; b < j_bnds; b++)
+    if (j_id[b] == iid && j_cpu[b] == icpu) { j_to_i = 1; break; }
+  if (!i_to_j && j_to_i) {
+    int isite;
+    if (ipar1[ORC_I_FIX_SITE] == 0)
+      isite = (ipar1[ORC_I_SITE1] == 1) ? 2 : 1;
+    else
+      isite = ipar1[ORC_I_FIX_SITE];
+    if (i_bnds <= 3) {
+      add_bond_slot(ipar1, i_bnds, jid, jcpu, isite, i_id, i_cpu, i_site);
+      i_bnds++;
+    }
+  } else if (i_to_j && !j_to_i) {
+    int jsite;
+    if (ipar2[ORC_I_FIX_SITE] == 0)
+      jsite = (ipar2[ORC_I_SITE1] == 1) ? 2 : 1;
+    else
+      jsite = ipar2[ORC_I_FIX_SITE];
+    if (j_bnds <= 3) {
+      add_bond_slot(ipar2, j_bnds, iid, icpu, jsite, j_id, j_cpu, j_site);
+      j_bnds++;
+    }
+  }
+
+  /* K.H:553-612: segment end points, j at the origin, i at r12 */
+  double L1 = par1[ORC_R_CLENGTH], L2 = par2[ORC_R_CLENGTH];
+  double th = par1[ORC_R_THETA], ph = par1[ORC_R_PHI];
+  double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  double xi1 = r12[0] - 0.5 * nx * L1, yi1 = r12[1] - 0.5 * ny * L1,
+         zi1 = r12[2] - 0.5 * nz * L1;
+  double xi2 = r12[0] + 0.5 * nx * L1, yi2 = r12[1] + 0.5 * ny * L1,
+         zi2 = r12[2] + 0.5 * nz * L1;
+  double Ri[3][3], Rj[3][3];
+  seg_rotation(ct, st, cp, sp, Ri);
+  th = par2[ORC_R_THETA];
+  ph = par2[ORC_R_PHI];
+  ct = cos(th);
+  st = sin(th);
+  cp = cos(ph);
+  sp = sin(ph);
+  nx = st * cp;
+  ny = st * sp;
+  nz = ct;
+  seg_rotation(ct, st, cp, sp, Rj);
+  double xj1 = -0.5 * nx * L2, yj1 = -0.5 * ny * L2, zj1 = -0.5 * nz * L2;
+  double xj2 = 0.5 * nx * L2, yj2 = 0.5 * ny * L2, zj2 = 0.5 * nz * L2;
+
+  /* K.H:615-698: harmonic spring between bonded sites */
+  int ib, jb;
+  for (ib = 0; ib < i_bnds; ib++)
+    if (i_id[ib] == jid && i_cpu[ib] == jcpu) break;
+  for (jb = 0; jb < j_bnds; jb++)
+    if (j_id[jb] == iid && j_cpu[jb] == icpu) break;
+  double rx, ry, rz, r;
+  if (ib < i_bnds && jb < j_bnds) {
+    int itip = ipar1[ORC_I_FUSE_TIP], jtip = ipar2[ORC_I_FUSE_TIP];
+    if ((itip > 0 && itip == i_site[ib]) || (jtip > 0 && jtip == j_site[jb]))
+      kb = scale * kb;
+    int si = i_site[ib], sj = j_site[jb];
+    if (si == 1 && (sj == 1 || sj == 2)) {
+      rx = xi1 - (sj == 1 ? xj1 : xj2);
+      ry = yi1 - (sj == 1 ? yj1 : yj2);
+      rz = zi1 - (sj == 1 ? zj1 : zj2);
+      f1s1[0] -= kb * rx; f1s1[1] -= kb * ry; f1s1[2] -= kb * rz;
+      f2s2[0] += kb * rx; f2s2[1] += kb * ry; f2s2[2] += kb * rz;
+      bonded = 1;
+    } else if (si == 2 && (sj == 1 || sj == 2)) {
+      rx = xi2 - (sj == 1 ? xj1 : xj2);
+      ry = yi2 - (sj == 1 ? yj1 : yj2);
+      rz = zi2 - (sj == 1 ? zj1 : zj2);
+      f1s2[0] -= kb * rx; f1s2[1] -= kb * ry; f1s2[2] -= kb * rz;
+      f2s1[0] += kb * rx; f2s1[1] += kb * ry; f2s1[2] += kb * rz;
+      bonded = 1;
+    }
+  }
+  /* K.H:701-772: closest-point cubic contact for non-bonded segments */
+  if (!bonded) {
+    double ti, tj;
+    orc_min_distance(xi1, yi1, zi1, xi2, yi2, zi2, xj1, yj1, zj1, xj2, yj2,
+                     zj2, &ti, &tj);
+    double ri0 = xi2 - xi1, ri1 = yi2 - yi1, ri2 = zi2 - zi1;
+    double rj0 = xj2 - xj1, rj1 = yj2 - yj1, rj2 = zj2 - zj1;
+    double pi0 = xi1 + ti * ri0, pi1 = yi1 + ti * ri1, pi2 = zi1 + ti * ri2;
+    double pj0 = xj1 + tj * rj0, pj1 = yj1 + tj * rj1, pj2 = zj1 + tj * rj2;
+    rx = pi0 - pj0;
+    ry = pi1 - pj1;
+    rz = pi2 - pj2;
+    r = sqrt(rx * rx + ry * ry + rz * rz);
+    if (r > 1.0e-8) {
+      rx /= r;
+      ry /= r;
+      rz /= r;
+    }
+    /* else: the reference computes a cross-product direction into
+     * block-scoped variables that shadow rx/ry/rz (K.H:744-749), so the
+     * un-normalized separation is what reaches the force below. */
+    double F = (r < rA) ? -mu * (r - rA) * (r - rA) * (r - rS) : 0.0;
+    if (F != 0.0) {
+      f1s1[0] += F * rx * (1.0 - ti);
+      f1s1[1] += F * ry * (1.0 - ti);
+      f1s1[2] += F * rz * (1.0 - ti);
+      f1s2[0] += F * rx * ti;
+      f1s2[1] += F * ry * ti;
+      f1s2[2] += F * rz * ti;
+      f2s1[0] -= F * rx * (1.0 - tj);
+      f2s1[1] -= F * ry * (1.0 - tj);
+      f2s1[2] -= F * rz * (1.0 - tj);
+      f2s2[0] -= F * rx * tj;
+      f2s2[1] -= F * ry * tj;
+      f2s2[2] -= F * rz * tj;
+    }
+  }
+  /* K.H:776-810: drop the axial (stretching) part of the site forces */
+  double rxi = xi1 - xi2, ryi = yi1 - yi2, rzi = zi1 - zi2;
+  remove_axial(f1s1, f1s2, rxi, ryi, rzi);
+  double rxj = xj1 - xj2, ryj = yj1 - yj2, rzj = zj1 - zj2;
+  remove_axial(f2s1, f2s2, rxj, ryj, rzj);
+
+  /* K.H:812-970: two-site mobility */
+  double dii[3][3], dij[3][3], vi1[3], vi2[3], vj1[3], vj2[3];
+  seg_mobility(par1[ORC_R_AREA], L1, visc, Ri, dii, dij);
+  site_velocities(dii, dij, f1s1, f1s2, vi1, vi2);
+  seg_mobility(par2[ORC_R_AREA], L2, visc, Rj, dii, dij);
+  site_velocities(dii, dij, f2s1, f2s2, vj1, vj2);
+
+  /* K.H:973-992 */
+  vel1[0] = 0.5 * (vi1[0] + vi2[0]);
+  vel1[1] = 0.5 * (vi1[1] + vi2[1]);
+  vel1[2] = 0.5 * (vi1[2] + vi2[2]);
+  vel2[0] = 0.5 * (vj1[0] + vj2[0]);
+  vel2[1] = 0.5 * (vj1[1] + vj2[1]);
+  vel2[2] = 0.5 * (vj1[2] + vj2[2]);
+  rotv1[0] = 0.5 * (ryi * vi1[2] - rzi * vi1[1]);
+  rotv1[1] = 0.5 * (rzi * vi1[0] - rxi * vi1[2]);
+  rotv1[2] = 0.5 * (rxi * vi1[1] - ryi * vi1[0]);
+  rotv1[0] += -0.5 * (ryi * vi2[2] - rzi * vi2[1]);
+  rotv1[1] += -0.5 * (rzi * vi2[0] - rxi * vi2[2]);
+  rotv1[2] += -0.5 * (rxi * vi2[1] - ryi * vi2[0]);
+  rotv2[0] = 0.5 * (ryj * vj1[2] - rzj * vj1[1]);
+  rotv2[1] = 0.5 * (rzj * vj1[0] - rxj * vj1[2]);
+  rotv2[2] = 0.5 * (rxj * vj1[1] - ryj * vj1[0]);
+  rotv2[0] += -0.5 * (ryj * vj2[2] - rzj * vj2[1]);
+  rotv2[1] += -0.5 * (rzj * vj2[0] - rxj * vj2[2]);
+  rotv2[2] += -0.5 * (rxj * vj2[1] - ryj * vj2[0]);
+}
+
+/* Cubic wall law for one site or sphere (K.H:1019-1030, 1099-1121). */
+static double wall_law(double z, double zw, double rS, double WZ, double xi,
+                       double g) {
+  double dz = z - zw;
+  double rA = rS + WZ;
+  double v;
+  if (dz > 0.0 && dz < rA)
+    v = -xi * (dz - rA) * (dz - rA) * (dz - rS);
+  else if (dz <= 0.0)
+    v = xi * rA * rA * rS;
+  else
+    v = 0.0;
+  v -= g * xi * rA * rA * rS;
+  return v;
+}
+
+/* K.H:1005-1283 */
+void orc_surface_force(const double *pos, double *par, int32_t *ipar,
+                       double *vel, double *rotv, const double *fpar) {
+  double zw = fpar[ORC_F_ZWALL], WZ = fpar[ORC_F_WZ], xi = fpar[ORC_F_XI],
+         g = fpar[ORC_F_GRAV];
+  if (ipar[ORC_I_CELL_TYPE] == ORC_YEAST) {
+    vel[0] = 0.0;
+    vel[1] = 0.0;
+    rotv[0] = rotv[1] = rotv[2] = 0.0;
+    vel[2] = wall_law(pos[2], zw, par[ORC_R_RADIUS], WZ, xi, g);
+  } else if (ipar[ORC_I_CELL_TYPE] == ORC_FUNGI) {
+    ipar[ORC_I_FIX_SITE] = 0;
+    double L = par[ORC_R_CLENGTH];
+    double th = par[ORC_R_THETA], ph = par[ORC_R_PHI];
+    double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+    double nx = st * cp, ny = st * sp, nz = ct;
+    double x1 = pos[0] - 0.5 * nx * L, y1 = pos[1] - 0.5 * ny * L,
+           z1 = pos[2] - 0.5 * nz * L;
+    double x2 = pos[0] + 0.5 * nx * L, y2 = pos[1] + 0.5 * ny * L,
+           z2 = pos[2] + 0.5 * nz * L;
+    double fs1[3] = {0.0, 0.0, 0.0}, fs2[3] = {0.0, 0.0, 0.0};
+    fs1[2] = wall_law(z1, zw, par[ORC_R_RADIUS], WZ, xi, g);
+    fs2[2] = wall_law(z2, zw, par[ORC_R_RADIUS], WZ, xi, g);
+    double rx = x1 - x2, ry = y1 - y2, rz = z1 - z2;
+    remove_axial(fs1, fs2, rx, ry, rz);
+    /* K.H:1176-1265 compute mobility velocities that K.H:1266-1281 then
+     * overwrite; only the overwriting force-as-velocity result survives. */
+    vel[0] = 0.5 * (fs1[0] + fs2[0]);
+    vel[1] = 0.5 * (fs1[1] + fs2[1]);
+    vel[2] = 0.5 * (fs1[2] + fs2[2]);
+    for (int a = 0; a < 3; a++) {
+      fs1[a] -= 0.5 * vel[a];
+      fs2[a] -= 0.5 * vel[a];
+    }
+    rotv[0] = 0.5 * (ry * fs1[2] - rz * fs1[1]);
+    rotv[1] = 0.5 * (rz * fs1[0] - rx * fs1[2]);
+    rotv[2] = 0.5 * (rx * fs1[1] - ry * fs1[0]);
+    rotv[0] += -0.5 * (ry * fs2[2] - rz * fs2[1]);
+    rotv[1] += -0.5 * (rz * fs2[0] - rx * fs2[2]);
+    rotv[2] += -0.5 * (rx * fs2[1] - ry * fs2[0]);
+  }
+}
+
+/* K.H:1285-1444.  Writes only into the slots 0..3; a fifth bond (which the
+ * reference writes past the bond slots) is reported instead. */
+int orc_fix_bonds(const double *r12, const double *ripar, int32_t *ipar,
+                  const double *rjpar, int32_t *jpar, double rlim) {
+  if (!(ipar[ORC_I_CELL_TYPE] == ORC_FUNGI &&
+        jpar[ORC_I_CELL_TYPE] == ORC_FUNGI))
+    return 0;
+  int i_bnds = ipar[ORC_I_N_BNDS], j_bnds = jpar[ORC_I_N_BNDS];
+  int iid = ipar[ORC_I_ID], icpu = ipar[ORC_I_CPU];
+  int jid = jpar[ORC_I_ID], jcpu = jpar[ORC_I_CPU];
+  double L1 = ripar[ORC_R_CLENGTH], L2 = rjpar[ORC_R_CLENGTH];
+  double th = ripar[ORC_R_THETA], ph = ripar[ORC_R_PHI];
+  double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  double xi1 = r12[0] - 0.5 * nx * L1, yi1 = r12[1] - 0.5 * ny * L1,
+         zi1 = r12[2] - 0.5 * nz * L1;
+  double xi2 = r12[0] + 0.5 * nx * L1, yi2 = r12[1] + 0.5 * ny * L1,
+         zi2 = r12[2] + 0.5 * nz * L1;
+  th = rjpar[ORC_R_THETA];
+  ph = rjpar[ORC_R_PHI];
+  ct = cos(th);
+  st = sin(th);
+  cp = cos(ph);
+  sp = sin(ph);
+  nx = st * cp;
+  ny = st * sp;
+  nz = ct;
+  double xj1 = -0.5 * nx * L2, yj1 = -0.5 * ny * L2, zj1 = -0.5 * nz * L2;
+  double xj2 = 0.5 * nx * L2, yj2 = 0.5 * ny * L2, zj2 = 0.5 * nz * L2;
+  /* site pairs in the reference's order: 1-1, 2-1, 1-2, 2-2 */
+  const double *ei[4][3] = {{&xi1, &yi1, &zi1}, {&xi2, &yi2, &zi2},
+                            {&xi1, &yi1, &zi1}, {&xi2, &yi2, &zi2}};
+  const double *ej[4][3] = {{&xj1, &yj1, &zj1}, {&xj1, &yj1, &zj1},
+                            {&xj2, &yj2, &zj2}, {&xj2, &yj2, &zj2}};
+  const int si[4] = {1, 2, 1, 2}, sj[4] = {1, 1, 2, 2};
+  for (int c = 0; c < 4; c++) {
+    double rx = *ei[c][0] - *ej[c][0];
+    double ry = *ei[c][1] - *ej[c][1];
+    double rz = *ei[c][2] - *ej[c][2];
+    double r = sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < rlim) {
+      if (i_bnds < 0 || i_bnds > 3) return 1;
+      ipar[ORC_I_SEG1_ID1 + i_bnds] = jid;
+      ipar[ORC_I_SEG1_ID2 + i_bnds] = jcpu;
+      ipar[ORC_I_SITE1 + i_bnds] = si[c];
+      ipar[ORC_I_N_BNDS] = i_bnds + 1;
+      /* the partner's record is a per-pair copy in the reference
+       * (PCI:592), so these writes only matter to a caller that keeps it */
+      if (j_bnds >= 0 && j_bnds <= 3) {
+        jpar[ORC_I_SEG1_ID1 + j_bnds] = iid;
+        jpar[ORC_I_SEG1_ID2 + j_bnds] = icpu;
+        jpar[ORC_I_SITE1 + j_bnds] = sj[c];
+        jpar[ORC_I_N_BNDS] = j_bnds + 1;
+      }
+      return 0;
+    }
+  }
+  return 0;
+}
+
+/* PCI:317-436 */
+void orc_update_particle(orc_particle *p, const double fc[6], double subdt) {
+  double *rd = p->rdata;
+  double ppos[3] = {p->pos[0], p->pos[1], p->pos[2]};
+  rd[ORC_R_VELX] = fc[0];
+  rd[ORC_R_VELY] = fc[1];
+  rd[ORC_R_VELZ] = fc[2];
+  rd[ORC_R_WX] = fc[3];
+  rd[ORC_R_WY] = fc[4];
+  rd[ORC_R_WZ] = fc[5];
+  ppos[0] += subdt * rd[ORC_R_VELX];
+  ppos[1] += subdt * rd[ORC_R_VELY];
+  ppos[2] += subdt * rd[ORC_R_VELZ];
+  if (p->idata[ORC_I_CELL_TYPE] == ORC_FUNGI) {
+    double theta = rd[ORC_R_THETA], phi = rd[ORC_R_PHI];
+    double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+    double R[3][3];
+    seg_rotation(ct, st, cp, sp, R);
+    double lm[3] = {rd[ORC_R_WX], rd[ORC_R_WY], rd[ORC_R_WZ]};
+    double om[3];
+    for (int a = 0; a < 3; a++) {
+      om[a] = 0.0;
+      for (int b = 0; b < 3; b++) om[a] += R[b][a] * lm[b];
+    }
+    double clen = rd[ORC_R_CLENGTH];
+    om[0] = 0.0;
+    om[1] *= 2.0 / (clen * clen);
+    om[2] *= 2.0 / (clen * clen);
+    double on = sqrt(om[1] * om[1] + om[2] * om[2]);
+    if (on > 0.0) {
+      om[1] /= on;
+      om[2] /= on;
+    }
+    ct = om[2];
+    st = om[1];
+    double orot[3][3] = {{1.0, 0.0, 0.0}, {0.0, ct, -st}, {0.0, st, ct}};
+    double dtheta = on * subdt;
+    double ndir[3] = {cos(dtheta), sin(dtheta), 0.0};
+    for (int a = 0; a < 3; a++) {
+      om[a] = 0.0;
+      for (int b = 0; b < 3; b++) om[a] += orot[b][a] * ndir[b];
+    }
+    for (int a = 0; a < 3; a++) {
+      ndir[a] = 0.0;
+      for (int b = 0; b < 3; b++) ndir[a] += R[a][b] * om[b];
+    }
+    theta = acos(ndir[2]);
+    on = sqrt(ndir[0] * ndir[0] + ndir[1] * ndir[1]);
+    if (on > 0.0) ndir[0] /= on;
+    phi = acos(ndir[0]);
+    if (ndir[1] < 0.0) phi = 2.0 * ORC_PI - phi;
+    rd[ORC_R_THETA] = theta;
+    rd[ORC_R_PHI] = phi;
+  }
+  p->pos[0] = ppos[0];
+  p->pos[1] = ppos[1];
+  p->pos[2] = ppos[2];
+}
+
+/* ---- geometry ---------------------------------------------------------- */
+
+void orc_wrap(orc_particle *p, int64_t n, const orc_geom *g) {
+  for (int d = 0; d < 3; d++) {
+    if (!g->periodic[d]) continue;
+    double lo = g->plo[d], hi = g->phi[d], L = hi - lo;
+    for (int64_t i = 0; i < n; i++) {
+      double x = p[i].pos[d];
+      if (x >= hi) x -= L;
+      if (x < lo) x += L;
+      if (x >= hi) x = lo;
+      p[i].pos[d] = x;
+    }
+  }
+}
+
+int64_t orc_bin_index(const double *x, const orc_geom *g, int32_t b3[3]) {
+  for (int d = 0; d < 3; d++) {
+    double dx = (g->phi[d] - g->plo[d]) / (double)g->nbin[d];
+    double dxi = 1.0 / dx;
+    double lx = (x[d] - g->plo[d]) * dxi;
+    double f = floor(lx);
+    int b;
+    if (f < 0.0)
+      b = 0;
+    else if (f > (double)(g->nbin[d] - 1))
+      b = g->nbin[d] - 1;
+    else
+      b = (int)f;
+    b3[d] = b;
+  }
+  return ((int64_t)b3[2] * g->nbin[1] + b3[1]) * g->nbin[0] + b3[0];
+}
+
+void orc_pair_separation(const double *xi, const double *xj, const orc_geom *g,
+                         double d[3]) {
+  for (int a = 0; a < 3; a++) {
+    double xg = xj[a];
+    if (g->periodic[a]) {
+      double L = g->phi[a] - g->plo[a];
+      double t = xi[a] - xj[a];
+      if (t > 0.5 * L)
+        xg = xj[a] + L;
+      else if (t < -0.5 * L)
+        xg = xj[a] - L;
+    }
+    d[a] = xi[a] - xg;
+  }
+}
+
+static int accept_pair(const double d[3], int mode, double radius) {
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (mode == ORC_NBR_STENCIL) return r2 <= radius;
+  return r2 <= radius * radius;
+}
+
+static int check_geom(const orc_geom *g, int mode, double radius) {
+  for (int d = 0; d < 3; d++) {
+    if (g->nbin[d] < 1 || !(g->phi[d] > g->plo[d])) return -2;
+    if (g->periodic[d] && g->nbin[d] < (mode == ORC_NBR_STENCIL ? 4 : 3))
+      return -2;
+    if (mode == ORC_NBR_RADIUS &&
+        (g->phi[d] - g->plo[d]) / (double)g->nbin[d] < radius)
+      return -2;
+  }
+  return 0;
+}
+
+/* Visits the +-1 stencil of particle i in canonical order; with out == NULL
+ * only counts the accepted candidates. */
+static int64_t stencil_row(const orc_particle *p, int64_t i, const orc_geom *g,
+                           int mode, double radius, const int64_t *start,
+                           const int32_t *sorted, int32_t *out) {
+  int32_t b3[3];
+  int64_t cnt = 0;
+  orc_bin_index(p[i].pos, g, b3);
+  for (int oz = -1; oz <= 1; oz++) {
+    int bz = b3[2] + oz;
+    if (g->periodic[2]) bz = (bz + g->nbin[2]) % g->nbin[2];
+    else if (bz < 0 || bz >= g->nbin[2]) continue;
+    for (int oy = -1; oy <= 1; oy++) {
+      int by = b3[1] + oy;
+      if (g->periodic[1]) by = (by + g->nbin[1]) % g->nbin[1];
+      else if (by < 0 || by >= g->nbin[1]) continue;
+      for (int ox = -1; ox <= 1; ox++) {
+        int bx = b3[0] + ox;
+        if (g->periodic[0]) bx = (bx + g->nbin[0]) % g->nbin[0];
+        else if (bx < 0 || bx >= g->nbin[0]) continue;
+        int64_t b = ((int64_t)bz * g->nbin[1] + by) * g->nbin[0] + bx;
+        for (int64_t s = start[b]; s < start[b + 1]; s++) {
+          int32_t j = sorted[s];
+          if (j == i) continue;
+          double d[3];
+          orc_pair_separation(p[i].pos, p[j].pos, g, d);
+          if (!accept_pair(d, mode, radius)) continue;
+          if (out) out[cnt] = j;
+          cnt++;
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
+int orc_build_neighbors(const orc_particle *p, int64_t n, const orc_geom *g,
+                        int mode, double radius, int64_t *offsets,
+                        int32_t **indices) {
+  int rc = check_geom(g, mode, radius);
+  if (rc) return rc;
+  int64_t nb = (int64_t)g->nbin[0] * g->nbin[1] * g->nbin[2];
+  int64_t *start = calloc((size_t)nb + 1, sizeof(int64_t));
+  int64_t *bin = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  int32_t *sorted = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int64_t *fill = malloc(sizeof(int64_t) * (size_t)(nb + 1));
+  if (!start || !bin || !sorted || !fill) {
+    free(start); free(bin); free(sorted); free(fill);
+    return -1;
+  }
+  /* stable counting sort: ascending particle index inside each bin */
+  int32_t b3[3];
+  for (int64_t i = 0; i < n; i++) {
+    bin[i] = orc_bin_index(p[i].pos, g, b3);
+    start[bin[i] + 1]++;
+  }
+  for (int64_t b = 0; b < nb; b++) start[b + 1] += start[b];
+  memcpy(fill, start, sizeof(int64_t) * (size_t)(nb + 1));
+  for (int64_t i = 0; i < n; i++) sorted[fill[bin[i]]++] = (int32_t)i;
+  free(fill);
+  free(bin);
+  /* count, prefix, fill; rows are independent */
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < n; i++)
+    offsets[i + 1] = stencil_row(p, i, g, mode, radius, start, sorted, NULL);
+  offsets[0] = 0;
+  for (int64_t i = 0; i < n; i++) offsets[i + 1] += offsets[i];
+  int32_t *out = malloc(sizeof(int32_t) * (size_t)(offsets[n] > 0 ? offsets[n] : 1));
+  if (!out) { free(start); free(sorted); return -1; }
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < n; i++)
+    stencil_row(p, i, g, mode, radius, start, sorted, out + offsets[i]);
+  free(start);
+  free(sorted);
+  *indices = out;
+  return 0;
+}
+
+int orc_build_neighbors_brute(const orc_particle *p, int64_t n,
+                              const orc_geom *g, int mode, double radius,
+                              int64_t *offsets, int32_t **indices) {
+  size_t cap = (size_t)(n > 0 ? n : 1) * 16, used = 0;
+  int32_t *out = malloc(sizeof(int32_t) * cap);
+  if (!out) return -1;
+  for (int64_t i = 0; i < n; i++) {
+    offsets[i] = (int64_t)used;
+    for (int64_t j = 0; j < n; j++) {
+      if (j == i) continue;
+      double d[3];
+      orc_pair_separation(p[i].pos, p[j].pos, g, d);
+      if (!accept_pair(d, mode, radius)) continue;
+      if (used == cap) {
+        cap *= 2;
+        int32_t *nw = realloc(out, sizeof(int32_t) * cap);
+        if (!nw) { free(out); return -1; }
+        out = nw;
+      }
+      out[used++] = (int32_t)j;
+    }
+  }
+  offsets[n] = (int64_t)used;
+  *indices = out;
+  return 0;
+}
+
+/* ---- driver ------------------------------------------------------------ */
+
+struct orc_state {
+  orc_particle *p;
+  int64_t n;
+  orc_geom g;
+  double fpar[9];
+  int mode;
+  double radius;
+  int rebuild_every;
+  int64_t step;
+  int64_t last_build;
+  int64_t *off;
+  int32_t *idx;
+  orc_particle *snap;
+};
+
+orc_state *orc_state_create(orc_particle *p, int64_t n, const orc_geom *g,
+                            const double fpar[9], int nbr_mode,
+                            double nbr_radius, int rebuild_every) {
+  orc_state *s = calloc(1, sizeof(orc_state));
+  if (!s) return NULL;
+  s->p = p;
+  s->n = n;
+  s->g = *g;
+  memcpy(s->fpar, fpar, sizeof(double) * 9);
+  s->mode = nbr_mode;
+  s->radius = nbr_radius;
+  s->rebuild_every = rebuild_every < 1 ? 1 : rebuild_every;
+  s->last_build = -1;
+  s->off = malloc(sizeof(int64_t) * (size_t)(n + 1));
+  s->snap = malloc(sizeof(orc_particle) * (size_t)(n > 0 ? n : 1));
+  if (!s->off || !s->snap) {
+    orc_state_destroy(s);
+    return NULL;
+  }
+  orc_wrap(p, n, g);
+  return s;
+}
+
+void orc_state_destroy(orc_state *s) {
+  if (!s) return;
+  free(s->off);
+  free(s->idx);
+  free(s->snap);
+  free(s);
+}
+
+static int rebuild(orc_state *s) {
+  free(s->idx);
+  s->idx = NULL;
+  int rc = orc_build_neighbors(s->p, s->n, &s->g, s->mode, s->radius, s->off,
+                               &s->idx);
+  if (rc == 0) s->last_build = s->step;
+  return rc;
+}
+
+/* PCI:170-279.  Neighbor records are read as they stood when the pass
+ * began; particle i's own integer record evolves through its loop.  Each
+ * particle writes only its own record and fc entries, so the loop runs in
+ * parallel (as the reference's OpenMP tiles do) with identical results. */
+static void force_pass(orc_state *s, double *fc) {
+  int64_t n = s->n;
+  /* Only segment pairs change integer fields another particle reads; with
+   * spheres alone the live array is its own snapshot. */
+  int any_fungi = 0;
+  for (int64_t i = 0; i < n && !any_fungi; i++)
+    any_fungi = s->p[i].idata[ORC_I_CELL_TYPE] == ORC_FUNGI;
+  const orc_particle *src = s->p;
+  if (any_fungi) {
+    memcpy(s->snap, s->p, sizeof(orc_particle) * (size_t)n);
+    src = s->snap;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; i++) {
+    orc_particle *pi = &s->p[i];
+    pi->idata[ORC_I_FUSE_FLAG] = 0;
+    pi->idata[ORC_I_SPLIT_FLAG] = 0;
+    pi->idata[ORC_I_NEW_FLAG] = 0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t k = s->off[i]; k < s->off[i + 1]; k++) {
+      const orc_particle *pj = &src[s->idx[k]];
+      double d[3];
+      orc_pair_separation(pi->pos, pj->pos, &s->g, d);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double r_lm = orc_max_interaction_distance(
+          pi->rdata, pj->rdata, pi->idata, pj->idata, s->fpar[ORC_F_WA]);
+      if (r2 <= (r_lm - 1.0e-15) * (r_lm - 1.0e-15)) {
+        double v1[3] = {0, 0, 0}, v2[3] = {0, 0, 0};
+        double w1[3] = {0, 0, 0}, w2[3] = {0, 0, 0};
+        /* the neighbor's record is a per-pair copy (PCI:188) */
+        orc_particle p2 = *pj;
+        orc_evaluate_force(d, pi->rdata, p2.rdata, pi->idata, p2.idata, v1,
+                           v2, w1, w2, s->fpar);
+        acc[0] += v1[0];
+        acc[1] += v1[1];
+        acc[2] += v1[2];
+        acc[3] += w1[0];
+        acc[4] += w1[1];
+        acc[5] += w1[2];
+      }
+    }
+    double vc[3] = {0, 0, 0}, vr[3] = {0, 0, 0};
+    orc_surface_force(pi->pos, pi->rdata, pi->idata, vc, vr, s->fpar);
+    acc[0] += vc[0];
+    acc[1] += vc[1];
+    acc[2] += vc[2];
+    acc[3] += vr[0];
+    acc[4] += vr[1];
+    acc[5] += vr[2];
+    for (int c = 0; c < 6; c++) fc[(int64_t)c * n + i] = acc[c];
+  }
+}
+
+int orc_state_evolve(orc_state *s, double dt, int nsteps, double *fc_out) {
+  int64_t n = s->n;
+  double *fc = malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
+  if (!fc) return -1;
+  for (int st = 0; st < nsteps; st++) {
+    if (s->idx == NULL || s->step - s->last_build >= s->rebuild_every) {
+      int rc = rebuild(s);
+      if (rc) { free(fc); return rc; }
+    }
+    force_pass(s, fc);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      double f6[6];
+      for (int c = 0; c < 6; c++) f6[c] = fc[(int64_t)c * n + i];
+      orc_update_particle(&s->p[i], f6, dt);
+    }
+    orc_wrap(s->p, n, &s->g);
+    s->step++;
+  }
+  if (fc_out && nsteps > 0)
+    memcpy(fc_out, fc, sizeof(double) * 6 * (size_t)n);
+  free(fc);
+  return 0;
+}
+
+/* PCI:500-668 (rlim = 1e-7, PCI:508) */
+int orc_state_init_bonds(orc_state *s) {
+  int rc = rebuild(s);
+  if (rc) return rc;
+  int bad = 0;
+  for (int64_t i = 0; i < s->n; i++) {
+    orc_particle *pi = &s->p[i];
+    for (int64_t k = s->off[i]; k < s->off[i + 1]; k++) {
+      orc_particle p2 = s->p[s->idx[k]];
+      double d[3];
+      orc_pair_separation(pi->pos, p2.pos, &s->g, d);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double r_lm = orc_max_interaction_distance(
+          pi->rdata, p2.rdata, pi->idata, p2.idata, s->fpar[ORC_F_WA]);
+      if (r2 <= (r_lm - 1.0e-15) * (r_lm - 1.0e-15))
+        bad |= orc_fix_bonds(d, pi->rdata, pi->idata, p2.rdata, p2.idata,
+                             1.0e-7);
+    }
+  }
+  /* the next evolve rebuilds from scratch, as the reference does */
+  s->last_build = -1;
+  free(s->idx);
+  s->idx = NULL;
+  return bad ? -3 : 0;
+}
+
+int64_t orc_state_neighbor_count(const orc_state *s) {
+  return s->idx ? s->off[s->n] : 0;
+}
+
+void orc_state_get_neighbors(const orc_state *s, int64_t *offsets,
+                             int32_t *indices) {
+  if (!s->idx) return;
+  memcpy(offsets, s->off, sizeof(int64_t) * (size_t)(s->n + 1));
+  memcpy(indices, s->idx, sizeof(int32_t) * (size_t)s->off[s->n]);
+}
+
+int64_t orc_state_step(const orc_state *s) { return s->step; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}

@@ -1,0 +1,220 @@
+"""GPU parity of the sphere path against the CPU oracle, through the C ABI.
+
+Sphere arithmetic is +, -, *, / and sqrt only, rounded as written on both
+sides (no contraction), and both sides sum each particle's pair terms in the
+same canonical order, so forces, neighbor lists, bins and trajectories must
+match bit for bit.  Across neighbor policies (Verlet list vs the reference's
+per-step stencil list) only the summation order differs: forces then agree
+to 1e-12 of the largest force and 10-step trajectories to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from boltzmannmfx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def verlet_radius(col):
+    rmax = col.recs["rdata"][:, 0].max()
+    return 2.0 * rmax + col.fpar[0] + col.skin
+
+
+def gpu_ctx(bmx, col, mode=0, K=10, n_cell=None, neighborhood=1.0, maxn=0,
+            block=256, remap=1):
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, n_cell or col.n_cell)
+    ctx.set_neighbor_params(mode=mode, rebuild_every=K, skin=col.skin,
+                            neighborhood=neighborhood, max_neighbors=maxn)
+    ctx.set_launch_params(block, remap)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    return ctx, recs
+
+
+def oracle_state(oracle, col, nbin, mode=0, K=10, radius=None):
+    recs = col.recs.copy()
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, nbin)
+    r = verlet_radius(col) if radius is None else radius
+    return oracle.State(recs, g, col.fpar, mode, r, K), recs
+
+
+def rows(off, idx):
+    return [list(idx[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return W.c1_small()
+
+
+def test_c1_first_step_bit_exact(bmx, oracle, gpu, c1):
+    ctx, _ = gpu_ctx(bmx, c1)
+    ctx.evolve(W.DT, 1)
+    nb = ctx.bin_dims()
+    st, orecs = oracle_state(oracle, c1, nb)
+    fo = st.evolve(W.DT, 1)
+    fg = ctx.fc()
+    assert np.array_equal(fg, fo)
+    assert np.abs(fo[:3]).max() > 0
+    assert not fg[3:].any()
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    og, ig = ctx.neighbors()
+    oo, io = st.neighbors()
+    assert rows(og, ig) == rows(oo, io)          # same pairs, same order
+    bins = ctx.bins()
+    g = oracle.make_geom(c1.prob_lo, c1.prob_hi, c1.periodic, nb)
+    # bins of the rebuild, i.e. of the initial positions
+    expect = [oracle.bin_index(p, g)[0] for p in c1.recs["pos"]]
+    assert np.array_equal(bins, expect)
+
+
+def test_c1_trajectory_100_steps_bit_exact(bmx, oracle, gpu, c1):
+    ctx, recs = gpu_ctx(bmx, c1)
+    nb = ctx.bin_dims()
+    st, orecs = oracle_state(oracle, c1, nb)
+    for chunk in (1, 9, 40, 50):
+        ctx.evolve(W.DT, chunk)
+        fo = st.evolve(W.DT, chunk)
+        assert np.array_equal(ctx.positions(), orecs["pos"])
+        assert np.array_equal(ctx.fc(), fo)
+    ctx.sync()
+    assert np.array_equal(recs["pos"], orecs["pos"])
+    assert np.array_equal(recs["rdata"][:, 6:12], orecs["rdata"][:, 6:12])
+    assert np.array_equal(recs["idata"], orecs["idata"])
+    moved = np.abs(orecs["pos"] - c1.recs["pos"]).max()
+    assert moved > 0
+
+
+def test_stencil_mode_bit_exact(bmx, oracle, gpu, c1):
+    """The reference's own policy: mesh-cell bins, +-1 stencil, every pair
+    with |dx|^2 <= neighborhood, rebuilt every step."""
+    L = c1.prob_hi - c1.prob_lo
+    n_cell = tuple(int(L[d] // 2.0e-4) for d in range(3))
+    hood = 1.1 * (2 * 7e-5 + 1e-2)   # DEM::neighborhood (bmx_chem.cpp:124)
+    ctx, _ = gpu_ctx(bmx, c1, mode=1, K=1, n_cell=n_cell, neighborhood=hood, maxn=160)
+    st, orecs = oracle_state(oracle, c1, n_cell, mode=1, K=1, radius=hood)
+    for _ in range(3):
+        ctx.evolve(W.DT, 5)
+        fo = st.evolve(W.DT, 5)
+        assert np.array_equal(ctx.fc(), fo)
+        assert np.array_equal(ctx.positions(), orecs["pos"])
+    og, ig = ctx.neighbors()
+    oo, io = st.neighbors()
+    assert rows(og, ig) == rows(oo, io)
+
+
+def test_verlet_matches_reference_policy(bmx, gpu, c1):
+    L = c1.prob_hi - c1.prob_lo
+    n_cell = tuple(int(L[d] // 2.0e-4) for d in range(3))
+    a, _ = gpu_ctx(bmx, c1)
+    b, _ = gpu_ctx(bmx, c1, mode=1, K=1, n_cell=n_cell, neighborhood=1.0, maxn=160)
+    for _ in range(10):
+        a.evolve(W.DT, 1)
+        b.evolve(W.DT, 1)
+        fa, fb = a.fc(), b.fc()
+        np.testing.assert_allclose(fa, fb, rtol=0, atol=1e-12 * np.abs(fb).max())
+    np.testing.assert_allclose(a.positions(), b.positions(), rtol=0, atol=1e-10)
+
+
+def test_graph_replay_matches_eager(bmx, gpu, c1):
+    a, _ = gpu_ctx(bmx, c1)
+    b, _ = gpu_ctx(bmx, c1)
+    a.evolve(W.DT, 30)
+    for _ in range(3):
+        b.evolve_graph(W.DT, 10)
+    assert np.array_equal(a.positions(), b.positions())
+    assert np.array_equal(a.fc(), b.fc())
+
+
+@pytest.mark.parametrize("block,remap", [(64, 0), (128, 1), (512, 1)])
+def test_launch_shape_does_not_change_results(bmx, gpu, c1, block, remap):
+    a, _ = gpu_ctx(bmx, c1)
+    b, _ = gpu_ctx(bmx, c1, block=block, remap=remap)
+    a.evolve(W.DT, 12)
+    b.evolve(W.DT, 12)
+    assert np.array_equal(a.positions(), b.positions())
+
+
+def test_colonies_match_separate_runs(bmx, oracle, gpu):
+    cols = [W.sphere_colony(n, phi, s, W_A=2e-5) for n, phi, s in
+            [(700, 0.15, 1), (1500, 0.3, 2), (520, 0.35, 3), (3000, 0.1, 4)]]
+    recs, starts, geoms = W.concat_colonies(cols)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(cols[0].fpar)
+    ctx.set_neighbor_params(mode=0, rebuild_every=10, skin=cols[0].skin)
+    ctx.bind_colonies(recs, starts, geoms)
+    ctx.evolve(W.DT, 25)
+    pos = ctx.positions()
+    for k, c in enumerate(cols):
+        st, orecs = oracle_state(oracle, c, ctx.bin_dims(k))
+        st.evolve(W.DT, 25)
+        assert np.array_equal(pos[starts[k]:starts[k + 1]], orecs["pos"]), k
+
+
+def test_edge_cases(bmx, oracle, gpu, c1):
+    # a single particle: wall term only
+    one = W.sphere_colony(1, 0.01, 3)
+    ctx, _ = gpu_ctx(bmx, one)
+    ctx.evolve(W.DT, 5)
+    st, orecs = oracle_state(oracle, one, ctx.bin_dims())
+    st.evolve(W.DT, 5)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    # particles above the box top (bins clamp) and below the surface
+    col = W.c1_small()
+    col.recs["pos"][::50, 2] += 2 * (col.prob_hi[2] - col.prob_lo[2])
+    col.recs["pos"][1::50, 2] = -1e-4
+    ctx, _ = gpu_ctx(bmx, col)
+    ctx.evolve(W.DT, 12)
+    st, orecs = oracle_state(oracle, col, ctx.bin_dims())
+    st.evolve(W.DT, 12)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    # no particles
+    empty = W.c1_small()
+    empty.recs = empty.recs[:0].copy()
+    ctx, _ = gpu_ctx(bmx, empty)
+    ctx.evolve(W.DT, 3)
+    assert ctx.positions().shape == (0, 3)
+
+
+def test_errors_are_reported(bmx, gpu, c1):
+    col = W.c1_small()
+    col.recs["pos"][1] = col.recs["pos"][0]          # coincident pair
+    ctx, _ = gpu_ctx(bmx, col)
+    with pytest.raises(bmx.BMXError) as ei:
+        ctx.evolve(W.DT, 1)
+    assert ei.value.code == 4
+    ctx, _ = gpu_ctx(bmx, c1, maxn=2)                 # rows overflow
+    with pytest.raises(bmx.BMXError) as ei:
+        ctx.evolve(W.DT, 1)
+    assert ei.value.code == 5
+    fast = W.c1_small()
+    fast.fpar[1] = 1e12                               # very stiff: moves > skin/2
+    fast.skin = 1e-9
+    ctx, _ = gpu_ctx(bmx, fast, K=10)
+    with pytest.raises(bmx.BMXError) as ei:
+        ctx.evolve(W.DT, 10)
+    assert ei.value.code == 6
+    with pytest.raises(bmx.BMXError):
+        ctx.set_neighbor_params(mode=1, rebuild_every=5)
+    bad = W.c1_small()
+    bad.recs["rdata"][3, 0] = -1.0
+    with pytest.raises(bmx.BMXError) as ei:
+        gpu_ctx(bmx, bad)
+    assert ei.value.code == 1
+
+
+def test_one_million_spheres_10_steps_bit_exact(bmx, oracle, gpu):
+    """The 1,000,000-sphere workload at volume fraction 0.3: 10 steps (one
+    Verlet rebuild) on the GPU and on the CPU oracle."""
+    col = W.c2_large()
+    ctx, _ = gpu_ctx(bmx, col)
+    ctx.evolve(W.DT, 10)
+    st, orecs = oracle_state(oracle, col, ctx.bin_dims())
+    fo = st.evolve(W.DT, 10)
+    assert np.array_equal(ctx.fc(), fo)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    og, ig = ctx.neighbors()
+    oo, io = st.neighbors()
+    assert np.array_equal(og, oo) and np.array_equal(ig, io)

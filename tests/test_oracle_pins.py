@@ -1,0 +1,565 @@
+"""Pins the CPU oracle to facts fixed by mathematics, not by its own code.
+
+The reference ships no golden vectors or known-answer tests for this path
+(its only "tests" are input decks run by hand), and its force kernels need
+AMReX headers this image lacks, so each oracle function is pinned here by:
+exact known answers of the cubic laws (model1.tex:293-345), closed forms of
+the two-site mobility for aligned segments, rotation covariance, brute force
+(closest points, neighbor sets), Rodrigues' rotation for the orientation
+update, conservation laws and the relaxation behaviour the reference's
+exec/pair_test/sep.pl and exec/wall_test/zloc.pl scripts look at.  Each check
+is chosen so a dropped term, a flipped sign, a swapped index or a transposed
+operand fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+NR, NI = 34, 35
+
+
+def sphere(radius):
+    r = np.zeros(NR)
+    r[0] = radius
+    r[1] = radius
+    i = np.zeros(NI, np.int32)
+    return r, i
+
+
+def rod(radius, L, theta, phi, rid, area=None, bscale=1.0):
+    r = np.zeros(NR)
+    r[0], r[1], r[2], r[3] = radius, L, theta, phi
+    r[4] = 2 * math.pi * radius * L if area is None else area
+    r[24] = bscale
+    i = np.zeros(NI, np.int32)
+    i[5] = 1
+    i[33] = rid
+    i[34] = 0
+    return r, i
+
+
+def fpar(W_A=0.0, mu=0.0, zw=0.0, WZ=0.0, xi=0.0, g=0.0, kb=0.0, visc=1.0):
+    return np.array([W_A, mu, zw, WZ, xi, g, kb, 5e-4, visc])
+
+
+def direction(theta, phi):
+    return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi),
+                     math.cos(theta)])
+
+
+def angles(n):
+    n = n / np.linalg.norm(n)
+    th = math.acos(max(-1.0, min(1.0, n[2])))
+    ph = math.atan2(n[1], n[0]) % (2 * math.pi)
+    return th, ph
+
+
+# ---------------------------------------------------------------- spheres --
+@pytest.mark.parametrize("r12,expect", [
+    ((0.5, 0, 0), (1.125, 0, 0)),        # repulsive: -(0.5-2)^2 (0.5-1)
+    ((0, 1.5, 0), (0, -0.125, 0)),       # attractive band: -(1.5-2)^2 (1.5-1)
+    ((0, 0, 1.0), (0, 0, 0)),            # r = R_S: equilibrium
+    ((0, 0, -2.0), (0, 0, 0)),           # r = R_A: vanishes
+    ((3.0, 0, 0), (0, 0, 0)),            # beyond R_A
+])
+def test_sphere_law_known_answers(oracle, r12, expect):
+    p1, i1 = sphere(0.5)
+    p2, i2 = sphere(0.5)
+    v1, v2, w1, w2, _, _ = oracle.evaluate_force(r12, p1, p2, i1, i2, fpar(W_A=1.0, mu=1.0))
+    assert np.array_equal(v1, np.array(expect, float))
+    assert np.array_equal(v2, -v1)           # Newton 3 exactly (K.H:268-270)
+    assert not w1.any() and not w2.any()
+
+
+def test_sphere_law_scaling_and_direction(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        ra, rb = rng.uniform(0.4, 0.8, 2)
+        W, mu = rng.uniform(0.05, 0.5), rng.uniform(0.5, 3.0)
+        d = rng.normal(size=3)
+        d *= rng.uniform(0.05, 1.0) * (ra + rb + W) / np.linalg.norm(d)
+        p1, i1 = sphere(ra)
+        p2, i2 = sphere(rb)
+        v1, _, _, _, _, _ = oracle.evaluate_force(d, p1, p2, i1, i2, fpar(W_A=W, mu=mu))
+        rn = np.linalg.norm(d)
+        # F(r) from its roots: zero at R_S (simple) and R_A (double), F(0) = mu R_A^2 R_S
+        rS, rA = ra + rb, ra + rb + W
+        F = mu * (rA - rn) ** 2 * (rS - rn)
+        np.testing.assert_allclose(v1, F * d / rn, rtol=1e-12, atol=1e-300)
+
+
+def test_max_interaction_distance(oracle):
+    p1, i1 = sphere(0.3)
+    p2, i2 = sphere(0.2)
+    assert oracle.max_interaction_distance(p1, p2, i1, i2, 0.25) == 0.75
+    r1, j1 = rod(0.1, 2.0, 1.0, 0.0, 1)
+    r2, j2 = rod(0.1, 4.0, 1.0, 0.0, 2)
+    assert oracle.max_interaction_distance(r1, r2, j1, j2, 0.5) == 9.5  # 1.5(L1+L2)+W
+    assert oracle.max_interaction_distance(p1, r2, i1, j2, 0.5) == 0.0  # mixed
+
+
+# -------------------------------------------------------------------- wall --
+@pytest.mark.parametrize("z,expect", [
+    (0.5, 1.125 - 1.0),    # H(0.5) - g xi R_A^2 R_S with R_S=1, R_A=2, g=0.25
+    (-1.0, 4.0 - 1.0),     # below the surface: xi R_A^2 R_S
+    (0.0, 4.0 - 1.0),      # dz <= 0 branch includes dz == 0
+    (1.0, -1.0),           # H(R_S) = 0
+    (3.0, -1.0),           # beyond R_A: gravity only
+])
+def test_sphere_wall_known_answers(oracle, z, expect):
+    p, i = sphere(1.0)
+    v, w, _ = oracle.surface_force((7.0, -3.0, z), p, i,
+                                   fpar(zw=0.0, WZ=1.0, xi=1.0, g=0.25))
+    assert v[0] == 0.0 and v[1] == 0.0 and not w.any()
+    assert v[2] == expect
+
+
+def H(dz, rS, WZ, xi, g):
+    rA = rS + WZ
+    if 0 < dz < rA:
+        h = -xi * (dz - rA) ** 2 * (dz - rS)
+    elif dz <= 0:
+        h = xi * rA * rA * rS
+    else:
+        h = 0.0
+    return h - g * xi * rA * rA * rS
+
+
+def test_rod_wall_closed_form(oracle):
+    """Site forces H(z1), H(z2) along z: the centre moves with their mean and
+    the angular term is r x (H1 - H2) z / 2 (axial parts cancel)."""
+    rng = np.random.default_rng(2)
+    fp = fpar(zw=0.1, WZ=0.5, xi=2.0, g=0.1, visc=3.0)
+    for _ in range(100):
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(0, 2 * math.pi)
+        L, rad = 1.5, 0.4
+        pos = np.array([0.3, -0.2, rng.uniform(-0.5, 1.5)])
+        p, i = rod(rad, L, th, ph, 1)
+        i[32] = 2
+        v, w, iout = oracle.surface_force(pos, p, i, fp)
+        n = direction(th, ph)
+        x1, x2 = pos - 0.5 * L * n, pos + 0.5 * L * n
+        h1 = H(x1[2] - 0.1, rad, 0.5, 2.0, 0.1)
+        h2 = H(x2[2] - 0.1, rad, 0.5, 2.0, 0.1)
+        scale = abs(h1) + abs(h2)
+        np.testing.assert_allclose(v, [0, 0, 0.5 * (h1 + h2)], atol=1e-14 * scale)
+        r = x1 - x2
+        np.testing.assert_allclose(w, 0.5 * np.cross(r, [0, 0, h1 - h2]),
+                                   atol=1e-14 * scale * L)
+        assert iout[32] == 0   # fix_site reset (K.H:1060)
+
+
+# ---------------------------------------------------------- closest point --
+def seg_point(seg, t):
+    a, b = np.array(seg[:3]), np.array(seg[3:])
+    return a + t * (b - a)
+
+
+def brute_min(si, sj, m=401):
+    """Exact constrained minimum of |p_i(s) - p_j(t)| over [0,1]^2: the
+    quadratic is convex, so its minimiser is the interior stationary point or
+    lies on an edge, where clamping the 1-D projection is exact.  A dense
+    grid cross-checks it."""
+    a, b = np.array(si[:3], float), np.array(si[3:], float)
+    c, d = np.array(sj[:3], float), np.array(sj[3:], float)
+    u, v = b - a, d - c
+    cands = []
+    A = np.array([[u @ u, -(u @ v)], [-(u @ v), v @ v]])
+    if abs(np.linalg.det(A)) > 1e-14 * (u @ u) * (v @ v):
+        s, t = np.linalg.solve(A, [-(a - c) @ u, (a - c) @ v])
+        if 0 <= s <= 1 and 0 <= t <= 1:
+            cands.append(np.linalg.norm(a + s * u - c - t * v))
+    for p in (a, b):   # end of i against segment j
+        t = min(1.0, max(0.0, ((p - c) @ v) / (v @ v)))
+        cands.append(np.linalg.norm(p - c - t * v))
+    for q in (c, d):   # end of j against segment i
+        s = min(1.0, max(0.0, ((q - a) @ u) / (u @ u)))
+        cands.append(np.linalg.norm(a + s * u - q))
+    exact = min(cands)
+    tt = np.linspace(0, 1, m)
+    grid = np.linalg.norm((a + tt[:, None] * u)[:, None, :] -
+                          (c + tt[:, None] * v)[None, :, :], axis=2).min()
+    assert exact <= grid + 1e-12
+    return exact
+
+
+def test_min_distance_interior_solution(oracle):
+    """When the unconstrained minimiser lies in [0,1]^2 the result is the
+    exact solution of the 2x2 normal equations."""
+    rng = np.random.default_rng(3)
+    hits = 0
+    for _ in range(2000):
+        si = rng.normal(size=6)
+        sj = rng.normal(size=6)
+        a, b = np.array(si[:3]), np.array(si[3:])
+        c, d = np.array(sj[:3]), np.array(sj[3:])
+        u, v = b - a, d - c
+        A = np.array([[u @ u, -(u @ v)], [-(u @ v), v @ v]])
+        rhs = np.array([-(a - c) @ u, (a - c) @ v])
+        tt = np.linalg.solve(A, rhs)
+        if not (0 < tt[0] < 1 and 0 < tt[1] < 1):
+            continue
+        hits += 1
+        ti, tj = oracle.min_distance(si, sj)
+        np.testing.assert_allclose([ti, tj], tt, rtol=1e-9, atol=1e-12)
+        dist = np.linalg.norm(seg_point(si, ti) - seg_point(sj, tj))
+        assert dist <= brute_min(si, sj, 101) + 1e-12
+    assert hits > 100
+
+
+def test_min_distance_clamped_branches(oracle):
+    """Outside the interior case the result is a segment end paired with the
+    orthogonal projection on the other line, or the closest of the four end
+    pairs, and never closer than the true minimum."""
+    rng = np.random.default_rng(4)
+    for _ in range(1500):
+        si = rng.normal(size=6)
+        sj = rng.normal(size=6)
+        u = np.array(si[3:]) - si[:3]
+        v = np.array(sj[3:]) - sj[:3]
+        if abs(u @ v) == 0.0:
+            continue
+        ti, tj = oracle.min_distance(si, sj)
+        pi, pj = seg_point(si, ti), seg_point(sj, tj)
+        dist = np.linalg.norm(pi - pj)
+        assert dist >= brute_min(si, sj, 201) - 1e-9
+        if ti in (0.0, 1.0) and tj in (0.0, 1.0):
+            ends = [np.linalg.norm(seg_point(si, a) - seg_point(sj, b))
+                    for a in (0.0, 1.0) for b in (0.0, 1.0)]
+            if not (0 < ti < 1 or 0 < tj < 1):
+                assert dist == pytest.approx(min(ends), rel=1e-12) or \
+                    abs(((pi - pj) @ u)) < 1e-9 * np.linalg.norm(u) or \
+                    abs(((pi - pj) @ v)) < 1e-9 * np.linalg.norm(v)
+        elif tj in (0.0, 1.0):
+            assert 0 <= ti <= 1
+            assert abs((pi - pj) @ u) <= 1e-9 * np.linalg.norm(u) * (1 + dist)
+        elif ti in (0.0, 1.0):
+            assert 0 <= tj <= 1
+            assert abs((pi - pj) @ v) <= 1e-9 * np.linalg.norm(v) * (1 + dist)
+        else:
+            assert 0 <= ti <= 1 and 0 <= tj <= 1
+
+
+def test_min_distance_special_cases(oracle):
+    # crossing at the midpoints
+    ti, tj = oracle.min_distance((-1, 0, 0, 1, 0, 0), (0, -1, 1e-3, 0, 1, 1e-3))
+    assert (ti, tj) == (0.5, 0.5)
+    # exactly perpendicular (Aij == 0): unclamped projections (K.H:83-86)
+    ti, tj = oracle.min_distance((0, 0, 0, 1, 0, 0), (3, -1, 0, 3, 1, 0))
+    assert (ti, tj) == (3.0, 0.5)
+    # exactly parallel (det == 0, K.H:87-104): end 0 of i projects to
+    # tau_j = -0.5 (off the segment), so end 1 of i is used, tau_j = 0.5
+    ti, tj = oracle.min_distance((0, 0, 0, 2, 0, 0), (1, 1, 0, 3, 1, 0))
+    assert (ti, tj) == (1.0, 0.5)
+    ti, tj = oracle.min_distance((0, 0, 0, 2, 0, 0), (-1, 1, 0, 1, 1, 0))
+    assert (ti, tj) == (0.0, 0.5)
+    # parallel, disjoint along the axis: end-pair search
+    ti, tj = oracle.min_distance((0, 0, 0, 1, 0, 0), (3, 1, 0, 4, 1, 0))
+    assert (ti, tj) == (1.0, 0.0)
+
+
+# ------------------------------------------------------------- rod pairs --
+def rp_coeffs(area, L, visc):
+    """Two-site mobility of an area-equivalent sphere pair (model1.tex:624-843):
+    self term 1/(6 pi a eta); pair term (L^2 -+ 2a^2/3)/(4 or 8 pi L^3 eta)
+    for L > 2a, overlapping-sphere form otherwise."""
+    a = math.sqrt(area / 2 / (4 * math.pi))
+    d0 = 1.0 / (6 * math.pi * a * visc)
+    if L > 2 * a:
+        e0 = (L * L - 2 * a * a / 3) / (4 * math.pi * L ** 3 * visc)
+        e1 = (L * L + 2 * a * a / 3) / (8 * math.pi * L ** 3 * visc)
+    else:
+        e0 = (1 - 3 * L / (16 * a)) / (6 * math.pi * a * visc)
+        e1 = (1 - 9 * L / (32 * a)) / (6 * math.pi * a * visc)
+    return d0, e0, e1
+
+
+@pytest.mark.parametrize("area", [0.6, 30.0])   # both mobility branches
+def test_bonded_pair_closed_form(oracle, area):
+    """Segments along x bonded i.site1 -- j.site2 with gap g: the spring
+    -k g acts on i's site 1, the axial part of the site-force difference is
+    removed, and the aligned mobility gives
+      vel = -k/2 (d0 + D1) g,  rot = k (d0 - e1) (0, -g_z, g_y)."""
+    L, k, visc = 2.0, 0.7, 1.3
+    g = np.array([1e-3, 2e-3, -1.5e-3])
+    fp = fpar(W_A=0.1, mu=5.0, kb=k, visc=visc)
+    pi_, ii = rod(0.2, L, math.pi / 2, 0.0, 11, area=area)
+    pj_, ij = rod(0.2, L, math.pi / 2, 0.0, 22, area=area)
+    ii[6], ii[7], ii[11], ii[15] = 1, 22, 0, 1      # i: partner 22 on site 1
+    ij[6], ij[7], ij[11], ij[15] = 1, 11, 0, 2      # j: partner 11 on site 2
+    r12 = np.array([L, 0.0, 0.0]) + g               # i.site1 - j.site2 = g
+    v1, _, w1, _, iout, _ = oracle.evaluate_force(r12, pi_, pj_, ii, ij, fp)
+    d0, e0, e1 = rp_coeffs(area, L, visc)
+    D1 = np.array([e0, e1, e1])
+    np.testing.assert_allclose(v1, -0.5 * k * (d0 + D1) * g, rtol=1e-9, atol=1e-16)
+    np.testing.assert_allclose(w1, k * (d0 - e1) * np.array([0, -g[2], g[1]]),
+                               rtol=1e-9, atol=1e-16)
+    assert list(iout[26:32]) == [-1] * 6                # dltd_* reset
+
+
+def test_crossed_contact_closed_form(oracle):
+    """Perpendicular segments crossing at their midpoints, separation s < R_A
+    along z: force F(s) split 1/2, 1/2 on i's sites, no angular term."""
+    L, visc, area = 2.0, 1.0, 2.0
+    rad, W, mu = 0.3, 0.2, 3.0
+    s = 0.5
+    fp = fpar(W_A=W, mu=mu, visc=visc)
+    pi_, ii = rod(rad, L, math.pi / 2, 0.0, 1, area=area)
+    pj_, ij = rod(rad, L, math.pi / 2, math.pi / 2, 2, area=area)
+    v1, _, w1, _, _, _ = oracle.evaluate_force([0, 0, s], pi_, pj_, ii, ij, fp)
+    rS, rA = 2 * rad, 2 * rad + W
+    F = mu * (rA - s) ** 2 * (rS - s)
+    d0, e0, e1 = rp_coeffs(area, L, visc)
+    np.testing.assert_allclose(v1, [0, 0, 0.5 * (d0 + e1) * F], rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(w1, 0.0, atol=1e-12 * abs(F))
+
+
+def rot_matrix(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_rod_pair_rotation_covariance(oracle):
+    """Rotating the whole pair rotates both outputs (the mobility is axially
+    symmetric, so the roll of the body frame does not matter).  A transposed
+    R D R^T or a wrong cross product breaks this."""
+    rng = np.random.default_rng(5)
+    fp = fpar(W_A=0.3, mu=2.0, visc=1.7)
+    checked = 0
+    for _ in range(300):
+        ti, pi_a = rng.uniform(0.3, 2.8), rng.uniform(0, 2 * math.pi)
+        tj, pj_a = rng.uniform(0.3, 2.8), rng.uniform(0, 2 * math.pi)
+        r12 = rng.normal(size=3) * 0.6
+        a = rod(0.25, 1.5, ti, pi_a, 1, area=1.0)
+        b = rod(0.25, 1.8, tj, pj_a, 2, area=1.2)
+        v1, _, w1, _, _, _ = oracle.evaluate_force(r12, a[0], b[0], a[1], b[1], fp)
+        if np.linalg.norm(v1) == 0:
+            continue
+        Q = rot_matrix(rng)
+        ni, nj = Q @ direction(ti, pi_a), Q @ direction(tj, pj_a)
+        a2 = rod(0.25, 1.5, *angles(ni), 1, area=1.0)
+        b2 = rod(0.25, 1.8, *angles(nj), 2, area=1.2)
+        v1r, _, w1r, _, _, _ = oracle.evaluate_force(Q @ r12, a2[0], b2[0], a2[1], b2[1], fp)
+        sv = np.linalg.norm(v1) + 1e-300
+        np.testing.assert_allclose(v1r, Q @ v1, atol=1e-8 * sv)
+        np.testing.assert_allclose(w1r, Q @ w1, atol=1e-8 * (np.linalg.norm(w1) + sv))
+        checked += 1
+    assert checked > 50
+
+
+def test_bond_repair_and_demotion(oracle):
+    """j registered i but i did not: i adds j on the free site (K.H:390-467);
+    a SECOND_1 segment whose site-1 partner is not a tip becomes INTERIOR."""
+    fp = fpar(W_A=0.1, mu=1.0, kb=1.0)
+    a = rod(0.2, 2.0, math.pi / 2, 0.0, 5)
+    b = rod(0.2, 2.0, math.pi / 2, 0.0, 6)
+    a[1][6], a[1][15] = 0, 1          # i: no bonds, site1 marker = 1
+    b[1][6], b[1][7], b[1][11], b[1][15] = 1, 5, 0, 2
+    _, _, _, _, iout, _ = oracle.evaluate_force([2.0, 0, 0], a[0], b[0], a[1], b[1], fp)
+    assert iout[6] == 1 and iout[7] == 6 and iout[11] == 0 and iout[15] == 2
+    # demotion
+    a = rod(0.2, 2.0, math.pi / 2, 0.0, 5)
+    b = rod(0.2, 2.0, math.pi / 2, 0.0, 6)
+    a[1][6], a[1][7], a[1][15], a[1][19] = 1, 6, 1, 1   # SECOND_1 bonded to 6 on site 1
+    b[1][6], b[1][7], b[1][15], b[1][19] = 1, 5, 2, 3   # partner is INTERIOR
+    _, _, _, _, iout, _ = oracle.evaluate_force([2.0, 0, 0], a[0], b[0], a[1], b[1], fp)
+    assert iout[19] == 3
+    b[1][19] = 0                                         # partner is a TIP
+    a[1][19] = 1
+    _, _, _, _, iout, _ = oracle.evaluate_force([2.0, 0, 0], a[0], b[0], a[1], b[1], fp)
+    assert iout[19] == 1
+
+
+def test_fix_bonds(oracle):
+    L = 2.0
+    a = rod(0.2, L, math.pi / 2, 0.0, 1)
+    b = rod(0.2, L, math.pi / 2, 0.0, 2)
+    # i.site1 touches j.site2 (gap 1e-8 < rlim)
+    rc, ii, ij = oracle.fix_bonds([L + 1e-8, 0, 0], a[0], a[1], b[0], b[1])
+    assert rc == 0 and ii[6] == 1 and ii[7] == 2 and ii[15] == 1
+    assert ij[6] == 1 and ij[7] == 1 and ij[15] == 2
+    # i.site2 touches j.site1
+    rc, ii, _ = oracle.fix_bonds([-L, 0, 0], a[0], a[1], b[0], b[1])
+    assert ii[6] == 1 and ii[15] == 2
+    # gap beyond rlim: nothing
+    rc, ii, _ = oracle.fix_bonds([L + 2e-7, 0, 0], a[0], a[1], b[0], b[1])
+    assert ii[6] == 0
+    # a fifth bond is reported, not written
+    full = a[1].copy()
+    full[6] = 4
+    rc, ii, _ = oracle.fix_bonds([L, 0, 0], a[0], full, b[0], b[1])
+    assert rc == 1 and ii[6] == 4 and ii[11] == 0
+
+
+# ---------------------------------------------------------------- update --
+def test_sphere_update_is_explicit_euler(oracle):
+    rec = np.zeros(1, dtype=oracle.RECORD_DTYPE)[0]
+    rec["pos"] = (1.0, 2.0, 3.0)
+    rec["rdata"][0] = 0.5
+    out = oracle.update_particle(rec, [0.5, -0.25, 2.0, 0.0, 0.0, 0.0], 0.25)
+    assert tuple(out["pos"]) == (1.125, 1.9375, 3.5)
+    assert tuple(out["rdata"][6:12]) == (0.5, -0.25, 2.0, 0.0, 0.0, 0.0)
+
+
+def test_rod_orientation_update_rodrigues(oracle):
+    """The body-frame angular term (x part dropped, scaled 2/L^2) turns the
+    axis n about the lab vector W = R w_b by |W| dt (Rodrigues)."""
+    rng = np.random.default_rng(6)
+    for _ in range(200):
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(0, 2 * math.pi)
+        L, dt = rng.uniform(0.5, 2.0), rng.uniform(0.01, 0.3)
+        w = rng.normal(size=3)
+        rec = np.zeros(1, dtype=oracle.RECORD_DTYPE)[0]
+        rec["idata"][5] = 1
+        rec["rdata"][1], rec["rdata"][2], rec["rdata"][3] = L, th, ph
+        out = oracle.update_particle(rec, [0, 0, 0, *w], dt)
+        ct, st, cp, sp = math.cos(th), math.sin(th), math.cos(ph), math.sin(ph)
+        R = np.array([[cp * st, -sp, -cp * ct], [sp * st, cp, -sp * ct], [ct, 0, st]])
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        n = R[:, 0]
+        wb = R.T @ w
+        wb[0] = 0.0
+        wb *= 2.0 / (L * L)
+        W = R @ wb
+        a = np.linalg.norm(W) * dt
+        k = W / np.linalg.norm(W)
+        expect = n * math.cos(a) + np.cross(k, n) * math.sin(a)
+        got = direction(out["rdata"][2], out["rdata"][3])
+        np.testing.assert_allclose(got, expect, atol=2e-7 if abs(expect[2]) > 0.9999 else 1e-11)
+
+
+# ------------------------------------------------------ binning / lists --
+def test_bin_index_known_answers(oracle):
+    g = oracle.make_geom((0, 0, 0), (1.0, 2.0, 4.0), (1, 1, 0), (4, 8, 16))
+    assert oracle.bin_index((0.0, 0.0, 0.0), g)[0] == 0
+    b, b3 = oracle.bin_index((0.3, 0.3, 0.3), g)
+    assert tuple(b3) == (1, 1, 1) and b == (1 * 8 + 1) * 4 + 1
+    b, b3 = oracle.bin_index((1.0 - 1e-16, 1.999, 9.0), g)   # clamps high
+    assert tuple(b3) == (3, 7, 15)
+    b, b3 = oracle.bin_index((-0.5, 0.0, -3.0), g)           # clamps low
+    assert tuple(b3) == (0, 0, 0)
+
+
+def test_pair_separation_minimum_image(oracle):
+    g = oracle.make_geom((0, 0, 0), (1.0, 1.0, 1.0), (1, 1, 0), (4, 4, 4))
+    d = oracle.pair_separation((0.05, 0.5, 0.05), (0.95, 0.5, 0.95), g)
+    np.testing.assert_allclose(d, [0.05 - (0.95 - 1.0), 0.0, 0.05 - 0.95])
+
+
+def random_recs(oracle, n, L, rng, rmin=0.01, rmax=0.02):
+    recs = np.zeros(n, dtype=oracle.RECORD_DTYPE)
+    recs["pos"] = rng.uniform(0, 1, (n, 3)) * L
+    recs["rdata"][:, 0] = rng.uniform(rmin, rmax, n)
+    return recs
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 0), (1, 1, 1), (0, 0, 0)])
+def test_cell_list_equals_brute_force(oracle, periodic):
+    rng = np.random.default_rng(7)
+    L = np.array([1.0, 0.8, 0.6])
+    recs = random_recs(oracle, 600, L, rng)
+    R = 0.09
+    nb = tuple(int(math.floor(L[d] / R)) for d in range(3))
+    g = oracle.make_geom((0, 0, 0), L, periodic, nb)
+    off, idx = oracle.build_neighbors(recs, g, oracle.NBR_RADIUS, R)
+    offb, idxb = oracle.build_neighbors(recs, g, oracle.NBR_RADIUS, R, brute=True)
+    for i in range(len(recs)):
+        assert sorted(idx[off[i]:off[i + 1]]) == list(idxb[offb[i]:offb[i + 1]])
+    assert off[-1] > 1000
+    # stencil mode: every pair in the +-1 stencil with r^2 <= neighborhood,
+    # here a value beyond the stencil reach, i.e. the whole stencil
+    nc = (6, 5, 4)
+    gs = oracle.make_geom((0, 0, 0), L, periodic, nc)
+    off, idx = oracle.build_neighbors(recs, gs, oracle.NBR_STENCIL, 10.0)
+    for i in range(0, len(recs), 37):
+        _, bi = oracle.bin_index(recs["pos"][i], gs)
+        expect = []
+        for j in range(len(recs)):
+            if j == i:
+                continue
+            _, bj = oracle.bin_index(recs["pos"][j], gs)
+            ok = True
+            for d in range(3):
+                diff = bj[d] - bi[d]
+                if periodic[d]:
+                    diff = (diff + nc[d] + 1) % nc[d] - 1
+                ok &= abs(diff) <= 1
+            if ok:
+                expect.append(j)
+        assert sorted(idx[off[i]:off[i + 1]]) == expect
+
+
+# ---------------------------------------------------------------- driver --
+def test_momentum_conservation(oracle):
+    """No wall (xi = 0) and a fully periodic box: pair terms cancel in total."""
+    rng = np.random.default_rng(8)
+    L = np.array([0.5, 0.5, 0.5])
+    recs = random_recs(oracle, 800, L, rng, 0.01, 0.014)
+    fp = fpar(W_A=0.004, mu=1e3)
+    R = 2 * 0.014 + 0.004
+    g = oracle.make_geom((0, 0, 0), L, (1, 1, 1), [int(L[d] // R) for d in range(3)])
+    st = oracle.State(recs, g, fp, oracle.NBR_RADIUS, R, 1)
+    fc = st.evolve(0.1, 1)
+    tot = fc[:3].sum(axis=1)
+    assert np.all(np.abs(tot) <= 1e-13 * np.abs(fc[:3]).sum(axis=1).max())
+    assert np.abs(fc[:3]).max() > 0
+
+
+def test_pair_relaxes_to_contact(oracle):
+    """exec/pair_test: two spheres inside the attractive band approach R_S
+    monotonically and stop there."""
+    recs = np.zeros(2, dtype=oracle.RECORD_DTYPE)
+    recs["rdata"][:, 0] = 5e-5
+    recs["pos"][0] = (0.003, 0.003, 0.003)
+    recs["pos"][1] = (0.003 + 1.2e-4, 0.003, 0.003)
+    fp = fpar(W_A=5e-5, mu=1e8)
+    g = oracle.make_geom((0, 0, 0), (0.006, 0.006, 0.006), (1, 1, 0), (4, 4, 4))
+    st = oracle.State(recs, g, fp, oracle.NBR_STENCIL, 1.0, 1)
+    seps = []
+    for _ in range(400):
+        st.evolve(0.25, 1)
+        seps.append(recs["pos"][1, 0] - recs["pos"][0, 0])
+    assert all(b <= a + 1e-18 for a, b in zip(seps, seps[1:]))
+    assert seps[-1] == pytest.approx(1e-4, rel=1e-6)
+
+
+def test_wall_equilibrium_height(oracle):
+    """exec/wall_test: a cell settles where the surface law balances the
+    gravity term, H(dz) = g xi R_A^2 R_S."""
+    rS, WZ, xi, gr = 5e-5, 1e-4, 1e7, 0.01
+    recs = np.zeros(1, dtype=oracle.RECORD_DTYPE)
+    recs["rdata"][0, 0] = rS
+    recs["pos"][0] = (0.001, 0.001, 0.0048 + 1.2e-4)
+    fp = fpar(zw=0.0048, WZ=WZ, xi=xi, g=gr)
+    g = oracle.make_geom((0, 0, 0), (0.002, 0.002, 0.01), (1, 1, 0), (4, 4, 4))
+    st = oracle.State(recs, g, fp, oracle.NBR_STENCIL, 1.0, 1)
+    st.evolve(0.25, 4000)
+    rA = rS + WZ
+    c = gr * rA * rA * rS
+    roots = np.roots([-1.0, 2 * rA + rS, -(rA * rA + 2 * rA * rS), rA * rA * rS - c])
+    # gravity pulls down, so the balance sits in the repulsive range 0 < dz < R_S
+    dz = [r.real for r in roots if abs(r.imag) < 1e-12 and 0 < r.real < rS][0]
+    assert recs["pos"][0, 2] - 0.0048 == pytest.approx(dz, rel=1e-8)
+
+
+def test_verlet_and_stencil_lists_agree(oracle):
+    """The reference rebuilds a stencil list every step; a Verlet list with a
+    skin rebuilt every 10 steps gives the same interacting pairs, so the same
+    forces up to summation order."""
+    rng = np.random.default_rng(9)
+    L = np.array([0.4, 0.4, 0.2])
+    recs = random_recs(oracle, 1500, L, rng, 0.006, 0.009)
+    recs2 = recs.copy()
+    fp = fpar(W_A=0.002, mu=1e4, zw=0.0, WZ=0.01, xi=1e3, g=0.01)
+    R = 2 * 0.009 + 0.002 + 0.004
+    g1 = oracle.make_geom((0, 0, 0), L, (1, 1, 0), [int(L[d] // R) for d in range(3)])
+    g2 = oracle.make_geom((0, 0, 0), L, (1, 1, 0), (16, 16, 8))
+    a = oracle.State(recs, g1, fp, oracle.NBR_RADIUS, R, 10)
+    b = oracle.State(recs2, g2, fp, oracle.NBR_STENCIL, 1.0, 1)
+    for _ in range(12):
+        fa = a.evolve(0.05, 1)
+        fb = b.evolve(0.05, 1)
+        scale = np.abs(fb).max()
+        np.testing.assert_allclose(fa, fb, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(recs["pos"], recs2["pos"], rtol=0, atol=1e-12)
