@@ -108,9 +108,13 @@ struct bmx_ctx {
   // outputs and flags
   DevBuf<double> d_fc;
   bool wrote_step = false;
-  DevBuf<int32_t> d_err, d_stats;
-  DevBuf<unsigned long long> d_total, d_vmax, d_ustats;
+  DevBuf<int32_t> d_err, d_stats, d_bstats;
+  DevBuf<unsigned long long> d_total, d_ustats;
+  DevBuf<double> d_vpart;      // [slot][block] max dt*|vel| per force block
+  DevBuf<unsigned long long> d_slotmax;  // per-slot maxima (skin check)
   int32_t vmax_slots = 1;
+  int32_t vstride = 1;         // blocks per slot (smallest block size)
+  int32_t row_nblk = 0;        // blocks of the last row build
   int32_t *h_flags = nullptr;  // pinned: err, max row, pad, pad, total(2)
 
   // timing
@@ -285,6 +289,20 @@ bmx_status ensure_rows(bmx_ctx *c) {
   return BMX_OK;
 }
 
+// Per-block partial arrays sized for the smallest block size; the skin
+// check reads every slot, so unused tails stay zero.
+bmx_status alloc_partials(bmx_ctx *c) {
+  c->vmax_slots = std::max(1, c->nbp.rebuild_every);
+  c->vstride = std::max(1, row_blocks(c->ncap, 64));
+  HIPCHK(c->d_vpart.alloc((size_t)c->vmax_slots * c->vstride));
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
+                         (int64_t)c->vmax_slots * c->vstride, 0ull, c->stream));
+  HIPCHK(c->d_bstats.alloc(2 * (size_t)c->vstride));
+  HIPCHK(c->d_slotmax.alloc(c->vmax_slots));
+  HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, c->stream));
+  return BMX_OK;
+}
+
 // Binning + gather into the other buffers + list build.
 bmx_status rebuild(bmx_ctx *c) {
   hipStream_t s = c->stream;
@@ -312,12 +330,13 @@ bmx_status rebuild(bmx_ctx *c) {
   c->cd = 1 - c->cd;
   StaticArrays st = static_arrays(c, c->cs);
   DynArrays dy = dyn_arrays(c, c->cd);
-  HIPCHK(launch_fill_i32(c->d_stats.p, 1, 0, s));
-  HIPCHK(launch_fill_u64(c->d_total.p, 1, 0ull, s));
-  HIPCHK(launch_build_nbr(dy.pos, st.col, c->d_boxes.p, g0, multi,
-                          c->d_start.p, st.key, c->n, c->crit, c->d_nbr.p,
+  HIPCHK(launch_build_nbr(dy.pos, st.col, st.type, dy.rod, c->d_boxes.p, g0,
+                          multi, c->any_fungi ? 1 : 0, c->d_start.p, st.key,
+                          c->n, c->nbp.mode == BMX_NBR_STENCIL ? 1 : 0,
+                          c->crit, c->fpar[0], c->nbp.skin, c->d_nbr.p,
                           c->d_ncnt.p, c->ncap, c->maxn, c->d_err.p,
-                          c->d_stats.p, c->d_total.p, shape(c), s));
+                          c->d_bstats.p, shape(c), s));
+  c->row_nblk = row_blocks(c->n, c->lp.block_size);
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, s));
     c->ev_rebuild.push_back({e0, e1});
@@ -337,14 +356,16 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
               (c->step - c->last_build) >= (int64_t)c->nbp.rebuild_every;
   if (need) {
     if (c->have_list && verlet_checked(c))
-      HIPCHK(launch_skin_check(c->d_vmax.p, c->vmax_slots,
-                               0.5 * c->nbp.skin, c->d_err.p, s));
+      HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                               0.5 * c->nbp.skin, 1, c->d_slotmax.p,
+                               c->d_err.p, s));
     bmx_status st = rebuild(c);
     if (st != BMX_OK) return st;
   }
   int slot = (int)(c->step - c->last_build);
-  unsigned long long *vm =
-      verlet_checked(c) ? c->d_vmax.p + (slot % c->vmax_slots) : nullptr;
+  double *vm = verlet_checked(c)
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
+                   : nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = next_event(c);
@@ -381,6 +402,9 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
 // Reads the error word and list statistics; clears the error word.
 bmx_status check_flags(bmx_ctx *c) {
   hipStream_t s = c->stream;
+  if (c->row_nblk > 0)
+    HIPCHK(launch_reduce_rowstats(c->d_bstats.p, c->row_nblk, c->d_stats.p,
+                                  c->d_total.p, s));
   HIPCHK(hipMemcpyAsync(c->h_flags, c->d_err.p, sizeof(int32_t),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->h_flags + 1, c->d_stats.p, sizeof(int32_t),
@@ -549,9 +573,9 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   bs = ensure_rows(c);
   if (bs != BMX_OK) return bs;
   HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
-  c->vmax_slots = std::max(1, c->nbp.rebuild_every);
-  HIPCHK(c->d_vmax.alloc(c->vmax_slots));
-  HIPCHK(launch_fill_u64(c->d_vmax.p, c->vmax_slots, 0ull, s));
+  bs = alloc_partials(c);
+  if (bs != BMX_OK) return bs;
+  c->row_nblk = 0;
   HIPCHK(hipStreamSynchronize(s));
   c->step = 0;
   c->last_build = -1;
@@ -650,8 +674,10 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_fc.release();
   c->d_err.release();
   c->d_stats.release();
+  c->d_bstats.release();
   c->d_total.release();
-  c->d_vmax.release();
+  c->d_vpart.release();
+  c->d_slotmax.release();
   c->d_ustats.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -707,9 +733,8 @@ bmx_status bmx_set_neighbor_params(bmx_ctx *c, const bmx_neighbor_params *p) {
     if (st != BMX_OK) return st;
     st = ensure_rows(c);
     if (st != BMX_OK) return st;
-    c->vmax_slots = std::max(1, c->nbp.rebuild_every);
-    HIPCHK(c->d_vmax.alloc(c->vmax_slots));
-    HIPCHK(launch_fill_u64(c->d_vmax.p, c->vmax_slots, 0ull, c->stream));
+    st = alloc_partials(c);
+    if (st != BMX_OK) return st;
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   return BMX_OK;
@@ -797,8 +822,9 @@ bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
     if (st != BMX_OK) return st;
   }
   if (verlet_checked(c) && nsteps > 0 && c->have_list)
-    HIPCHK(launch_skin_check(c->d_vmax.p, c->vmax_slots, 0.5 * c->nbp.skin,
-                             c->d_err.p, c->stream));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 0, c->d_slotmax.p, c->d_err.p,
+                             c->stream));
   return check_flags(c);
 }
 
@@ -828,7 +854,8 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     for (int32_t k = 0; k < nsteps && cst == BMX_OK; k++)
       cst = enqueue_step(c, dt, k == nsteps - 1);
     if (cst == BMX_OK && verlet_checked(c))
-      (void)launch_skin_check(c->d_vmax.p, c->vmax_slots, 0.5 * c->nbp.skin,
+      (void)launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                              0.5 * c->nbp.skin, 0, c->d_slotmax.p,
                               c->d_err.p, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);

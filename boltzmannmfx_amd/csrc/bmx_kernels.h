@@ -49,40 +49,50 @@ hipError_t launch_gather(const unsigned long long *permk, int64_t n,
                          const int32_t *key, int multi, int rods,
                          hipStream_t s);
 
-// ---- neighbor list (column-major rows: nbr[k*ncap + i])
-// mode 0: r2 <= crit (crit = radius^2); mode 1: r2 <= crit (neighborhood)
-// stats: [0] max row, total_entries accumulates row lengths
+// ---- neighbor list (column-major rows: nbr[k*ncap + i]; ncnt holds the row
+// length, bit 30 set for particles that need no periodic image test)
+// mode 0: r2 <= (reach_i + reach_j + wa + skin)^2 (reach = radius, or 1.5 *
+// c_length for segments); mode 1: r2 <= crit (the neighborhood value)
 hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
+                            const int32_t *type, const double4 *rod,
                             const BoxGeom *boxes, BoxGeom g0, int multi,
-                            const int32_t *start, const int32_t *key,
-                            int64_t n, double crit, int32_t *nbr,
-                            int32_t *ncnt, int64_t ncap, int32_t maxn,
-                            int32_t *err, int32_t *stats,
-                            unsigned long long *total, LaunchShape ls,
-                            hipStream_t s);
+                            int rods, const int32_t *start, const int32_t *key,
+                            int64_t n, int mode, double crit, double wa,
+                            double skin, int32_t *nbr, int32_t *ncnt,
+                            int64_t ncap, int32_t maxn, int32_t *err,
+                            int32_t *bstats, LaunchShape ls, hipStream_t s);
+// blocks of a row build / force launch over n particles (partial arrays)
+int32_t row_blocks(int64_t n, int block);
+// stats[0] = longest row, *total = entries, from the per-block partials
+hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
+                                  int32_t *stats, unsigned long long *total,
+                                  hipStream_t s);
 
 // ---- force + overdamped update, fused.  fc (6*ncap) is written when
-// non-null; vmax receives max(dt*|vel|) as double bits when non-null.
+// non-null; vmax[block] receives the block's max dt*|vel| when non-null.
 hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
                                const int32_t *nbr, const int32_t *ncnt,
                                int64_t ncap, int64_t n, const int32_t *col,
                                const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc,
-                               unsigned long long *vmax, int32_t *err,
-                               LaunchShape ls, hipStream_t s);
+                               ForceParams fp, double *fc, double *vmax,
+                               int32_t *err, LaunchShape ls, hipStream_t s);
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *nbr, const int32_t *ncnt,
                                 int64_t ncap, int64_t n, const BoxGeom *boxes,
                                 BoxGeom g0, int multi, ForceParams fp,
-                                double *fc, unsigned long long *vmax,
-                                int32_t *err, LaunchShape ls, hipStream_t s);
+                                double *fc, double *vmax, int32_t *err,
+                                LaunchShape ls, hipStream_t s);
 hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
                              const int32_t *ncnt, int64_t ncap, int64_t n,
                              const BoxGeom *boxes, BoxGeom g0, int multi,
                              ForceParams fp, int32_t *err, hipStream_t s);
-// sum of vmax[0..nslots) > limit -> err |= ERRBIT_SKIN; then clear slots
-hipError_t launch_skin_check(unsigned long long *vmax, int32_t nslots,
-                             double limit, int32_t *err, hipStream_t s);
+// sum over slots of max over blocks of vpart[slot*stride + b] > limit ->
+// err |= ERRBIT_SKIN; clear = 1 zeroes the partials (at a rebuild).
+// slotmax: nslots zeroed words of scratch (left zeroed).
+hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
+                             double limit, int clear,
+                             unsigned long long *slotmax, int32_t *err,
+                             hipStream_t s);
 
 // ---- exports to original order
 hipError_t launch_export_fc(const double *fc, int64_t ncap, const int32_t *oid,

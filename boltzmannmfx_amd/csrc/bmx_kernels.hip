@@ -21,6 +21,7 @@ namespace bmx {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kUnroll = 4;  // neighbors / candidates in flight per thread
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
   return (unsigned long long)__double_as_longlong(x);
@@ -58,6 +59,34 @@ __device__ __forceinline__ double wave_max_f64(double v) {
   return v;
 }
 
+// Block-wide max / sum through LDS (sh: one slot per wave); every thread
+// must call it; the result is valid in thread 0.
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x / kWave); w++) v = sh[w] > v ? sh[w] : v;
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x / kWave); w++) v += sh[w];
+  return v;
+}
+
 template <typename T>
 __global__ void k_fill(T *p, int64_t n, T v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,8 +112,12 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
                              int64_t n, const int32_t *__restrict__ col,
                              const BoxGeom *__restrict__ boxes, StaticArrays st,
                              DynArrays dy, unsigned long long *stats) {
+  __shared__ double shd[16];
+  __shared__ unsigned long long shu[16];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  double rmax = 0.0, lmax = 0.0;
+  unsigned long long flags = 0, nbad = 0;
+  if (t < n) {
   const uint8_t *r = aos + t * (int64_t)lay.stride;
   const double *pos = reinterpret_cast<const double *>(r + lay.pos);
   const double *rd = reinterpret_cast<const double *>(r + lay.rdata);
@@ -100,9 +133,9 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
   st.type[t] = ty;
   if (st.col) st.col[t] = col[t];
   bool bad = !(rad >= 0.0) || !(x == x) || !(y == y) || !(z == z);
-  if (ty == rec::YEAST) atomicOr(&stats[2], 1ull);
-  if (ty == rec::FUNGI) atomicOr(&stats[3], 1ull);
-  if (rad >= 0.0) atomicMax(&stats[0], dbits(rad));
+  if (ty == rec::YEAST) flags |= 1ull;
+  if (ty == rec::FUNGI) flags |= 2ull;
+  if (rad >= 0.0) rmax = rad;
   if (dy.rod) {
     double L = rd[rec::c_length], th = rd[rec::theta], ph = rd[rec::phi];
     dy.rod[t] = make_double4(L, th, ph, rd[rec::area]);
@@ -124,11 +157,28 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
     tp.pad = 0;
     dy.topo[t] = tp;
     if (ty == rec::FUNGI) {
-      if (L >= 0.0) atomicMax(&stats[1], dbits(L));
+      if (L >= 0.0) lmax = L;
       if (tp.n_bnds < 0 || tp.n_bnds > 4 || !(L > 0.0)) bad = true;
     }
   }
-  if (bad) atomicAdd(&stats[4], 1ull);
+  if (bad) nbad = 1;
+  }
+  rmax = block_max(rmax, shd);
+  __syncthreads();
+  lmax = block_max(lmax, shd);
+  __syncthreads();
+  nbad = block_sum(nbad, shu);
+  __syncthreads();
+  unsigned long long fy = block_max(flags & 1ull, shu);
+  __syncthreads();
+  unsigned long long ff = block_max(flags & 2ull, shu);
+  if (threadIdx.x == 0) {
+    if (rmax > 0.0) atomicMax(&stats[0], dbits(rmax));
+    if (lmax > 0.0) atomicMax(&stats[1], dbits(lmax));
+    if (fy) atomicOr(&stats[2], 1ull);
+    if (ff) atomicOr(&stats[3], 1ull);
+    if (nbad) atomicAdd(&stats[4], nbad);
+  }
 }
 
 __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
@@ -332,15 +382,77 @@ __global__ void k_gather(const unsigned long long *__restrict__ permk,
 
 // -------------------------------------------------------- neighbor rows ---
 
-template <bool MULTI>
+// Interaction reach of a particle: r_lm = reach_i + reach_j + W_A for equal
+// cell types (K.H:23-40).
+template <bool RODS>
+__device__ __forceinline__ double reach_of(const double4 &p, int ty,
+                                           const double4 *rod, int64_t u) {
+  if (RODS && ty == rec::FUNGI) return 1.5 * rod[u].x;
+  return p.w;
+}
+
+// Scans one contiguous candidate range [ua, ub) of the sorted arrays.
+// IMG = false when no periodic image can be involved (the stencil does not
+// wrap), which gives the same separation bits as the image test would.
+template <bool RODS, bool IMG>
+__device__ __forceinline__ void scan_range(
+    int ua, int ub, int64_t t, const double4 &p, double reach_i,
+    const BoxGeom &g, const double4 *__restrict__ pos,
+    const int32_t *__restrict__ type, const double4 *__restrict__ rod,
+    int mode, double crit, double wa, double skin, int32_t *__restrict__ nbr,
+    int64_t ncap, int32_t maxn, int &cnt) {
+  for (int u0 = ua; u0 < ub; u0 += kUnroll) {
+    double4 o[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++) o[j] = pos[(u0 + j < ub) ? u0 + j : u0];
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++) {
+      int u = u0 + j;
+      if (u >= ub) break;
+      if (u == t) continue;
+      double d0, d1, d2;
+      if (IMG) {
+        d0 = sep_axis(p.x, o[j].x, g.L[0], g.halfL[0], g.periodic[0]);
+        d1 = sep_axis(p.y, o[j].y, g.L[1], g.halfL[1], g.periodic[1]);
+        d2 = sep_axis(p.z, o[j].z, g.L[2], g.halfL[2], g.periodic[2]);
+      } else {
+        d0 = p.x - o[j].x;
+        d1 = p.y - o[j].y;
+        d2 = p.z - o[j].z;
+      }
+      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      bool ok;
+      if (mode == 1) {
+        ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
+      } else {
+        double rj = RODS ? reach_of<true>(o[j], type[u], rod, u) : o[j].w;
+        double cut = reach_i + rj + wa + skin;
+        ok = r2 <= cut * cut;
+      }
+      if (ok) {
+        if (cnt < maxn) nbr[(int64_t)cnt * ncap + t] = u;
+        cnt++;
+      }
+    }
+  }
+}
+
+// Bit 30 of a row count marks a particle whose neighbors can need no
+// periodic image until the next rebuild (two bins or more from every
+// periodic face); force kernels then skip the image test.
+constexpr int32_t kInteriorBit = 1 << 30;
+__device__ __forceinline__ int row_len(int32_t raw) { return raw & (kInteriorBit - 1); }
+
+template <bool MULTI, bool RODS>
 __global__ __launch_bounds__(512) void k_build_nbr(
     const double4 *__restrict__ pos, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ type, const double4 *__restrict__ rod,
     const BoxGeom *__restrict__ boxes, BoxGeom g0,
     const int32_t *__restrict__ start, const int32_t *__restrict__ key,
-    int64_t n, double crit, int32_t *__restrict__ nbr,
-    int32_t *__restrict__ ncnt, int64_t ncap, int32_t maxn,
-    int32_t *__restrict__ err, int32_t *__restrict__ stats,
-    unsigned long long *__restrict__ total, int remap) {
+    int64_t n, int mode, double crit, double wa, double skin,
+    int32_t *__restrict__ nbr, int32_t *__restrict__ ncnt, int64_t ncap,
+    int32_t maxn, int32_t *__restrict__ err, int32_t *__restrict__ bstats,
+    int remap) {
   int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   int cnt = 0;
   // Candidates visited per particle are capped (a pile of particles in a few
@@ -348,12 +460,33 @@ __global__ __launch_bounds__(512) void k_build_nbr(
   // as an overflow.  After a non-finite force no rows are built.
   const int64_t scan_cap = 64 * (int64_t)maxn + 4096;
   int64_t scanned = 0;
-  if (t < n && !(*err & ERRBIT_NAN)) {
-    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
-    const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
-    double4 p = pos[t];
+  bool active = t < n && !(*err & ERRBIT_NAN);
+  const BoxGeom &g = MULTI ? boxes[active ? col[t] : 0] : g0;
+  const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
+  int bx = 0, by = 0, bz = 0;
+  double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+  double reach_i = 0.0;
+  bool noimg = false, interior = false;
+  if (active) {
+    p = pos[t];
+    reach_i = RODS ? reach_of<true>(p, type[t], rod, t) : p.w;
     int lb = key[t] - g.bin_offset;
-    int bx = lb % nbx, rr = lb / nbx, by = rr % nby, bz = rr / nby;
+    bx = lb % nbx;
+    int rr = lb / nbx;
+    by = rr % nby;
+    bz = rr / nby;
+    const int b3[3] = {bx, by, bz};
+    noimg = true;
+    interior = true;
+    for (int a = 0; a < 3; a++) {
+      if (!g.periodic[a]) continue;
+      int nb = g.nbin[a];
+      noimg = noimg && nb >= 4 && b3[a] >= 1 && b3[a] + 1 < nb;
+      interior = interior && nb >= 6 && b3[a] >= 2 && b3[a] + 2 < nb;
+    }
+  }
+  const bool wave_noimg = __all(noimg || !active);
+  if (active) {
     for (int oz = -1; oz <= 1; oz++) {
       int cz = bz + oz;
       if (g.periodic[2]) {
@@ -396,41 +529,123 @@ __global__ __launch_bounds__(512) void k_build_nbr(
             ranges[q][1] = ranges[q][0];
           }
           scanned += len;
-          for (int u = ranges[q][0]; u < ranges[q][1]; u++) {
-            if (u == t) continue;
-            double4 o = pos[u];
-            double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-            double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-            double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-            double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-            if (r2 <= crit) {
-              if (cnt < maxn) nbr[(int64_t)cnt * ncap + t] = u;
-              cnt++;
-            }
-          }
+          if (wave_noimg)
+            scan_range<RODS, false>(ranges[q][0], ranges[q][1], t, p, reach_i,
+                                    g, pos, type, rod, mode, crit, wa, skin,
+                                    nbr, ncap, maxn, cnt);
+          else
+            scan_range<RODS, true>(ranges[q][0], ranges[q][1], t, p, reach_i,
+                                   g, pos, type, rod, mode, crit, wa, skin,
+                                   nbr, ncap, maxn, cnt);
         }
       }
     }
-    ncnt[t] = cnt < maxn ? cnt : maxn;
+    ncnt[t] = (cnt < maxn ? cnt : maxn) | (interior ? kInteriorBit : 0);
     if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
   } else if (t < n) {
     ncnt[t] = 0;
   }
-  int m = wave_max_i32(cnt);
-  long long sm = wave_sum_i64(cnt);
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicMax(&stats[0], m);
-    atomicAdd(total, (unsigned long long)sm);
+  __shared__ int32_t shi[16];
+  int m = block_max(cnt, shi);
+  __syncthreads();
+  int sm = block_sum(cnt, shi);
+  if (threadIdx.x == 0) {
+    bstats[2 * blockIdx.x] = m;
+    bstats[2 * blockIdx.x + 1] = sm;
+  }
+}
+
+// Reduces the per-block row statistics of the last build.
+__global__ __launch_bounds__(1024) void k_reduce_rowstats(
+    const int32_t *__restrict__ bstats, int32_t nblk,
+    int32_t *__restrict__ stats, unsigned long long *__restrict__ total) {
+  __shared__ int32_t shi[16];
+  __shared__ unsigned long long shu[16];
+  int m = 0;
+  unsigned long long sm = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    int v = bstats[2 * b];
+    m = v > m ? v : m;
+    sm += (unsigned long long)bstats[2 * b + 1];
+  }
+  m = block_max(m, shi);
+  __syncthreads();
+  sm = block_sum(sm, shu);
+  if (threadIdx.x == 0) {
+    stats[0] = m;
+    *total = sm;
   }
 }
 
 // ------------------------------------------------------- force + update ---
 
-__device__ __forceinline__ void block_vmax(double v2, double dt,
-                                           unsigned long long *vmax) {
-  double m = wave_max_f64(v2);
-  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0.0)
-    atomicMax(vmax, dbits(dt * sqrt(m)));
+// Largest step displacement dt*|vel| of the block, one store per block.
+__device__ __forceinline__ void block_vmax(double v2, double dt, double *vpart) {
+  __shared__ double shd[16];
+  double m = block_max(v2, shd);
+  if (threadIdx.x == 0) vpart[blockIdx.x] = dt * sqrt(m);
+}
+
+// Sphere row, two passes over up to 64 neighbors at a time: first the cheap
+// interaction test (K.H:23-40 with PCI:212) for each neighbor, recorded as a
+// bit mask; then the cubic law (K.H:243-270) only for the set bits, in row
+// order.  The second pass recomputes the separation with the same
+// operations, so the terms and their order are those of a single pass.
+template <bool IMG>
+__device__ __forceinline__ void sphere_row(const double4 &p, int c,
+                                           const int32_t *__restrict__ row,
+                                           int64_t ncap, int64_t t,
+                                           const double4 *__restrict__ pos_in,
+                                           const BoxGeom &g,
+                                           const ForceParams &fp, double &a0,
+                                           double &a1, double &a2) {
+  for (int base = 0; base < c; base += 64) {
+    const int m = (c - base) < 64 ? (c - base) : 64;
+    unsigned long long mask = 0ull;
+    for (int k = 0; k < m; k += kUnroll) {
+      int u[kUnroll];
+      double4 o[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++)
+        u[j] = (k + j < m) ? row[(int64_t)(base + k + j) * ncap] : (int)t;
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) o[j] = pos_in[u[j]];
+#pragma unroll
+      for (int j = 0; j < kUnroll; j++) {
+        if (k + j >= m) break;
+        double d0, d1, d2;
+        if (IMG) {
+          d0 = sep_axis(p.x, o[j].x, g.L[0], g.halfL[0], g.periodic[0]);
+          d1 = sep_axis(p.y, o[j].y, g.L[1], g.halfL[1], g.periodic[1]);
+          d2 = sep_axis(p.z, o[j].z, g.L[2], g.halfL[2], g.periodic[2]);
+        } else {
+          d0 = p.x - o[j].x;
+          d1 = p.y - o[j].y;
+          d2 = p.z - o[j].z;
+        }
+        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double lim = p.w + o[j].w + fp.WA - 1.0e-15;  // r_lm - small_number
+        if (r2 <= lim * lim) mask |= 1ull << (k + j);
+      }
+    }
+    while (mask) {
+      int k = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      double4 o = pos_in[row[(int64_t)(base + k) * ncap]];
+      double d0, d1, d2;
+      if (IMG) {
+        d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+        d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+        d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+      } else {
+        d0 = p.x - o.x;
+        d1 = p.y - o.y;
+        d2 = p.z - o.z;
+      }
+      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      sphere_pair(d0, d1, d2, r2, p.w + o.w, fp, a0, a1, a2);
+    }
+  }
 }
 
 template <bool MULTI>
@@ -439,27 +654,22 @@ __global__ __launch_bounds__(512) void k_force_sphere(
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
     int64_t ncap, int64_t n, const int32_t *__restrict__ col,
     const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
-    double *__restrict__ fc, unsigned long long *__restrict__ vmax,
+    double *__restrict__ fc, double *__restrict__ vmax,
     int32_t *__restrict__ err, int remap) {
   int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   double v2 = 0.0;
-  if (t < n) {
+  const bool active = t < n;
+  const int32_t raw = active ? ncnt[t] : kInteriorBit;
+  const bool wave_interior = __all((raw & kInteriorBit) != 0);
+  if (active) {
     const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
     double4 p = pos_in[t];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    int c = ncnt[t];
-    const int32_t *row = nbr + t;
-    for (int k = 0; k < c; k++) {
-      int u = row[(int64_t)k * ncap];
-      double4 o = pos_in[u];
-      double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-      double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-      double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      double rS = p.w + o.w;
-      double lim = rS + fp.WA - 1.0e-15;  // r_lm - small_number (PCI:212)
-      if (r2 <= lim * lim) sphere_pair(d0, d1, d2, r2, rS, fp, a0, a1, a2);
-    }
+    const int c = row_len(raw);
+    if (wave_interior)
+      sphere_row<false>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
+    else
+      sphere_row<true>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
     // growth surface (K.H:1009-1030); vcom x/y are zero
     a0 += 0.0;
     a1 += 0.0;
@@ -490,7 +700,7 @@ __global__ __launch_bounds__(256) void k_force_general(
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
     int64_t ncap, int64_t n, const BoxGeom *__restrict__ boxes, BoxGeom g0,
     ForceParams fp, double *__restrict__ fc,
-    unsigned long long *__restrict__ vmax, int32_t *__restrict__ err,
+    double *__restrict__ vmax, int32_t *__restrict__ err,
     int remap) {
   int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   double v2 = 0.0;
@@ -502,7 +712,7 @@ __global__ __launch_bounds__(256) void k_force_general(
     const double4 tr_i = in.trig[t];
     Topo ti = in.topo[t];
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int c = ncnt[t];
+    const int c = row_len(ncnt[t]);
     const int32_t *row = nbr + t;
     if (ty == rec::YEAST) {
       for (int k = 0; k < c; k++) {
@@ -604,7 +814,7 @@ __global__ void k_init_bonds(DynArrays dy, StaticArrays st,
   double Li = rod_i.x;
   double nxi = tr.y * tr.z, nyi = tr.y * tr.w, nzi = tr.x;
   Topo ti = dy.topo[t];
-  int c = ncnt[t];
+  int c = row_len(ncnt[t]);
   for (int k = 0; k < c; k++) {
     int u = nbr[(int64_t)k * ncap + t];
     if (st.type[u] != rec::FUNGI) continue;
@@ -655,15 +865,40 @@ __global__ void k_init_bonds(DynArrays dy, StaticArrays st,
   }
 }
 
-__global__ void k_skin_check(unsigned long long *vmax, int32_t nslots,
-                             double limit, int32_t *err) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = 0.0;
-  for (int i = 0; i < nslots; i++) {
-    s += __longlong_as_double((long long)vmax[i]);
-    vmax[i] = 0ull;
+// Sum over the steps since the last rebuild of the largest displacement in
+// any block bounds every particle's displacement; more than half the skin
+// means a pair may have entered interaction range unlisted.  Stage 1: each
+// block takes the per-slot max over a chunk of force blocks into slotmax
+// (one atomic per slot per block, values >= 0 compare as integers).
+__global__ __launch_bounds__(256) void k_skin_partial(
+    double *__restrict__ vpart, int32_t nslots, int32_t stride, int clear,
+    unsigned long long *__restrict__ slotmax) {
+  __shared__ double shd[4];
+  for (int sl = 0; sl < nslots; sl++) {
+    double m = 0.0;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < stride;
+         b += (int64_t)gridDim.x * blockDim.x) {
+      double v = vpart[(int64_t)sl * stride + b];
+      m = v > m ? v : m;
+      if (clear) vpart[(int64_t)sl * stride + b] = 0.0;
+    }
+    m = block_max(m, shd);
+    __syncthreads();
+    if (threadIdx.x == 0 && m > 0.0) atomicMax(&slotmax[sl], dbits(m));
   }
-  if (s > limit) atomicOr(err, ERRBIT_SKIN);
+}
+
+// Stage 2: sum the slot maxima, compare, reset the slot maxima.
+__global__ void k_skin_final(unsigned long long *__restrict__ slotmax,
+                             int32_t nslots, double limit,
+                             int32_t *__restrict__ err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double total = 0.0;
+  for (int sl = 0; sl < nslots; sl++) {
+    total += __longlong_as_double((long long)slotmax[sl]);
+    slotmax[sl] = 0ull;
+  }
+  if (total > limit) atomicOr(err, ERRBIT_SKIN);
 }
 
 // ------------------------------------------------------------- exports ---
@@ -709,7 +944,7 @@ __global__ void k_export_counts(const int32_t *__restrict__ ncnt,
                                 int64_t *__restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  out[oid[t]] = ncnt[t];
+  out[oid[t]] = row_len(ncnt[t]);
 }
 __global__ void k_export_nbr(const int32_t *__restrict__ nbr,
                              const int32_t *__restrict__ ncnt, int64_t ncap,
@@ -719,7 +954,7 @@ __global__ void k_export_nbr(const int32_t *__restrict__ nbr,
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int64_t base = row_off[oid[t]];
-  int c = ncnt[t];
+  int c = row_len(ncnt[t]);
   for (int k = 0; k < c; k++) out[base + k] = oid[nbr[(int64_t)k * ncap + t]];
 }
 
@@ -820,23 +1055,39 @@ hipError_t launch_gather(const unsigned long long *permk, int64_t n,
 }
 
 hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
+                            const int32_t *type, const double4 *rod,
                             const BoxGeom *boxes, BoxGeom g0, int multi,
-                            const int32_t *start, const int32_t *key,
-                            int64_t n, double crit, int32_t *nbr,
-                            int32_t *ncnt, int64_t ncap, int32_t maxn,
-                            int32_t *err, int32_t *stats,
-                            unsigned long long *total, LaunchShape ls,
-                            hipStream_t s) {
+                            int rods, const int32_t *start, const int32_t *key,
+                            int64_t n, int mode, double crit, double wa,
+                            double skin, int32_t *nbr, int32_t *ncnt,
+                            int64_t ncap, int32_t maxn, int32_t *err,
+                            int32_t *bstats, LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
-  if (multi)
-    hipLaunchKernelGGL(k_build_nbr<true>, dim3(grid_for(n, b)), dim3(b), 0, s,
-                       pos, col, boxes, g0, start, key, n, crit, nbr, ncnt,
-                       ncap, maxn, err, stats, total, ls.remap);
+#define BMX_BUILD(M, RD)                                                      \
+  hipLaunchKernelGGL((k_build_nbr<M, RD>), dim3(grid_for(n, b)), dim3(b), 0, s, \
+                     pos, col, type, rod, boxes, g0, start, key, n, mode,     \
+                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats,      \
+                     ls.remap)
+  if (multi && rods)
+    BMX_BUILD(true, true);
+  else if (multi)
+    BMX_BUILD(true, false);
+  else if (rods)
+    BMX_BUILD(false, true);
   else
-    hipLaunchKernelGGL(k_build_nbr<false>, dim3(grid_for(n, b)), dim3(b), 0, s,
-                       pos, col, boxes, g0, start, key, n, crit, nbr, ncnt,
-                       ncap, maxn, err, stats, total, ls.remap);
+    BMX_BUILD(false, false);
+#undef BMX_BUILD
+  return hipGetLastError();
+}
+
+int32_t row_blocks(int64_t n, int block) { return grid_for(n, block); }
+
+hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
+                                  int32_t *stats, unsigned long long *total,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_rowstats, dim3(1), dim3(1024), 0, s, bstats,
+                     nblk, stats, total);
   return hipGetLastError();
 }
 
@@ -844,9 +1095,8 @@ hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
                                const int32_t *nbr, const int32_t *ncnt,
                                int64_t ncap, int64_t n, const int32_t *col,
                                const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc,
-                               unsigned long long *vmax, int32_t *err,
-                               LaunchShape ls, hipStream_t s) {
+                               ForceParams fp, double *fc, double *vmax,
+                               int32_t *err, LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
   if (multi)
@@ -864,8 +1114,8 @@ hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *nbr, const int32_t *ncnt,
                                 int64_t ncap, int64_t n, const BoxGeom *boxes,
                                 BoxGeom g0, int multi, ForceParams fp,
-                                double *fc, unsigned long long *vmax,
-                                int32_t *err, LaunchShape ls, hipStream_t s) {
+                                double *fc, double *vmax, int32_t *err,
+                                LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block > 256 ? 256 : ls.block;
   if (multi)
@@ -893,9 +1143,16 @@ hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
   return hipGetLastError();
 }
 
-hipError_t launch_skin_check(unsigned long long *vmax, int32_t nslots,
-                             double limit, int32_t *err, hipStream_t s) {
-  hipLaunchKernelGGL(k_skin_check, dim3(1), dim3(64), 0, s, vmax, nslots,
+hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
+                             double limit, int clear,
+                             unsigned long long *slotmax, int32_t *err,
+                             hipStream_t s) {
+  int blocks = (stride + 255) / 256;
+  if (blocks > 64) blocks = 64;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_skin_partial, dim3(blocks), dim3(256), 0, s, vpart,
+                     nslots, stride, clear, slotmax);
+  hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(64), 0, s, slotmax, nslots,
                      limit, err);
   return hipGetLastError();
 }

@@ -703,9 +703,26 @@ void orc_pair_separation(const double *xi, const double *xj, const orc_geom *g,
   }
 }
 
-static int accept_pair(const double d[3], int mode, double radius) {
+/* Interaction reach of one particle: r_lm of a pair is reach_i + reach_j +
+ * W_A for equal cell types (K.H:23-40). */
+static double reach(const orc_particle *q) {
+  return q->idata[ORC_I_CELL_TYPE] == ORC_FUNGI ? 1.5 * q->rdata[ORC_R_CLENGTH]
+                                                : q->rdata[ORC_R_RADIUS];
+}
+
+/* Pair-cutoff mode: extra = {W_A, skin}; accept if r^2 <= (reach_i +
+ * reach_j + W_A + skin)^2, the pairs that can reach r_lm before the skin is
+ * used up. */
+static double g_pair_wa = 0.0;
+
+static int accept_pair3(const orc_particle *pi, const orc_particle *pj,
+                        const double d[3], int mode, double radius) {
   double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   if (mode == ORC_NBR_STENCIL) return r2 <= radius;
+  if (mode == ORC_NBR_PAIR) {
+    double cut = reach(pi) + reach(pj) + g_pair_wa + radius;
+    return r2 <= cut * cut;
+  }
   return r2 <= radius * radius;
 }
 
@@ -717,6 +734,7 @@ static int check_geom(const orc_geom *g, int mode, double radius) {
     if (mode == ORC_NBR_RADIUS &&
         (g->phi[d] - g->plo[d]) / (double)g->nbin[d] < radius)
       return -2;
+    if (mode == ORC_NBR_PAIR && g->periodic[d] && g->nbin[d] < 3) return -2;
   }
   return 0;
 }
@@ -747,7 +765,7 @@ static int64_t stencil_row(const orc_particle *p, int64_t i, const orc_geom *g,
           if (j == i) continue;
           double d[3];
           orc_pair_separation(p[i].pos, p[j].pos, g, d);
-          if (!accept_pair(d, mode, radius)) continue;
+          if (!accept_pair3(&p[i], &p[j], d, mode, radius)) continue;
           if (out) out[cnt] = j;
           cnt++;
         }
@@ -756,6 +774,8 @@ static int64_t stencil_row(const orc_particle *p, int64_t i, const orc_geom *g,
   }
   return cnt;
 }
+
+void orc_set_pair_wa(double wa) { g_pair_wa = wa; }
 
 int orc_build_neighbors(const orc_particle *p, int64_t n, const orc_geom *g,
                         int mode, double radius, int64_t *offsets,
@@ -811,7 +831,7 @@ int orc_build_neighbors_brute(const orc_particle *p, int64_t n,
       if (j == i) continue;
       double d[3];
       orc_pair_separation(p[i].pos, p[j].pos, g, d);
-      if (!accept_pair(d, mode, radius)) continue;
+      if (!accept_pair3(&p[i], &p[j], d, mode, radius)) continue;
       if (used == cap) {
         cap *= 2;
         int32_t *nw = realloc(out, sizeof(int32_t) * cap);
@@ -877,6 +897,7 @@ void orc_state_destroy(orc_state *s) {
 static int rebuild(orc_state *s) {
   free(s->idx);
   s->idx = NULL;
+  g_pair_wa = s->fpar[ORC_F_WA];
   int rc = orc_build_neighbors(s->p, s->n, &s->g, s->mode, s->radius, s->off,
                                &s->idx);
   if (rc == 0) s->last_build = s->step;

@@ -111,13 +111,19 @@ int64_t orc_bin_index(const double *x, const orc_geom *g, int32_t b3[3]);
 
 /* Neighbor candidate predicate:
  *   ORC_NBR_RADIUS  : |pos_i - image(pos_j)|^2 <= radius^2
+ *   ORC_NBR_PAIR    : per-pair cutoff, see below
  *   ORC_NBR_STENCIL : |pos_i - image(pos_j)|^2 <= radius  (BMXCheckPair,
  *                     src/des/bmx_pc.H:63-81: squared distance vs the
  *                     'neighborhood' value)
  * Candidates are drawn from the +-1 bin stencil.  Rows are in canonical
  * order: stencil offsets (dz, dy, dx) lexicographic in -1..1, then ascending
  * particle index within a bin. */
-enum { ORC_NBR_RADIUS = 0, ORC_NBR_STENCIL = 1 };
+enum { ORC_NBR_RADIUS = 0, ORC_NBR_STENCIL = 1, ORC_NBR_PAIR = 2 };
+/* ORC_NBR_PAIR: |d|^2 <= (reach_i + reach_j + W_A + radius)^2 with reach =
+ * radius (spheres) or 1.5 c_length (segments), 'radius' the skin and W_A
+ * set by orc_set_pair_wa (the driver sets it from fpar).  The bins must be
+ * no smaller than the largest such cutoff. */
+void orc_set_pair_wa(double wa);
 
 /* returns 0, -1 on allocation failure, -2 on bad geometry */
 int orc_build_neighbors(const orc_particle *p, int64_t n, const orc_geom *g,

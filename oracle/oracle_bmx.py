@@ -24,7 +24,7 @@ RECORD_DTYPE = np.dtype(
         "itemsize": 448,
     }
 )
-NBR_RADIUS, NBR_STENCIL = 0, 1
+NBR_RADIUS, NBR_STENCIL, NBR_PAIR = 0, 1, 2
 
 
 class Geom(C.Structure):
@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
             "orc_state_get_neighbors": (None, [vp, P(I64), P(I32)]),
             "orc_state_step": (I64, [vp]),
             "orc_num_threads": (C.c_int, []),
+            "orc_set_pair_wa": (None, [C.c_double]),
             "orc_set_num_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sigs.items():
@@ -183,9 +184,11 @@ def pair_separation(xi, xj, geom: Geom):
     return d
 
 
-def build_neighbors(recs, geom: Geom, mode: int, radius: float, brute=False):
-    """CSR (offsets, indices) over particle indices."""
+def build_neighbors(recs, geom: Geom, mode: int, radius: float, brute=False, W_A=0.0):
+    """CSR (offsets, indices) over particle indices.  For NBR_PAIR, radius is
+    the skin and W_A the adhesion band."""
     recs = np.ascontiguousarray(recs, dtype=RECORD_DTYPE)
+    lib().orc_set_pair_wa(W_A)
     n = len(recs)
     off = np.zeros(n + 1, np.int64)
     ptr = C.POINTER(C.c_int32)()

@@ -15,11 +15,6 @@ from boltzmannmfx_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def verlet_radius(col):
-    rmax = col.recs["rdata"][:, 0].max()
-    return 2.0 * rmax + col.fpar[0] + col.skin
-
-
 def gpu_ctx(bmx, col, mode=0, K=10, n_cell=None, neighborhood=1.0, maxn=0,
             block=256, remap=1):
     ctx = bmx.Context(0)
@@ -33,11 +28,14 @@ def gpu_ctx(bmx, col, mode=0, K=10, n_cell=None, neighborhood=1.0, maxn=0,
     return ctx, recs
 
 
-def oracle_state(oracle, col, nbin, mode=0, K=10, radius=None):
+def oracle_state(oracle, col, nbin, mode=None, K=10, radius=None):
+    """The oracle with the library's policy: mode None is the Verlet policy
+    (per-pair cutoff r_lm + skin, NBR_PAIR with radius = skin)."""
     recs = col.recs.copy()
     g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, nbin)
-    r = verlet_radius(col) if radius is None else radius
-    return oracle.State(recs, g, col.fpar, mode, r, K), recs
+    if mode is None:
+        mode, radius = oracle.NBR_PAIR, col.skin
+    return oracle.State(recs, g, col.fpar, mode, radius, K), recs
 
 
 def rows(off, idx):

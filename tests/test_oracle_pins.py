@@ -467,6 +467,17 @@ def test_cell_list_equals_brute_force(oracle, periodic):
     for i in range(len(recs)):
         assert sorted(idx[off[i]:off[i + 1]]) == list(idxb[offb[i]:offb[i + 1]])
     assert off[-1] > 1000
+    # per-pair cutoff (reach_i + reach_j + W_A + skin) against brute force
+    W_A, skin = 0.01, 0.02
+    off, idx = oracle.build_neighbors(recs, g, oracle.NBR_PAIR, skin, W_A=W_A)
+    offb, idxb = oracle.build_neighbors(recs, g, oracle.NBR_PAIR, skin, brute=True, W_A=W_A)
+    r = recs["rdata"][:, 0]
+    for i in range(len(recs)):
+        got = sorted(idx[off[i]:off[i + 1]])
+        assert got == list(idxb[offb[i]:offb[i + 1]])
+        for j in got[:3]:
+            d = oracle.pair_separation(recs["pos"][i], recs["pos"][j], g)
+            assert np.dot(d, d) <= (r[i] + r[j] + W_A + skin) ** 2
     # stencil mode: every pair in the +-1 stencil with r^2 <= neighborhood,
     # here a value beyond the stencil reach, i.e. the whole stencil
     nc = (6, 5, 4)
