@@ -81,6 +81,22 @@ def cpu_baseline(col, budget_s: float = 12.0):
     }
 
 
+def calibration_copy(nbytes: int) -> None:
+    """One device-to-device copy of nbytes (a plain copy kernel of known
+    traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +107,9 @@ def main() -> int:
     ap.add_argument("--remap", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (the "
+                         "reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -101,6 +120,9 @@ def main() -> int:
     import numpy as np
     import boltzmannmfx_amd as bmx
     from boltzmannmfx_amd import workloads as W
+
+    if args.calibrate_copy > 0:
+        calibration_copy(args.calibrate_copy << 20)
 
     col = W.c2_large(args.n)
     ctx = bmx.Context(0)

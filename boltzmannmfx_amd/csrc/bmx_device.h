@@ -8,6 +8,7 @@
 #pragma once
 
 #include "bmx_internal.h"
+#include "bmx_math.h"
 
 namespace bmx {
 
@@ -241,7 +242,7 @@ __device__ inline void rod_mobility(double area, double c_length, double visc,
   double d0 = 1.0 / (6.0 * kPi * arad * visc);
   double e0, e1;
   if (c_length > 2.0 * arad) {
-    double l3 = pow(c_length, 3.0);
+    double l3 = bmx_cube(c_length);
     e0 = (c_length * c_length - 2.0 * arad * arad / 3.0) /
          (4.0 * kPi * l3 * visc);
     e1 = (c_length * c_length + 2.0 * arad * arad / 3.0) /
@@ -524,7 +525,7 @@ __device__ inline void rod_rotate(double &theta, double &phi,
   double ct = om[2], st = om[1];
   double orot[3][3] = {{1.0, 0.0, 0.0}, {0.0, ct, -st}, {0.0, st, ct}};
   double dth = on * subdt;
-  double nd[3] = {cos(dth), sin(dth), 0.0};
+  double nd[3] = {bmx_cos(dth), bmx_sin(dth), 0.0};
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     double s = 0.0;
@@ -539,10 +540,10 @@ __device__ inline void rod_rotate(double &theta, double &phi,
     for (int b = 0; b < 3; b++) s += R[a][b] * om[b];
     nd[a] = s;
   }
-  theta = acos(nd[2]);
+  theta = bmx_acos(nd[2]);
   on = sqrt(nd[0] * nd[0] + nd[1] * nd[1]);
   if (on > 0.0) nd[0] /= on;
-  phi = acos(nd[0]);
+  phi = bmx_acos(nd[0]);
   if (nd[1] < 0.0) phi = 2.0 * kPi - phi;
 }
 

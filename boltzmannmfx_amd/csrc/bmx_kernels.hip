@@ -139,7 +139,7 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
   if (dy.rod) {
     double L = rd[rec::c_length], th = rd[rec::theta], ph = rd[rec::phi];
     dy.rod[t] = make_double4(L, th, ph, rd[rec::area]);
-    dy.trig[t] = make_double4(cos(th), sin(th), cos(ph), sin(ph));
+    dy.trig[t] = make_double4(bmx_cos(th), bmx_sin(th), bmx_cos(ph), bmx_sin(ph));
     st.bscale[t] = rd[rec::bond_scale];
     Topo tp;
     tp.n_bnds = id[rec::n_bnds];
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_force_general(
       double w[3] = {acc[3], acc[4], acc[5]};
       rod_rotate(th, ph, tr_i, rod_i.x, w, fp.dt);
       out.rod[t] = make_double4(rod_i.x, th, ph, rod_i.w);
-      out.trig[t] = make_double4(cos(th), sin(th), cos(ph), sin(ph));
+      out.trig[t] = make_double4(bmx_cos(th), bmx_sin(th), bmx_cos(ph), bmx_sin(ph));
     } else {
       out.rod[t] = rod_i;
       out.trig[t] = tr_i;

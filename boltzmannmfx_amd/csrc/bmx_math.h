@@ -1,0 +1,142 @@
+/*
+ * bmx_math.h -- sin, cos and acos built from IEEE +, -, *, /, sqrt and
+ * floor only, so that a host build and a gfx950 build of the same call give
+ * the same bits (with floating-point contraction off).  The segment force
+ * law turns orientations (theta, phi) into axes with sin/cos and the update
+ * turns axes back with acos (src/chemistry/bmx_cell_interaction_K.H:559-612,
+ * src/des/bmx_pc_interaction.cpp:337-429); nearly parallel segments make the
+ * closest-point split ill-conditioned, so a one-ulp difference in an axis can
+ * move a force by 1e-11 of its size.  Using one implementation on both sides
+ * keeps the segment path bit-reproducible.
+ *
+ * Method: Cody-Waite reduction by pi/2 (three-part constant, the first two
+ * parts 33 significant bits so k*C1 and k*C2 are exact for |k| < 2^20),
+ * Taylor polynomials of sin (to r^17) and cos (to r^18) on [-pi/4, pi/4] in
+ * Horner form; acos through the arcsine series (to y^61) on |y| <= 1/2 with
+ * acos(x) = 2 asin(sqrt((1-x)/2)) for x > 1/2 and pi - 2 asin(sqrt((1+x)/2))
+ * for x < -1/2.  Coefficients are the exact series terms 1/n! and
+ * (2n)! / (4^n (n!)^2 (2n+1)) rounded to double.  Valid for |x| < 1e6;
+ * agreement with glibc is checked by the tests to a few ulp.
+ */
+#ifndef BMX_MATH_H
+#define BMX_MATH_H
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define BMX_MATH_FN __host__ __device__ static inline
+#else
+#define BMX_MATH_FN static inline
+#endif
+
+#define BMX_PIO2_1 0x1.921fb544p+0
+#define BMX_PIO2_2 0x1.0b4611a6p-34
+#define BMX_PIO2_3 0x1.3198a2e037073p-69
+#define BMX_2_OVER_PI 0x1.45f306dc9c883p-1
+#define BMX_PIO2 0x1.921fb54442d18p+0
+#define BMX_PI 0x1.921fb54442d18p+1
+
+/* sin r for |r| <= pi/4 */
+BMX_MATH_FN double bmx_sin_kernel(double r) {
+  double z = r * r;
+  double p = 2.8114572543455206e-15;   /* 1/17! */
+  p = p * z - 7.647163731819816e-13;   /* 1/15! */
+  p = p * z + 1.6059043836821613e-10;  /* 1/13! */
+  p = p * z - 2.505210838544172e-08;   /* 1/11! */
+  p = p * z + 2.7557319223985893e-06;  /* 1/9! */
+  p = p * z - 0.0001984126984126984;   /* 1/7! */
+  p = p * z + 0.008333333333333333;    /* 1/5! */
+  p = p * z - 0.16666666666666666;     /* 1/3! */
+  return r + r * z * p;
+}
+
+/* cos r for |r| <= pi/4 */
+BMX_MATH_FN double bmx_cos_kernel(double r) {
+  double z = r * r;
+  double p = -1.5619206968586225e-16;  /* 1/18! */
+  p = p * z + 4.779477332387385e-14;   /* 1/16! */
+  p = p * z - 1.1470745597729725e-11;  /* 1/14! */
+  p = p * z + 2.08767569878681e-09;    /* 1/12! */
+  p = p * z - 2.755731922398589e-07;   /* 1/10! */
+  p = p * z + 2.48015873015873e-05;    /* 1/8! */
+  p = p * z - 0.001388888888888889;    /* 1/6! */
+  p = p * z + 0.041666666666666664;    /* 1/4! */
+  return (1.0 - 0.5 * z) + z * z * p;
+}
+
+/* x = k pi/2 + r, |r| <= pi/4 (plus rounding); returns r, *q = k mod 4 */
+BMX_MATH_FN double bmx_reduce_pio2(double x, int *q) {
+  double k = floor(x * BMX_2_OVER_PI + 0.5);
+  double r = ((x - k * BMX_PIO2_1) - k * BMX_PIO2_2) - k * BMX_PIO2_3;
+  *q = (int)((long long)k & 3);
+  return r;
+}
+
+BMX_MATH_FN double bmx_sin(double x) {
+  int q;
+  double r = bmx_reduce_pio2(x, &q);
+  switch (q) {
+    case 0: return bmx_sin_kernel(r);
+    case 1: return bmx_cos_kernel(r);
+    case 2: return -bmx_sin_kernel(r);
+    default: return -bmx_cos_kernel(r);
+  }
+}
+
+BMX_MATH_FN double bmx_cos(double x) {
+  int q;
+  double r = bmx_reduce_pio2(x, &q);
+  switch (q) {
+    case 0: return bmx_cos_kernel(r);
+    case 1: return -bmx_sin_kernel(r);
+    case 2: return -bmx_cos_kernel(r);
+    default: return bmx_sin_kernel(r);
+  }
+}
+
+/* asin y for |y| <= 1/2 */
+BMX_MATH_FN double bmx_asin_kernel(double y) {
+  double z = y * y;
+  double p = 0.0017680811205154183;
+  p = p * z + 0.0018622264064031275;
+  p = p * z + 0.0019650336162772837;
+  p = p * z + 0.0020776610325181676;
+  p = p * z + 0.0022014739737101384;
+  p = p * z + 0.002338091892111975;
+  p = p * z + 0.0024894486782468836;
+  p = p * z + 0.00265787063820729;
+  p = p * z + 0.002846178401108942;
+  p = p * z + 0.0030578216492580306;
+  p = p * z + 0.003297059503473485;
+  p = p * z + 0.0035692053938259347;
+  p = p * z + 0.003880964558837669;
+  p = p * z + 0.004240907093679363;
+  p = p * z + 0.004660143486915096;
+  p = p * z + 0.005153309682319905;
+  p = p * z + 0.005740037670841924;
+  p = p * z + 0.006447210311889649;
+  p = p * z + 0.0073125258735988454;
+  p = p * z + 0.008390335809616815;
+  p = p * z + 0.009761609529194078;
+  p = p * z + 0.011551800896139705;
+  p = p * z + 0.01396484375;
+  p = p * z + 0.017352764423076924;
+  p = p * z + 0.022372159090909092;
+  p = p * z + 0.030381944444444444;
+  p = p * z + 0.044642857142857144;
+  p = p * z + 0.075;
+  p = p * z + 0.16666666666666666;
+  return y + y * z * p;
+}
+
+BMX_MATH_FN double bmx_acos(double x) {
+  if (!(x >= -1.0 && x <= 1.0)) return (x - x) / (x - x); /* NaN, as acos */
+  if (x <= 0.5 && x >= -0.5) return BMX_PIO2 - bmx_asin_kernel(x);
+  if (x > 0.0) return 2.0 * bmx_asin_kernel(sqrt((1.0 - x) * 0.5));
+  return BMX_PI - 2.0 * bmx_asin_kernel(sqrt((1.0 + x) * 0.5));
+}
+
+/* c^3, the reference's pow(c_length, 3) (K.H:832-834) */
+BMX_MATH_FN double bmx_cube(double c) { return c * c * c; }
+
+#endif

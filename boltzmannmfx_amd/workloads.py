@@ -88,7 +88,7 @@ def c3_hyphae(n: int = 200_000, nroots: int = 2000, seed: int = 43343,
     hi = np.array([side, side, height])
     recs = generate_hyphae(n, nroots, L, r, 0.05, 30.0, 60.0, seed, lo, hi, 0.0)
     return Colony(recs, lo, hi, (1, 1, 0), hyphae_fpar(0.0), n_cell=(ncx, ncx, 8),
-                  skin=2.0e-4, meta={"n": n, "nroots": nroots, "seed": seed})
+                  skin=1.0e-3, meta={"n": n, "nroots": nroots, "seed": seed})
 
 
 def c4_ensemble(ncol: int = 256, seed: int = 2024, nmin: int = 500,

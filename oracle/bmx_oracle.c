@@ -11,6 +11,8 @@
  * is rounded exactly as the reference's CPU build rounds it.
  */
 #include "bmx_oracle.h"
+/* the same sin/cos/acos the HIP library uses (IEEE operations only) */
+#include "../boltzmannmfx_amd/csrc/bmx_math.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -177,9 +179,9 @@ static void seg_mobility(double area, double c_length, double visc,
     }
   if (c_length > 2.0 * arad) {
     d1[0][0] = (c_length * c_length - 2.0 * arad * arad / 3.0) /
-               (4.0 * ORC_PI * pow(c_length, 3) * visc);
+               (4.0 * ORC_PI * bmx_cube(c_length) * visc);
     d1[1][1] = (c_length * c_length + 2.0 * arad * arad / 3.0) /
-               (8.0 * ORC_PI * pow(c_length, 3) * visc);
+               (8.0 * ORC_PI * bmx_cube(c_length) * visc);
     d1[2][2] = d1[1][1];
   } else {
     d1[0][0] = (1.0 - 3.0 * c_length / (16.0 * arad)) /
@@ -355,7 +357,7 @@ void orc_evaluate_force(const double *r12, double *par1, double *par2,
   /* K.H:553-612: segment end points, j at the origin, i at r12 */
   double L1 = par1[ORC_R_CLENGTH], L2 = par2[ORC_R_CLENGTH];
   double th = par1[ORC_R_THETA], ph = par1[ORC_R_PHI];
-  double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+  double ct = bmx_cos(th), st = bmx_sin(th), cp = bmx_cos(ph), sp = bmx_sin(ph);
   double nx = st * cp, ny = st * sp, nz = ct;
   double xi1 = r12[0] - 0.5 * nx * L1, yi1 = r12[1] - 0.5 * ny * L1,
          zi1 = r12[2] - 0.5 * nz * L1;
@@ -365,10 +367,10 @@ void orc_evaluate_force(const double *r12, double *par1, double *par2,
   seg_rotation(ct, st, cp, sp, Ri);
   th = par2[ORC_R_THETA];
   ph = par2[ORC_R_PHI];
-  ct = cos(th);
-  st = sin(th);
-  cp = cos(ph);
-  sp = sin(ph);
+  ct = bmx_cos(th);
+  st = bmx_sin(th);
+  cp = bmx_cos(ph);
+  sp = bmx_sin(ph);
   nx = st * cp;
   ny = st * sp;
   nz = ct;
@@ -505,7 +507,7 @@ void orc_surface_force(const double *pos, double *par, int32_t *ipar,
     ipar[ORC_I_FIX_SITE] = 0;
     double L = par[ORC_R_CLENGTH];
     double th = par[ORC_R_THETA], ph = par[ORC_R_PHI];
-    double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+    double ct = bmx_cos(th), st = bmx_sin(th), cp = bmx_cos(ph), sp = bmx_sin(ph);
     double nx = st * cp, ny = st * sp, nz = ct;
     double x1 = pos[0] - 0.5 * nx * L, y1 = pos[1] - 0.5 * ny * L,
            z1 = pos[2] - 0.5 * nz * L;
@@ -546,7 +548,7 @@ int orc_fix_bonds(const double *r12, const double *ripar, int32_t *ipar,
   int jid = jpar[ORC_I_ID], jcpu = jpar[ORC_I_CPU];
   double L1 = ripar[ORC_R_CLENGTH], L2 = rjpar[ORC_R_CLENGTH];
   double th = ripar[ORC_R_THETA], ph = ripar[ORC_R_PHI];
-  double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+  double ct = bmx_cos(th), st = bmx_sin(th), cp = bmx_cos(ph), sp = bmx_sin(ph);
   double nx = st * cp, ny = st * sp, nz = ct;
   double xi1 = r12[0] - 0.5 * nx * L1, yi1 = r12[1] - 0.5 * ny * L1,
          zi1 = r12[2] - 0.5 * nz * L1;
@@ -554,10 +556,10 @@ int orc_fix_bonds(const double *r12, const double *ripar, int32_t *ipar,
          zi2 = r12[2] + 0.5 * nz * L1;
   th = rjpar[ORC_R_THETA];
   ph = rjpar[ORC_R_PHI];
-  ct = cos(th);
-  st = sin(th);
-  cp = cos(ph);
-  sp = sin(ph);
+  ct = bmx_cos(th);
+  st = bmx_sin(th);
+  cp = bmx_cos(ph);
+  sp = bmx_sin(ph);
   nx = st * cp;
   ny = st * sp;
   nz = ct;
@@ -609,7 +611,7 @@ void orc_update_particle(orc_particle *p, const double fc[6], double subdt) {
   ppos[2] += subdt * rd[ORC_R_VELZ];
   if (p->idata[ORC_I_CELL_TYPE] == ORC_FUNGI) {
     double theta = rd[ORC_R_THETA], phi = rd[ORC_R_PHI];
-    double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+    double cp = bmx_cos(phi), sp = bmx_sin(phi), ct = bmx_cos(theta), st = bmx_sin(theta);
     double R[3][3];
     seg_rotation(ct, st, cp, sp, R);
     double lm[3] = {rd[ORC_R_WX], rd[ORC_R_WY], rd[ORC_R_WZ]};
@@ -631,7 +633,7 @@ void orc_update_particle(orc_particle *p, const double fc[6], double subdt) {
     st = om[1];
     double orot[3][3] = {{1.0, 0.0, 0.0}, {0.0, ct, -st}, {0.0, st, ct}};
     double dtheta = on * subdt;
-    double ndir[3] = {cos(dtheta), sin(dtheta), 0.0};
+    double ndir[3] = {bmx_cos(dtheta), bmx_sin(dtheta), 0.0};
     for (int a = 0; a < 3; a++) {
       om[a] = 0.0;
       for (int b = 0; b < 3; b++) om[a] += orot[b][a] * ndir[b];
@@ -640,10 +642,10 @@ void orc_update_particle(orc_particle *p, const double fc[6], double subdt) {
       ndir[a] = 0.0;
       for (int b = 0; b < 3; b++) ndir[a] += R[a][b] * om[b];
     }
-    theta = acos(ndir[2]);
+    theta = bmx_acos(ndir[2]);
     on = sqrt(ndir[0] * ndir[0] + ndir[1] * ndir[1]);
     if (on > 0.0) ndir[0] /= on;
-    phi = acos(ndir[0]);
+    phi = bmx_acos(ndir[0]);
     if (ndir[1] < 0.0) phi = 2.0 * ORC_PI - phi;
     rd[ORC_R_THETA] = theta;
     rd[ORC_R_PHI] = phi;
@@ -1039,4 +1041,13 @@ void orc_set_num_threads(int n) {
 #else
   (void)n;
 #endif
+}
+
+/* The shared sin/cos/acos (bmx_math.h), exported for its accuracy test. */
+double orc_math(int which, double x) {
+  switch (which) {
+    case 0: return bmx_sin(x);
+    case 1: return bmx_cos(x);
+    default: return bmx_acos(x);
+  }
 }

@@ -154,6 +154,8 @@ int64_t orc_state_neighbor_count(const orc_state *s);
 void orc_state_get_neighbors(const orc_state *s, int64_t *offsets,
                              int32_t *indices);
 int64_t orc_state_step(const orc_state *s);
+/* 0: sin, 1: cos, 2: acos of the shared implementation (bmx_math.h) */
+double orc_math(int which, double x);
 /* OpenMP threads the force pass uses (1 without OpenMP) */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
             "orc_state_step": (I64, [vp]),
             "orc_num_threads": (C.c_int, []),
             "orc_set_pair_wa": (None, [C.c_double]),
+            "orc_math": (C.c_double, [C.c_int, C.c_double]),
             "orc_set_num_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sigs.items():

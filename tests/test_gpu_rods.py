@@ -1,0 +1,134 @@
+"""GPU parity of the segment (hyphae) path against the CPU oracle.
+
+Segment pairs go through sin/cos (orientations) and acos (the orientation
+update).  Both sides use the same IEEE-only implementation of those
+(bmx_math.h, within 2 ulp of glibc), so forces, orientations, trajectories
+and the integer state (bond topology, positions in the hyphae, the reset
+fields) must match bit for bit.  The 1e-12 / 1e-10 tolerance helpers below
+are the bar against a glibc build; the tests assert equality.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from boltzmannmfx_amd import workloads as W
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def small_hyphae(n=3000, nroots=30, seed=11):
+    return W.c3_hyphae(n=n, nroots=nroots, seed=seed)
+
+
+def make_pair(bmx, oracle, col, mode=0, K=10, hood=1.0, maxn=0):
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=mode, rebuild_every=K, skin=col.skin,
+                            neighborhood=hood, max_neighbors=maxn)
+    grecs = col.recs.copy()
+    ctx.bind(grecs)
+    nb = ctx.bin_dims() if mode == 0 else col.n_cell
+    orecs = col.recs.copy()
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, nb)
+    if mode == 0:
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_PAIR, col.skin, K)
+    else:
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, hood, 1)
+    return ctx, grecs, st, orecs
+
+
+def close_fc(fg, fo, rel=1e-12):
+    assert np.array_equal(fg, fo)
+    for blk in (slice(0, 3), slice(3, 6)):
+        scale = np.abs(fo[blk]).max()
+        err = np.abs(fg[blk] - fo[blk]).max()
+        assert err <= rel * scale + 1e-300, (err, scale)
+
+
+def close_traj(ctx, orecs, box, rel=1e-10):
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    assert np.array_equal(ctx.orientations(), orecs["rdata"][:, 2:4])
+    err = np.abs(ctx.positions() - orecs["pos"]).max()
+    assert err <= rel * box, err
+    ori = ctx.orientations()
+    oo = orecs["rdata"][:, 2:4]
+    dphi = np.abs(ori - oo)
+    dphi[:, 1] = np.minimum(dphi[:, 1], 2 * math.pi - dphi[:, 1])
+    assert dphi.max() <= rel * 2 * math.pi, dphi.max()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_hyphae_init_bonds_and_steps(bmx, oracle, gpu, mode):
+    col = small_hyphae()
+    hood = 1.1 * (2 * 5.47e-4 + 1e-2)  # fungi deck: 1.1 (2 r(max_vol) + width)
+    ctx, grecs, st, orecs = make_pair(bmx, oracle, col, mode=mode,
+                                      K=10 if mode == 0 else 1, hood=hood, maxn=256)
+    ctx.init_bonds()
+    assert st.init_bonds() == 0
+    gi = ctx.int_fields()
+    assert np.array_equal(gi[:, 6:19], orecs["idata"][:, 6:19])   # bond slots
+    nb = orecs["idata"][:, 6]
+    # chains are bonded end to end; the reference's mesh stencil (2e-3 cm
+    # cells, 3e-3 cm segments) misses bonded pairs two cells apart
+    assert nb.sum() > (1.5 if mode == 0 else 1.0) * len(orecs)
+    box = (col.prob_hi - col.prob_lo).max()
+    ctx.evolve(col.dt, 1)
+    fo = st.evolve(col.dt, 1)
+    close_fc(ctx.fc(), fo)
+    assert np.array_equal(ctx.int_fields(), orecs["idata"])
+    for _ in range(3):
+        ctx.evolve(col.dt, 3)
+        fo = st.evolve(col.dt, 3)
+        close_fc(ctx.fc(), fo)
+    close_traj(ctx, orecs, box)
+    ctx.sync()
+    assert np.array_equal(grecs["idata"], orecs["idata"])
+    np.testing.assert_allclose(grecs["rdata"][:, 6:12], orecs["rdata"][:, 6:12],
+                               rtol=0, atol=1e-12 * np.abs(orecs["rdata"][:, 6:12]).max())
+
+
+def test_mixed_spheres_and_segments(bmx, oracle, gpu):
+    """Spheres and segments never interact (r_lm = 0, K.H:39) but share the
+    lists, the wall and the update."""
+    hy = small_hyphae(n=1500, nroots=15, seed=3)
+    L = hy.prob_hi - hy.prob_lo
+    sp, _, _ = bmx.generate_spheres(1500, 1.5e-4, 2.5e-4, 0.3, 5, aspect=(L[0], L[1], 4e-3),
+                                    periodic=(1, 1, 0), overlap_free=False)
+    sp["pos"][:, :2] *= L[:2] / sp["pos"][:, :2].max(axis=0)
+    sp["pos"][:, 2] = np.clip(sp["pos"][:, 2], 0, L[2] * 0.9)
+    both = np.concatenate([hy.recs, sp]).astype(bmx.RECORD_DTYPE)
+    col = W.Colony(both, hy.prob_lo, hy.prob_hi, hy.periodic,
+                   hy.fpar, n_cell=hy.n_cell, skin=hy.skin)
+    col.recs["id"] = np.arange(1, len(col.recs) + 1)
+    col.recs["idata"][:, 33] = col.recs["id"]
+    # overlapping random spheres move fast: rebuild every step
+    ctx, grecs, st, orecs = make_pair(bmx, oracle, col, K=1, maxn=256)
+    ctx.init_bonds()
+    st.init_bonds()
+    for _ in range(4):
+        ctx.evolve(col.dt, 3)
+        fo = st.evolve(col.dt, 3)
+        close_fc(ctx.fc(), fo)
+    close_traj(ctx, orecs, L.max())
+    assert np.array_equal(ctx.int_fields(), orecs["idata"])
+
+
+@pytest.mark.parametrize("fname", ["fungi_init_cfg.dat", "wall_test_fungi_init_cfg.dat"])
+def test_reference_single_segment_decks(bmx, oracle, gpu, fname):
+    """The reference's own fungi and wall_test inputs: one segment settling on
+    the growth surface (fluid.surface_location = 0.048, fungi deck forces)."""
+    recs = bmx.read_particles_ascii(os.path.join(GOLDEN, "ref_inputs", fname))
+    lo, hi = np.zeros(3), np.array([0.128, 0.128, 0.064])
+    col = W.Colony(recs, lo, hi, (1, 1, 0), W.hyphae_fpar(z_wall=0.048, k_bond=1.0),
+                   n_cell=(64, 64, 32), skin=2e-4)
+    ctx, _, st, orecs = make_pair(bmx, oracle, col)
+    for _ in range(5):
+        ctx.evolve(0.25, 20)
+        fo = st.evolve(0.25, 20)
+        close_fc(ctx.fc(), fo)
+    close_traj(ctx, orecs, 0.128)
+    assert orecs["pos"][0, 2] != recs["pos"][0, 2]   # it moved

@@ -574,3 +574,31 @@ def test_verlet_and_stencil_lists_agree(oracle):
         scale = np.abs(fb).max()
         np.testing.assert_allclose(fa, fb, rtol=0, atol=1e-12 * scale)
     np.testing.assert_allclose(recs["pos"], recs2["pos"], rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------ shared trig --
+def ulp_diff(a, b):
+    ia = np.frombuffer(np.float64(a).tobytes(), np.int64)[0]
+    ib = np.frombuffer(np.float64(b).tobytes(), np.int64)[0]
+    if ia < 0:
+        ia = -(ia & 0x7fffffffffffffff)
+    if ib < 0:
+        ib = -(ib & 0x7fffffffffffffff)
+    return abs(int(ia) - int(ib))
+
+
+def test_shared_trig_close_to_libm(oracle):
+    """sin, cos and acos of bmx_math.h (used by both the oracle and the
+    kernels) stay within 2 ulp of the C library over the angles the
+    segment path sees."""
+    rng = np.random.default_rng(10)
+    L = oracle.lib()
+    worst = [0, 0, 0]
+    for x in np.concatenate([rng.uniform(-4, 12, 20000),
+                             np.arange(0, 8) * math.pi / 4]):
+        worst[0] = max(worst[0], ulp_diff(L.orc_math(0, x), math.sin(x)))
+        worst[1] = max(worst[1], ulp_diff(L.orc_math(1, x), math.cos(x)))
+    for y in np.concatenate([rng.uniform(-1, 1, 20000), [-1.0, -0.5, 0.0, 0.5, 1.0]]):
+        worst[2] = max(worst[2], ulp_diff(L.orc_math(2, y), math.acos(y)))
+    assert worst[0] <= 2 and worst[1] <= 2 and worst[2] <= 2, worst
+    assert math.isnan(L.orc_math(2, 1.0 + 1e-15))
