@@ -71,6 +71,16 @@ def build(arch: str = "gfx950", verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB_PATH)
+    # deck driver linked against the library (found next to it at run time)
+    exe = os.path.join(HERE, "bmx_run")
+    cmd = [cc, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}",
+           os.path.join(CSRC, "bmx_run.cpp"), "-o", exe + ".tmp", f"-L{HERE}",
+           "-lbmx_mi355x", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("bmx_run link failed")
+    os.replace(exe + ".tmp", exe)
     return LIB_PATH
 
 

@@ -1,0 +1,194 @@
+// bmx_container.cpp -- BMXParticleContainer over the C ABI, and the deck
+// reader.
+#include "bmx_container.h"
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace bmx {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+std::string trim(const std::string &s) {
+  size_t a = s.find_first_not_of(" \t\r\n");
+  if (a == std::string::npos) return "";
+  size_t b = s.find_last_not_of(" \t\r\n");
+  return s.substr(a, b - a + 1);
+}
+}  // namespace
+
+Deck read_deck(const std::string &path) {
+  std::ifstream ifs(path);
+  if (!ifs.good()) throw Error(BMX_ERR_IO, "cannot open deck " + path);
+  Deck d;
+  std::string line;
+  while (std::getline(ifs, line)) {
+    size_t h = line.find('#');
+    if (h != std::string::npos) line = line.substr(0, h);
+    size_t eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    std::string key = trim(line.substr(0, eq));
+    std::string rest = line.substr(eq + 1);
+    for (char &c : rest)
+      if (c == '"') c = ' ';
+    std::istringstream vs(rest);
+    std::vector<std::string> vals;
+    std::string v;
+    while (vs >> v) vals.push_back(v);
+    if (!key.empty()) d.kv[key] = vals;  // later entries override, as ParmParse::query
+  }
+  return d;
+}
+
+double Deck::real(const std::string &k, int i) const {
+  auto it = kv.find(k);
+  if (it == kv.end() || (int)it->second.size() <= i)
+    throw Error(BMX_ERR_BAD_ARG, "deck key missing: " + k);
+  return std::stod(it->second[i]);
+}
+double Deck::real_or(const std::string &k, double dflt, int i) const {
+  return has(k) ? real(k, i) : dflt;
+}
+int Deck::integer(const std::string &k, int i) const {
+  auto it = kv.find(k);
+  if (it == kv.end() || (int)it->second.size() <= i)
+    throw Error(BMX_ERR_BAD_ARG, "deck key missing: " + k);
+  return std::stoi(it->second[i]);
+}
+int Deck::integer_or(const std::string &k, int dflt, int i) const {
+  return has(k) ? integer(k, i) : dflt;
+}
+std::string Deck::str(const std::string &k) const {
+  auto it = kv.find(k);
+  if (it == kv.end() || it->second.empty())
+    throw Error(BMX_ERR_BAD_ARG, "deck key missing: " + k);
+  return it->second[0];
+}
+
+bmx_geometry Deck::finest_geometry() const {
+  bmx_geometry g;
+  int lev = integer_or("amr.max_level", 0);
+  int ratio = integer_or("amr.ref_ratio", 2);
+  for (int d = 0; d < 3; d++) {
+    g.prob_lo[d] = real("geometry.prob_lo", d);
+    g.prob_hi[d] = real("geometry.prob_hi", d);
+    g.periodic[d] = integer_or("geometry.is_periodic", 0, d);
+    int nc = integer("amr.n_cell", d);
+    for (int l = 0; l < lev; l++) nc *= ratio;
+    g.n_cell[d] = nc;
+  }
+  return g;
+}
+
+// BMXCellInteraction::setParams (src/chemistry/bmx_cell_interaction.cpp:54-93)
+void Deck::force_params(double fpar[9]) const {
+  fpar[0] = real("cell_force.boundary_width");
+  fpar[1] = real("cell_force.stiffness");
+  fpar[2] = real_or("fluid.surface_location", 0.0);
+  fpar[3] = real("cell_force.wall_boundary_width");
+  fpar[4] = real("cell_force.wall_stiffness");
+  fpar[5] = real_or("cell_force.gravity", 0.0);
+  fpar[6] = real("cell_force.bond_strength");
+  fpar[7] = real("cell_force.bond_cutoff");
+  fpar[8] = real("cell_force.viscous_drag");
+}
+
+// DEM::neighborhood = 1.1 (2 r(max_vol) + neighbor_width)
+// (src/chemistry/bmx_chem.cpp:114-124)
+double Deck::neighborhood() const {
+  double vol = real_or("chem_species.max_vol", 0.0);
+  double radius = std::pow(3.0 * vol / (4.0 * kPi), 1.0 / 3.0);
+  return 1.1 * (2.0 * radius + real("cell_force.neighbor_width"));
+}
+
+BMXParticleContainer::BMXParticleContainer(const bmx_geometry &geom,
+                                           const double fpar[9],
+                                           const bmx_neighbor_params &nbp,
+                                           int device) {
+  check(bmx_ctx_create(&ctx_, device));
+  check(bmx_set_force_params(ctx_, fpar));
+  check(bmx_set_geometry(ctx_, &geom));
+  check(bmx_set_neighbor_params(ctx_, &nbp));
+}
+
+static bmx_neighbor_params stencil_params(const Deck &deck) {
+  bmx_neighbor_params p;
+  p.mode = BMX_NBR_STENCIL;
+  p.rebuild_every = 1;
+  p.max_neighbors = 256;
+  p.bin_factor_pct = 100;
+  p.skin = 0.0;
+  p.neighborhood = deck.neighborhood();
+  return p;
+}
+
+static std::vector<double> fpar_of(const Deck &deck) {
+  std::vector<double> f(9);
+  deck.force_params(f.data());
+  return f;
+}
+
+BMXParticleContainer::BMXParticleContainer(const Deck &deck, int device)
+    : BMXParticleContainer(deck.finest_geometry(), fpar_of(deck).data(),
+                           stencil_params(deck), device) {}
+
+BMXParticleContainer::~BMXParticleContainer() {
+  if (ctx_) bmx_ctx_destroy(ctx_);
+}
+
+void BMXParticleContainer::check(bmx_status st) const {
+  if (st != BMX_OK) throw Error(st, bmx_last_error());
+}
+
+void BMXParticleContainer::InitParticlesAscii(const std::string &file) {
+  int64_t n = 0;
+  bmx_layout L = bmx_default_layout();
+  check(bmx_read_particles_ascii(file.c_str(), nullptr, 0, &n, &L));
+  std::vector<uint8_t> recs((size_t)n * L.stride_bytes);
+  check(bmx_read_particles_ascii(file.c_str(), recs.data(), n, &n, &L));
+  InitParticles(recs.data(), n);
+}
+
+void BMXParticleContainer::InitParticles(const void *records, int64_t n) {
+  bmx_layout L = bmx_default_layout();
+  host_.assign(static_cast<const uint8_t *>(records),
+               static_cast<const uint8_t *>(records) + (size_t)n * L.stride_bytes);
+  n_ = n;
+  check(bmx_bind_particles_aos(ctx_, host_.data(), n, &L, 0));
+  dirty_ = false;
+}
+
+void BMXParticleContainer::InitBonds() {
+  check(bmx_init_bonds(ctx_));
+  dirty_ = true;
+}
+
+void BMXParticleContainer::EvolveParticles(double dt, int &nsubsteps) {
+  nsubsteps = 1;
+  check(bmx_evolve(ctx_, dt, 1));
+  dirty_ = true;
+}
+
+void BMXParticleContainer::EvolveSteps(double dt, int nsteps) {
+  check(bmx_evolve(ctx_, dt, nsteps));
+  dirty_ = true;
+}
+
+const std::vector<uint8_t> &BMXParticleContainer::Records() {
+  if (dirty_) {
+    check(bmx_sync_to_aos(ctx_, host_.data()));
+    dirty_ = false;
+  }
+  return host_;
+}
+
+void BMXParticleContainer::WriteAsciiFileForInit(const std::string &file) {
+  const std::vector<uint8_t> &r = Records();
+  bmx_layout L = bmx_default_layout();
+  check(bmx_write_particles_ascii(file.c_str(), r.data(), n_, &L));
+}
+
+}  // namespace bmx

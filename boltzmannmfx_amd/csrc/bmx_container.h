@@ -1,0 +1,92 @@
+// bmx_container.h -- C++ host layer with the reference's particle-container
+// entry points, over the C ABI of include/bmx_mi355x.h.
+//
+//   BMXParticleContainer::InitParticlesAscii   src/des/bmx_pc_init.cpp:17-132
+//   BMXParticleContainer::InitBonds            src/des/bmx_pc_interaction.cpp:500-668
+//   BMXParticleContainer::EvolveParticles      src/des/bmx_pc_interaction.cpp:13-495
+//   BMXParticleContainer::WriteAsciiFileForInit src/des/bmx_pc.H:154
+//   Deck / read_deck                           the ParmParse keys the path reads
+//                                              (src/chemistry/bmx_cell_interaction.cpp:54-93,
+//                                               src/chemistry/bmx_chem.cpp:114-124,
+//                                               src/setup/bmx_init.cpp:57-79)
+//
+// The load-balancing cost arguments of EvolveParticles/InitBonds
+// (src/des/bmx_pc_interaction.cpp:460-475) have no counterpart: one context
+// holds one level on one device.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bmx_mi355x.h"
+
+namespace bmx {
+
+class Error : public std::runtime_error {
+ public:
+  Error(bmx_status code, const std::string &msg)
+      : std::runtime_error(msg), code_(code) {}
+  bmx_status code() const { return code_; }
+
+ private:
+  bmx_status code_;
+};
+
+// The input-deck values the particle path uses.
+struct Deck {
+  std::map<std::string, std::vector<std::string>> kv;  // every key = value
+  bool has(const std::string &k) const { return kv.count(k) != 0; }
+  double real(const std::string &k, int i = 0) const;
+  double real_or(const std::string &k, double dflt, int i = 0) const;
+  int integer(const std::string &k, int i = 0) const;
+  int integer_or(const std::string &k, int dflt, int i = 0) const;
+  std::string str(const std::string &k) const;
+
+  // derived
+  bmx_geometry finest_geometry() const;  // level mesh of amr.max_level
+  void force_params(double fpar[9]) const;
+  double neighborhood() const;           // DEM::neighborhood
+};
+
+// Parses "key = v1 v2 ..." lines; '#' starts a comment; quotes are dropped.
+Deck read_deck(const std::string &path);
+
+class BMXParticleContainer {
+ public:
+  BMXParticleContainer(const bmx_geometry &geom, const double fpar[9],
+                       const bmx_neighbor_params &nbp, int device = 0);
+  // Container configured from a deck: the level mesh of amr.max_level, the
+  // cell_force parameters and the reference's per-step stencil list.
+  explicit BMXParticleContainer(const Deck &deck, int device = 0);
+  ~BMXParticleContainer();
+  BMXParticleContainer(const BMXParticleContainer &) = delete;
+  BMXParticleContainer &operator=(const BMXParticleContainer &) = delete;
+
+  void InitParticlesAscii(const std::string &file);
+  // Records supplied by the caller (host memory), e.g. a restart.
+  void InitParticles(const void *records, int64_t n);
+  void InitBonds();
+  // One step of size dt; nsubsteps is set to 1 as in the reference
+  // (dtsolid = dt, src/des/bmx_pc_interaction.cpp:56-65).
+  void EvolveParticles(double dt, int &nsubsteps);
+  // n steps without copying back in between.
+  void EvolveSteps(double dt, int nsteps);
+  void WriteAsciiFileForInit(const std::string &file);
+
+  int64_t NumberOfParticles() const { return n_; }
+  // Host copy of the records, brought up to date.
+  const std::vector<uint8_t> &Records();
+  bmx_ctx *context() { return ctx_; }
+
+ private:
+  void check(bmx_status st) const;
+  bmx_ctx *ctx_ = nullptr;
+  std::vector<uint8_t> host_;
+  int64_t n_ = 0;
+  bool dirty_ = false;
+};
+
+}  // namespace bmx
