@@ -106,10 +106,9 @@ class Timing(C.Structure):
     ]
 
 
-class AdjointParams(C.Structure):
-    _fields_ = [
-        ("checkpoint_every", C.c_int32), ("pad", C.c_int32),
-    ]
+class AdjointTiming(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double),
+                ("segments", C.c_int32), ("pad", C.c_int32)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -160,6 +159,9 @@ def load_library() -> C.CDLL:
         "bmx_get_timing": (C.c_int, [vp, P(Timing)]),
         "bmx_reset_timing": (C.c_int, [vp]),
         "bmx_synchronize": (C.c_int, [vp]),
+        "bmx_adjoint": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int32, P(C.c_double),
+                                  P(C.c_double), P(C.c_double), P(C.c_double),
+                                  P(AdjointTiming)]),
         "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
                                                P(C.c_int64), P(Layout)]),
         "bmx_write_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout)]),
@@ -287,6 +289,22 @@ class Context:
 
     def evolve_graph(self, dt: float, nsteps: int) -> None:
         _check(self.lib.bmx_evolve_graph(self.h, float(dt), int(nsteps)))
+
+    def adjoint(self, dt: float, nsteps: int, target: np.ndarray,
+                checkpoint_every: int = 20):
+        """(loss, [dL/dmu, dL/dW_A, dL/dgamma], dL/dradius, timing) of nsteps
+        sphere steps from the current state; the context ends in the final
+        state.  target: (n, 3) positions in original order."""
+        tg = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        assert tg.size == 3 * self.n
+        loss = C.c_double()
+        g3 = np.zeros(3)
+        gr = np.zeros(max(self.n, 1))
+        tm = AdjointTiming()
+        _check(self.lib.bmx_adjoint(self.h, float(dt), int(nsteps), int(checkpoint_every),
+                                    _dptr(tg), C.byref(loss), _dptr(g3), _dptr(gr),
+                                    C.byref(tm)))
+        return loss.value, g3, gr[: self.n], tm
 
     # inspection
     def fc(self) -> np.ndarray:

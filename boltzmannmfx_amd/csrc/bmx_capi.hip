@@ -399,6 +399,23 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   return BMX_OK;
 }
 
+// One force launch without bookkeeping of rebuilds (the adjoint drives its
+// own rebuild schedule).  slot >= 0 records the step's displacement
+// partials for the skin check in that slot.
+bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
+  ForceParams fp = force_params(c, dt);
+  double *vm = (slot >= 0 && verlet_checked(c))
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
+                   : nullptr;
+  HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_nbr.p,
+                             c->d_ncnt.p, c->ncap, c->n, nullptr, c->d_boxes.p,
+                             c->boxes[0], 0, fp, nullptr, vm, c->d_err.p,
+                             shape(c), c->stream));
+  c->cd = 1 - c->cd;
+  c->step++;
+  return BMX_OK;
+}
+
 // Reads the error word and list statistics; clears the error word.
 bmx_status check_flags(bmx_ctx *c) {
   hipStream_t s = c->stream;
@@ -900,6 +917,180 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     c->step++;
   }
   c->wrote_step = true;
+  return check_flags(c);
+}
+
+
+bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
+                       int32_t ckpt_every, const double *target, double *loss,
+                       double grad3[3], double *grad_r,
+                       bmx_adjoint_timing *timing) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (c->any_fungi || c->ncol != 1)
+    return fail(BMX_ERR_BAD_ARG, "the adjoint covers spheres in one box");
+  const int K = c->nbp.rebuild_every;
+  if (nsteps <= 0 || ckpt_every <= 0 || ckpt_every % K != 0 || !target ||
+      !loss || !grad3 || !grad_r || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG,
+                "bad adjoint arguments (checkpoint_every must be a multiple "
+                "of rebuild_every)");
+  (void)hipSetDevice(c->device);
+  reset_graph(c);
+  hipStream_t s = c->stream;
+  const int64_t n = c->n, cap = c->ncap;
+  const int C = ckpt_every;
+  const int S = (nsteps + C - 1) / C;
+  const int R = (C + K - 1) / K;  // rebuilds per segment
+  const size_t rowbuf = (size_t)c->maxn * (size_t)cap;
+  DevBuf<double> d_target, d_gr, d_acc, d_lpart, d_out;
+  DevBuf<double4> d_ckpt, d_hpos, d_lam0, d_lam1, d_lamo, d_final;
+  DevBuf<int32_t> d_hnbr, d_hcnt, d_hoid;
+  HIPCHK(d_target.alloc(3 * (size_t)std::max<int64_t>(n, 1)));
+  HIPCHK(d_gr.alloc((size_t)std::max<int64_t>(n, 1)));
+  HIPCHK(d_ckpt.alloc((size_t)S * cap));
+  HIPCHK(d_hpos.alloc((size_t)C * cap));
+  HIPCHK(d_hnbr.alloc((size_t)R * rowbuf));
+  HIPCHK(d_hcnt.alloc((size_t)R * cap));
+  HIPCHK(d_hoid.alloc((size_t)R * cap));
+  HIPCHK(d_lam0.alloc(cap));
+  HIPCHK(d_lam1.alloc(cap));
+  HIPCHK(d_lamo.alloc(cap));
+  HIPCHK(d_final.alloc(cap));
+  const int32_t nblk = adj_blocks(n);
+  HIPCHK(d_acc.alloc(3 * (size_t)nblk));
+  HIPCHK(d_lpart.alloc((size_t)nblk));
+  HIPCHK(d_out.alloc(4));
+  if (n)
+    HIPCHK(hipMemcpyAsync(d_target.p, target, sizeof(double) * 3 * n,
+                          hipMemcpyHostToDevice, s));
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_gr.p), n, 0ull, s));
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc.p),
+                         3 * (int64_t)nblk, 0ull, s));
+  hipEvent_t e0, e1, e2;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventCreate(&e2));
+  HIPCHK(hipEventRecord(e0, s));
+  const int64_t step0 = c->step;
+  const ForceParams fp = force_params(c, dt);
+  // ---- forward with checkpoints (rebuilds at every multiple of K)
+  // The skin check runs on this pass only: the recomputed segments repeat
+  // its arithmetic exactly.
+  c->have_list = false;
+  for (int t = 0; t < nsteps; t++) {
+    if (t % C == 0)
+      HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n,
+                             d_ckpt.p + (size_t)(t / C) * cap, s));
+    if (t % K == 0) {
+      if (t > 0 && verlet_checked(c))
+        HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                                 0.5 * c->nbp.skin, 1, c->d_slotmax.p,
+                                 c->d_err.p, s));
+      st = rebuild(c);
+      if (st != BMX_OK) return st;
+    }
+    st = enqueue_force_only(c, dt, t % K);
+    if (st != BMX_OK) return st;
+  }
+  if (verlet_checked(c))
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
+                             s));
+  HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n, d_final.p, s));
+  HIPCHK(launch_adj_loss(c->pos[c->cd].p, c->oid[c->cs].p, d_target.p, n,
+                         c->boxes[0], d_lam0.p, d_lpart.p, s));
+  HIPCHK(launch_to_orig4(d_lam0.p, c->oid[c->cs].p, n, d_lamo.p, s));
+  HIPCHK(hipEventRecord(e1, s));
+  {
+    // a stale list (skin exceeded) or a non-finite force ends the call here
+    int32_t ferr = 0;
+    HIPCHK(hipMemcpyAsync(&ferr, c->d_err.p, sizeof ferr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ferr) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
+      c->have_list = false;
+      return check_flags(c);
+    }
+  }
+  // ---- reverse, one segment at a time
+  double4 *lam_in = d_lam0.p, *lam_out = d_lam1.p;
+  for (int sg = S - 1; sg >= 0; sg--) {
+    const int t0 = sg * C;
+    const int len = std::min(C, nsteps - t0);
+    HIPCHK(launch_from_orig4(d_ckpt.p + (size_t)sg * cap, c->oid[c->cs].p, n,
+                             c->pos[c->cd].p, s));
+    c->have_list = false;
+    for (int t = 0; t < len; t++) {
+      if (t % K == 0) {
+        st = rebuild(c);
+        if (st != BMX_OK) return st;
+        const int r = t / K;
+        HIPCHK(hipMemcpyAsync(d_hnbr.p + (size_t)r * rowbuf, c->d_nbr.p,
+                              sizeof(int32_t) * rowbuf, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_hcnt.p + (size_t)r * cap, c->d_ncnt.p,
+                              sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_hoid.p + (size_t)r * cap, c->oid[c->cs].p,
+                              sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
+      }
+      HIPCHK(hipMemcpyAsync(d_hpos.p + (size_t)t * cap, c->pos[c->cd].p,
+                            sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
+      if (t + 1 < len) {  // the last state of the segment is not needed
+        st = enqueue_force_only(c, dt, -1);
+        if (st != BMX_OK) return st;
+      }
+    }
+    // lambda of the segment's last step, in that step's order
+    int r = (len - 1) / K;
+    HIPCHK(launch_from_orig4(d_lamo.p, d_hoid.p + (size_t)r * cap, n, lam_in, s));
+    for (int t = len - 1; t >= 0; t--) {
+      r = t / K;
+      const int32_t *oid = d_hoid.p + (size_t)r * cap;
+      HIPCHK(launch_adj_step(d_hpos.p + (size_t)t * cap, d_hnbr.p + (size_t)r * rowbuf,
+                             d_hcnt.p + (size_t)r * cap, cap, oid, n, c->boxes[0],
+                             fp, lam_in, lam_out, d_gr.p, d_acc.p, s));
+      std::swap(lam_in, lam_out);
+      if (t % K == 0 && t > 0) {  // back across a rebuild: reorder lambda
+        HIPCHK(launch_to_orig4(lam_in, oid, n, d_lamo.p, s));
+        HIPCHK(launch_from_orig4(d_lamo.p, d_hoid.p + (size_t)(r - 1) * cap, n,
+                                 lam_in, s));
+      }
+    }
+    HIPCHK(launch_to_orig4(lam_in, d_hoid.p, n, d_lamo.p, s));
+  }
+  HIPCHK(launch_sum_partials(d_acc.p, nblk, 3, d_out.p, s));
+  HIPCHK(launch_sum_partials(d_lpart.p, nblk, 1, d_out.p + 3, s));
+  HIPCHK(hipEventRecord(e2, s));
+  // leave the context in the final forward state
+  HIPCHK(launch_from_orig4(d_final.p, c->oid[c->cs].p, n, c->pos[c->cd].p, s));
+  c->have_list = false;
+  c->step = step0 + nsteps;
+  double out[4];
+  HIPCHK(hipMemcpyAsync(out, d_out.p, sizeof out, hipMemcpyDeviceToHost, s));
+  if (n)
+    HIPCHK(hipMemcpyAsync(grad_r, d_gr.p, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float fms = 0.f, bms = 0.f;
+  HIPCHK(hipEventElapsedTime(&fms, e0, e1));
+  HIPCHK(hipEventElapsedTime(&bms, e1, e2));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  const double m = c->mobility;
+  grad3[0] = out[0];
+  grad3[1] = out[1];
+  grad3[2] = -m * m * out[2];
+  *loss = out[3] / (double)std::max<int64_t>(n, 1);
+  if (timing) {
+    timing->forward_ms = fms;
+    timing->backward_ms = bms;
+    timing->segments = S;
+    timing->pad = 0;
+  }
+  c->wrote_step = false;
   return check_flags(c);
 }
 

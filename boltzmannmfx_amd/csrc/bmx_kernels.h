@@ -111,4 +111,22 @@ hipError_t launch_export_counts(const int32_t *ncnt, const int32_t *oid,
                                 int64_t n, int64_t *out_counts,
                                 hipStream_t s);
 
+// ---- adjoint of the sphere step (bmx_adjoint.hip)
+int32_t adj_blocks(int64_t n);
+hipError_t launch_adj_loss(const double4 *pos, const int32_t *oid,
+                           const double *target, int64_t n, BoxGeom g,
+                           double4 *lam, double *lpart, hipStream_t s);
+hipError_t launch_adj_step(const double4 *pos, const int32_t *nbr,
+                           const int32_t *ncnt, int64_t ncap,
+                           const int32_t *oid, int64_t n, BoxGeom g,
+                           ForceParams fp, const double4 *lam_in,
+                           double4 *lam_out, double *grad_r, double *acc,
+                           hipStream_t s);
+hipError_t launch_to_orig4(const double4 *in, const int32_t *oid, int64_t n,
+                           double4 *out, hipStream_t s);
+hipError_t launch_from_orig4(const double4 *in, const int32_t *oid, int64_t n,
+                             double4 *out, hipStream_t s);
+hipError_t launch_sum_partials(const double *part, int64_t nblk, int stride,
+                               double *out, hipStream_t s);
+
 }  // namespace bmx

@@ -161,6 +161,26 @@ bmx_status bmx_evolve(bmx_ctx *ctx, double dt, int32_t nsteps);
  * for this (dt, nsteps) and re-used while the configuration is unchanged). */
 bmx_status bmx_evolve_graph(bmx_ctx *ctx, double dt, int32_t nsteps);
 
+/* --- gradients (no counterpart in the reference) ------------------------ */
+typedef struct {
+  double forward_ms;   /* forward steps incl. checkpoints and the loss */
+  double backward_ms;  /* recompute of each segment + reverse steps */
+  int32_t segments;
+  int32_t pad;
+} bmx_adjoint_timing;
+/* Discrete adjoint of nsteps sphere steps (spheres only, one box):
+ * L = (1/N) sum_i |minimum image(x_T,i - target_i)|^2, target given as n*3
+ * doubles in original order.  Positions are checkpointed every
+ * checkpoint_every steps (a multiple of rebuild_every) and each segment is
+ * recomputed before its reverse pass.  Outputs: *loss, grad3 = {dL/dmu
+ * (stiffness), dL/dW_A (adhesion band), dL/dgamma with mobility 1/gamma} and
+ * grad_radius (n doubles, original order).  The context is left in the
+ * final forward state.  timing may be NULL. */
+bmx_status bmx_adjoint(bmx_ctx *ctx, double dt, int32_t nsteps,
+                       int32_t checkpoint_every, const double *target_xyz,
+                       double *loss, double grad3[3], double *grad_radius,
+                       bmx_adjoint_timing *timing);
+
 /* --- inspection (original particle order) ------------------------------ */
 /* fc of the last step, 6*n doubles, component-major: the reference's
  * fc[i + c*ntot] (src/des/bmx_pc_interaction.cpp:244-249). */

@@ -863,6 +863,7 @@ struct orc_state {
   int64_t *off;
   int32_t *idx;
   orc_particle *snap;
+  double mobility;
 };
 
 orc_state *orc_state_create(orc_particle *p, int64_t n, const orc_geom *g,
@@ -878,6 +879,7 @@ orc_state *orc_state_create(orc_particle *p, int64_t n, const orc_geom *g,
   s->radius = nbr_radius;
   s->rebuild_every = rebuild_every < 1 ? 1 : rebuild_every;
   s->last_build = -1;
+  s->mobility = 1.0;
   s->off = malloc(sizeof(int64_t) * (size_t)(n + 1));
   s->snap = malloc(sizeof(orc_particle) * (size_t)(n > 0 ? n : 1));
   if (!s->off || !s->snap) {
@@ -977,6 +979,9 @@ int orc_state_evolve(orc_state *s, double dt, int nsteps, double *fc_out) {
     for (int64_t i = 0; i < n; i++) {
       double f6[6];
       for (int c = 0; c < 6; c++) f6[c] = fc[(int64_t)c * n + i];
+      /* overdamped mobility m: vel = m fc (the reference has m = 1, for
+       * which m * fc == fc exactly) */
+      for (int c = 0; c < 3; c++) f6[c] = s->mobility * f6[c];
       orc_update_particle(&s->p[i], f6, dt);
     }
     orc_wrap(s->p, n, &s->g);
@@ -1050,4 +1055,161 @@ double orc_math(int which, double x) {
     case 1: return bmx_cos(x);
     default: return bmx_acos(x);
   }
+}
+
+void orc_state_set_mobility(orc_state *s, double m) { s->mobility = m; }
+
+/* ---- adjoint of the sphere step ------------------------------------------
+ * Forward map per step: x' = wrap(x + dt m F(x)), F_i = sum_j F(r_ij) u_ij +
+ * wall(z_i) e_z with F(r) = -mu (r - R_A)^2 (r - R_S), R_S = r_i + r_j,
+ * R_A = R_S + W_A (K.H:243-263, 1009-1030).  With A = dF/dd = (F/r) I +
+ * (F' - F/r) u u^T (symmetric, even in d) and symmetric interaction sets:
+ *   lambda_t,i = lambda_{t+1,i} + dt m [ sum_j A_ij (lambda_i - lambda_j)
+ *                                        + W'(z_i) lambda_i,z e_z ]
+ * Parameter terms per step: dL/dmu += dt m lambda_i . sum_j (F/mu) u_ij;
+ * dL/dW_A += dt m lambda_i . sum_j 2 mu (r - R_A)(r - R_S) u_ij;
+ * dL/dm += dt lambda_i . F_i; dL/dr_k += dt m [ sum_j dF/dR_S u_kj .
+ * (lambda_k - lambda_j) + lambda_k,z dW/dr_k ] with dF/dR_S = -F'(r) since
+ * F depends on r - R_S only.  dL/dgamma = -m^2 dL/dm for m = 1/gamma.
+ * Loss: (1/N) sum_i |minimum image(x_T,i - target_i)|^2. */
+static double img_axis(double d, const orc_geom *g, int a) {
+  if (g->periodic[a]) {
+    double L = g->phi[a] - g->plo[a];
+    if (d > 0.5 * L) d -= L;
+    else if (d < -0.5 * L) d += L;
+  }
+  return d;
+}
+
+int orc_adjoint(orc_particle *p, int64_t n, const orc_geom *g,
+                const double fpar[9], double mobility, double dt, int nsteps,
+                int rebuild_every, double skin, const double *target,
+                double *loss, double grad[3], double *grad_r) {
+  orc_state *s = orc_state_create(p, n, g, fpar, ORC_NBR_PAIR, skin,
+                                  rebuild_every);
+  if (!s) return -1;
+  s->mobility = mobility;
+  double *xs = malloc(sizeof(double) * 3 * (size_t)n * (size_t)(nsteps + 1));
+  int nb = (nsteps + s->rebuild_every - 1) / s->rebuild_every + 1;
+  int64_t **offs = calloc((size_t)nb, sizeof(int64_t *));
+  int32_t **idxs = calloc((size_t)nb, sizeof(int32_t *));
+  int *list_of_step = malloc(sizeof(int) * (size_t)(nsteps > 0 ? nsteps : 1));
+  double *lam = calloc(3 * (size_t)n, sizeof(double));
+  double *lam2 = calloc(3 * (size_t)n, sizeof(double));
+  if (!xs || !offs || !idxs || !list_of_step || !lam || !lam2) {
+    orc_state_destroy(s);
+    return -1;
+  }
+  int nl = -1;
+  for (int t = 0; t < nsteps; t++) {
+    for (int64_t i = 0; i < n; i++)
+      for (int a = 0; a < 3; a++)
+        xs[((size_t)t * n + i) * 3 + a] = p[i].pos[a];
+    int64_t before = s->last_build;
+    int rc = orc_state_evolve(s, dt, 1, NULL);
+    if (rc) { orc_state_destroy(s); return rc; }
+    if (s->last_build != before || nl < 0) {
+      nl++;
+      offs[nl] = malloc(sizeof(int64_t) * (size_t)(n + 1));
+      idxs[nl] = malloc(sizeof(int32_t) * (size_t)(s->off[n] > 0 ? s->off[n] : 1));
+      memcpy(offs[nl], s->off, sizeof(int64_t) * (size_t)(n + 1));
+      memcpy(idxs[nl], s->idx, sizeof(int32_t) * (size_t)s->off[n]);
+    }
+    list_of_step[t] = nl;
+  }
+  double L = 0.0;
+  for (int64_t i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) {
+      double d = img_axis(p[i].pos[a] - target[3 * i + a], g, a);
+      L += d * d;
+      lam[3 * i + a] = 2.0 / (double)n * d;
+    }
+  *loss = L / (double)n;
+  double gmu = 0.0, gwa = 0.0, gm = 0.0;
+  for (int64_t i = 0; i < n; i++) grad_r[i] = 0.0;
+  const double WA = fpar[ORC_F_WA], mu = fpar[ORC_F_MU], zw = fpar[ORC_F_ZWALL],
+               WZ = fpar[ORC_F_WZ], xi = fpar[ORC_F_XI], gr = fpar[ORC_F_GRAV];
+  const double m = mobility;
+  for (int t = nsteps - 1; t >= 0; t--) {
+    const double *x = xs + (size_t)t * n * 3;
+    const int64_t *off = offs[list_of_step[t]];
+    const int32_t *idx = idxs[list_of_step[t]];
+    for (int64_t i = 0; i < n; i++) {
+      const double *li = lam + 3 * i;
+      double ri = p[i].rdata[ORC_R_RADIUS];
+      double jl[3] = {0, 0, 0}, fsum[3] = {0, 0, 0};
+      double dmu = 0.0, dwa = 0.0, dr = 0.0;
+      for (int64_t k = off[i]; k < off[i + 1]; k++) {
+        int64_t j = idx[k];
+        double d[3];
+        orc_pair_separation(x + 3 * i, x + 3 * j, g, d);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double rj = p[j].rdata[ORC_R_RADIUS];
+        double rS = ri + rj, rA = rS + WA;
+        double lim = rS + WA - 1.0e-15;
+        if (!(r2 <= lim * lim)) continue;
+        double r = sqrt(r2);
+        if (!(r < rA)) continue;
+        double u[3] = {d[0] / r, d[1] / r, d[2] / r};
+        double F = -mu * (r - rA) * (r - rA) * (r - rS);
+        double Fp = -mu * (r - rA) * (3.0 * r - 2.0 * rS - rA);
+        const double *lj = lam + 3 * j;
+        double dl[3] = {li[0] - lj[0], li[1] - lj[1], li[2] - lj[2]};
+        double udl = u[0] * dl[0] + u[1] * dl[1] + u[2] * dl[2];
+        double uli = u[0] * li[0] + u[1] * li[1] + u[2] * li[2];
+        for (int a = 0; a < 3; a++) {
+          jl[a] += (F / r) * dl[a] + (Fp - F / r) * udl * u[a];
+          fsum[a] += F * u[a];
+        }
+        dmu += -(r - rA) * (r - rA) * (r - rS) * uli;
+        dwa += 2.0 * mu * (r - rA) * (r - rS) * uli;
+        dr += -Fp * udl;
+      }
+      /* wall (K.H:1019-1030) */
+      double dz = x[3 * i + 2] - zw;
+      double rAw = ri + WZ;
+      double w, wp, wr;
+      if (dz > 0.0 && dz < rAw) {
+        w = -xi * (dz - rAw) * (dz - rAw) * (dz - ri);
+        wp = -xi * (dz - rAw) * (3.0 * dz - 2.0 * ri - rAw);
+        wr = -wp;
+      } else if (dz <= 0.0) {
+        w = xi * rAw * rAw * ri;
+        wp = 0.0;
+        wr = xi * (2.0 * rAw * ri + rAw * rAw);
+      } else {
+        w = 0.0;
+        wp = 0.0;
+        wr = 0.0;
+      }
+      w -= gr * xi * rAw * rAw * ri;
+      wr -= gr * xi * (2.0 * rAw * ri + rAw * rAw);
+      fsum[2] += w;
+      lam2[3 * i + 0] = li[0] + dt * m * jl[0];
+      lam2[3 * i + 1] = li[1] + dt * m * jl[1];
+      lam2[3 * i + 2] = li[2] + dt * m * (jl[2] + wp * li[2]);
+      gmu += dt * m * dmu;
+      gwa += dt * m * dwa;
+      gm += dt * (li[0] * fsum[0] + li[1] * fsum[1] + li[2] * fsum[2]);
+      grad_r[i] += dt * m * (dr + wr * li[2]);
+    }
+    double *tmp = lam;
+    lam = lam2;
+    lam2 = tmp;
+  }
+  grad[0] = gmu;
+  grad[1] = gwa;
+  grad[2] = -m * m * gm;
+  for (int k = 0; k <= nl; k++) {
+    free(offs[k]);
+    free(idxs[k]);
+  }
+  free(offs);
+  free(idxs);
+  free(list_of_step);
+  free(xs);
+  free(lam);
+  free(lam2);
+  orc_state_destroy(s);
+  return 0;
 }

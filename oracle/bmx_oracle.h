@@ -154,6 +154,16 @@ int64_t orc_state_neighbor_count(const orc_state *s);
 void orc_state_get_neighbors(const orc_state *s, int64_t *offsets,
                              int32_t *indices);
 int64_t orc_state_step(const orc_state *s);
+/* overdamped mobility m (vel = m fc); 1 by default, as the reference */
+void orc_state_set_mobility(orc_state *s, double m);
+/* Gradient of L = (1/N) sum |min image(x_T - target)|^2 after nsteps sphere
+ * steps (Verlet pair lists, rebuild_every, skin) with respect to mu, W_A and
+ * gamma = 1/m (grad[0..2]) and every radius (grad_r), by the discrete
+ * adjoint; p is advanced to x_T.  All states are kept (no recompute). */
+int orc_adjoint(orc_particle *p, int64_t n, const orc_geom *g,
+                const double fpar[9], double mobility, double dt, int nsteps,
+                int rebuild_every, double skin, const double *target,
+                double *loss, double grad[3], double *grad_r);
 /* 0: sin, 1: cos, 2: acos of the shared implementation (bmx_math.h) */
 double orc_math(int which, double x);
 /* OpenMP threads the force pass uses (1 without OpenMP) */

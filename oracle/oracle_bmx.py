@@ -85,6 +85,9 @@ def lib() -> C.CDLL:
             "orc_num_threads": (C.c_int, []),
             "orc_set_pair_wa": (None, [C.c_double]),
             "orc_math": (C.c_double, [C.c_int, C.c_double]),
+            "orc_state_set_mobility": (None, [vp, C.c_double]),
+            "orc_adjoint": (C.c_int, [vp, I64, P(Geom), D, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.c_double, D, D, D, D]),
             "orc_set_num_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sigs.items():
@@ -235,6 +238,9 @@ class State:
             raise RuntimeError(f"orc_state_evolve failed: {rc}")
         return fc.reshape(6, self.n)
 
+    def set_mobility(self, m: float) -> None:
+        lib().orc_state_set_mobility(self.h, m)
+
     def init_bonds(self) -> int:
         return lib().orc_state_init_bonds(self.h)
 
@@ -244,3 +250,21 @@ class State:
         idx = np.zeros(max(tot, 1), np.int32)
         lib().orc_state_get_neighbors(self.h, _i64(off), _i(idx))
         return off, idx[:tot]
+
+
+def adjoint(recs, geom: Geom, fpar, mobility, dt, nsteps, rebuild_every, skin, target):
+    """(loss, [dL/dmu, dL/dW_A, dL/dgamma], dL/dr) of the sphere steps; recs
+    is advanced to the final state."""
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    n = len(recs)
+    fp = np.ascontiguousarray(fpar, np.float64)
+    tg = np.ascontiguousarray(target, np.float64).reshape(-1)
+    loss = np.zeros(1)
+    grad = np.zeros(3)
+    gr = np.zeros(n)
+    rc = lib().orc_adjoint(recs.ctypes.data_as(C.c_void_p), n, C.byref(geom), _d(fp),
+                           mobility, dt, nsteps, rebuild_every, skin, _d(tg), _d(loss),
+                           _d(grad), _d(gr))
+    if rc != 0:
+        raise RuntimeError(f"orc_adjoint failed: {rc}")
+    return loss[0], grad, gr

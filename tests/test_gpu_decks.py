@@ -45,18 +45,22 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
     assert res.stdout.count("Time per step") == steps
 
     kv = parse_deck(path)
-    f = lambda k, i=0: float(kv[k][i])
+    def f(k, i=0, default=None):
+        if k not in kv and default is not None:
+            return default
+        return float(kv[k][i])
     lev = int(kv.get("amr.max_level", ["0"])[0])
     n_cell = [int(kv["amr.n_cell"][d]) * 2 ** lev for d in range(3)]
     lo = [f("geometry.prob_lo", d) for d in range(3)]
     hi = [f("geometry.prob_hi", d) for d in range(3)]
     per = [int(kv["geometry.is_periodic"][d]) for d in range(3)]
+    # defaults of BMXCellInteraction::setParams (src/chemistry/bmx_cell_interaction.cpp:59-60)
     fpar = [f("cell_force.boundary_width"), f("cell_force.stiffness"),
-            f("fluid.surface_location"), f("cell_force.wall_boundary_width"),
-            f("cell_force.wall_stiffness"), f("cell_force.gravity"),
+            f("fluid.surface_location", default=0.0), f("cell_force.wall_boundary_width"),
+            f("cell_force.wall_stiffness"), f("cell_force.gravity", default=0.0),
             f("cell_force.bond_strength"), f("cell_force.bond_cutoff"),
             f("cell_force.viscous_drag")]
-    vol = f("chem_species.max_vol")
+    vol = f("chem_species.max_vol", default=0.0)
     hood = 1.1 * (2 * (3 * vol / (4 * math.pi)) ** (1 / 3) + f("cell_force.neighbor_width"))
     recs = bmx.read_particles_ascii(os.path.join(os.path.dirname(path),
                                                  kv["particles.input_file"][0]))

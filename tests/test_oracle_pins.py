@@ -602,3 +602,42 @@ def test_shared_trig_close_to_libm(oracle):
         worst[2] = max(worst[2], ulp_diff(L.orc_math(2, y), math.acos(y)))
     assert worst[0] <= 2 and worst[1] <= 2 and worst[2] <= 2, worst
     assert math.isnan(L.orc_math(2, 1.0 + 1e-15))
+
+
+# ---------------------------------------------------------------- adjoint --
+def test_adjoint_matches_central_differences(oracle, bmx):
+    """The oracle's discrete adjoint (the CPU reference gradient) against
+    central differences of its own forward map, h = 1e-6 relative, for the
+    stiffness, the adhesion band, the damping and a random radius direction."""
+    from boltzmannmfx_amd import workloads as W
+    col = W.sphere_colony(400, 0.3, 5, W_A=2e-5, overlap_free=False)
+    rng = np.random.default_rng(1)
+    target = col.recs["pos"] + rng.normal(size=(400, 3)) * 1e-6
+    fp = col.fpar.copy()
+    fp[1] = 1e8
+    R = 2 * col.recs["rdata"][:, 0].max() + fp[0] + col.skin
+    L = col.prob_hi - col.prob_lo
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
+                         [int(L[d] // R) for d in range(3)])
+    r0 = col.recs["rdata"][:, 0].copy()
+
+    def run(f, m, radii):
+        r = col.recs.copy()
+        r["rdata"][:, 0] = radii
+        return oracle.adjoint(r, g, f, m, 0.25, 40, 10, col.skin, target)
+
+    loss, grad, gr = run(fp, 1.0, r0)
+    for k, gi in ((1, 0), (0, 1)):
+        h = fp[k] * 1e-6
+        a, b = fp.copy(), fp.copy()
+        a[k] += h
+        b[k] -= h
+        fd = (run(a, 1.0, r0)[0] - run(b, 1.0, r0)[0]) / (2 * h)
+        assert grad[gi] == pytest.approx(fd, rel=1e-6)
+    h = 1e-6
+    fd = (run(fp, 1 / (1 + h), r0)[0] - run(fp, 1 / (1 - h), r0)[0]) / (2 * h)
+    assert grad[2] == pytest.approx(fd, rel=1e-6)
+    v = rng.normal(size=400)
+    h = 1e-6 * 5e-5
+    fd = (run(fp, 1.0, r0 + h * v)[0] - run(fp, 1.0, r0 - h * v)[0]) / (2 * h)
+    assert gr @ v == pytest.approx(fd, rel=1e-6)
