@@ -81,6 +81,107 @@ def cpu_baseline(col, budget_s: float = 12.0):
     }
 
 
+def ensemble_bench(bmx, W, steps: int, seq: bool = True):
+    """256 colonies (cell counts log-uniform in [500, 50000], volume fraction
+    uniform in [0.1, 0.35], each in its own periodic box) advanced `steps`
+    steps by one captured-graph launch after autotuning, against the same
+    colonies run one after another through the plain per-step path."""
+    import numpy as np
+    cols = W.c4_ensemble()
+    recs, starts, geoms = W.concat_colonies(cols)
+    n = len(recs)
+    K = 10
+    ctx = bmx.Context(0)
+    ctx.set_force_params(cols[0].fpar)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K, skin=cols[0].skin,
+                            max_neighbors=64)
+    ctx.bind_colonies(recs, starts, geoms)
+    t0 = time.perf_counter()
+    tune = ctx.autotune(W.DT, 2 * K)
+    tune["seconds"] = time.perf_counter() - t0
+    ctx.evolve(W.DT, K)                      # into the steady rebuild phase
+    t0 = time.perf_counter()
+    ctx.evolve_graph(W.DT, steps)            # capture + first replay
+    capture_s = time.perf_counter() - t0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.evolve_graph(W.DT, steps)            # one graph launch, timed
+    batched_s = time.perf_counter() - t0
+    ctx.close()
+    out = {
+        "workload": f"{len(cols)} sphere colonies, {n} spheres, sizes "
+                    f"{min(len(c.recs) for c in cols)}..{max(len(c.recs) for c in cols)}, "
+                    f"phi U[0.1,0.35], {steps} steps, Verlet list every {K} steps",
+        "colonies": len(cols), "particles": n, "steps": steps,
+        "batched_particle_steps_per_s": n * steps / batched_s,
+        "batched_ms_per_step": batched_s * 1e3 / steps,
+        "graph_launches_per_call": 1,
+        "capture_and_first_run_s": capture_s,
+        "tuning": tune,
+    }
+    if seq:
+        total = 0.0
+        for col in cols:
+            c = bmx.Context(0)
+            c.set_force_params(col.fpar)
+            c.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+            c.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K, skin=col.skin,
+                                  max_neighbors=64)
+            c.bind(col.recs.copy())
+            c.synchronize()
+            t0 = time.perf_counter()
+            c.evolve(W.DT, steps)
+            total += time.perf_counter() - t0
+            c.close()
+        out["sequential_particle_steps_per_s"] = n * steps / total
+        out["sequential_s"] = total
+        out["speedup_vs_sequential"] = total / batched_s
+    return out
+
+
+def adjoint_bench(bmx, W, n: int = 100_000, steps: int = 200):
+    """Checkpointed adjoint (checkpoints every 20 steps) of `steps` sphere
+    steps on n spheres at volume fraction 0.3, against the plain forward
+    evolve of the same steps."""
+    import numpy as np
+    col = W.sphere_colony(n, 0.3, 11, W_A=6e-6, overlap_free=False, skin=5e-5)
+    target = col.recs["pos"] + np.random.default_rng(8).normal(size=(n, 3)) * 1e-5
+
+    def ctx_for():
+        c = bmx.Context(0)
+        c.set_force_params(col.fpar)
+        c.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+        c.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+        c.bind(col.recs.copy())
+        return c
+
+    c = ctx_for()
+    c.adjoint(W.DT, 20, target)                          # warm up
+    c.close()
+    c = ctx_for()
+    c.synchronize()
+    t0 = time.perf_counter()
+    c.evolve(W.DT, steps)
+    fwd_s = time.perf_counter() - t0
+    c.close()
+    c = ctx_for()
+    c.synchronize()
+    t0 = time.perf_counter()
+    loss, g3, gr, tm = c.adjoint(W.DT, steps, target, checkpoint_every=20)
+    adj_s = time.perf_counter() - t0
+    c.close()
+    return {
+        "workload": f"{n} spheres, phi=0.3, {steps} steps, checkpoints every 20 steps, "
+                    f"loss = mean squared distance to a perturbed target",
+        "plain_forward_ms": fwd_s * 1e3,
+        "adjoint_forward_ms": tm.forward_ms,
+        "adjoint_backward_ms": tm.backward_ms,
+        "adjoint_call_ms": adj_s * 1e3,
+        "backward_over_forward": tm.backward_ms / (fwd_s * 1e3),
+        "adjoint_call_over_forward": adj_s / fwd_s,
+    }
+
+
 def calibration_copy(nbytes: int) -> None:
     """One device-to-device copy of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
@@ -107,6 +208,11 @@ def main() -> int:
     ap.add_argument("--remap", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the 256 one-colony runs the ensemble is compared with")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -194,12 +300,20 @@ def main() -> int:
             "algorithmic_bytes_per_launch": alg_bytes,
         },
     }
+    ctx.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(bmx, W, args.ensemble_steps,
+                                         seq=not args.no_sequential)
+        print(f"# ensemble done: {out['ensemble'].get('speedup_vs_sequential')}",
+              file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(bmx, W)
+        print("# adjoint done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(col, args.cpu_budget)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)
-    ctx.close()
     return 0
 
 

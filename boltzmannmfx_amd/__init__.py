@@ -111,6 +111,16 @@ class AdjointTiming(C.Structure):
                 ("segments", C.c_int32), ("pad", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("xcd_remap", C.c_int32),
+                ("bin_factor_pct", C.c_int32), ("from_cache", C.c_int32),
+                ("us_per_step", C.c_double), ("candidates", C.c_int32),
+                ("pad", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 _lib: Optional[C.CDLL] = None
 
 
@@ -162,6 +172,7 @@ def load_library() -> C.CDLL:
         "bmx_adjoint": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int32, P(C.c_double),
                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(AdjointTiming)]),
+        "bmx_autotune": (C.c_int, [vp, C.c_double, C.c_int32, C.c_char_p, P(Tuning)]),
         "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
                                                P(C.c_int64), P(Layout)]),
         "bmx_write_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout)]),
@@ -305,6 +316,15 @@ class Context:
                                     _dptr(tg), C.byref(loss), _dptr(g3), _dptr(gr),
                                     C.byref(tm)))
         return loss.value, g3, gr[: self.n], tm
+
+    def autotune(self, dt: float, trial_steps: int = 20,
+                 cache: Optional[str] = None) -> dict:
+        """Times the launch shapes and bin edges on the bound particles,
+        applies the fastest and returns it; the state is left unchanged."""
+        t = Tuning()
+        _check(self.lib.bmx_autotune(self.h, float(dt), int(trial_steps),
+                                     cache.encode() if cache else None, C.byref(t)))
+        return t.as_dict()
 
     # inspection
     def fc(self) -> np.ndarray:

@@ -1264,3 +1264,246 @@ bmx_status bmx_synchronize(bmx_ctx *c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+namespace {
+
+// Copy of the particle arrays a step reads, so trials can start from the
+// same state.
+struct StateSnap {
+  DevBuf<int32_t> oid, type, col, key;
+  DevBuf<double> bscale;
+  DevBuf<double4> pos, rod, trig;
+  DevBuf<Topo> topo;
+  int cs = 0, cd = 0;
+  int64_t step = 0;
+};
+
+template <typename T>
+hipError_t dcopy(T *dst, const T *src, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  return hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n,
+                        hipMemcpyDeviceToDevice, s);
+}
+
+bmx_status snap_take(bmx_ctx *c, StateSnap &z) {
+  const int64_t cap = c->ncap;
+  hipStream_t s = c->stream;
+  HIPCHK(z.oid.alloc(cap));
+  HIPCHK(z.type.alloc(cap));
+  HIPCHK(z.key.alloc(cap));
+  HIPCHK(z.bscale.alloc(cap));
+  HIPCHK(z.pos.alloc(cap));
+  HIPCHK(z.rod.alloc(cap));
+  HIPCHK(z.trig.alloc(cap));
+  HIPCHK(z.topo.alloc(cap));
+  HIPCHK(dcopy(z.oid.p, c->oid[c->cs].p, cap, s));
+  HIPCHK(dcopy(z.type.p, c->type[c->cs].p, cap, s));
+  HIPCHK(dcopy(z.key.p, c->key[c->cs].p, cap, s));
+  HIPCHK(dcopy(z.bscale.p, c->bscale[c->cs].p, cap, s));
+  HIPCHK(dcopy(z.pos.p, c->pos[c->cd].p, cap, s));
+  HIPCHK(dcopy(z.rod.p, c->rod[c->cd].p, cap, s));
+  HIPCHK(dcopy(z.trig.p, c->trig[c->cd].p, cap, s));
+  HIPCHK(dcopy(z.topo.p, c->topo[c->cd].p, cap, s));
+  if (c->ncol > 1) {
+    HIPCHK(z.col.alloc(cap));
+    HIPCHK(dcopy(z.col.p, c->col[c->cs].p, cap, s));
+  }
+  z.cs = c->cs;
+  z.cd = c->cd;
+  z.step = c->step;
+  return BMX_OK;
+}
+
+// Puts the saved state back; the next step rebuilds the lists.
+bmx_status snap_restore(bmx_ctx *c, const StateSnap &z) {
+  const int64_t cap = c->ncap;
+  hipStream_t s = c->stream;
+  HIPCHK(dcopy(c->oid[z.cs].p, z.oid.p, cap, s));
+  HIPCHK(dcopy(c->type[z.cs].p, z.type.p, cap, s));
+  HIPCHK(dcopy(c->key[z.cs].p, z.key.p, cap, s));
+  HIPCHK(dcopy(c->bscale[z.cs].p, z.bscale.p, cap, s));
+  HIPCHK(dcopy(c->pos[z.cd].p, z.pos.p, cap, s));
+  HIPCHK(dcopy(c->rod[z.cd].p, z.rod.p, cap, s));
+  HIPCHK(dcopy(c->trig[z.cd].p, z.trig.p, cap, s));
+  HIPCHK(dcopy(c->topo[z.cd].p, z.topo.p, cap, s));
+  if (c->ncol > 1) HIPCHK(dcopy(c->col[z.cs].p, z.col.p, cap, s));
+  c->cs = z.cs;
+  c->cd = z.cd;
+  c->step = z.step;
+  c->have_list = false;
+  c->last_build = -1;
+  c->wrote_step = false;
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
+                         (int64_t)c->vmax_slots * c->vstride, 0ull, s));
+  HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, s));
+  return BMX_OK;
+}
+
+std::string tune_key(const bmx_ctx *c) {
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+    arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+  }
+  double vol = 0.0;
+  for (const bmx_geometry &g : c->colony_geom)
+    vol += (g.prob_hi[0] - g.prob_lo[0]) * (g.prob_hi[1] - g.prob_lo[1]) *
+           (g.prob_hi[2] - g.prob_lo[2]);
+  // packing from the largest radius: enough to tell workloads apart
+  double phi = vol > 0.0 ? (double)c->n * 4.18879020478639 * c->rmax *
+                               c->rmax * c->rmax / vol
+                         : 0.0;
+  char buf[256];
+  std::snprintf(buf, sizeof buf,
+                "%s mode=%d K=%d ncol=%d n=%.2g phi=%.2g rods=%d", arch.c_str(),
+                c->nbp.mode, c->nbp.rebuild_every, c->ncol, (double)c->n, phi,
+                c->any_fungi ? 1 : 0);
+  return buf;
+}
+
+bool json_int(const std::string &line, const char *name, int32_t *v) {
+  std::string pat = std::string("\"") + name + "\":";
+  size_t at = line.find(pat);
+  return at != std::string::npos &&
+         std::sscanf(line.c_str() + at + pat.size(), " %d", v) == 1;
+}
+
+// Last cache line for key; false when none.
+bool read_cache(const char *path, const std::string &key, bmx_tuning *t) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  std::string want = "\"key\": \"" + key + "\"";
+  bool found = false;
+  char buf[1024];
+  while (std::fgets(buf, sizeof buf, f)) {
+    std::string line(buf);
+    if (line.find(want) == std::string::npos) continue;
+    bmx_tuning r{};
+    if (json_int(line, "block_size", &r.block_size) &&
+        json_int(line, "xcd_remap", &r.xcd_remap) &&
+        json_int(line, "bin_factor_pct", &r.bin_factor_pct)) {
+      *t = r;
+      found = true;
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+}  // namespace
+
+extern "C" bmx_status bmx_autotune(bmx_ctx *c, double dt, int32_t trial_steps,
+                                   const char *cache_path, bmx_tuning *out) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out || !std::isfinite(dt) || trial_steps < 0)
+    return fail(BMX_ERR_BAD_ARG, "bad autotune arguments");
+  (void)hipSetDevice(c->device);
+  std::memset(out, 0, sizeof *out);
+  const std::string key = tune_key(c);
+  bmx_neighbor_params nb = c->nbp;
+  bmx_launch_params lp = c->lp;
+  bmx_tuning hit{};
+  if (cache_path && read_cache(cache_path, key, &hit)) {
+    nb.bin_factor_pct = hit.bin_factor_pct;
+    lp.block_size = hit.block_size;
+    lp.xcd_remap = hit.xcd_remap;
+    st = bmx_set_neighbor_params(c, &nb);
+    if (st == BMX_OK) st = bmx_set_launch_params(c, &lp);
+    if (st != BMX_OK) return st;
+    *out = hit;
+    out->from_cache = 1;
+    return BMX_OK;
+  }
+  const bool was_timing = c->timing;
+  c->timing = false;
+  const int K = c->nbp.rebuild_every;
+  const int steps = std::max(K, ((trial_steps + K - 1) / K) * K);
+  StateSnap z;
+  st = snap_take(c, z);
+  if (st != BMX_OK) return st;
+  std::vector<int32_t> pcts;
+  if (c->nbp.mode == BMX_NBR_VERLET)
+    pcts = {100, 150, 200};
+  else
+    pcts = {c->nbp.bin_factor_pct};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  bmx_tuning best{};
+  best.us_per_step = 1.0e300;
+  bmx_status err = BMX_OK;
+  for (int32_t pct : pcts) {
+    nb.bin_factor_pct = pct;
+    st = bmx_set_neighbor_params(c, &nb);
+    if (st == BMX_ERR_BAD_ARG) continue;  // too few bins at this edge
+    if (st != BMX_OK) { err = st; break; }
+    for (int32_t block : {64, 128, 256, 512}) {
+      for (int32_t remap : {0, 1}) {
+        lp.block_size = block;
+        lp.xcd_remap = remap;
+        st = bmx_set_launch_params(c, &lp);
+        if (st == BMX_OK) st = snap_restore(c, z);
+        if (st == BMX_OK) st = bmx_evolve_graph(c, dt, steps);  // capture
+        float bestc = 1.0e30f;
+        for (int rep = 0; rep < 3 && st == BMX_OK; rep++) {
+          st = snap_restore(c, z);
+          if (st != BMX_OK) break;
+          if (hipEventRecord(e0, c->stream) != hipSuccess) { st = BMX_ERR_HIP; break; }
+          st = bmx_evolve_graph(c, dt, steps);
+          if (st != BMX_OK) break;
+          float ms = 0.f;
+          if (hipEventRecord(e1, c->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            st = fail(BMX_ERR_HIP, "autotune timing failed");
+            break;
+          }
+          bestc = std::min(bestc, ms);
+        }
+        if (st != BMX_OK) { err = st; break; }
+        out->candidates++;
+        double us = 1000.0 * bestc / steps;
+        if (us < best.us_per_step) {
+          best.us_per_step = us;
+          best.block_size = block;
+          best.xcd_remap = remap;
+          best.bin_factor_pct = pct;
+        }
+      }
+      if (err != BMX_OK) break;
+    }
+    if (err != BMX_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = was_timing;
+  if (err != BMX_OK) return err;
+  if (out->candidates == 0)
+    return fail(BMX_ERR_BAD_ARG, "no launchable configuration");
+  nb.bin_factor_pct = best.bin_factor_pct;
+  lp.block_size = best.block_size;
+  lp.xcd_remap = best.xcd_remap;
+  st = bmx_set_neighbor_params(c, &nb);
+  if (st == BMX_OK) st = bmx_set_launch_params(c, &lp);
+  if (st == BMX_OK) st = snap_restore(c, z);
+  if (st != BMX_OK) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int32_t ncand = out->candidates;
+  *out = best;
+  out->candidates = ncand;
+  out->from_cache = 0;
+  if (cache_path) {
+    FILE *f = std::fopen(cache_path, "a");
+    if (!f) return fail(BMX_ERR_IO, std::string("cannot append to ") + cache_path);
+    std::fprintf(f,
+                 "{\"key\": \"%s\", \"block_size\": %d, \"xcd_remap\": %d, "
+                 "\"bin_factor_pct\": %d, \"us_per_step\": %.3f}\n",
+                 key.c_str(), best.block_size, best.xcd_remap,
+                 best.bin_factor_pct, best.us_per_step);
+    std::fclose(f);
+  }
+  return BMX_OK;
+}

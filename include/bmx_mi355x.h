@@ -125,6 +125,28 @@ bmx_status bmx_set_geometry(bmx_ctx *ctx, const bmx_geometry *geom);
 bmx_status bmx_set_neighbor_params(bmx_ctx *ctx,
                                    const bmx_neighbor_params *p);
 bmx_status bmx_set_launch_params(bmx_ctx *ctx, const bmx_launch_params *p);
+
+/* Result of bmx_autotune. */
+typedef struct {
+  int32_t block_size;       /* chosen launch shape ... */
+  int32_t xcd_remap;
+  int32_t bin_factor_pct;   /* ... and Verlet bin edge, % of the radius */
+  int32_t from_cache;       /* 1: read from the cache file, no trials run */
+  double us_per_step;       /* best measured time per step (0 from cache) */
+  int32_t candidates;       /* configurations timed */
+  int32_t pad;
+} bmx_tuning;
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed by
+ * AMReX): times graph-replayed rebuild periods of the bound particles for
+ * every block size, XCD mapping and (Verlet policy) bin edge of 100, 150
+ * and 200% of the interaction radius, applies the fastest and returns it.
+ * The particle state is restored afterwards, so tuning does not advance the
+ * run; the next step rebuilds the lists.  When cache_path is not NULL the
+ * choice is looked up in / appended to that file (one JSON object per line,
+ * keyed by architecture, policy, rebuild interval, colony count, particle
+ * count and packing).  trial_steps is rounded up to whole rebuild periods. */
+bmx_status bmx_autotune(bmx_ctx *ctx, double dt, int32_t trial_steps,
+                        const char *cache_path, bmx_tuning *out);
 bmx_layout bmx_default_layout(void);
 
 /* --- particles --------------------------------------------------------- */
