@@ -119,6 +119,8 @@ struct bmx_ctx {
 
   // timing
   bool timing = false;
+  int32_t timing_every = 1;  // time one force launch in this many
+  int64_t timing_seq = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_force, ev_rebuild;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -182,7 +184,10 @@ ForceParams force_params(const bmx_ctx *c, double dt) {
 hipEvent_t next_event(bmx_ctx *c) {
   if (c->ev_used == c->ev_pool.size()) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing only: read after the stream is synchronized, so no system
+    // fence (cache write-back) is needed when the event is recorded
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+      return nullptr;
     c->ev_pool.push_back(e);
   }
   return c->ev_pool[c->ev_used++];
@@ -367,7 +372,8 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
                    ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
                    : nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->timing) {
+  const bool timed = c->timing && (c->timing_seq++ % c->timing_every) == 0;
+  if (timed) {
     e0 = next_event(c);
     e1 = next_event(c);
     if (!e0 || !e1) return fail(BMX_ERR_HIP, "event allocation failed");
@@ -389,7 +395,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
                                 c->boxes[0], multi, fp, fc, vm, c->d_err.p,
                                 shape(c), s));
   }
-  if (c->timing) {
+  if (timed) {
     HIPCHK(hipEventRecord(e1, s));
     c->ev_force.push_back({e0, e1});
   }
@@ -1233,7 +1239,10 @@ bmx_status bmx_get_bin_dims(bmx_ctx *c, int32_t colony, int32_t out3[3]) {
 
 bmx_status bmx_set_timing(bmx_ctx *c, int32_t enable) {
   if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  if (enable < 0) return fail(BMX_ERR_BAD_ARG, "negative timing period");
   c->timing = enable != 0;
+  c->timing_every = enable > 0 ? enable : 1;
+  c->timing_seq = 0;
   return BMX_OK;
 }
 

@@ -21,7 +21,7 @@ namespace bmx {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kUnroll = 4;  // neighbors / candidates in flight per thread
+constexpr int kUnroll = 1;  // candidates in flight per thread (list build)
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
   return (unsigned long long)__double_as_longlong(x);
@@ -391,25 +391,27 @@ __device__ __forceinline__ double reach_of(const double4 &p, int ty,
   return p.w;
 }
 
-// Scans one contiguous candidate range [ua, ub) of the sorted arrays.
-// IMG = false when no periodic image can be involved (the stencil does not
-// wrap), which gives the same separation bits as the image test would.
+// Scans one contiguous candidate range [ua, ub) of the sorted arrays and
+// appends the accepted candidates to the row (out walks the column-major
+// row, one stride of ncap per entry).  The candidate loop is branch-free
+// apart from the append: loads past the range end repeat the last entry and
+// are masked out with the self pair.  IMG = false when no periodic image can
+// be involved (the stencil does not wrap), which gives the same separation
+// bits as the image test would.  was = W_A + skin.
 template <bool RODS, bool IMG>
 __device__ __forceinline__ void scan_range(
-    int ua, int ub, int64_t t, const double4 &p, double reach_i,
+    int ua, int ub, int t, const double4 &p, double reach_i,
     const BoxGeom &g, const double4 *__restrict__ pos,
     const int32_t *__restrict__ type, const double4 *__restrict__ rod,
-    int mode, double crit, double wa, double skin, int32_t *__restrict__ nbr,
-    int64_t ncap, int32_t maxn, int &cnt) {
+    int mode, double crit, double was, int32_t *&out, int64_t ncap,
+    int32_t maxn, int &cnt) {
   for (int u0 = ua; u0 < ub; u0 += kUnroll) {
     double4 o[kUnroll];
 #pragma unroll
-    for (int j = 0; j < kUnroll; j++) o[j] = pos[(u0 + j < ub) ? u0 + j : u0];
+    for (int j = 0; j < kUnroll; j++) o[j] = pos[min(u0 + j, ub - 1)];
 #pragma unroll
     for (int j = 0; j < kUnroll; j++) {
-      int u = u0 + j;
-      if (u >= ub) break;
-      if (u == t) continue;
+      const int u = u0 + j;
       double d0, d1, d2;
       if (IMG) {
         d0 = sep_axis(p.x, o[j].x, g.L[0], g.halfL[0], g.periodic[0]);
@@ -425,12 +427,17 @@ __device__ __forceinline__ void scan_range(
       if (mode == 1) {
         ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
       } else {
-        double rj = RODS ? reach_of<true>(o[j], type[u], rod, u) : o[j].w;
-        double cut = reach_i + rj + wa + skin;
+        double rj = RODS ? reach_of<true>(o[j], type[min(u, ub - 1)], rod,
+                                          min(u, ub - 1))
+                         : o[j].w;
+        double cut = reach_i + rj + was;
         ok = r2 <= cut * cut;
       }
-      if (ok) {
-        if (cnt < maxn) nbr[(int64_t)cnt * ncap + t] = u;
+      if (ok && u < ub && u != t) {
+        if (cnt < maxn) {
+          *out = u;
+          out += ncap;
+        }
         cnt++;
       }
     }
@@ -453,7 +460,10 @@ __global__ __launch_bounds__(512) void k_build_nbr(
     int32_t *__restrict__ nbr, int32_t *__restrict__ ncnt, int64_t ncap,
     int32_t maxn, int32_t *__restrict__ err, int32_t *__restrict__ bstats,
     int remap) {
-  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  const int ti = (int)t;  // n < 2^31
+  const double was = wa + skin;
+  int32_t *out = nbr + t;
   int cnt = 0;
   // Candidates visited per particle are capped (a pile of particles in a few
   // bins would otherwise make this quadratic); hitting the cap is reported
@@ -530,13 +540,13 @@ __global__ __launch_bounds__(512) void k_build_nbr(
           }
           scanned += len;
           if (wave_noimg)
-            scan_range<RODS, false>(ranges[q][0], ranges[q][1], t, p, reach_i,
-                                    g, pos, type, rod, mode, crit, wa, skin,
-                                    nbr, ncap, maxn, cnt);
+            scan_range<RODS, false>(ranges[q][0], ranges[q][1], ti, p,
+                                    reach_i, g, pos, type, rod, mode, crit,
+                                    was, out, ncap, maxn, cnt);
           else
-            scan_range<RODS, true>(ranges[q][0], ranges[q][1], t, p, reach_i,
-                                   g, pos, type, rod, mode, crit, wa, skin,
-                                   nbr, ncap, maxn, cnt);
+            scan_range<RODS, true>(ranges[q][0], ranges[q][1], ti, p, reach_i,
+                                   g, pos, type, rod, mode, crit, was, out,
+                                   ncap, maxn, cnt);
         }
       }
     }
@@ -586,11 +596,43 @@ __device__ __forceinline__ void block_vmax(double v2, double dt, double *vpart) 
   if (threadIdx.x == 0) vpart[blockIdx.x] = dt * sqrt(m);
 }
 
-// Sphere row, two passes over up to 64 neighbors at a time: first the cheap
-// interaction test (K.H:23-40 with PCI:212) for each neighbor, recorded as a
-// bit mask; then the cubic law (K.H:243-270) only for the set bits, in row
-// order.  The second pass recomputes the separation with the same
-// operations, so the terms and their order are those of a single pass.
+// Sphere row in chunks of kRowChunk neighbors.  All indices of a chunk are
+// loaded, then all their coordinates, so a row of up to kRowChunk entries
+// waits for memory twice.  The cheap interaction test (K.H:23-40 with
+// PCI:212) marks the neighbors that interact; the cubic law (K.H:243-270)
+// then runs only for the marked ones, in row order, on the coordinates still
+// held in registers.  The separation is recomputed with the same operations,
+// so the terms and their order are those of a single pass over the row.
+constexpr int kRowChunk = 4;
+
+__device__ __forceinline__ double4 pick4(const double4 (&o)[kRowChunk], int k) {
+  double x = o[0].x, y = o[0].y, z = o[0].z, w = o[0].w;
+#pragma unroll
+  for (int j = 1; j < kRowChunk; j++) {
+    const bool h = k == j;
+    x = h ? o[j].x : x;
+    y = h ? o[j].y : y;
+    z = h ? o[j].z : z;
+    w = h ? o[j].w : w;
+  }
+  return make_double4(x, y, z, w);
+}
+
+template <bool IMG>
+__device__ __forceinline__ void sphere_sep(const double4 &p, const double4 &o,
+                                           const BoxGeom &g, double &d0,
+                                           double &d1, double &d2) {
+  if (IMG) {
+    d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+    d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+    d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+  } else {
+    d0 = p.x - o.x;
+    d1 = p.y - o.y;
+    d2 = p.z - o.z;
+  }
+}
+
 template <bool IMG>
 __device__ __forceinline__ void sphere_row(const double4 &p, int c,
                                            const int32_t *__restrict__ row,
@@ -599,51 +641,34 @@ __device__ __forceinline__ void sphere_row(const double4 &p, int c,
                                            const BoxGeom &g,
                                            const ForceParams &fp, double &a0,
                                            double &a1, double &a2) {
-  for (int base = 0; base < c; base += 64) {
-    const int m = (c - base) < 64 ? (c - base) : 64;
-    unsigned long long mask = 0ull;
-    for (int k = 0; k < m; k += kUnroll) {
-      int u[kUnroll];
-      double4 o[kUnroll];
+  const int32_t *rp = row;
+  for (int base = 0; base < c; base += kRowChunk) {
+    int u[kRowChunk];
+    double4 o[kRowChunk];
 #pragma unroll
-      for (int j = 0; j < kUnroll; j++)
-        u[j] = (k + j < m) ? row[(int64_t)(base + k + j) * ncap] : (int)t;
+    for (int j = 0; j < kRowChunk; j++) {
+      u[j] = (base + j < c) ? *rp : (int)t;
+      rp += ncap;
+    }
 #pragma unroll
-      for (int j = 0; j < kUnroll; j++) o[j] = pos_in[u[j]];
+    for (int j = 0; j < kRowChunk; j++) o[j] = pos_in[u[j]];
+    unsigned mask = 0u;
 #pragma unroll
-      for (int j = 0; j < kUnroll; j++) {
-        if (k + j >= m) break;
-        double d0, d1, d2;
-        if (IMG) {
-          d0 = sep_axis(p.x, o[j].x, g.L[0], g.halfL[0], g.periodic[0]);
-          d1 = sep_axis(p.y, o[j].y, g.L[1], g.halfL[1], g.periodic[1]);
-          d2 = sep_axis(p.z, o[j].z, g.L[2], g.halfL[2], g.periodic[2]);
-        } else {
-          d0 = p.x - o[j].x;
-          d1 = p.y - o[j].y;
-          d2 = p.z - o[j].z;
-        }
-        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        double lim = p.w + o[j].w + fp.WA - 1.0e-15;  // r_lm - small_number
-        if (r2 <= lim * lim) mask |= 1ull << (k + j);
-      }
+    for (int j = 0; j < kRowChunk; j++) {
+      double d0, d1, d2;
+      sphere_sep<IMG>(p, o[j], g, d0, d1, d2);
+      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      double lim = p.w + o[j].w + fp.WA - 1.0e-15;  // r_lm - small_number
+      if (base + j < c && r2 <= lim * lim) mask |= 1u << j;
     }
     while (mask) {
-      int k = __builtin_ctzll(mask);
-      mask &= mask - 1ull;
-      double4 o = pos_in[row[(int64_t)(base + k) * ncap]];
+      const int k = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const double4 q = pick4(o, k);
       double d0, d1, d2;
-      if (IMG) {
-        d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-        d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-        d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-      } else {
-        d0 = p.x - o.x;
-        d1 = p.y - o.y;
-        d2 = p.z - o.z;
-      }
+      sphere_sep<IMG>(p, q, g, d0, d1, d2);
       double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      sphere_pair(d0, d1, d2, r2, p.w + o.w, fp, a0, a1, a2);
+      sphere_pair(d0, d1, d2, r2, p.w + q.w, fp, a0, a1, a2);
     }
   }
 }

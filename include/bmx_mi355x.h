@@ -226,8 +226,10 @@ bmx_status bmx_get_bins(bmx_ctx *ctx, int32_t *out_n);
 bmx_status bmx_get_bin_dims(bmx_ctx *ctx, int32_t colony, int32_t out3[3]);
 
 /* --- measurement ------------------------------------------------------- */
-/* When enabled, every force and rebuild launch is bracketed by HIP events on
- * the context's stream. */
+/* enable = 0 turns timing off.  enable = k > 0 brackets every rebuild and
+ * one force launch in k (1: every launch) with HIP events on the context's
+ * stream; force_ms / force_launches is then the mean over the timed
+ * launches.  bmx_evolve_graph refuses to run while timing is on. */
 bmx_status bmx_set_timing(bmx_ctx *ctx, int32_t enable);
 bmx_status bmx_get_timing(bmx_ctx *ctx, bmx_timing *out);
 bmx_status bmx_reset_timing(bmx_ctx *ctx);

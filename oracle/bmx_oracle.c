@@ -713,8 +713,8 @@ static double reach(const orc_particle *q) {
 }
 
 /* Pair-cutoff mode: extra = {W_A, skin}; accept if r^2 <= (reach_i +
- * reach_j + W_A + skin)^2, the pairs that can reach r_lm before the skin is
- * used up. */
+ * reach_j + (W_A + skin))^2, the pairs that can reach r_lm before the skin
+ * is used up (W_A + skin is summed first, once per list build). */
 static double g_pair_wa = 0.0;
 
 static int accept_pair3(const orc_particle *pi, const orc_particle *pj,
@@ -722,7 +722,7 @@ static int accept_pair3(const orc_particle *pi, const orc_particle *pj,
   double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
   if (mode == ORC_NBR_STENCIL) return r2 <= radius;
   if (mode == ORC_NBR_PAIR) {
-    double cut = reach(pi) + reach(pj) + g_pair_wa + radius;
+    double cut = reach(pi) + reach(pj) + (g_pair_wa + radius);
     return r2 <= cut * cut;
   }
   return r2 <= radius * radius;
