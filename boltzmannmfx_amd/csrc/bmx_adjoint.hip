@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
     const double4 p = pos[t];
     const double4 li = lam_in[t];
     const double ri = p.w;
-    const int c = ncnt[t] & ((1 << 30) - 1);
+    const int c = ncnt[t] & ((1 << 28) - 1);  // bits 28..30: image axes
     double jl0 = 0.0, jl1 = 0.0, jl2 = 0.0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     double dmu = 0.0, dwa = 0.0, dr = 0.0;

@@ -741,7 +741,8 @@ bmx_status bmx_set_neighbor_params(bmx_ctx *c, const bmx_neighbor_params *p) {
   if (!c || !p) return fail(BMX_ERR_BAD_ARG, "null argument");
   if (p->mode != BMX_NBR_VERLET && p->mode != BMX_NBR_STENCIL)
     return fail(BMX_ERR_BAD_ARG, "unknown neighbor mode");
-  if (p->rebuild_every < 1 || p->max_neighbors < 0 || !(p->skin >= 0.0) ||
+  if (p->rebuild_every < 1 || p->max_neighbors < 0 ||
+      p->max_neighbors > (1 << 20) || !(p->skin >= 0.0) ||
       p->bin_factor_pct < 0)
     return fail(BMX_ERR_BAD_ARG, "bad neighbor parameters");
   if (p->mode == BMX_NBR_STENCIL && p->rebuild_every != 1)

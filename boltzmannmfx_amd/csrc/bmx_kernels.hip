@@ -391,14 +391,23 @@ __device__ __forceinline__ double reach_of(const double4 &p, int ty,
   return p.w;
 }
 
+// Separation along axis A: the periodic image test of sep_axis when bit A of
+// M is set, the plain difference otherwise (used only where no image can be
+// involved on that axis, which gives the same bits).
+template <int M, int A>
+__device__ __forceinline__ double sep_m(double xi, double xj, const BoxGeom &g) {
+  if (M & (1 << A)) return sep_axis(xi, xj, g.L[A], g.halfL[A], g.periodic[A]);
+  return xi - xj;
+}
+
 // Scans one contiguous candidate range [ua, ub) of the sorted arrays and
 // appends the accepted candidates to the row (out walks the column-major
 // row, one stride of ncap per entry).  The candidate loop is branch-free
 // apart from the append: loads past the range end repeat the last entry and
-// are masked out with the self pair.  IMG = false when no periodic image can
-// be involved (the stencil does not wrap), which gives the same separation
-// bits as the image test would.  was = W_A + skin.
-template <bool RODS, bool IMG>
+// are masked out with the self pair.  Bit a of M asks for the periodic image
+// test along axis a; an axis left out is one along which no candidate of
+// the wave can need an image.  was = W_A + skin.
+template <bool RODS, int M>
 __device__ __forceinline__ void scan_range(
     int ua, int ub, int t, const double4 &p, double reach_i,
     const BoxGeom &g, const double4 *__restrict__ pos,
@@ -412,16 +421,9 @@ __device__ __forceinline__ void scan_range(
 #pragma unroll
     for (int j = 0; j < kUnroll; j++) {
       const int u = u0 + j;
-      double d0, d1, d2;
-      if (IMG) {
-        d0 = sep_axis(p.x, o[j].x, g.L[0], g.halfL[0], g.periodic[0]);
-        d1 = sep_axis(p.y, o[j].y, g.L[1], g.halfL[1], g.periodic[1]);
-        d2 = sep_axis(p.z, o[j].z, g.L[2], g.halfL[2], g.periodic[2]);
-      } else {
-        d0 = p.x - o[j].x;
-        d1 = p.y - o[j].y;
-        d2 = p.z - o[j].z;
-      }
+      const double d0 = sep_m<M, 0>(p.x, o[j].x, g);
+      const double d1 = sep_m<M, 1>(p.y, o[j].y, g);
+      const double d2 = sep_m<M, 2>(p.z, o[j].z, g);
       double r2 = d0 * d0 + d1 * d1 + d2 * d2;
       bool ok;
       if (mode == 1) {
@@ -444,11 +446,20 @@ __device__ __forceinline__ void scan_range(
   }
 }
 
-// Bit 30 of a row count marks a particle whose neighbors can need no
-// periodic image until the next rebuild (two bins or more from every
-// periodic face); force kernels then skip the image test.
-constexpr int32_t kInteriorBit = 1 << 30;
-__device__ __forceinline__ int row_len(int32_t raw) { return raw & (kInteriorBit - 1); }
+// Bits 28..30 of a row count mark the axes (x, y, z) along which a neighbor
+// of the particle can need a periodic image before the next rebuild: the
+// axis is periodic and the particle's bin is within two bins of a face (or
+// the box has fewer than six bins along it).  Force kernels test images
+// only along the axes marked somewhere in the wave.
+constexpr int kImgShift = 28;
+constexpr int32_t kRowMask = (1 << kImgShift) - 1;
+__device__ __forceinline__ int row_len(int32_t raw) { return raw & kRowMask; }
+__device__ __forceinline__ int row_img(int32_t raw) { return (raw >> kImgShift) & 7; }
+
+// OR of a 3-bit axis mask over the wave.
+__device__ __forceinline__ int wave_or3(int m) {
+  return (__any(m & 1) ? 1 : 0) | (__any(m & 2) ? 2 : 0) | (__any(m & 4) ? 4 : 0);
+}
 
 template <bool MULTI, bool RODS>
 __global__ __launch_bounds__(512) void k_build_nbr(
@@ -476,7 +487,9 @@ __global__ __launch_bounds__(512) void k_build_nbr(
   int bx = 0, by = 0, bz = 0;
   double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
   double reach_i = 0.0;
-  bool noimg = false, interior = false;
+  // axes along which a candidate of this list build (scan) or a neighbor
+  // before the next rebuild (force) can need a periodic image
+  int scan_img = 0, force_img = 0;
   if (active) {
     p = pos[t];
     reach_i = RODS ? reach_of<true>(p, type[t], rod, t) : p.w;
@@ -486,16 +499,14 @@ __global__ __launch_bounds__(512) void k_build_nbr(
     by = rr % nby;
     bz = rr / nby;
     const int b3[3] = {bx, by, bz};
-    noimg = true;
-    interior = true;
     for (int a = 0; a < 3; a++) {
       if (!g.periodic[a]) continue;
       int nb = g.nbin[a];
-      noimg = noimg && nb >= 4 && b3[a] >= 1 && b3[a] + 1 < nb;
-      interior = interior && nb >= 6 && b3[a] >= 2 && b3[a] + 2 < nb;
+      if (!(nb >= 4 && b3[a] >= 1 && b3[a] + 1 < nb)) scan_img |= 1 << a;
+      if (!(nb >= 6 && b3[a] >= 2 && b3[a] + 2 < nb)) force_img |= 1 << a;
     }
   }
-  const bool wave_noimg = __all(noimg || !active);
+  const int wave_img = wave_or3(scan_img);
   if (active) {
     for (int oz = -1; oz <= 1; oz++) {
       int cz = bz + oz;
@@ -539,18 +550,22 @@ __global__ __launch_bounds__(512) void k_build_nbr(
             ranges[q][1] = ranges[q][0];
           }
           scanned += len;
-          if (wave_noimg)
-            scan_range<RODS, false>(ranges[q][0], ranges[q][1], ti, p,
-                                    reach_i, g, pos, type, rod, mode, crit,
-                                    was, out, ncap, maxn, cnt);
+          if (wave_img == 0)
+            scan_range<RODS, 0>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                pos, type, rod, mode, crit, was, out, ncap,
+                                maxn, cnt);
+          else if (wave_img == 1)
+            scan_range<RODS, 1>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                pos, type, rod, mode, crit, was, out, ncap,
+                                maxn, cnt);
           else
-            scan_range<RODS, true>(ranges[q][0], ranges[q][1], ti, p, reach_i,
-                                   g, pos, type, rod, mode, crit, was, out,
-                                   ncap, maxn, cnt);
+            scan_range<RODS, 7>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                pos, type, rod, mode, crit, was, out, ncap,
+                                maxn, cnt);
         }
       }
     }
-    ncnt[t] = (cnt < maxn ? cnt : maxn) | (interior ? kInteriorBit : 0);
+    ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
     if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
   } else if (t < n) {
     ncnt[t] = 0;
@@ -618,22 +633,16 @@ __device__ __forceinline__ double4 pick4(const double4 (&o)[kRowChunk], int k) {
   return make_double4(x, y, z, w);
 }
 
-template <bool IMG>
+template <int M>
 __device__ __forceinline__ void sphere_sep(const double4 &p, const double4 &o,
                                            const BoxGeom &g, double &d0,
                                            double &d1, double &d2) {
-  if (IMG) {
-    d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-    d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-    d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-  } else {
-    d0 = p.x - o.x;
-    d1 = p.y - o.y;
-    d2 = p.z - o.z;
-  }
+  d0 = sep_m<M, 0>(p.x, o.x, g);
+  d1 = sep_m<M, 1>(p.y, o.y, g);
+  d2 = sep_m<M, 2>(p.z, o.z, g);
 }
 
-template <bool IMG>
+template <int M>
 __device__ __forceinline__ void sphere_row(const double4 &p, int c,
                                            const int32_t *__restrict__ row,
                                            int64_t ncap, int64_t t,
@@ -656,7 +665,7 @@ __device__ __forceinline__ void sphere_row(const double4 &p, int c,
 #pragma unroll
     for (int j = 0; j < kRowChunk; j++) {
       double d0, d1, d2;
-      sphere_sep<IMG>(p, o[j], g, d0, d1, d2);
+      sphere_sep<M>(p, o[j], g, d0, d1, d2);
       double r2 = d0 * d0 + d1 * d1 + d2 * d2;
       double lim = p.w + o[j].w + fp.WA - 1.0e-15;  // r_lm - small_number
       if (base + j < c && r2 <= lim * lim) mask |= 1u << j;
@@ -666,7 +675,7 @@ __device__ __forceinline__ void sphere_row(const double4 &p, int c,
       mask &= mask - 1u;
       const double4 q = pick4(o, k);
       double d0, d1, d2;
-      sphere_sep<IMG>(p, q, g, d0, d1, d2);
+      sphere_sep<M>(p, q, g, d0, d1, d2);
       double r2 = d0 * d0 + d1 * d1 + d2 * d2;
       sphere_pair(d0, d1, d2, r2, p.w + q.w, fp, a0, a1, a2);
     }
@@ -684,17 +693,19 @@ __global__ __launch_bounds__(512) void k_force_sphere(
   int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   const bool active = t < n;
-  const int32_t raw = active ? ncnt[t] : kInteriorBit;
-  const bool wave_interior = __all((raw & kInteriorBit) != 0);
+  const int32_t raw = active ? ncnt[t] : 0;
+  const int wave_img = wave_or3(row_img(raw));
   if (active) {
     const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
     double4 p = pos_in[t];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const int c = row_len(raw);
-    if (wave_interior)
-      sphere_row<false>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
+    if (wave_img == 0)
+      sphere_row<0>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
+    else if (wave_img == 1)
+      sphere_row<1>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
     else
-      sphere_row<true>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
+      sphere_row<7>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
     // growth surface (K.H:1009-1030); vcom x/y are zero
     a0 += 0.0;
     a1 += 0.0;
