@@ -234,12 +234,32 @@ __global__ void k_bin_count(const double4 *__restrict__ pos,
                             int64_t n, int32_t *__restrict__ key,
                             int32_t *__restrict__ rank,
                             int32_t *__restrict__ count) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const BoxGeom &g = MULTI ? boxes[col[s]] : g0;
-  int b = bin_of(pos[s], g);
-  key[s] = b;
-  rank[s] = atomicAdd(&count[b], 1);
+  // The input is in the order of the previous rebuild, so neighboring lanes
+  // mostly share a bin: each run of equal bins in the wave takes its slots
+  // with one atomic (the head's) and numbers its lanes from there.  Only the
+  // slot set of a bin matters: the bins are sorted by original index next.
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = s < n;
+  int b = -1;
+  if (active) {
+    const BoxGeom &g = MULTI ? boxes[col[s]] : g0;
+    b = bin_of(pos[s], g);
+    key[s] = b;
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int prev = __shfl_up(b, 1, kWave);
+  const bool head = active && (lane == 0 || prev != b);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long act = __ballot(active);
+  // run of this lane: from the last head at or below it to the next head
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);
+  const int first = 63 - __builtin_clzll(below | 1ull);
+  const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+  const int stop = above ? __builtin_ctzll(above) : 64 - __builtin_clzll(act);
+  int base = 0;
+  if (head) base = atomicAdd(&count[b], stop - lane);
+  base = __shfl(base, first, kWave);
+  if (active) rank[s] = base + (lane - first);
 }
 
 // Exclusive scan: 256 threads x 8 items per block, wave prefix by shuffles.
@@ -462,7 +482,7 @@ __device__ __forceinline__ int wave_or3(int m) {
 }
 
 template <bool MULTI, bool RODS>
-__global__ __launch_bounds__(512) void k_build_nbr(
+__global__ __launch_bounds__(512, 8) void k_build_nbr(
     const double4 *__restrict__ pos, const int32_t *__restrict__ col,
     const int32_t *__restrict__ type, const double4 *__restrict__ rod,
     const BoxGeom *__restrict__ boxes, BoxGeom g0,
@@ -683,7 +703,7 @@ __device__ __forceinline__ void sphere_row(const double4 &p, int c,
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(512) void k_force_sphere(
+__global__ __launch_bounds__(512, 6) void k_force_sphere(
     const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
     int64_t ncap, int64_t n, const int32_t *__restrict__ col,
