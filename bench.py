@@ -282,7 +282,7 @@ def main() -> int:
         "config": {
             "workload": "1M FP64 spheres, r~U[0.5e-4,0.7e-4] cm, phi=0.3, 2:2:1 box "
                         "periodic x/y with growth surface, cubic law W_A=6e-6, dt=0.25, "
-                        "Verlet list (skin 2e-5) rebuilt every 10 steps",
+                        f"Verlet list (skin {col.skin:g}) rebuilt every 10 steps",
             "n": n,
             "mean_row": mean_row,
             "rebuilds": tm.rebuilds,
