@@ -182,6 +182,62 @@ def adjoint_bench(bmx, W, n: int = 100_000, steps: int = 200):
     }
 
 
+def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
+    """200,000 hyphal segments (2,000 roots, branch probability 0.05 per
+    segment at 30-60 degrees, 30e-4 cm long, radius 2e-4 cm, bond springs
+    k = 1e-2) on the growth surface, in the reference's policy: bins are the
+    level mesh cells (2e-3 cm) and the +-1 stencil list is rebuilt every
+    step; InitBonds first, then `steps` timed steps."""
+    col = W.c3_hyphae()
+    hood = 1.1 * (2.0 * 2.0e-4 + 1.0e-2)  # DEM::neighborhood (bmx_chem.cpp:124)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=bmx.NBR_STENCIL, rebuild_every=1, neighborhood=hood,
+                            max_neighbors=128)
+    ctx.bind(col.recs.copy())
+    ctx.init_bonds()
+    ctx.evolve(col.dt, 10)
+    ctx.set_timing(True, every=7)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.evolve(col.dt, steps)
+    secs = time.perf_counter() - t0
+    tm = ctx.timing()
+    ctx.close()
+    n = len(col.recs)
+    cpu = None
+    if cpu_steps > 0:
+        # the oracle on the same segments (InitBonds, then whole steps), all
+        # host threads
+        import oracle_bmx as orc
+        recs = col.recs.copy()
+        g = orc.make_geom(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+        st = orc.State(recs, g, col.fpar, orc.NBR_STENCIL, hood, 1)
+        st.init_bonds()
+        t0 = time.perf_counter()
+        st.evolve(col.dt, cpu_steps)
+        el = time.perf_counter() - t0
+        st.close()
+        nt = orc.lib().orc_num_threads()
+        cpu = {"value": n * cpu_steps / el, "unit": "particle-steps/s", "cores": nt,
+               "kind": "port", "sample": f"{cpu_steps} steps of the {n}-segment workload "
+                                         f"on {nt} OpenMP threads"}
+    return {
+        "workload": f"{n} hyphal segments (2,000 roots), mesh bins {col.n_cell}, "
+                    f"reference stencil list every step, {steps} steps",
+        "cpu_baseline": cpu,
+        "particle_steps_per_s": n * steps / secs,
+        "ms_per_step": secs * 1e3 / steps,
+        "force_kernel": "k_force_general",
+        "force_kernel_avg_ms": tm.force_ms / max(tm.force_launches, 1),
+        "rebuild_avg_ms": tm.rebuild_ms / max(tm.rebuilds, 1),
+        "mean_row": tm.neighbor_entries / n,
+        "bound": "fp64-valu",
+    }
+
+
 def calibration_copy(nbytes: int) -> None:
     """One device-to-device copy of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
@@ -213,6 +269,7 @@ def main() -> int:
                     help="skip the 256 one-colony runs the ensemble is compared with")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-rods", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -312,6 +369,9 @@ def main() -> int:
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(bmx, W)
         print("# adjoint done", file=sys.stderr, flush=True)
+    if not args.no_rods:
+        out["rods"] = rods_bench(bmx, W, cpu_steps=0 if args.no_cpu else 10)
+        print("# rods done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(col, args.cpu_budget)
         out["cpu_baseline"] = cb

@@ -146,3 +146,51 @@ def test_adjoint_reports_skin_violation(bmx, gpu):
         ctx.adjoint(W.DT, 80, col.recs["pos"], checkpoint_every=20)
     assert e.value.code == 6
     ctx.close()
+
+
+def test_adjoint_as_torch_autograd_function(bmx, gpu):
+    """The C-ABI adjoint wrapped in a torch.autograd.Function: gradients of a
+    torch expression of the loss flow through bmx_adjoint (the parameters are
+    CPU float64 tensors; the library owns the device state)."""
+    import torch
+
+    n, steps = 2000, 40
+    col = W.sphere_colony(n, 0.3, 21, W_A=6e-6, overlap_free=False, skin=5e-5)
+    target = col.recs["pos"] + np.random.default_rng(5).normal(size=(n, 3)) * 1e-5
+
+    class BMXLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, mu, wa, gamma, radii):
+            f = col.fpar.copy()
+            f[1] = float(mu)
+            f[0] = float(wa)
+            c, _ = setup(bmx, col, f, radii=radii.numpy(), mobility=1.0 / float(gamma))
+            loss, g3, gr, _ = c.adjoint(W.DT, steps, target, checkpoint_every=20)
+            c.close()
+            ctx.save_for_backward(torch.tensor(g3), torch.from_numpy(gr.copy()))
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, go):
+            g3, gr = ctx.saved_tensors
+            return go * g3[0], go * g3[1], go * g3[2], go * gr
+
+    mu = torch.tensor(col.fpar[1], dtype=torch.float64, requires_grad=True)
+    wa = torch.tensor(col.fpar[0], dtype=torch.float64, requires_grad=True)
+    gamma = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    radii = torch.from_numpy(col.recs["rdata"][:, 0].copy()).requires_grad_(True)
+    out = 3.0 * BMXLoss.apply(mu, wa, gamma, radii)
+    out.backward()
+    assert radii.grad.shape == (n,) and torch.isfinite(radii.grad).all()
+
+    # central differences of the same torch expression along one direction
+    v = torch.from_numpy(np.random.default_rng(6).normal(size=n))
+    scale = (1e-6 * col.fpar[1], 1e-6 * col.fpar[0], 1e-6, 1e-6 * radii.detach())
+    with torch.no_grad():
+        def f(s):
+            return 3.0 * BMXLoss.apply(mu + s * scale[0], wa + s * scale[1],
+                                       gamma + s * scale[2], radii + s * scale[3] * v)
+        fd = (f(1.0) - f(-1.0)) / 2.0
+    ad = (mu.grad * scale[0] + wa.grad * scale[1] + gamma.grad * scale[2]
+          + (radii.grad * scale[3] * v).sum())
+    assert float(abs(fd - ad) / abs(ad)) < 1e-6
