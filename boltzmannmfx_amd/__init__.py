@@ -172,6 +172,7 @@ def load_library() -> C.CDLL:
         "bmx_adjoint": (C.c_int, [vp, C.c_double, C.c_int32, C.c_int32, P(C.c_double),
                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(AdjointTiming)]),
+        "bmx_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
         "bmx_autotune": (C.c_int, [vp, C.c_double, C.c_int32, C.c_char_p, P(Tuning)]),
         "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
                                                P(C.c_int64), P(Layout)]),
@@ -316,6 +317,13 @@ class Context:
                                     _dptr(tg), C.byref(loss), _dptr(g3), _dptr(gr),
                                     C.byref(tm)))
         return loss.value, g3, gr[: self.n], tm
+
+    def exchange(self, xpar, dt: float) -> None:
+        """ParticleExchange between bonded segments (species in rdata
+        25..27, increments in 28..30; written back by sync())."""
+        xp = np.ascontiguousarray(xpar, dtype=np.float64)
+        assert xp.size == 3
+        _check(self.lib.bmx_exchange(self.h, _dptr(xp), float(dt)))
 
     def autotune(self, dt: float, trial_steps: int = 20,
                  cache: Optional[str] = None) -> dict:

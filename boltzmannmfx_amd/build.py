@@ -22,7 +22,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB_NAME = "libbmx_mi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["bmx_kernels.hip", "bmx_capi.hip", "bmx_adjoint.hip"]
+SOURCES = ["bmx_kernels.hip", "bmx_capi.hip", "bmx_adjoint.hip", "bmx_exchange.hip"]
 HOST_SOURCES = ["bmx_container.cpp", "bmx_io.cpp"]
 
 

@@ -102,6 +102,11 @@ struct bmx_ctx {
   DevBuf<unsigned long long> d_permk;
   int32_t maxn = 96;
   DevBuf<int32_t> d_nbr, d_ncnt;
+  // species A, B, C and volume, and their exchange increments, in original
+  // order (loaded from the records at the first exchange)
+  DevBuf<double4> d_chem, d_chem_inc;
+  bool have_chem = false;
+  int32_t n_real = 0;
   bool have_list = false;
   int64_t step = 0, last_build = -1;
 
@@ -481,6 +486,9 @@ bmx_status pack_records(bmx_ctx *c) {
                          dyn_arrays(c, c->cd), c->d_fc.p, c->ncap,
                          c->mobility, c->wrote_step ? 1 : 0,
                          c->any_fungi ? 1 : 0, c->stream));
+  if (c->have_chem)
+    HIPCHK(launch_chem_pack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+                            c->d_chem_inc.p, c->stream));
   return BMX_OK;
 }
 
@@ -604,6 +612,8 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   c->last_build = -1;
   c->have_list = false;
   c->wrote_step = false;
+  c->have_chem = false;
+  c->n_real = L.n_real;
   c->bound = true;
   return BMX_OK;
 }
@@ -1274,6 +1284,38 @@ bmx_status bmx_synchronize(bmx_ctx *c) {
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------- exchange
+extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!xpar || !std::isfinite(dt) || !std::isfinite(xpar[0]) ||
+      !std::isfinite(xpar[1]) || !std::isfinite(xpar[2]))
+    return fail(BMX_ERR_BAD_ARG, "bad exchange arguments");
+  if (c->n_real < rec::first_data + 6)
+    return fail(BMX_ERR_BAD_ARG,
+                "the record layout has no species fields (rdata[25..30])");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  if (!c->have_chem) {
+    HIPCHK(c->d_chem.alloc((size_t)c->ncap));
+    HIPCHK(c->d_chem_inc.alloc((size_t)c->ncap));
+    HIPCHK(launch_chem_unpack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+                              c->d_chem_inc.p, s));
+    c->have_chem = true;
+  }
+  // the rows are rebuilt at the current positions (xchng.cpp:68-77)
+  if (c->have_list && verlet_checked(c))
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
+                             s));
+  st = rebuild(c);
+  if (st != BMX_OK) return st;
+  HIPCHK(launch_exchange(c->pos[c->cd].p, c->type[c->cs].p, c->topo[c->cd].p,
+                         c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
+                         c->n, c->d_chem.p, c->d_chem_inc.p, xpar, dt, s));
+  return check_flags(c);
+}
 
 // ---------------------------------------------------------------- autotuner
 namespace {

@@ -133,7 +133,38 @@ static std::vector<double> fpar_of(const Deck &deck) {
 
 BMXParticleContainer::BMXParticleContainer(const Deck &deck, int device)
     : BMXParticleContainer(deck.finest_geometry(), fpar_of(deck).data(),
-                           stencil_params(deck), device) {}
+                           stencil_params(deck), device) {
+  deck.exchange_params(xpar_);
+  deck.init_conc(conc_);
+}
+
+void Deck::exchange_params(double xpar[3]) const {
+  xpar[0] = real_or("chem_species.mass_transfer_A", 0.0);
+  xpar[1] = real_or("chem_species.mass_transfer_B", 0.0);
+  xpar[2] = real_or("chem_species.mass_transfer_C", 0.0);
+}
+
+void Deck::init_conc(double conc[3]) const {
+  for (int c = 0; c < 3; c++) {
+    auto it = kv.find("fluid.init_conc_species");
+    conc[c] = (it != kv.end() && (int)it->second.size() > c)
+                  ? real("fluid.init_conc_species", c)
+                  : 0.0;
+  }
+}
+
+void BMXParticleContainer::set_exchange_params(const double xpar[3]) {
+  for (int c = 0; c < 3; c++) xpar_[c] = xpar[c];
+}
+
+void BMXParticleContainer::set_init_conc(const double conc[3]) {
+  for (int c = 0; c < 3; c++) conc_[c] = conc[c];
+}
+
+void BMXParticleContainer::ParticleExchange(double dt) {
+  check(bmx_exchange(ctx_, xpar_, dt));
+  dirty_ = true;
+}
 
 BMXParticleContainer::~BMXParticleContainer() {
   if (ctx_) bmx_ctx_destroy(ctx_);
@@ -149,6 +180,12 @@ void BMXParticleContainer::InitParticlesAscii(const std::string &file) {
   check(bmx_read_particles_ascii(file.c_str(), nullptr, 0, &n, &L));
   std::vector<uint8_t> recs((size_t)n * L.stride_bytes);
   check(bmx_read_particles_ascii(file.c_str(), recs.data(), n, &n, &L));
+  // rdata[first_data + c] = init_conc[c] (src/des/bmx_pc_init.cpp:87-91)
+  for (int64_t i = 0; i < n; i++) {
+    double *rd = reinterpret_cast<double *>(recs.data() + (size_t)i * L.stride_bytes +
+                                            L.rdata_offset);
+    for (int c = 0; c < 3; c++) rd[25 + c] = conc_[c];
+  }
   InitParticles(recs.data(), n);
 }
 

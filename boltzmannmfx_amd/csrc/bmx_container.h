@@ -49,6 +49,10 @@ struct Deck {
   bmx_geometry finest_geometry() const;  // level mesh of amr.max_level
   void force_params(double fpar[9]) const;
   double neighborhood() const;           // DEM::neighborhood
+  // chem_species.mass_transfer_A/B/C (src/chemistry/bmx_chem.cpp:100-105)
+  void exchange_params(double xpar[3]) const;
+  // fluid.init_conc_species (FLUID::init_conc), zeros where absent
+  void init_conc(double conc[3]) const;
 };
 
 // Parses "key = v1 v2 ..." lines; '#' starts a comment; quotes are dropped.
@@ -65,6 +69,8 @@ class BMXParticleContainer {
   BMXParticleContainer(const BMXParticleContainer &) = delete;
   BMXParticleContainer &operator=(const BMXParticleContainer &) = delete;
 
+  // Reads the 27-field file; the species start at init_conc (the deck's
+  // fluid.init_conc_species for a deck-built container, else zero).
   void InitParticlesAscii(const std::string &file);
   // Records supplied by the caller (host memory), e.g. a restart.
   void InitParticles(const void *records, int64_t n);
@@ -72,6 +78,11 @@ class BMXParticleContainer {
   // One step of size dt; nsubsteps is set to 1 as in the reference
   // (dtsolid = dt, src/des/bmx_pc_interaction.cpp:56-65).
   void EvolveParticles(double dt, int &nsubsteps);
+  // ParticleExchange (src/des/bmx_pc_xchng.cpp:13-257) with the transfer
+  // rates of the deck (or set_exchange_params).
+  void ParticleExchange(double dt);
+  void set_exchange_params(const double xpar[3]);
+  void set_init_conc(const double conc[3]);
   // n steps without copying back in between.
   void EvolveSteps(double dt, int nsteps);
   void WriteAsciiFileForInit(const std::string &file);
@@ -84,6 +95,8 @@ class BMXParticleContainer {
  private:
   void check(bmx_status st) const;
   bmx_ctx *ctx_ = nullptr;
+  double xpar_[3] = {0.0, 0.0, 0.0};
+  double conc_[3] = {0.0, 0.0, 0.0};
   std::vector<uint8_t> host_;
   int64_t n_ = 0;
   bool dirty_ = false;

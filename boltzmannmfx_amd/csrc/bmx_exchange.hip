@@ -1,0 +1,153 @@
+// bmx_exchange.hip -- ParticleExchange: transfer of the species A, B, C
+// between bonded segments over the neighbour rows
+// (src/des/bmx_pc_xchng.cpp:13-257; initExchange / evaluateExchange /
+// applyIncrements, src/chemistry/bmx_chem_K.H:1951-2112).
+//
+// The concentrations and volumes live in original particle order
+// (chem = {A, B, C, vol}, inc = {dA, dB, dC, -}, the record's rdata[25..27],
+// rdata[5] and rdata[28..30]); the rows, bond tables and radii are the
+// sorted arrays of the last rebuild.  Each row adds only into its own
+// increments, in row order: the neighbour is seen through a copy in the
+// reference, so its side of the flux is dropped there too.
+#include "bmx_kernels.h"
+
+namespace bmx {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;  // M_PI of <math.h>
+
+__global__ void k_chem_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
+                              int64_t n, double4 *__restrict__ chem,
+                              double4 *__restrict__ inc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *rd =
+      reinterpret_cast<const double *>(aos + i * (int64_t)lay.stride + lay.rdata);
+  chem[i] = make_double4(rd[rec::first_data], rd[rec::first_data + 1],
+                         rd[rec::first_data + 2], rd[rec::vol]);
+  inc[i] = make_double4(rd[rec::first_data + 3], rd[rec::first_data + 4],
+                        rd[rec::first_data + 5], 0.0);
+}
+
+__global__ void k_chem_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
+                            const double4 *__restrict__ chem,
+                            const double4 *__restrict__ inc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double *rd = reinterpret_cast<double *>(aos + i * (int64_t)lay.stride + lay.rdata);
+  const double4 c = chem[i], d = inc[i];
+  rd[rec::first_data] = c.x;
+  rd[rec::first_data + 1] = c.y;
+  rd[rec::first_data + 2] = c.z;
+  rd[rec::first_data + 3] = d.x;
+  rd[rec::first_data + 4] = d.y;
+  rd[rec::first_data + 5] = d.z;
+}
+
+// The flux of one species from i to j (evaluateExchange, bmx_chem_K.H:
+// 2041-2068): the area-weighted difference, clamped so that neither side
+// gives more than its share per bond.
+__device__ __forceinline__ double species_flux(double c1, double c2, double area,
+                                               double mt, double dt, double v1,
+                                               double v2, int nb1, int nb2) {
+  double d = (c1 - c2) * area * mt * dt;
+  if (d / v1 > c1 / (double)nb1) d = c1 * v1 / (double)nb1;
+  if (-d / v2 > c2 / (double)nb2) d = -c2 * v2 / (double)nb2;
+  return d;
+}
+
+__global__ void k_exchange_rows(const double4 *__restrict__ pos,
+                                const int32_t *__restrict__ type,
+                                const Topo *__restrict__ topo,
+                                const int32_t *__restrict__ oid,
+                                const int32_t *__restrict__ nbr,
+                                const int32_t *__restrict__ ncnt, int64_t ncap,
+                                int64_t n, const double4 *__restrict__ chem,
+                                double4 *__restrict__ inc, double mtA,
+                                double mtB, double mtC, double dt) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || type[t] != rec::FUNGI) return;
+  const Topo ti = topo[t];
+  const double r1 = pos[t].w;
+  const double4 c1 = chem[oid[t]];
+  double ia = 0.0, ib = 0.0, ic = 0.0;  // initExchange
+  const int c = ncnt[t] & ((1 << 28) - 1);  // bits 28..30: image axes
+  const int32_t *rp = nbr + t;
+  for (int k = 0; k < c; k++, rp += ncap) {
+    const int u = *rp;
+    if (type[u] != rec::FUNGI) continue;
+    const Topo tj = topo[u];
+    int bi, bj;
+    for (bi = 0; bi < ti.n_bnds; bi++)
+      if (ti.seg_id[bi & 3] == tj.self_id && ti.seg_cpu[bi & 3] == tj.self_cpu) break;
+    for (bj = 0; bj < tj.n_bnds; bj++)
+      if (tj.seg_id[bj & 3] == ti.self_id && tj.seg_cpu[bj & 3] == ti.self_cpu) break;
+    if (!(bi < ti.n_bnds && bj < tj.n_bnds)) continue;
+    const double r2 = pos[u].w;
+    const double4 c2 = chem[oid[u]];
+    const double area1 = kPi * r1 * r1, area2 = kPi * r2 * r2;
+    const double area = 0.5 * (area1 + area2);
+    const double dA = species_flux(c1.x, c2.x, area, mtA, dt, c1.w, c2.w, ti.n_bnds, tj.n_bnds);
+    const double dB = species_flux(c1.y, c2.y, area, mtB, dt, c1.w, c2.w, ti.n_bnds, tj.n_bnds);
+    const double dC = species_flux(c1.z, c2.z, area, mtC, dt, c1.w, c2.w, ti.n_bnds, tj.n_bnds);
+    ia -= dA / c1.w;
+    ib -= dB / c1.w;
+    ic -= dC / c1.w;
+  }
+  inc[oid[t]] = make_double4(ia, ib, ic, 0.0);
+}
+
+// applyIncrements (bmx_chem_K.H:2092-2112)
+__global__ void k_exchange_apply(const int32_t *__restrict__ type,
+                                 const int32_t *__restrict__ oid, int64_t n,
+                                 double4 *__restrict__ chem,
+                                 const double4 *__restrict__ inc) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || type[t] != rec::FUNGI) return;
+  const int64_t o = oid[t];
+  double4 c = chem[o];
+  const double4 d = inc[o];
+  c.x += d.x;
+  c.y += d.y;
+  c.z += d.z;
+  chem[o] = c;
+}
+
+int grid256(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+}  // namespace
+
+hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
+                              double4 *chem, double4 *inc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chem_unpack, dim3(grid256(n)), dim3(256), 0, s, aos, lay,
+                     n, chem, inc);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
+                            const double4 *chem, const double4 *inc,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chem_pack, dim3(grid256(n)), dim3(256), 0, s, aos, lay,
+                     n, chem, inc);
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange(const double4 *pos, const int32_t *type,
+                           const Topo *topo, const int32_t *oid,
+                           const int32_t *nbr, const int32_t *ncnt,
+                           int64_t ncap, int64_t n, double4 *chem,
+                           double4 *inc, const double xpar[3], double dt,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_exchange_rows, dim3(grid256(n)), dim3(256), 0, s, pos,
+                     type, topo, oid, nbr, ncnt, ncap, n, chem, inc, xpar[0],
+                     xpar[1], xpar[2], dt);
+  hipLaunchKernelGGL(k_exchange_apply, dim3(grid256(n)), dim3(256), 0, s, type,
+                     oid, n, chem, inc);
+  return hipGetLastError();
+}
+
+}  // namespace bmx

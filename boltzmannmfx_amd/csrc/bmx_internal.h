@@ -10,7 +10,8 @@ namespace bmx {
 namespace rec {
 enum : int {
   radius = 0, c_length = 1, theta = 2, phi = 3, area = 4, vol = 5,
-  velx = 6, vely = 7, velz = 8, wx = 9, wy = 10, wz = 11, bond_scale = 24
+  velx = 6, vely = 7, velz = 8, wx = 9, wy = 10, wz = 11, bond_scale = 24,
+  first_data = 25  // species A, B, C, then their exchange increments
 };
 enum : int {
   cell_type = 5, n_bnds = 6, seg1_id1 = 7, seg1_id2 = 11, site1 = 15,

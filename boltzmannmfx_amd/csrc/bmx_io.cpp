@@ -68,7 +68,19 @@ struct Rec {
 // Defaults set for every new particle after its 27 fields are read
 // (src/des/bmx_pc_init.cpp:87-100): bond_scale = 1, fuse_tip = 0,
 // id/cpu mirrored into idata.
+// What InitParticlesAscii sets beyond the file's fields
+// (src/des/bmx_pc_init.cpp:87-99): the chemistry model's integer header
+// (BMXChemistry::setIntegers, bmx_chem.cpp:50-60 and 76-83: 3 species, 0
+// extra ints, 9 reals, 0 ints, increments at offset 6), bond_scale = 1,
+// fuse_tip = 0, and the id and cpu.  The species start at zero here; the
+// deck's fluid.init_conc_species is applied by the container.
 void finish_record(Rec &r, int64_t i, int32_t pid) {
+  int32_t *hdr = r.idata(i);
+  hdr[0] = 3;
+  hdr[1] = 0;
+  hdr[2] = 9;
+  hdr[3] = 0;
+  hdr[4] = 6;
   r.rd(i)[24] = 1.0;
   r.idata(i)[25] = 0;
   *r.id(i) = pid;

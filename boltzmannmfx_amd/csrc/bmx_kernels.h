@@ -129,4 +129,17 @@ hipError_t launch_from_orig4(const double4 *in, const int32_t *oid, int64_t n,
 hipError_t launch_sum_partials(const double *part, int64_t nblk, int stride,
                                double *out, hipStream_t s);
 
+// ParticleExchange (bmx_exchange.hip)
+hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
+                              double4 *chem, double4 *inc, hipStream_t s);
+hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
+                            const double4 *chem, const double4 *inc,
+                            hipStream_t s);
+hipError_t launch_exchange(const double4 *pos, const int32_t *type,
+                           const Topo *topo, const int32_t *oid,
+                           const int32_t *nbr, const int32_t *ncnt,
+                           int64_t ncap, int64_t n, double4 *chem,
+                           double4 *inc, const double xpar[3], double dt,
+                           hipStream_t s);
+
 }  // namespace bmx

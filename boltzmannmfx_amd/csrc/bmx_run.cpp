@@ -5,8 +5,9 @@
 // Mirrors the DEM part of the reference's time loop (src/main.cpp:313-342,
 // src/timestepping/bmx_evolve.cpp:93-97): read the deck, InitParticlesAscii
 // from particles.input_file (relative to the deck's directory), InitBonds at
-// time 0, then EvolveParticles every step of bmx.fixed_dt.  The fluid,
-// growth and fusion steps of bmx::Evolve are not part of this path.  Prints
+// time 0, then EvolveParticles and ParticleExchange every step of
+// bmx.fixed_dt.  The fluid, growth (split_particles) and fusion steps of
+// bmx::Evolve are not part of this path.  Prints
 // one line per step with the time per step and, at the end, writes the
 // particles in the input format.
 #include <chrono>
@@ -41,6 +42,7 @@ int main(int argc, char **argv) {
       if (time == 0.0) pc.InitBonds();
       int nsub = 0;
       pc.EvolveParticles(dt, nsub);
+      pc.ParticleExchange(dt);  // bmx_evolve.cpp:99 (split_particles before it is growth)
       auto t1 = std::chrono::steady_clock::now();
       time += dt;
       std::printf("Step %d time %g Time per step %.6f\n", step + 1, time,

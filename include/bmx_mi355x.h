@@ -203,6 +203,15 @@ bmx_status bmx_adjoint(bmx_ctx *ctx, double dt, int32_t nsteps,
                        double *loss, double grad3[3], double *grad_radius,
                        bmx_adjoint_timing *timing);
 
+/* ParticleExchange (src/des/bmx_pc_xchng.cpp:13-257, kernels
+ * src/chemistry/bmx_chem_K.H:1951-2112): rebuilds the rows at the current
+ * positions, then moves species A, B, C (rdata[25..27]) between bonded
+ * segments at rates xpar = {mass_transfer_A, _B, _C}
+ * (BMXChemistry::getExchangeParameters, bmx_chem.cpp:221-228) over dt, and
+ * leaves the increments in rdata[28..30].  The records must carry at least
+ * 31 reals.  bmx_sync_to_aos writes the species back. */
+bmx_status bmx_exchange(bmx_ctx *ctx, const double xpar[3], double dt);
+
 /* --- inspection (original particle order) ------------------------------ */
 /* fc of the last step, 6*n doubles, component-major: the reference's
  * fc[i + c*ntot] (src/des/bmx_pc_interaction.cpp:244-249). */

@@ -16,6 +16,9 @@
 
 #include <math.h>
 #include <stdlib.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846 /* as <math.h> defines it */
+#endif
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1017,6 +1020,86 @@ int orc_state_init_bonds(orc_state *s) {
   free(s->idx);
   s->idx = NULL;
   return bad ? -3 : 0;
+}
+
+/* ParticleExchange (src/des/bmx_pc_xchng.cpp:13-257) with its kernels
+ * initExchange / evaluateExchange / applyIncrements
+ * (src/chemistry/bmx_chem_K.H:1951-2112).  The concentrations A, B, C are
+ * rdata[25..27] and their increments rdata[28..30] (first_data = 25).
+ * Particle i sees each neighbour through a copy (xchng.cpp: "auto p2 =
+ * *mit"), so the neighbour-side increments are dropped; i's increments are
+ * summed in row order and applied after all rows. */
+void orc_exchange_init(double *rd, const int32_t *id) {
+  if (id[ORC_I_CELL_TYPE] == ORC_FUNGI) {
+    double *c = rd + ORC_R_FIRST_DATA;
+    c[3] = 0.0;
+    c[4] = 0.0;
+    c[5] = 0.0;
+  }
+}
+
+void orc_exchange_pair(double *rd1, double *rd2, const int32_t *id1,
+                       const int32_t *id2, const double xpar[3], double dt) {
+  if (id1[ORC_I_CELL_TYPE] != ORC_FUNGI || id2[ORC_I_CELL_TYPE] != ORC_FUNGI)
+    return;
+  const int ib_n = id1[ORC_I_N_BNDS], jb_n = id2[ORC_I_N_BNDS];
+  const int iid = id1[ORC_I_ID], icpu = id1[ORC_I_CPU];
+  const int jid = id2[ORC_I_ID], jcpu = id2[ORC_I_CPU];
+  int ib, jb;
+  for (ib = 0; ib < ib_n; ib++)
+    if (id1[ORC_I_SEG1_ID1 + ib] == jid && id1[ORC_I_SEG1_ID2 + ib] == jcpu) break;
+  for (jb = 0; jb < jb_n; jb++)
+    if (id2[ORC_I_SEG1_ID1 + jb] == iid && id2[ORC_I_SEG1_ID2 + jb] == icpu) break;
+  if (!(ib < ib_n && jb < jb_n)) return;
+  double *c1 = rd1 + ORC_R_FIRST_DATA, *c2 = rd2 + ORC_R_FIRST_DATA;
+  const double r1 = rd1[ORC_R_RADIUS], r2 = rd2[ORC_R_RADIUS];
+  const double v1 = rd1[ORC_R_VOL], v2 = rd2[ORC_R_VOL];
+  const double area1 = M_PI * r1 * r1, area2 = M_PI * r2 * r2;
+  const double area = 0.5 * (area1 + area2);
+  double d[3];
+  for (int k = 0; k < 3; k++) d[k] = (c1[k] - c2[k]) * area * xpar[k] * dt;
+  /* keep either side from going negative (one bond's share) */
+  for (int k = 0; k < 3; k++) {
+    if (d[k] / v1 > c1[k] / (double)ib_n) d[k] = c1[k] * v1 / (double)ib_n;
+    if (-d[k] / v2 > c2[k] / (double)jb_n) d[k] = -c2[k] * v2 / (double)jb_n;
+  }
+  for (int k = 0; k < 3; k++) c1[3 + k] -= d[k] / v1;
+  for (int k = 0; k < 3; k++) c2[3 + k] += d[k] / v2;
+}
+
+void orc_exchange_apply(double *rd, const int32_t *id) {
+  if (id[ORC_I_CELL_TYPE] == ORC_FUNGI) {
+    double *c = rd + ORC_R_FIRST_DATA;
+    c[0] += c[3];
+    c[1] += c[4];
+    c[2] += c[5];
+  }
+}
+
+/* One ParticleExchange call: the list is rebuilt at the current positions
+ * (xchng.cpp:68-77 builds it at substep 0), then every row is evaluated,
+ * then the increments are applied (one substep: dt = dtsolid). */
+int orc_state_exchange(orc_state *s, const double xpar[3], double dt) {
+  int rc = rebuild(s);
+  if (rc) return rc;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < s->n; i++) {
+    orc_particle *pi = &s->p[i];
+    orc_exchange_init(pi->rdata, pi->idata);
+    for (int64_t k = s->off[i]; k < s->off[i + 1]; k++) {
+      /* the copy of the neighbour: everything up to its concentrations
+       * (its increments are being written by its own row meanwhile and are
+       * never read) */
+      const orc_particle *pj = &s->p[s->idx[k]];
+      double rd2[34];
+      memcpy(rd2, pj->rdata, sizeof(double) * (ORC_R_FIRST_DATA + 3));
+      for (int c = ORC_R_FIRST_DATA + 3; c < 34; c++) rd2[c] = 0.0;
+      orc_exchange_pair(pi->rdata, rd2, pi->idata, pj->idata, xpar, dt);
+    }
+  }
+  for (int64_t i = 0; i < s->n; i++)
+    orc_exchange_apply(s->p[i].rdata, s->p[i].idata);
+  return 0;
 }
 
 int64_t orc_state_neighbor_count(const orc_state *s) {

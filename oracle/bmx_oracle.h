@@ -149,6 +149,15 @@ void orc_state_destroy(orc_state *s);
 int orc_state_evolve(orc_state *s, double dt, int nsteps, double *fc_out);
 /* t = 0 bond detection over the neighbor list (rlim = 1e-7) */
 int orc_state_init_bonds(orc_state *s);
+/* ParticleExchange (src/des/bmx_pc_xchng.cpp:13-257; kernels
+ * src/chemistry/bmx_chem_K.H:1951-2112): rebuild the list at the current
+ * positions, evaluate every row, apply the increments.  xpar = {mtA, mtB,
+ * mtC} (BMXChemistry::getExchangeParameters, bmx_chem.cpp:221-228). */
+int orc_state_exchange(orc_state *s, const double xpar[3], double dt);
+void orc_exchange_init(double *rdata, const int32_t *idata);
+void orc_exchange_pair(double *rdata1, double *rdata2, const int32_t *idata1,
+                       const int32_t *idata2, const double xpar[3], double dt);
+void orc_exchange_apply(double *rdata, const int32_t *idata);
 /* neighbor list used by the last step (CSR over particle indices) */
 int64_t orc_state_neighbor_count(const orc_state *s);
 void orc_state_get_neighbors(const orc_state *s, int64_t *offsets,

@@ -79,6 +79,9 @@ def lib() -> C.CDLL:
             "orc_state_destroy": (None, [vp]),
             "orc_state_evolve": (C.c_int, [vp, C.c_double, C.c_int, D]),
             "orc_state_init_bonds": (C.c_int, [vp]),
+            "orc_state_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
+            "orc_exchange_pair": (None, [P(C.c_double), P(C.c_double), P(I32), P(I32),
+                                         P(C.c_double), C.c_double]),
             "orc_state_neighbor_count": (I64, [vp]),
             "orc_state_get_neighbors": (None, [vp, P(I64), P(I32)]),
             "orc_state_step": (I64, [vp]),
@@ -244,12 +247,31 @@ class State:
     def init_bonds(self) -> int:
         return lib().orc_state_init_bonds(self.h)
 
+    def exchange(self, xpar, dt: float) -> None:
+        """ParticleExchange: rebuild at the current positions, exchange
+        between bonded segments, apply the increments (rdata 25..30)."""
+        xp = np.ascontiguousarray(xpar, np.float64)
+        rc = lib().orc_state_exchange(self.h, _d(xp), float(dt))
+        if rc != 0:
+            raise RuntimeError(f"orc_state_exchange failed: {rc}")
+
     def neighbors(self):
         tot = lib().orc_state_neighbor_count(self.h)
         off = np.zeros(self.n + 1, np.int64)
         idx = np.zeros(max(tot, 1), np.int32)
         lib().orc_state_get_neighbors(self.h, _i64(off), _i(idx))
         return off, idx[:tot]
+
+
+def exchange_pair(rd1, rd2, id1, id2, xpar, dt):
+    """evaluateExchange for one pair; rd1/rd2 (34 doubles) are updated."""
+    r1 = np.array(rd1, np.float64)  # copies: the caller's arrays stay as given
+    r2 = np.array(rd2, np.float64)
+    i1 = np.ascontiguousarray(id1, np.int32)
+    i2 = np.ascontiguousarray(id2, np.int32)
+    xp = np.ascontiguousarray(xpar, np.float64)
+    lib().orc_exchange_pair(_d(r1), _d(r2), _i(i1), _i(i2), _d(xp), float(dt))
+    return r1, r2
 
 
 def adjoint(recs, geom: Geom, fpar, mobility, dt, nsteps, rebuild_every, skin, target):

@@ -4,8 +4,10 @@ bmx_run reads a BMX input deck (exec/fungi, exec/wall_test,
 exec/nlev_large_test, copied under tests/golden/ref_decks), builds
 BMXParticleContainer on the finest level mesh with the reference's per-step
 stencil list, calls InitParticlesAscii, InitBonds at time 0 and
-EvolveParticles each step, and writes the particles back in the input
-format.  The same run on the oracle must give the same bits.
+EvolveParticles and ParticleExchange each step, and writes the particles
+back in the input format (which holds no species: those are compared by
+test_gpu_exchange.py).  The same run on the oracle must give the same
+bits.
 """
 import math
 import os
@@ -64,10 +66,18 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
     hood = 1.1 * (2 * (3 * vol / (4 * math.pi)) ** (1 / 3) + f("cell_force.neighbor_width"))
     recs = bmx.read_particles_ascii(os.path.join(os.path.dirname(path),
                                                  kv["particles.input_file"][0]))
+    # species start at fluid.init_conc_species (bmx_pc_init.cpp:87-91)
+    conc = [float(v) for v in kv.get("fluid.init_conc_species", [])][:3]
+    recs["rdata"][:, 25:25 + len(conc)] = conc
+    # chem_species.mass_transfer_A/B/C (bmx_chem.cpp:100-105)
+    xpar = [f("chem_species.mass_transfer_" + s, default=0.0) for s in "ABC"]
     g = oracle.make_geom(lo, hi, per, n_cell)
     st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
     st.init_bonds()
-    st.evolve(f("bmx.fixed_dt"), steps)
+    dt = f("bmx.fixed_dt")
+    for _ in range(steps):   # EvolveParticles, then ParticleExchange
+        st.evolve(dt, 1)
+        st.exchange(xpar, dt)
     got = bmx.read_particles_ascii(out)
     assert np.array_equal(got["pos"], recs["pos"])
     assert np.array_equal(got["rdata"][:, :23], recs["rdata"][:, :23])

@@ -1,0 +1,104 @@
+"""GPU ParticleExchange against the oracle.
+
+Species A, B, C (rdata 25..27) move between bonded segments at the
+transfer rates over the rows rebuilt at the current positions; each row
+adds into its own increments (rdata 28..30) in row order, then all rows
+apply them.  Only +, -, *, / are involved, rounded as written on both sides,
+so the species and increments must match bit for bit, with the reference's
+per-step stencil rows and with the Verlet rows.
+"""
+import numpy as np
+import pytest
+
+from boltzmannmfx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+XPAR = [2e-5 * 50, 2e-5 * 70, 2e-5 * 30]
+HOOD = 1.1 * (2 * 2e-4 + 1e-2)
+
+
+def mat(n=3000, nroots=30, seed=17):
+    col = W.c3_hyphae(n=n, nroots=nroots, seed=seed)
+    rng = np.random.default_rng(seed)
+    r = col.recs["rdata"]
+    r[:, 5] = r[:, 4] * 1e-4 * rng.uniform(0.5, 1.5, n)   # volumes
+    r[:, 25:28] = rng.uniform(0.0, 2e-5, (n, 3))
+    return col
+
+
+def pair(bmx, oracle, col, mode, K):
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=mode, rebuild_every=K, skin=col.skin,
+                            neighborhood=HOOD, max_neighbors=384)
+    grecs = col.recs.copy()
+    ctx.bind(grecs)
+    nb = ctx.bin_dims() if mode == bmx.NBR_VERLET else col.n_cell
+    orecs = col.recs.copy()
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, nb)
+    if mode == bmx.NBR_VERLET:
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_PAIR, col.skin, K)
+    else:
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, HOOD, 1)
+    return ctx, grecs, st, orecs
+
+
+@pytest.mark.parametrize("mode,K", [(1, 1), (0, 10)])
+def test_exchange_matches_oracle(bmx, oracle, gpu, mode, K):
+    col = mat()
+    ctx, grecs, st, orecs = pair(bmx, oracle, col, mode, K)
+    ctx.init_bonds()
+    assert st.init_bonds() == 0
+    for steps in (4, 3):
+        ctx.evolve(col.dt, steps)
+        st.evolve(col.dt, steps)
+        ctx.exchange(XPAR, 1.0)
+        st.exchange(XPAR, 1.0)
+    ctx.sync()
+    assert np.abs(orecs["rdata"][:, 28:31]).max() > 0
+    assert np.array_equal(grecs["rdata"][:, 25:31], orecs["rdata"][:, 25:31])
+    assert np.array_equal(grecs["pos"], orecs["pos"])
+    assert np.array_equal(grecs["idata"], orecs["idata"])
+    # the steps after an exchange go on from the same state
+    ctx.evolve(col.dt, 5)
+    st.evolve(col.dt, 5)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    ctx.close()
+
+
+def test_exchange_conserves_species_with_symmetric_rows(bmx, gpu):
+    """Verlet rows are symmetric and hold every bonded pair, so each flux
+    leaves one segment and enters the other: sum(vol * c) is conserved."""
+    col = mat(seed=23)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin,
+                            max_neighbors=384)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    ctx.init_bonds()
+    m0 = (recs["rdata"][:, 5:6] * recs["rdata"][:, 25:28]).sum(0)
+    ctx.exchange(XPAR, 1.0)
+    ctx.sync()
+    m1 = (recs["rdata"][:, 5:6] * recs["rdata"][:, 25:28]).sum(0)
+    assert np.abs(recs["rdata"][:, 28:31]).max() > 0
+    np.testing.assert_allclose(m1, m0, rtol=1e-13, atol=0)
+    ctx.close()
+
+
+def test_exchange_leaves_spheres_alone(bmx, gpu):
+    col = W.c1_small()
+    col.recs["rdata"][:, 25:31] = np.random.default_rng(1).uniform(0, 1e-5, (2000, 6))
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    ctx.exchange(XPAR, 1.0)
+    ctx.sync()
+    assert np.array_equal(recs["rdata"][:, 25:31], col.recs["rdata"][:, 25:31])
+    ctx.close()
