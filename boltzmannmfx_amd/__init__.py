@@ -111,6 +111,12 @@ class AdjointTiming(C.Structure):
                 ("segments", C.c_int32), ("pad", C.c_int32)]
 
 
+class CheckpointHeader(C.Structure):
+    _fields_ = [("version", C.c_int32), ("n_real", C.c_int32), ("n_int", C.c_int32),
+                ("finest_level", C.c_int32), ("n_particles", C.c_int64),
+                ("next_id", C.c_int64)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("xcd_remap", C.c_int32),
                 ("bin_factor_pct", C.c_int32), ("from_cache", C.c_int32),
@@ -173,6 +179,12 @@ def load_library() -> C.CDLL:
                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(AdjointTiming)]),
         "bmx_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
+        "bmx_checkpoint_info": (C.c_int, [C.c_char_p, P(CheckpointHeader)]),
+        "bmx_read_checkpoint": (C.c_int, [C.c_char_p, vp, C.c_int64, P(C.c_int64),
+                                          P(Layout), P(C.c_int32), C.c_int32,
+                                          P(C.c_int32), C.c_int32]),
+        "bmx_write_checkpoint": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout),
+                                           P(Geometry), C.c_int32, C.c_int64]),
         "bmx_autotune": (C.c_int, [vp, C.c_double, C.c_int32, C.c_char_p, P(Tuning)]),
         "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
                                                P(C.c_int64), P(Layout)]),
@@ -446,3 +458,91 @@ def generate_hyphae(n: int, nroots: int, seg_len: float, seg_rad: float,
 
 def mean_sphere_volume(rmin: float, rmax: float) -> float:
     return load_library().bmx_mean_sphere_volume(rmin, rmax)
+
+
+# --- particle checkpoints -----------------------------------------------------
+def checkpoint_info(path: str) -> dict:
+    """Header of a particle checkpoint directory (the "particles" part of a
+    BMX checkpoint)."""
+    h = CheckpointHeader()
+    _check(load_library().bmx_checkpoint_info(path.encode(), C.byref(h)))
+    return {k: getattr(h, k) for k, _ in h._fields_}
+
+
+def _legacy_maps():
+    """Component maps of the two older record layouts in the checkpoints the
+    reference ships, onto the current Particle<34,35> record.  Inferred from
+    the decoded files (field values and the record layout of
+    src/chemistry/bmx_chem.H:38-118), not from the older sources, which the
+    reference does not hold.
+
+    31 reals / 5 ints (exec/rst_test/chk20000, exec/nlev_large_test/dbg/
+    chk300000, spheres): reals = ellipsoid axes a, b, c, Euler angles psi,
+    theta, phi, area, vol, vel[3], w[3], f[3], tau[3], two rates, then the
+    9 species values; ints = the chemistry header only (all cells yeast).
+
+    33 reals / 33 ints (exec/fungi/chk357900, segments): the current reals
+    without bond_scale (species start at 24), the current ints without
+    fuse_tip and fix_site."""
+    r31 = [0, 0, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19,
+           -1, -1, -1, 20, 21, -1, -1] + list(range(22, 31))
+    i5 = [0, 1, 2, 3, 4]
+    r33 = list(range(0, 24)) + [-1] + list(range(24, 33))
+    i33 = list(range(0, 25)) + [-1] + list(range(25, 31)) + [-1, 31, 32]
+    return {(31, 5): (r31, i5), (33, 33): (r33, i33)}
+
+
+LEGACY_CHECKPOINT_MAPS = _legacy_maps()
+
+
+def read_checkpoint(path: str, real_map=None, int_map=None) -> np.ndarray:
+    """Particles of a checkpoint directory as Particle<34,35> records.  The
+    current layout (34 reals, 35 ints) is read as is; the older layouts of
+    the shipped checkpoints go through LEGACY_CHECKPOINT_MAPS, after which
+    bond_scale is 1 where the file had none and idata id/cpu are the
+    particle's id words, as InitParticlesAscii sets them."""
+    lib = load_library()
+    info = checkpoint_info(path)
+    legacy = False
+    if real_map is None and int_map is None and (info["n_real"], info["n_int"]) != (34, 35):
+        key = (info["n_real"], info["n_int"])
+        if key not in LEGACY_CHECKPOINT_MAPS:
+            raise BMXError(1, f"no component map for {key[0]} reals / {key[1]} ints")
+        real_map, int_map = LEGACY_CHECKPOINT_MAPS[key]
+        legacy = True
+    recs = np.zeros(max(info["n_particles"], 1), dtype=RECORD_DTYPE)
+    n = C.c_int64()
+    lay = lib.bmx_default_layout()
+
+    def arr(m):
+        if m is None:
+            return None, 0
+        a = np.ascontiguousarray(m, dtype=np.int32)
+        return a, a.size
+    rm, nr = arr(real_map)
+    im, ni = arr(int_map)
+    _check(lib.bmx_read_checkpoint(
+        path.encode(), recs.ctypes.data_as(C.c_void_p), len(recs), C.byref(n),
+        C.byref(lay), rm.ctypes.data_as(C.POINTER(C.c_int32)) if rm is not None else None,
+        nr, im.ctypes.data_as(C.POINTER(C.c_int32)) if im is not None else None, ni))
+    recs = recs[: n.value].copy()
+    if legacy:
+        rm_l = list(real_map)
+        if rm_l[R.bond_scale] < 0:
+            recs["rdata"][:, R.bond_scale] = 1.0
+        recs["idata"][:, I.id] = recs["id"]
+        recs["idata"][:, I.cpu] = recs["cpu"]
+    return recs
+
+
+def write_checkpoint(path: str, recs: np.ndarray, geometry: "Geometry", level: int = 0,
+                     next_id: Optional[int] = None) -> None:
+    """Writes recs as a Version_Two_Dot_One_double particle checkpoint with
+    one grid spanning geometry.n_cell at `level`."""
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    lib = load_library()
+    lay = lib.bmx_default_layout()
+    nid = int(next_id) if next_id is not None else (int(recs["id"].max()) + 1 if len(recs) else 1)
+    _check(lib.bmx_write_checkpoint(path.encode(), recs.ctypes.data_as(C.c_void_p),
+                                    len(recs), C.byref(lay), C.byref(geometry),
+                                    int(level), nid))

@@ -391,8 +391,8 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
     HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p,
                                c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
                                multi ? c->col[c->cs].p : nullptr,
-                               c->d_boxes.p, c->boxes[0], multi, fp, fc, vm,
-                               c->d_err.p, shape(c), s));
+                               c->d_boxes.p, c->boxes[0], multi, fp, fc,
+                               c->oid[c->cs].p, vm, c->d_err.p, shape(c), s));
   } else {
     HIPCHK(launch_force_general(dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd),
                                 static_arrays(c, c->cs), c->d_nbr.p,
@@ -420,8 +420,8 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
                    : nullptr;
   HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_nbr.p,
                              c->d_ncnt.p, c->ncap, c->n, nullptr, c->d_boxes.p,
-                             c->boxes[0], 0, fp, nullptr, vm, c->d_err.p,
-                             shape(c), c->stream));
+                             c->boxes[0], 0, fp, nullptr, nullptr, vm,
+                             c->d_err.p, shape(c), c->stream));
   c->cd = 1 - c->cd;
   c->step++;
   return BMX_OK;
@@ -1119,8 +1119,7 @@ bmx_status bmx_get_fc(bmx_ctx *c, double *out) {
   (void)hipSetDevice(c->device);
   DevBuf<double> tmp;
   HIPCHK(tmp.alloc(6 * (size_t)std::max<int64_t>(c->n, 1)));
-  HIPCHK(launch_export_fc(c->d_fc.p, c->ncap, c->oid[c->cs].p, c->n, tmp.p,
-                          c->stream));
+  HIPCHK(launch_export_fc(c->d_fc.p, c->ncap, c->n, tmp.p, c->stream));
   if (c->n)
     HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * 6 * c->n,
                           hipMemcpyDeviceToHost, c->stream));

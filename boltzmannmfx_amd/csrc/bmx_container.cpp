@@ -112,6 +112,28 @@ BMXParticleContainer::BMXParticleContainer(const bmx_geometry &geom,
   check(bmx_set_force_params(ctx_, fpar));
   check(bmx_set_geometry(ctx_, &geom));
   check(bmx_set_neighbor_params(ctx_, &nbp));
+  geom_ = geom;
+}
+
+void BMXParticleContainer::Checkpoint(const std::string &dir, int level) {
+  const std::vector<uint8_t> &recs = Records();
+  bmx_layout L = bmx_default_layout();
+  int64_t next_id = 1;
+  for (int64_t i = 0; i < n_; i++) {
+    int32_t id;
+    std::memcpy(&id, recs.data() + (size_t)i * L.stride_bytes + L.id_offset, sizeof id);
+    if ((int64_t)id + 1 > next_id) next_id = (int64_t)id + 1;
+  }
+  check(bmx_write_checkpoint(dir.c_str(), recs.data(), n_, &L, &geom_, level, next_id));
+}
+
+void BMXParticleContainer::Restart(const std::string &dir) {
+  int64_t n = 0;
+  bmx_layout L = bmx_default_layout();
+  check(bmx_read_checkpoint(dir.c_str(), nullptr, 0, &n, &L, nullptr, 0, nullptr, 0));
+  std::vector<uint8_t> recs((size_t)n * L.stride_bytes);
+  check(bmx_read_checkpoint(dir.c_str(), recs.data(), n, &n, &L, nullptr, 0, nullptr, 0));
+  InitParticles(recs.data(), n);
 }
 
 static bmx_neighbor_params stencil_params(const Deck &deck) {

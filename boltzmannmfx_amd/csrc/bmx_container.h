@@ -86,6 +86,11 @@ class BMXParticleContainer {
   // n steps without copying back in between.
   void EvolveSteps(double dt, int nsteps);
   void WriteAsciiFileForInit(const std::string &file);
+  // The particle part of a BMX checkpoint (pc->Checkpoint(name, "particles"),
+  // src/io/bmx_chk.cpp:156-159; pc->Restart, src/io/bmx_restart.cpp:104-107):
+  // dir is the "particles" directory, written at this container's level.
+  void Checkpoint(const std::string &dir, int level = 0);
+  void Restart(const std::string &dir);
 
   int64_t NumberOfParticles() const { return n_; }
   // Host copy of the records, brought up to date.
@@ -95,6 +100,7 @@ class BMXParticleContainer {
  private:
   void check(bmx_status st) const;
   bmx_ctx *ctx_ = nullptr;
+  bmx_geometry geom_{};
   double xpar_[3] = {0.0, 0.0, 0.0};
   double conc_[3] = {0.0, 0.0, 0.0};
   std::vector<uint8_t> host_;

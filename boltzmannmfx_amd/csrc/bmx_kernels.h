@@ -68,13 +68,16 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
                                   int32_t *stats, unsigned long long *total,
                                   hipStream_t s);
 
-// ---- force + overdamped update, fused.  fc (6*ncap) is written when
-// non-null; vmax[block] receives the block's max dt*|vel| when non-null.
+// ---- force + overdamped update, fused.  fc (6*ncap, original order:
+// fc[oid + c*ncap]) is written when non-null, with fc_oid the original
+// index of each sorted row; vmax[block] receives the block's max dt*|vel|
+// when non-null.
 hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
                                const int32_t *nbr, const int32_t *ncnt,
                                int64_t ncap, int64_t n, const int32_t *col,
                                const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc, double *vmax,
+                               ForceParams fp, double *fc,
+                               const int32_t *fc_oid, double *vmax,
                                int32_t *err, LaunchShape ls, hipStream_t s);
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *nbr, const int32_t *ncnt,
@@ -95,8 +98,8 @@ hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
                              hipStream_t s);
 
 // ---- exports to original order
-hipError_t launch_export_fc(const double *fc, int64_t ncap, const int32_t *oid,
-                            int64_t n, double *out, hipStream_t s);
+hipError_t launch_export_fc(const double *fc, int64_t ncap, int64_t n,
+                            double *out, hipStream_t s);
 hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,
                              double *out, hipStream_t s);
 hipError_t launch_export_orient(const double4 *rod, const int32_t *oid,

@@ -197,12 +197,12 @@ __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
   pos[1] = p.y;
   pos[2] = p.z;
   if (wrote_step) {
-    rd[rec::velx] = mob * fc[t];
-    rd[rec::vely] = mob * fc[t + ncap];
-    rd[rec::velz] = mob * fc[t + 2 * ncap];
-    rd[rec::wx] = fc[t + 3 * ncap];
-    rd[rec::wy] = fc[t + 4 * ncap];
-    rd[rec::wz] = fc[t + 5 * ncap];
+    rd[rec::velx] = mob * fc[o];  // fc is in original order
+    rd[rec::vely] = mob * fc[o + ncap];
+    rd[rec::velz] = mob * fc[o + 2 * ncap];
+    rd[rec::wx] = fc[o + 3 * ncap];
+    rd[rec::wy] = fc[o + 4 * ncap];
+    rd[rec::wz] = fc[o + 5 * ncap];
     id[rec::fuse_flag] = 0;
     id[rec::split_flag] = 0;
     id[rec::new_flag] = 0;
@@ -708,8 +708,8 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere(
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
     int64_t ncap, int64_t n, const int32_t *__restrict__ col,
     const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
-    double *__restrict__ fc, double *__restrict__ vmax,
-    int32_t *__restrict__ err, int remap) {
+    double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
+    double *__restrict__ vmax, int32_t *__restrict__ err, int remap) {
   int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   double v2 = 0.0;
   const bool active = t < n;
@@ -736,13 +736,14 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere(
     y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
     z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
     pos_out[t] = make_double4(x, y, z, p.w);
-    if (fc) {
-      fc[t] = a0;
-      fc[t + ncap] = a1;
-      fc[t + 2 * ncap] = a2;
-      fc[t + 3 * ncap] = 0.0;
-      fc[t + 4 * ncap] = 0.0;
-      fc[t + 5 * ncap] = 0.0;
+    if (fc) {  // original order, so a later rebuild does not reorder it
+      const int64_t o = fc_oid[t];
+      fc[o] = a0;
+      fc[o + ncap] = a1;
+      fc[o + 2 * ncap] = a2;
+      fc[o + 3 * ncap] = 0.0;
+      fc[o + 4 * ncap] = 0.0;
+      fc[o + 5 * ncap] = 0.0;
     }
     if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
     v2 = vx * vx + vy * vy + vz * vz;
@@ -838,9 +839,10 @@ __global__ __launch_bounds__(256) void k_force_general(
       out.trig[t] = tr_i;
     }
     out.topo[t] = ti;
-    if (fc) {
+    if (fc) {  // original order
+      const int64_t o = st.oid[t];
 #pragma unroll
-      for (int k = 0; k < 6; k++) fc[t + k * ncap] = acc[k];
+      for (int k = 0; k < 6; k++) fc[o + k * ncap] = acc[k];
     }
     bool fin = true;
 #pragma unroll
@@ -959,13 +961,12 @@ __global__ void k_skin_final(unsigned long long *__restrict__ slotmax,
 
 // ------------------------------------------------------------- exports ---
 
+// fc is kept in original order with row stride ncap
 __global__ void k_export_fc(const double *__restrict__ fc, int64_t ncap,
-                            const int32_t *__restrict__ oid, int64_t n,
-                            double *__restrict__ out) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t o = oid[t];
-  for (int c = 0; c < 6; c++) out[c * n + o] = fc[c * ncap + t];
+                            int64_t n, double *__restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < 6; c++) out[c * n + i] = fc[c * ncap + i];
 }
 __global__ void k_export_pos(const double4 *__restrict__ pos,
                              const int32_t *__restrict__ oid, int64_t n,
@@ -1151,18 +1152,19 @@ hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
                                const int32_t *nbr, const int32_t *ncnt,
                                int64_t ncap, int64_t n, const int32_t *col,
                                const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc, double *vmax,
+                               ForceParams fp, double *fc,
+                               const int32_t *fc_oid, double *vmax,
                                int32_t *err, LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
   if (multi)
     hipLaunchKernelGGL(k_force_sphere<true>, dim3(grid_for(n, b)), dim3(b), 0,
                        s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
-                       fp, fc, vmax, err, ls.remap);
+                       fp, fc, fc_oid, vmax, err, ls.remap);
   else
     hipLaunchKernelGGL(k_force_sphere<false>, dim3(grid_for(n, b)), dim3(b), 0,
                        s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
-                       fp, fc, vmax, err, ls.remap);
+                       fp, fc, fc_oid, vmax, err, ls.remap);
   return hipGetLastError();
 }
 
@@ -1213,11 +1215,11 @@ hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
   return hipGetLastError();
 }
 
-hipError_t launch_export_fc(const double *fc, int64_t ncap, const int32_t *oid,
-                            int64_t n, double *out, hipStream_t s) {
+hipError_t launch_export_fc(const double *fc, int64_t ncap, int64_t n,
+                            double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_export_fc, dim3(grid_for(n, 256)), dim3(256), 0, s, fc,
-                     ncap, oid, n, out);
+                     ncap, n, out);
   return hipGetLastError();
 }
 hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,

@@ -212,6 +212,36 @@ bmx_status bmx_adjoint(bmx_ctx *ctx, double dt, int32_t nsteps,
  * 31 reals.  bmx_sync_to_aos writes the species back. */
 bmx_status bmx_exchange(bmx_ctx *ctx, const double xpar[3], double dt);
 
+/* --- particle checkpoints ------------------------------------------------
+ * The "particles" directory of a BMX checkpoint, AMReX's particle
+ * checkpoint format as BMX writes it (pc->Checkpoint(name, "particles"),
+ * src/io/bmx_chk.cpp:156-159) and reads it on restart (pc->Restart,
+ * src/io/bmx_restart.cpp:104-107). */
+typedef struct {
+  int32_t version;      /* 20: Version_Two_Dot_Zero_double, 21: Two_Dot_One */
+  int32_t n_real;       /* real components after x, y, z */
+  int32_t n_int;        /* int components after the two id words */
+  int32_t finest_level;
+  int64_t n_particles;
+  int64_t next_id;
+} bmx_checkpoint_header;
+bmx_status bmx_checkpoint_info(const char *dir, bmx_checkpoint_header *out);
+/* Reads every particle of every level into records (NULL: only *n_out).
+ * real_map[k] / int_map[k] name the file component that fills rdata[k] /
+ * idata[k] (-1: left zero); NULL maps take component k for k (the file
+ * must then hold all of the record's components).  Maps let checkpoints of
+ * older record layouts restart. */
+bmx_status bmx_read_checkpoint(const char *dir, void *records, int64_t capacity,
+                               int64_t *n_out, const bmx_layout *layout,
+                               const int32_t *real_map, int32_t n_real_map,
+                               const int32_t *int_map, int32_t n_int_map);
+/* Writes Version_Two_Dot_One_double with all n_real / n_int components, one
+ * grid spanning geom->n_cell at the given level. */
+bmx_status bmx_write_checkpoint(const char *dir, const void *records, int64_t n,
+                                const bmx_layout *layout,
+                                const bmx_geometry *geom, int32_t level,
+                                int64_t next_id);
+
 /* --- inspection (original particle order) ------------------------------ */
 /* fc of the last step, 6*n doubles, component-major: the reference's
  * fc[i + c*ntot] (src/des/bmx_pc_interaction.cpp:244-249). */

@@ -133,3 +133,101 @@ def test_hyphae_generator(bmx):
         d[:, :2] -= L[:2] * np.round(d[:, :2] / L[:2])
         linked += np.linalg.norm(d, axis=1).min() < 1e-9
     assert linked == len(range(20, 2000, 97))
+
+
+# ---------------------------------------------------------- checkpoints --
+CKPT = os.path.join(GOLDEN, "ref_checkpoints")
+
+
+def test_shipped_checkpoint_headers(bmx):
+    a = bmx.checkpoint_info(os.path.join(CKPT, "rst_test_chk20000"))
+    assert (a["version"], a["n_real"], a["n_int"], a["finest_level"], a["n_particles"]) == \
+        (20, 31, 5, 1, 238)
+    b = bmx.checkpoint_info(os.path.join(CKPT, "fungi_chk357900"))
+    assert (b["version"], b["n_real"], b["n_int"], b["finest_level"], b["n_particles"]) == \
+        (21, 33, 33, 2, 410)
+
+
+def test_read_shipped_sphere_checkpoint(bmx):
+    recs = bmx.read_checkpoint(os.path.join(CKPT, "rst_test_chk20000"))
+    assert len(recs) == 238
+    r = recs["rdata"][:, 0]
+    assert r.min() > 4e-5 and r.max() < 7e-5
+    # a sphere's area and volume are those of its radius
+    np.testing.assert_allclose(recs["rdata"][:, 4], 4 * np.pi * r ** 2, rtol=1e-12)
+    np.testing.assert_allclose(recs["rdata"][:, 5], 4 / 3 * np.pi * r ** 3, rtol=1e-12)
+    assert (recs["idata"][:, 5] == bmx.YEAST).all()
+    assert (recs["idata"][:, 0:5] == [3, 0, 9, 0, 6]).all()     # chemistry header
+    ids = list(zip(recs["id"], recs["cpu"]))
+    assert len(set(ids)) == len(ids)
+    lo, hi = np.zeros(3), np.full(3, 0.0064)                    # the rst_test deck
+    assert ((recs["pos"] >= lo) & (recs["pos"] < hi)).all()
+
+
+def test_read_shipped_segment_checkpoint(bmx):
+    recs = bmx.read_checkpoint(os.path.join(CKPT, "fungi_chk357900"))
+    assert len(recs) == 410
+    assert (recs["idata"][:, 5] == bmx.FUNGI).all()
+    assert np.allclose(recs["rdata"][:, 0], 2.5e-4)
+    # segment lengths up to chem_species.max_seg_length (tips still growing)
+    cl = recs["rdata"][:, 1]
+    assert cl.min() > 0 and cl.max() <= 35e-4 and np.isclose(cl, 3e-3).sum() > 200
+    # the packed id words of Two_Dot_One decode to the ids the record holds
+    # in its own int components
+    assert (recs["idata"][:, 33] == recs["id"]).all()
+    assert (recs["idata"][:, 34] == recs["cpu"]).all()
+    # every bond names a segment of the file (id, cpu), and the bonds are
+    # reciprocal but for 3 of 999 (one-sided bonds the reference's own
+    # fusion left behind)
+    key = {(int(i), int(c)): k for k, (i, c) in enumerate(zip(recs["id"], recs["cpu"]))}
+    nb = recs["idata"][:, 6]
+    assert nb.max() <= 4 and (nb > 0).sum() == 410
+    bonds = recip = 0
+    for k in range(len(recs)):
+        me = (int(recs["id"][k]), int(recs["cpu"][k]))
+        for b in range(nb[k]):
+            j = key[(int(recs["idata"][k, 7 + b]), int(recs["idata"][k, 11 + b]))]
+            back = {(int(recs["idata"][j, 7 + c]), int(recs["idata"][j, 11 + c]))
+                    for c in range(nb[j])}
+            bonds += 1
+            recip += me in back
+    assert (bonds, recip) == (999, 996)
+
+
+def test_checkpoint_round_trip(bmx, tmp_path):
+    from boltzmannmfx_amd import workloads as W
+    col = W.c3_hyphae(n=500, nroots=5, seed=3)
+    recs = col.recs.copy()
+    recs["rdata"][:, 25:34] = np.random.default_rng(4).normal(size=(500, 9))
+    recs["idata"][:, 26:32] = -1
+    geom = col.geometry()
+    d = str(tmp_path / "particles")
+    bmx.write_checkpoint(d, recs, geom, level=2, next_id=777)
+    info = bmx.checkpoint_info(d)
+    assert (info["version"], info["n_real"], info["n_int"], info["finest_level"],
+            info["n_particles"], info["next_id"]) == (21, 34, 35, 2, 500, 777)
+    head = open(os.path.join(d, "Header")).read().split()
+    assert head[:3] == ["Version_Two_Dot_One_double", "3", "34"]
+    back = bmx.read_checkpoint(d)
+
+    def same(a, b):  # every field (the 4 tail padding bytes are not data)
+        return all(np.array_equal(a[f], b[f]) for f in a.dtype.names)
+    assert same(back, recs)
+    # a shipped checkpoint through the current layout and back
+    old = bmx.read_checkpoint(os.path.join(CKPT, "fungi_chk357900"))
+    d2 = str(tmp_path / "p2")
+    bmx.write_checkpoint(d2, old, geom)
+    assert same(bmx.read_checkpoint(d2), old)
+
+
+def test_checkpoint_errors(bmx, tmp_path):
+    with pytest.raises(bmx.BMXError):
+        bmx.checkpoint_info(str(tmp_path / "missing"))
+    d = tmp_path / "bad"
+    d.mkdir()
+    (d / "Header").write_text("Version_Nine\n3\n")
+    with pytest.raises(bmx.BMXError):
+        bmx.checkpoint_info(str(d))
+    # an older layout without a map
+    with pytest.raises(bmx.BMXError):
+        bmx.read_checkpoint(os.path.join(CKPT, "fungi_chk357900"), real_map=list(range(34)))
