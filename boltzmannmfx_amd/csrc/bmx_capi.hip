@@ -99,6 +99,7 @@ struct bmx_ctx {
   double crit = 0.0;
   double verlet_radius = 0.0;
   DevBuf<int32_t> d_keytmp, d_rank, d_count, d_start, d_scan;
+  bool count_clear = false;  // d_count holds zeros (the scan resets it)
   DevBuf<unsigned long long> d_permk;
   int32_t maxn = 96;
   DevBuf<int32_t> d_nbr, d_ncnt;
@@ -285,6 +286,7 @@ bmx_status setup_boxes(bmx_ctx *c) {
                         sizeof(BoxGeom) * c->ncol, hipMemcpyHostToDevice,
                         c->stream));
   HIPCHK(c->d_count.alloc(c->nbins + 1));
+  c->count_clear = false;
   HIPCHK(c->d_start.alloc(c->nbins + 1));
   HIPCHK(c->d_scan.alloc(scan_scratch_elems(c->nbins) + 1));
   c->have_list = false;
@@ -327,10 +329,12 @@ bmx_status rebuild(bmx_ctx *c) {
   const BoxGeom g0 = c->boxes[0];
   StaticArrays ss = static_arrays(c, c->cs), ds = static_arrays(c, 1 - c->cs);
   DynArrays sd = dyn_arrays(c, c->cd), dd = dyn_arrays(c, 1 - c->cd);
-  HIPCHK(launch_fill_i32(c->d_count.p, c->nbins + 1, 0, s));
+  // the scan leaves the counts at zero; fill only a fresh array
+  if (!c->count_clear) HIPCHK(launch_fill_i32(c->d_count.p, c->nbins + 1, 0, s));
   HIPCHK(launch_bin_count(sd.pos, ss.col, c->d_boxes.p, g0, multi, c->n,
                           c->d_keytmp.p, c->d_rank.p, c->d_count.p, s));
   HIPCHK(launch_scan(c->d_count.p, c->d_start.p, c->nbins, c->d_scan.p, s));
+  c->count_clear = true;
   HIPCHK(launch_bin_scatter(c->d_keytmp.p, c->d_rank.p, c->d_start.p, ss.oid,
                             c->n, c->d_permk.p, s));
   HIPCHK(launch_bin_sort(c->d_start.p, c->nbins, c->d_permk.p, c->d_err.p, s));

@@ -34,7 +34,7 @@ hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
                             int32_t *count, hipStream_t s);
 // exclusive scan of count[0..nb) into start[0..nb]; scratch >= scan_scratch(nb)
 int64_t scan_scratch_elems(int64_t nb);
-hipError_t launch_scan(const int32_t *count, int32_t *start, int64_t nb,
+hipError_t launch_scan(int32_t *count, int32_t *start, int64_t nb,
                        int32_t *scratch, hipStream_t s);
 hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
                               const int32_t *start, const int32_t *oid,

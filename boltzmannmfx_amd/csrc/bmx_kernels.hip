@@ -288,8 +288,10 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t s, int32_t *wsum,
   return pre + incl - s;
 }
 
+// The counts are consumed here: each is reset to zero once read, which
+// leaves the count array cleared for the next binning.
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(
-    const int32_t *__restrict__ in, int32_t *__restrict__ out,
+    int32_t *__restrict__ in, int32_t *__restrict__ out,
     int32_t *__restrict__ bsum, int64_t n) {
   __shared__ int32_t wsum[kScanThreads / kWave];
   int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -297,7 +299,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(
   int32_t s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    int32_t x = (base + i < n) ? in[base + i] : 0;
+    int32_t x = 0;
+    if (base + i < n) {
+      x = in[base + i];
+      in[base + i] = 0;
+    }
     v[i] = s;
     s += x;
   }
@@ -928,22 +934,23 @@ __global__ void k_init_bonds(DynArrays dy, StaticArrays st,
 // means a pair may have entered interaction range unlisted.  Stage 1: each
 // block takes the per-slot max over a chunk of force blocks into slotmax
 // (one atomic per slot per block, values >= 0 compare as integers).
+// Block (x, y) takes slot y over a chunk of kSkinChunk force blocks.
+constexpr int kSkinChunk = 4096;
 __global__ __launch_bounds__(256) void k_skin_partial(
     double *__restrict__ vpart, int32_t nslots, int32_t stride, int clear,
     unsigned long long *__restrict__ slotmax) {
   __shared__ double shd[4];
-  for (int sl = 0; sl < nslots; sl++) {
-    double m = 0.0;
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < stride;
-         b += (int64_t)gridDim.x * blockDim.x) {
-      double v = vpart[(int64_t)sl * stride + b];
-      m = v > m ? v : m;
-      if (clear) vpart[(int64_t)sl * stride + b] = 0.0;
-    }
-    m = block_max(m, shd);
-    __syncthreads();
-    if (threadIdx.x == 0 && m > 0.0) atomicMax(&slotmax[sl], dbits(m));
+  const int sl = blockIdx.y;
+  const int64_t b0 = (int64_t)blockIdx.x * kSkinChunk;
+  const int64_t b1 = b0 + kSkinChunk < stride ? b0 + kSkinChunk : stride;
+  double m = 0.0;
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    double v = vpart[(int64_t)sl * stride + b];
+    m = v > m ? v : m;
+    if (clear) vpart[(int64_t)sl * stride + b] = 0.0;
   }
+  m = block_max(m, shd);
+  if (threadIdx.x == 0 && m > 0.0) atomicMax(&slotmax[sl], dbits(m));
 }
 
 // Stage 2: sum the slot maxima, compare, reset the slot maxima.
@@ -1070,7 +1077,7 @@ int64_t scan_scratch_elems(int64_t nb) {
   return (nb + kScanTile - 1) / kScanTile + 1;
 }
 
-hipError_t launch_scan(const int32_t *count, int32_t *start, int64_t nb,
+hipError_t launch_scan(int32_t *count, int32_t *start, int64_t nb,
                        int32_t *scratch, hipStream_t s) {
   int64_t nt = (nb + kScanTile - 1) / kScanTile;
   if (nt < 1) nt = 1;
@@ -1205,11 +1212,10 @@ hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
                              double limit, int clear,
                              unsigned long long *slotmax, int32_t *err,
                              hipStream_t s) {
-  int blocks = (stride + 255) / 256;
-  if (blocks > 64) blocks = 64;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_skin_partial, dim3(blocks), dim3(256), 0, s, vpart,
-                     nslots, stride, clear, slotmax);
+  int chunks = (stride + kSkinChunk - 1) / kSkinChunk;
+  if (chunks < 1) chunks = 1;
+  hipLaunchKernelGGL(k_skin_partial, dim3(chunks, nslots > 0 ? nslots : 1),
+                     dim3(256), 0, s, vpart, nslots, stride, clear, slotmax);
   hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(64), 0, s, slotmax, nslots,
                      limit, err);
   return hipGetLastError();
