@@ -58,10 +58,12 @@ FPAR_NAMES = (
 )
 
 NBR_VERLET, NBR_STENCIL = 0, 1
+DEPOSIT_ONE_TO_ONE, DEPOSIT_TRILINEAR = 0, 1
 
 STATUS = {
     0: "OK", 1: "BAD_ARG", 2: "OOM", 3: "HIP", 4: "NAN", 5: "NBR_OVERFLOW",
     6: "SKIN", 7: "STATE", 8: "IO", 9: "BOND_OVERFLOW", 10: "NO_DEVICE",
+    11: "MESH", 12: "CHEM",
 }
 
 
@@ -179,6 +181,12 @@ def load_library() -> C.CDLL:
                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(AdjointTiming)]),
         "bmx_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
+        "bmx_calc_volume_fraction": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
+        "bmx_calc_txfr_fluid": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
+        "bmx_count_particles_per_cell": (C.c_int, [vp, vp, C.c_int32]),
+        "bmx_calc_txfr_particle": (C.c_int, [vp, P(C.c_double), vp, vp, C.c_double,
+                                             C.c_int32, C.c_int32, C.c_int32,
+                                             C.c_int32]),
         "bmx_checkpoint_info": (C.c_int, [C.c_char_p, P(CheckpointHeader)]),
         "bmx_read_checkpoint": (C.c_int, [C.c_char_p, vp, C.c_int64, P(C.c_int64),
                                           P(Layout), P(C.c_int32), C.c_int32,
@@ -209,6 +217,10 @@ def load_library() -> C.CDLL:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _is_device_tensor(a) -> bool:
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
 
 
 def _check(code: int) -> None:
@@ -266,6 +278,63 @@ class Context:
     def set_geometry(self, prob_lo, prob_hi, periodic, n_cell=(1, 1, 1)) -> None:
         g = make_geometry(prob_lo, prob_hi, periodic, n_cell)
         _check(self.lib.bmx_set_geometry(self.h, C.byref(g)))
+        self._n_cell = tuple(int(v) for v in n_cell)
+
+    # particle <-> mesh; host meshes are numpy arrays [comp, k, j, i], device
+    # meshes torch tensors on the context's device
+    def _mesh_out(self, ncomp: int, out):
+        nx, ny, nz = self._n_cell
+        if out is None:
+            out = np.zeros((ncomp, nz, ny, nx))
+        on_dev = int(_is_device_tensor(out))
+        assert (out.numel() if on_dev else out.size) == ncomp * nx * ny * nz
+        return out, on_dev
+
+    @staticmethod
+    def _ptr(a, on_dev: int):
+        if on_dev:
+            assert a.is_contiguous() and str(a.dtype) == "torch.float64"
+            return C.c_void_p(a.data_ptr())
+        assert a.flags.c_contiguous and a.dtype == np.float64
+        return a.ctypes.data_as(C.c_void_p)
+
+    def volume_fraction(self, scheme: int = DEPOSIT_ONE_TO_ONE, out=None):
+        """bmx_calc_volume_fraction: vf per cell, shape (nz, ny, nx)."""
+        out, dev = self._mesh_out(1, out)
+        _check(self.lib.bmx_calc_volume_fraction(self.h, int(scheme),
+                                                 self._ptr(out, dev), dev))
+        return out.reshape(out.shape[1:]) if out.ndim == 4 else out
+
+    def txfr_fluid(self, scheme: int = DEPOSIT_ONE_TO_ONE, out=None):
+        """bmx_calc_txfr_fluid: rdata[31..33] summed per cell, (3, nz, ny, nx)."""
+        out, dev = self._mesh_out(3, out)
+        _check(self.lib.bmx_calc_txfr_fluid(self.h, int(scheme), self._ptr(out, dev), dev))
+        return out
+
+    def count_per_cell(self, out=None):
+        out, dev = self._mesh_out(1, out)
+        _check(self.lib.bmx_count_particles_per_cell(self.h, self._ptr(out, dev), dev))
+        return out.reshape(out.shape[1:]) if out.ndim == 4 else out
+
+    def txfr_particle(self, chempar, x_k, vf, dt: float, nloop: int = 4,
+                      cnc_scheme: int = DEPOSIT_ONE_TO_ONE,
+                      vf_scheme: int = DEPOSIT_ONE_TO_ONE) -> None:
+        """bmx_calc_txfr_particle: x_k (3, nz+2, ny+2, nx+2) and vf
+        (nz+2, ny+2, nx+2) with one ghost layer; updates the records (read
+        back by sync())."""
+        cp = np.ascontiguousarray(chempar, dtype=np.float64)
+        assert cp.size == 10
+        dev = int(_is_device_tensor(x_k))
+        if not dev:
+            x_k = np.ascontiguousarray(x_k, dtype=np.float64)
+            vf = np.ascontiguousarray(vf, dtype=np.float64)
+        nx, ny, nz = self._n_cell
+        ng = (nx + 2) * (ny + 2) * (nz + 2)
+        assert (x_k.numel() if dev else x_k.size) == 3 * ng
+        assert (vf.numel() if dev else vf.size) == ng
+        _check(self.lib.bmx_calc_txfr_particle(
+            self.h, _dptr(cp), self._ptr(x_k, dev), self._ptr(vf, dev), float(dt),
+            int(nloop), int(cnc_scheme), int(vf_scheme), dev))
 
     def set_neighbor_params(self, mode=NBR_VERLET, rebuild_every=10, skin=0.0,
                             max_neighbors=0, bin_factor_pct=100,

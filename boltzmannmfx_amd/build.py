@@ -22,7 +22,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB_NAME = "libbmx_mi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
-SOURCES = ["bmx_kernels.hip", "bmx_capi.hip", "bmx_adjoint.hip", "bmx_exchange.hip"]
+SOURCES = ["bmx_kernels.hip", "bmx_capi.hip", "bmx_adjoint.hip", "bmx_exchange.hip",
+           "bmx_mesh.hip"]
 HOST_SOURCES = ["bmx_container.cpp", "bmx_io.cpp", "bmx_checkpoint.cpp"]
 
 

@@ -107,6 +107,9 @@ struct bmx_ctx {
   // order (loaded from the records at the first exchange)
   DevBuf<double4> d_chem, d_chem_inc;
   bool have_chem = false;
+  // particle <-> mesh coupling: bins over the level mesh, staged fields
+  DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan, d_mperm;
+  DevBuf<double> d_mout, d_min;
   int32_t n_real = 0;
   bool have_list = false;
   int64_t step = 0, last_build = -1;
@@ -716,6 +719,16 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_vpart.release();
   c->d_slotmax.release();
   c->d_ustats.release();
+  c->d_chem.release();
+  c->d_chem_inc.release();
+  c->d_mkey.release();
+  c->d_mrank.release();
+  c->d_mcount.release();
+  c->d_mstart.release();
+  c->d_mscan.release();
+  c->d_mperm.release();
+  c->d_mout.release();
+  c->d_min.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1561,4 +1574,186 @@ extern "C" bmx_status bmx_autotune(bmx_ctx *c, double dt, int32_t trial_steps,
     std::fclose(f);
   }
   return BMX_OK;
+}
+
+// ------------------------------------------------------ particle <-> mesh
+namespace {
+
+bmx_status mesh_geom(bmx_ctx *c, MeshGeom &g) {
+  if (c->ncol != 1)
+    return fail(BMX_ERR_STATE, "mesh coupling needs a single-colony context");
+  const bmx_geometry &G = c->colony_geom[0];
+  double dx[3];
+  int64_t ng = 1;
+  for (int d = 0; d < 3; d++) {
+    if (G.n_cell[d] <= 0)
+      return fail(BMX_ERR_BAD_ARG, "the geometry has no level mesh (n_cell)");
+    g.plo[d] = G.prob_lo[d];
+    dx[d] = (G.prob_hi[d] - G.prob_lo[d]) / (double)G.n_cell[d];
+    g.dxi[d] = 1.0 / dx[d];
+    g.n[d] = G.n_cell[d];
+    g.periodic[d] = G.periodic[d] ? 1 : 0;
+    ng *= (int64_t)G.n_cell[d] + 2;
+  }
+  if (ng > (int64_t)1 << 30)
+    return fail(BMX_ERR_BAD_ARG, "level mesh too large (> 2^30 cells)");
+  g.grid_vol = dx[0] * dx[1] * dx[2];
+  g.inv_cell_vol = 1.0 / (dx[0] * dx[1] * dx[2]);
+  return BMX_OK;
+}
+
+int64_t mesh_cells(const MeshGeom &g, int ghost) {
+  return ((int64_t)g.n[0] + 2 * ghost) * (g.n[1] + 2 * ghost) *
+         (g.n[2] + 2 * ghost);
+}
+
+// Flushes the device state into the record copy and bins the particles.
+bmx_status mesh_bin(bmx_ctx *c, const MeshGeom &g, int tri, int sorted) {
+  (void)hipSetDevice(c->device);
+  bmx_status st = pack_records(c);
+  if (st != BMX_OK) return st;
+  int64_t nb = 1;
+  for (int d = 0; d < 3; d++)
+    nb *= (tri && !g.periodic[d]) ? (int64_t)g.n[d] + 1 : g.n[d];
+  size_t cap = (size_t)c->ncap;
+  HIPCHK(c->d_mkey.alloc(cap));
+  HIPCHK(c->d_mrank.alloc(cap));
+  HIPCHK(c->d_mperm.alloc(cap));
+  HIPCHK(c->d_mcount.alloc((size_t)nb + 1));
+  HIPCHK(c->d_mstart.alloc((size_t)nb + 1));
+  HIPCHK(c->d_mscan.alloc((size_t)scan_scratch_elems(nb) + 1));
+  HIPCHK(launch_mesh_bin(c->d_aos.p, c->lay, c->n, g, tri, c->d_mkey.p,
+                         c->d_mrank.p, c->d_mcount.p, c->d_mstart.p,
+                         c->d_mscan.p, c->d_mperm.p, sorted, c->d_err.p,
+                         c->stream));
+  return BMX_OK;
+}
+
+// Reads and clears the error word after the mesh kernels.
+bmx_status mesh_flags(bmx_ctx *c) {
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(c->h_flags, c->d_err.p, sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int32_t err = c->h_flags[0];
+  if (err & ERRBIT_MESH)
+    return fail(BMX_ERR_MESH,
+                "a particle lies outside the level mesh, or one cell holds "
+                "more than 4096 particles");
+  if (err & ERRBIT_CHEM)
+    return fail(BMX_ERR_CHEM,
+                "negative concentration in the particle chemistry");
+  if (err)
+    return fail(BMX_ERR_STATE, "unexpected error bits before the mesh call");
+  return BMX_OK;
+}
+
+// One deposited field (ncomp components) into out (host or device).
+bmx_status mesh_deposit(bmx_ctx *c, int scheme, int comp0, int ncomp, int mode,
+                        double *out, int32_t on_device) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!out) return fail(BMX_ERR_BAD_ARG, "null output mesh");
+  if (scheme != BMX_DEPOSIT_ONE_TO_ONE && scheme != BMX_DEPOSIT_TRILINEAR)
+    return fail(BMX_ERR_BAD_ARG, "unknown deposition scheme");
+  MeshGeom g;
+  st = mesh_geom(c, g);
+  if (st != BMX_OK) return st;
+  const int tri = scheme == BMX_DEPOSIT_TRILINEAR ? 1 : 0;
+  st = mesh_bin(c, g, tri, mode != 2);
+  if (st != BMX_OK) return st;
+  const int64_t nc = mesh_cells(g, 0);
+  double *dst = out;
+  if (!on_device) {
+    HIPCHK(c->d_mout.alloc((size_t)(nc * ncomp)));
+    dst = c->d_mout.p;
+  }
+  HIPCHK(launch_deposit(c->d_aos.p, c->lay, g, tri, c->d_mstart.p, c->d_mperm.p,
+                        comp0, ncomp, mode, dst, c->stream));
+  if (!on_device)
+    HIPCHK(hipMemcpyAsync(out, dst, sizeof(double) * (size_t)(nc * ncomp),
+                          hipMemcpyDeviceToHost, c->stream));
+  return mesh_flags(c);
+}
+
+}  // namespace
+
+extern "C" bmx_status bmx_calc_volume_fraction(bmx_ctx *c, int32_t scheme,
+                                               double *vf, int32_t on_device) {
+  return mesh_deposit(c, scheme, rec::vol, 1, 1, vf, on_device);
+}
+
+extern "C" bmx_status bmx_calc_txfr_fluid(bmx_ctx *c, int32_t scheme,
+                                          double *x_rhs, int32_t on_device) {
+  if (c && c->bound && c->n_real < rec::first_data + 9)
+    return fail(BMX_ERR_BAD_ARG,
+                "the record layout has no fluid increments (rdata[31..33])");
+  return mesh_deposit(c, scheme, rec::first_data + 6, 3, 0, x_rhs, on_device);
+}
+
+extern "C" bmx_status bmx_count_particles_per_cell(bmx_ctx *c, double *npart,
+                                                   int32_t on_device) {
+  return mesh_deposit(c, BMX_DEPOSIT_ONE_TO_ONE, 0, 1, 2, npart, on_device);
+}
+
+extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10],
+                                             const double *x_k, const double *vf,
+                                             double dt, int32_t nloop,
+                                             int32_t cnc_scheme,
+                                             int32_t vf_scheme,
+                                             int32_t on_device) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!chempar || !x_k || !vf) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (nloop < 1 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "need nloop >= 1 and a finite dt");
+  for (int k = 0; k < 10; k++)
+    if (!std::isfinite(chempar[k]))
+      return fail(BMX_ERR_BAD_ARG, "chemistry parameters must be finite");
+  for (int32_t sc : {cnc_scheme, vf_scheme})
+    if (sc != BMX_DEPOSIT_ONE_TO_ONE && sc != BMX_DEPOSIT_TRILINEAR)
+      return fail(BMX_ERR_BAD_ARG, "unknown interpolation scheme");
+  if (c->n_real < rec::first_data + 9)
+    return fail(BMX_ERR_BAD_ARG,
+                "the record layout has no species fields (rdata[25..33])");
+  MeshGeom g;
+  st = mesh_geom(c, g);
+  if (st != BMX_OK) return st;
+  st = mesh_bin(c, g, 0, 0);  // particles per cell
+  if (st != BMX_OK) return st;
+  hipStream_t s = c->stream;
+  const int64_t ng = mesh_cells(g, 1);
+  const double *X = x_k, *V = vf;
+  if (!on_device) {
+    HIPCHK(c->d_min.alloc((size_t)(4 * ng)));
+    HIPCHK(hipMemcpyAsync(c->d_min.p, x_k, sizeof(double) * (size_t)(3 * ng),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->d_min.p + 3 * ng, vf, sizeof(double) * (size_t)ng,
+                          hipMemcpyHostToDevice, s));
+    X = c->d_min.p;
+    V = c->d_min.p + 3 * ng;
+  }
+  HIPCHK(launch_mesh_to_particle(c->d_aos.p, c->lay, c->n, g, c->d_mstart.p, X,
+                                 V, cnc_scheme, vf_scheme, chempar, dt, nloop,
+                                 c->d_err.p, s));
+  // the records now hold the new sizes and species: the sorted arrays take
+  // the sizes, the species are reloaded at the next exchange, and the rows
+  // are rebuilt at the next step for the new reaches
+  c->have_chem = false;
+  HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
+  HIPCHK(launch_refresh_sizes(c->d_aos.p, c->lay, c->n, c->oid[c->cs].p,
+                              c->type[c->cs].p, c->pos[c->cd].p,
+                              c->rod[c->cd].p, c->d_ustats.p, s));
+  unsigned long long hs[8];
+  HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
+  st = mesh_flags(c);  // synchronizes the stream
+  if (st != BMX_OK) return st;
+  if (hs[4] != 0)
+    return fail(BMX_ERR_CHEM, "growth left a negative radius or a segment "
+                              "length <= 0");
+  std::memcpy(&c->rmax, &hs[0], sizeof(double));
+  std::memcpy(&c->lmax, &hs[1], sizeof(double));
+  reset_graph(c);
+  return setup_boxes(c);
 }

@@ -10,8 +10,10 @@ namespace bmx {
 namespace rec {
 enum : int {
   radius = 0, c_length = 1, theta = 2, phi = 3, area = 4, vol = 5,
-  velx = 6, vely = 7, velz = 8, wx = 9, wy = 10, wz = 11, bond_scale = 24,
-  first_data = 25  // species A, B, C, then their exchange increments
+  velx = 6, vely = 7, velz = 8, wx = 9, wy = 10, wz = 11, gx = 18, gy = 19,
+  gz = 20, dadt = 21, dvdt = 22, bond_scale = 24,
+  first_data = 25  // species A, B, C, their exchange increments, then the
+                   // amounts taken from the fluid
 };
 enum : int {
   cell_type = 5, n_bnds = 6, seg1_id1 = 7, seg1_id2 = 11, site1 = 15,
@@ -29,6 +31,8 @@ enum : int32_t {
   ERRBIT_OVERFLOW = 2,
   ERRBIT_SKIN = 4,
   ERRBIT_BOND = 8,
+  ERRBIT_MESH = 16,  // particle outside the mesh, or an over-full cell
+  ERRBIT_CHEM = 32,  // negative concentration in the particle chemistry
 };
 
 // One periodic box with its bins.
@@ -80,6 +84,17 @@ struct DynArrays {     // rewritten every step (ping-pong)
   double4 *rod;        // c_length, theta, phi, area
   double4 *trig;       // cos(theta), sin(theta), cos(phi), sin(phi)
   Topo *topo;
+};
+
+// The level mesh of the particle <-> mesh coupling: n cells per axis,
+// dx = (phi - plo) / n and dxi = 1 / dx as AMReX's Geometry forms them.
+struct MeshGeom {
+  double plo[3];
+  double dxi[3];
+  double grid_vol;      // dx[0] * dx[1] * dx[2]
+  double inv_cell_vol;  // 1 / grid_vol
+  int32_t n[3];
+  int32_t periodic[3];
 };
 
 struct AosLayout {

@@ -1,7 +1,8 @@
 /*
- * bmx_math.h -- sin, cos and acos built from IEEE +, -, *, /, sqrt and
- * floor only, so that a host build and a gfx950 build of the same call give
- * the same bits (with floating-point contraction off).  The segment force
+ * bmx_math.h -- sin, cos, acos and pow built from IEEE +, -, *, /, sqrt,
+ * fma, floor, frexp and ldexp only (each exactly or correctly rounded), so
+ * that a host build and a gfx950 build of the same call give the same bits
+ * (with floating-point contraction off).  The segment force
  * law turns orientations (theta, phi) into axes with sin/cos and the update
  * turns axes back with acos (src/chemistry/bmx_cell_interaction_K.H:559-612,
  * src/des/bmx_pc_interaction.cpp:337-429); nearly parallel segments make the
@@ -138,5 +139,145 @@ BMX_MATH_FN double bmx_acos(double x) {
 
 /* c^3, the reference's pow(c_length, 3) (K.H:832-834) */
 BMX_MATH_FN double bmx_cube(double c) { return c * c * c; }
+
+/* ---- pow -----------------------------------------------------------------
+ * x^y for finite x > 0, as exp(y log x) carried in double-double arithmetic
+ * (a value hi + lo with |lo| <= ulp(hi)/2; Dekker/Knuth error-free sums and
+ * fma products, so every step is IEEE-rounded the same way on both sides).
+ * log: x = m 2^e with m in [1/sqrt2, sqrt2), s = (m-1)/(m+1),
+ * log m = 2 atanh s = 2 s sum_k s^2k/(2k+1) (k <= 20, |s^2| < 0.03).
+ * exp: t = k ln2 + r, |r| <= ln2/2, Taylor series to r^24.  The series
+ * coefficients are 1/(2k+1) and 1/n! split exactly into hi + lo.  The result
+ * is the double nearest the double-double value, so it agrees with a
+ * correctly rounded pow except in rare near-halfway cases. */
+typedef struct {
+  double hi, lo;
+} bmx_dd;
+
+BMX_MATH_FN bmx_dd bmx_dd_make(double hi, double lo) {
+  bmx_dd r;
+  r.hi = hi;
+  r.lo = lo;
+  return r;
+}
+/* a + b exactly, any magnitudes */
+BMX_MATH_FN bmx_dd bmx_dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return bmx_dd_make(s, e);
+}
+/* a + b exactly, |a| >= |b| */
+BMX_MATH_FN bmx_dd bmx_dd_fast_sum(double a, double b) {
+  double s = a + b;
+  double e = b - (s - a);
+  return bmx_dd_make(s, e);
+}
+BMX_MATH_FN bmx_dd bmx_dd_add(bmx_dd a, bmx_dd b) {
+  bmx_dd s = bmx_dd_two_sum(a.hi, b.hi);
+  bmx_dd t = bmx_dd_two_sum(a.lo, b.lo);
+  double e = s.lo + t.hi;
+  bmx_dd u = bmx_dd_fast_sum(s.hi, e);
+  e = u.lo + t.lo;
+  return bmx_dd_fast_sum(u.hi, e);
+}
+BMX_MATH_FN bmx_dd bmx_dd_mul(bmx_dd a, bmx_dd b) {
+  double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p);
+  double c = a.hi * b.lo;
+  double d = a.lo * b.hi;
+  e = e + (c + d);
+  return bmx_dd_fast_sum(p, e);
+}
+BMX_MATH_FN bmx_dd bmx_dd_div(bmx_dd a, bmx_dd b) {
+  double q1 = a.hi / b.hi;
+  bmx_dd r = bmx_dd_add(a, bmx_dd_mul(b, bmx_dd_make(-q1, 0.0)));
+  double q2 = r.hi / b.hi;
+  r = bmx_dd_add(r, bmx_dd_mul(b, bmx_dd_make(-q2, 0.0)));
+  double q3 = r.hi / b.hi;
+  bmx_dd q = bmx_dd_fast_sum(q1, q2);
+  return bmx_dd_add(q, bmx_dd_make(q3, 0.0));
+}
+
+#define BMX_LN2_HI 0x1.62e42fefa39efp-1
+#define BMX_LN2_LO 0x1.abc9e3b39803fp-56
+
+/* log x for finite x > 0 (normal range) */
+BMX_MATH_FN bmx_dd bmx_log_dd(double x) {
+  int ex;
+  double m = frexp(x, &ex); /* [0.5, 1) */
+  if (m < 0.70710678118654752) {
+    m = m * 2.0;
+    ex = ex - 1;
+  }
+  bmx_dd s = bmx_dd_div(bmx_dd_make(m - 1.0, 0.0), bmx_dd_two_sum(m, 1.0));
+  bmx_dd z = bmx_dd_mul(s, s);
+  bmx_dd p = bmx_dd_make(0.024390243902439025, -8.46206573647223e-19);
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02564102564102564, 8.896017825522087e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02702702702702703, -1.50030138462859e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02857142857142857, 8.921435019309293e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.030303030303030304, -8.410780489584519e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.03225806451612903, 8.953411488912552e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.034482758620689655, 4.785444071660157e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.037037037037037035, 2.05596856412066e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.04, -8.326672684688674e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.043478260869565216, 1.206764157201257e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.047619047619047616, 2.64338815386942e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.05263157894736842, 2.921639538487254e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.058823529411764705, 8.163404592832033e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.06666666666666667, 9.251858538542971e-19));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.07692307692307693, -4.270088556250602e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.09090909090909091, -2.523234146875356e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.1111111111111111, 6.1679056923619804e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.14285714285714285, 7.93016446160826e-18));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.2, -1.1102230246251566e-17));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.3333333333333333, 1.850371707708594e-17));
+  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(1.0, 0.0));
+  bmx_dd lm = bmx_dd_mul(bmx_dd_mul(s, p), bmx_dd_make(2.0, 0.0));
+  double k = (double)ex;
+  bmx_dd kl = bmx_dd_fast_sum(k * BMX_LN2_HI, fma(k, BMX_LN2_HI, -(k * BMX_LN2_HI)));
+  kl = bmx_dd_add(kl, bmx_dd_make(k * BMX_LN2_LO, 0.0));
+  return bmx_dd_add(kl, lm);
+}
+
+/* exp t for a double-double t with |t| < 700 */
+BMX_MATH_FN double bmx_exp_dd(bmx_dd t) {
+  double k = floor(t.hi * 1.4426950408889634 + 0.5);
+  bmx_dd kl = bmx_dd_two_sum(-(k * BMX_LN2_HI), -fma(k, BMX_LN2_HI, -(k * BMX_LN2_HI)));
+  kl = bmx_dd_add(kl, bmx_dd_make(-(k * BMX_LN2_LO), 0.0));
+  bmx_dd r = bmx_dd_add(t, kl);
+  bmx_dd e = bmx_dd_make(1.6117375710961184e-24, -3.6846573564509766e-41);
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(3.868170170630684e-23, -8.843177655482344e-40));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(8.896791392450574e-22, -7.911402614872376e-38));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.9572941063391263e-20, -1.3643503830087908e-36));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(4.110317623312165e-19, 1.4412973378659527e-36));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(8.22063524662433e-18, 2.2141894119604265e-34));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.5619206968586225e-16, 1.1910679660273754e-32));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.8114572543455206e-15, 1.6508842730861433e-31));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(4.779477332387385e-14, 4.399205485834081e-31));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(7.647163731819816e-13, 7.03872877733453e-30));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.1470745597729725e-11, 2.0655512752830745e-28));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.6059043836821613e-10, 1.2585294588752098e-26));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.08767569878681e-09, -1.20734505911326e-25));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.505210838544172e-08, -1.448814070935912e-24));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.755731922398589e-07, 2.3767714622250297e-23));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.7557319223985893e-06, -1.858393274046472e-22));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.48015873015873e-05, 2.1511947866775882e-23));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.0001984126984126984, 1.7209558293420705e-22));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.001388888888888889, -5.300543954373577e-20));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.008333333333333333, 1.1564823173178714e-19));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.041666666666666664, 2.3129646346357427e-18));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.16666666666666666, 9.25185853854297e-18));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.5, 0.0));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.0, 0.0));
+  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.0, 0.0));
+  return ldexp(e.hi + e.lo, (int)k);
+}
+
+BMX_MATH_FN double bmx_pow(double x, double y) {
+  if (!(x > 0.0) || x - x != 0.0) return (x - x) / (x - x); /* NaN */
+  if (x == 1.0 || y == 0.0) return 1.0;
+  return bmx_exp_dd(bmx_dd_mul(bmx_dd_make(y, 0.0), bmx_log_dd(x)));
+}
 
 #endif

@@ -40,7 +40,11 @@ typedef enum {
   BMX_ERR_STATE = 7,         /* call order (e.g. evolve before bind) */
   BMX_ERR_IO = 8,            /* file parse / write failure */
   BMX_ERR_BOND_OVERFLOW = 9, /* InitBonds found a fifth bond */
-  BMX_ERR_NO_DEVICE = 10     /* no HIP device / library not usable */
+  BMX_ERR_NO_DEVICE = 10,    /* no HIP device / library not usable */
+  BMX_ERR_MESH = 11,         /* a particle outside the level mesh, or more
+                                than 4096 particles in one mesh cell */
+  BMX_ERR_CHEM = 12          /* negative concentration in the particle
+                                chemistry (where the reference aborts) */
 } bmx_status;
 
 /* Where the fields of one particle record sit (bytes).  The default is
@@ -245,6 +249,48 @@ bmx_status bmx_write_checkpoint(const char *dir, const void *records, int64_t n,
 /* --- inspection (original particle order) ------------------------------ */
 /* fc of the last step, 6*n doubles, component-major: the reference's
  * fc[i + c*ntot] (src/des/bmx_pc_interaction.cpp:244-249). */
+/* --- particle <-> mesh coupling ------------------------------------------
+ * On the level mesh of bmx_set_geometry (n_cell; dx = (hi - lo) / n_cell).
+ * Meshes are arrays of doubles, x fastest: cell (i, j, k) of component m at
+ * m*N + i + n0*(j + n1*k), N = n0*n1*n2.  Meshes passed IN carry one ghost
+ * layer: (n0+2)(n1+2)(n2+2) values per component, cell (i, j, k) at
+ * (i+1) + (n0+2)*((j+1) + (n1+2)*(k+1)), ghosts filled by the caller
+ * (periodic images or boundary values, as FillBoundary and the species BCs
+ * leave them).  on_device != 0: the pointers are device memory of the
+ * context's device; otherwise host memory.  Single-colony contexts only.
+ * Sums over the particles of a cell run in ascending particle index. */
+enum { BMX_DEPOSIT_ONE_TO_ONE = 0, BMX_DEPOSIT_TRILINEAR = 1 };
+
+/* Volume fraction of the fluid: vf = max(1 - sum(vol)/(dx dy dz), 0.1) with
+ * vol = rdata[5] deposited one-to-one or trilinear (bmx::
+ * bmx_calc_volume_fraction, src/des/bmx_calc_volume_fraction.cpp:14-86,
+ * without the X_k rescaling).  vf: N values. */
+bmx_status bmx_calc_volume_fraction(bmx_ctx *ctx, int32_t scheme, double *vf,
+                                    int32_t on_device);
+/* Species taken from the fluid, rdata[31..33], summed per cell
+ * (bmx::bmx_calc_txfr_fluid, src/des/bmx_calc_txfr.cpp:22-66).  3N values. */
+bmx_status bmx_calc_txfr_fluid(bmx_ctx *ctx, int32_t scheme, double *x_rhs,
+                               int32_t on_device);
+/* Particles per cell (pc->Increment, src/des/bmx_calc_txfr.cpp:96-98). */
+bmx_status bmx_count_particles_per_cell(bmx_ctx *ctx, double *npart,
+                                        int32_t on_device);
+/* Mesh -> particle (bmx::bmx_calc_txfr_particle, src/des/bmx_calc_txfr.cpp:
+ * 72-396): the species X_k (3 ghosted components) and vf (1 ghosted) are
+ * interpolated to every particle (one-to-one or trilinear), the gradient of
+ * species A at its cell goes to rdata[18..20], and
+ * xferMeshToParticleAndUpdateChem (src/chemistry/bmx_chem_K.H:1358-1940)
+ * integrates uptake, reaction and growth over nloop substeps ("substeps",
+ * src/setup/bmx_init.cpp:85-86), updating radius, c_length, area, vol,
+ * dadt, dvdt and rdata[25..33].  chempar = {k1, k2, k3, kr1, kr2, kr3, kg,
+ * kv, radius_max, length_max} (BMXChemistry::getChemParams,
+ * src/chemistry/bmx_chem.cpp:203-216).  The next step rebuilds the rows for
+ * the new sizes.  BMX_ERR_CHEM where the reference's CPU build aborts
+ * (negative concentrations); the records are then unspecified. */
+bmx_status bmx_calc_txfr_particle(bmx_ctx *ctx, const double chempar[10],
+                                  const double *x_k, const double *vf,
+                                  double dt, int32_t nloop, int32_t cnc_scheme,
+                                  int32_t vf_scheme, int32_t on_device);
+
 bmx_status bmx_get_fc(bmx_ctx *ctx, double *out6n);
 /* positions, n*3 doubles (x,y,z per particle) */
 bmx_status bmx_get_positions(bmx_ctx *ctx, double *out3n);

@@ -1296,3 +1296,453 @@ int orc_adjoint(orc_particle *p, int64_t n, const orc_geom *g,
   orc_state_destroy(s);
   return 0;
 }
+
+/* ---- particle <-> mesh ----------------------------------------------------
+ * The level mesh is the orc_geom's bins (nbin = n_cell), dx = (phi-plo)/n,
+ * dxi = 1/dx as AMReX's Geometry forms them.  Output meshes hold the valid
+ * cells, x fastest, components slowest; input meshes carry one ghost layer
+ * ((n+2) per axis, cell i at i+1). */
+
+static void orc_mesh_dx(const orc_geom *g, double dx[3], double dxi[3]) {
+  for (int d = 0; d < 3; d++) {
+    dx[d] = (g->phi[d] - g->plo[d]) / (double)g->nbin[d];
+    dxi[d] = 1.0 / dx[d];
+  }
+}
+
+/* one-to-one cell floor((x - plo) * dxi) (bmx_deposition_K.H:78-85,
+ * bmx_interp_K.H:55-56); 0 when the particle lies outside the mesh */
+static int orc_mesh_cell(const double *x, const orc_geom *g, const double dxi[3],
+                         int c3[3]) {
+  for (int d = 0; d < 3; d++) {
+    double f = floor((x[d] - g->plo[d]) * dxi[d]);
+    if (!(f >= 0.0 && f < (double)g->nbin[d])) return 0;
+    c3[d] = (int)f;
+  }
+  return 1;
+}
+
+static int64_t orc_cell_lin(const orc_geom *g, int i, int j, int k) {
+  return ((int64_t)k * g->nbin[1] + j) * g->nbin[0] + i;
+}
+
+static int64_t orc_ghost_lin(const orc_geom *g, int i, int j, int k) {
+  return ((int64_t)(k + 1) * (g->nbin[1] + 2) + (j + 1)) * (g->nbin[0] + 2) +
+         (i + 1);
+}
+
+int64_t orc_deposit(const orc_particle *p, int64_t n, const orc_geom *g,
+                    int scheme, int comp0, int ncomp, double *out) {
+  /* ParticleToMesh with zero_out_input (bmx_calc_volume_fraction.cpp:23-38,
+   * bmx_calc_txfr.cpp:32-48): every particle, in order, adds into the mesh;
+   * trilinear deposits of periodic images wrap onto the valid cells and
+   * those beyond a non-periodic face are dropped. */
+  const int64_t nc = (int64_t)g->nbin[0] * g->nbin[1] * g->nbin[2];
+  double dx[3], dxi[3];
+  orc_mesh_dx(g, dx, dxi);
+  memset(out, 0, sizeof(double) * (size_t)(nc * ncomp));
+  int64_t outside = 0;
+  for (int64_t q = 0; q < n; q++) {
+    int c3[3];
+    if (!orc_mesh_cell(p[q].pos, g, dxi, c3)) {
+      outside++;
+      continue;
+    }
+    const double *rd = p[q].rdata;
+    if (scheme == 0) { /* OneToOneDeposition, bmx_deposition_K.H:64-90 */
+      int64_t c = orc_cell_lin(g, c3[0], c3[1], c3[2]);
+      for (int m = 0; m < ncomp; m++) out[m * nc + c] += rd[comp0 + m];
+      continue;
+    }
+    /* TrilinearDeposition, bmx_deposition_K.H:92-151 */
+    double lx = (p[q].pos[0] - g->plo[0]) * dxi[0] + 0.5;
+    double ly = (p[q].pos[1] - g->plo[1]) * dxi[1] + 0.5;
+    double lz = (p[q].pos[2] - g->plo[2]) * dxi[2] + 0.5;
+    int i = (int)floor(lx), j = (int)floor(ly), k = (int)floor(lz);
+    double xint = lx - i, yint = ly - j, zint = lz - k;
+    double wx[2] = {1. - xint, xint}, wy[2] = {1. - yint, yint},
+           wz[2] = {1. - zint, zint};
+    double w[2][2][2], total = 0.0;
+    for (int ii = 0; ii <= 1; ++ii)
+      for (int jj = 0; jj <= 1; ++jj)
+        for (int kk = 0; kk <= 1; ++kk) {
+          w[ii][jj][kk] = wx[ii] * wy[jj] * wz[kk];
+          total += w[ii][jj][kk];
+        }
+    for (int ii = 0; ii <= 1; ++ii)
+      for (int jj = 0; jj <= 1; ++jj)
+        for (int kk = 0; kk <= 1; ++kk) w[ii][jj][kk] /= total;
+    for (int kk = 0; kk <= 1; ++kk)
+      for (int jj = 0; jj <= 1; ++jj)
+        for (int ii = 0; ii <= 1; ++ii) {
+          int cc[3] = {i + ii - 1, j + jj - 1, k + kk - 1}, keep = 1;
+          for (int d = 0; d < 3; d++) {
+            if (cc[d] >= 0 && cc[d] < g->nbin[d]) continue;
+            if (g->periodic[d])
+              cc[d] = (cc[d] + g->nbin[d]) % g->nbin[d];
+            else
+              keep = 0;
+          }
+          if (!keep) continue;
+          int64_t c = orc_cell_lin(g, cc[0], cc[1], cc[2]);
+          for (int m = 0; m < ncomp; m++)
+            out[m * nc + c] += w[ii][jj][kk] * rd[comp0 + m];
+        }
+  }
+  return outside;
+}
+
+int64_t orc_volume_fraction(const orc_particle *p, int64_t n, const orc_geom *g,
+                            int scheme, double *vf) {
+  /* bmx_calc_volume_fraction.cpp:14-86: deposit vol, divide by the cell
+   * volume (ParticleToMesh's vol_weight: a multiply by 1/(dx dy dz)), then
+   * vf = 1 - vf and vf = max(vf, 0.1) */
+  int64_t outside = orc_deposit(p, n, g, scheme, ORC_R_VOL, 1, vf);
+  double dx[3], dxi[3];
+  orc_mesh_dx(g, dx, dxi);
+  const double inv = 1.0 / (dx[0] * dx[1] * dx[2]);
+  const int64_t nc = (int64_t)g->nbin[0] * g->nbin[1] * g->nbin[2];
+  for (int64_t c = 0; c < nc; c++) {
+    double v = vf[c] * inv;
+    v = v * -1.0;
+    v = v + 1.0;
+    vf[c] = v < 0.1 ? 0.1 : v;
+  }
+  return outside;
+}
+
+int64_t orc_count_per_cell(const orc_particle *p, int64_t n, const orc_geom *g,
+                           double *out) {
+  /* pc->Increment(temp_npart, lev) (bmx_calc_txfr.cpp:96-98) */
+  const int64_t nc = (int64_t)g->nbin[0] * g->nbin[1] * g->nbin[2];
+  double dx[3], dxi[3];
+  orc_mesh_dx(g, dx, dxi);
+  memset(out, 0, sizeof(double) * (size_t)nc);
+  int64_t outside = 0;
+  for (int64_t q = 0; q < n; q++) {
+    int c3[3];
+    if (!orc_mesh_cell(p[q].pos, g, dxi, c3)) {
+      outside++;
+      continue;
+    }
+    out[orc_cell_lin(g, c3[0], c3[1], c3[2])] += 1.0;
+  }
+  return outside;
+}
+
+/* trilinear_interp (bmx_interp_K.H:17-44) of ncomp components of a ghosted
+ * field */
+static void orc_trilinear_interp(const double *x, const double *f,
+                                 const orc_geom *g, const double dxi[3],
+                                 int ncomp, double *vals) {
+  const int64_t ng = (int64_t)(g->nbin[0] + 2) * (g->nbin[1] + 2) * (g->nbin[2] + 2);
+  double l[3], hi[3], lo[3];
+  int c[3];
+  for (int d = 0; d < 3; d++) {
+    l[d] = (x[d] - g->plo[d]) * dxi[d] + 0.5;
+    c[d] = (int)floor(l[d]);
+    hi[d] = l[d] - c[d];
+    lo[d] = 1 - hi[d];
+  }
+  const int i = c[0], j = c[1], k = c[2];
+  for (int m = 0; m < ncomp; m++) {
+    const double *a = f + m * ng;
+    vals[m] = lo[0] * lo[1] * lo[2] * a[orc_ghost_lin(g, i - 1, j - 1, k - 1)] +
+              lo[0] * lo[1] * hi[2] * a[orc_ghost_lin(g, i - 1, j - 1, k)] +
+              lo[0] * hi[1] * lo[2] * a[orc_ghost_lin(g, i - 1, j, k - 1)] +
+              lo[0] * hi[1] * hi[2] * a[orc_ghost_lin(g, i - 1, j, k)] +
+              hi[0] * lo[1] * lo[2] * a[orc_ghost_lin(g, i, j - 1, k - 1)] +
+              hi[0] * lo[1] * hi[2] * a[orc_ghost_lin(g, i, j - 1, k)] +
+              hi[0] * hi[1] * lo[2] * a[orc_ghost_lin(g, i, j, k - 1)] +
+              hi[0] * hi[1] * hi[2] * a[orc_ghost_lin(g, i, j, k)];
+  }
+}
+
+/* First and second uptake half-steps of one species
+ * (bmx_chem_K.H:1503-1583 and 1780-1851): the cell takes d from the fluid,
+ * undone and redone so that neither side goes negative.  The two forms
+ * differ when the fluid would go negative, as the reference writes them. */
+static void orc_uptake(double *c, double *f, double *d, double vol,
+                       double fluid_vol, int second) {
+  *c += *d / vol;
+  double df = *d / fluid_vol;
+  *f -= df;
+  if (*c < 0.0) {
+    *c -= *d / vol;
+    *f += df;
+    *d = -*c * vol;
+    *c = 0.0;
+    df = *d / fluid_vol;
+    *f -= df;
+  }
+  if (*f < 0.0) {
+    *c -= *d / vol;
+    *f += df;
+    if (!second) {
+      *d = *f * fluid_vol;
+      df = *d / fluid_vol;
+    } else {
+      *d = -*f * fluid_vol;
+      df = -*f;
+    }
+    *f = 0.0;
+    *c += *d / vol;
+  }
+}
+
+/* xferMeshToParticleAndUpdateChem (bmx_chem_K.H:1358-1940), with the
+ * checks of the reference's CPU build (the #ifndef AMREX_USE_GPU blocks).
+ * Returns error bits: 1 negative fluid concentration on entry, 2 after the
+ * first update, 4 after the second, 8 negative cell concentration. */
+int orc_chem_update(double grid_vol, double npart, double *cp, double *fm,
+                    double *pv, double dt, int nloop, const double *chempar,
+                    const int32_t *ip) {
+  int err = 0;
+  double k1 = chempar[0], k2 = chempar[1], k3 = chempar[2];
+  double kr1 = chempar[3], kr2 = chempar[4], kr3 = chempar[5];
+  double kg = chempar[6], kv = chempar[7];
+  double radius_max = chempar[8], length_max = chempar[9];
+  double fluid_vol = grid_vol / npart;
+  int cell_type = ip[ORC_I_CELL_TYPE];
+  int is_tip = ip[ORC_I_POSITION] == ORC_TIP;
+  if (cell_type == ORC_FUNGI && cp[ORC_R_RADIUS] >= radius_max &&
+      cp[ORC_R_CLENGTH] >= length_max) {
+    kg = 0.0;
+    kv = 0.0;
+  }
+  double orig_cell_vol = cp[ORC_R_VOL];
+  double f[3] = {fm[0], fm[1], fm[2]};
+  for (int s = 0; s < 3; s++)
+    if (f[s] < 0.0 && fabs(f[s]) < 1.0e-15) f[s] = 0.0;
+  if (f[0] < 0. || f[1] < 0. || f[2] < 0.) return 1;
+  double c[3] = {pv[0], pv[1], pv[2]};
+  double dtp = dt / (double)nloop;
+  pv[3] = c[0];
+  pv[4] = c[1];
+  pv[5] = c[2];
+  pv[6] = 0.0;
+  pv[7] = 0.0;
+  pv[8] = 0.0;
+  double new_cell_vol = cp[ORC_R_VOL];
+  double new_cell_area = cp[ORC_R_AREA];
+  int is_negative = 0;
+  for (int it = 0; it < nloop; it++) {
+    double d1[3];
+    d1[0] = 0.5 * dtp * new_cell_area * (k1 * f[0] - kr1 * c[0]);
+    d1[1] = 0.0;
+    d1[2] = 0.5 * dtp * new_cell_area * (k3 * f[2] - kr3 * c[2]);
+    for (int s = 0; s < 3; s++)
+      orc_uptake(&c[s], &f[s], &d1[s], orig_cell_vol, fluid_vol, 0);
+    for (int s = 0; s < 3; s++)
+      if (c[s] < 0.0 && fabs(c[s]) < 1.0e-15) c[s] = 0.0;
+    for (int s = 0; s < 3; s++)
+      if (c[s] < 0.0) is_negative = 1;
+    if (f[0] < 0. || f[1] < 0. || f[2] < 0.) return err | 2;
+
+    /* reaction A <-> B + C (:1620-1672) */
+    double rA = -k2 * c[0] + kr2 * c[1] * c[2];
+    double rB = k2 * c[0] - kr2 * c[1] * c[2];
+    double rC = k2 * c[0] - kr2 * c[1] * c[2];
+    double dA3 = dtp * rA, dB3 = dtp * rB, dC3 = dtp * rC;
+    c[0] += dA3;
+    c[1] += dB3;
+    c[2] += dC3;
+    if (c[0] < 0.0) {
+      c[0] -= dA3;
+      dA3 = -c[0];
+      dB3 = -dA3;
+      dC3 = -dA3;
+      c[0] += dA3;
+      c[1] += dB3;
+      c[2] += dC3;
+    }
+    if (c[1] < 0.0) {
+      c[1] -= dB3;
+      dB3 = -c[1];
+      dA3 = -dB3;
+      dC3 = dB3;
+      c[0] += dA3;
+      c[1] += dB3;
+      c[2] += dC3;
+    }
+    if (c[2] < 0.0) {
+      c[2] -= dC3;
+      dC3 = -c[2];
+      dA3 = -dC3;
+      dB3 = dC3;
+      c[0] += dA3;
+      c[1] += dB3;
+      c[2] += dC3;
+    }
+
+    /* growth consumes B (:1674-1686) */
+    double rV = kv * c[1] * orig_cell_vol;
+    rB = -kg * c[1];
+    dB3 = dtp * rB;
+    c[1] += dB3;
+    double dV = dtp * rV;
+    if (c[1] < 0.0) {
+      c[1] -= dB3;
+      double ttmp = -c[1] / rB;
+      dV = ttmp * rV;
+      c[1] = 0.0;
+    }
+    for (int s = 0; s < 3; s++)
+      if (c[s] < 0.0) is_negative = 1;
+
+    /* sizes (:1697-1778) */
+    if (cell_type == ORC_YEAST) {
+      double dvdt = rV;
+      cp[ORC_R_DVDT] = dvdt;
+      new_cell_vol += dV;
+      cp[ORC_R_VOL] = new_cell_vol;
+      double radius = bmx_pow((3.0 * new_cell_vol / (4.0 * BMX_PI)), 1.0 / 3.0);
+      cp[ORC_R_AREA] = 4.0 * BMX_PI * radius * radius;
+      new_cell_area = cp[ORC_R_AREA];
+      cp[ORC_R_DADT] = 2.0 * dvdt / radius;
+      cp[ORC_R_RADIUS] = radius;
+    } else if (cell_type == ORC_FUNGI) {
+      if (is_tip) {
+        cp[ORC_R_DVDT] = rV;
+        new_cell_vol += dV;
+        cp[ORC_R_VOL] = new_cell_vol;
+        double radius, c_length, tau;
+        if (cp[ORC_R_RADIUS] < radius_max && cp[ORC_R_CLENGTH] < length_max) {
+          tau = bmx_pow(new_cell_vol / orig_cell_vol, 1.0 / 3.0);
+          radius = tau * cp[ORC_R_RADIUS];
+          c_length = tau * cp[ORC_R_CLENGTH];
+          double dtdt = 1.0 / (3.0 * bmx_pow(new_cell_vol, 2.0 / 3.0)) * rV /
+                        bmx_pow(orig_cell_vol, 1.0 / 3.0);
+          double drdt = dtdt * cp[ORC_R_RADIUS];
+          double dldt = dtdt * cp[ORC_R_CLENGTH];
+          cp[ORC_R_DADT] = 2.0 * BMX_PI *
+                           (drdt * (radius + c_length) + radius * (drdt + dldt));
+          if (ip[ORC_I_N_BNDS] == 0)
+            cp[ORC_R_AREA] = 2.0 * BMX_PI * radius * (radius + c_length);
+          else
+            cp[ORC_R_AREA] = BMX_PI * radius * (radius + 2.0 * c_length);
+        } else if (cp[ORC_R_RADIUS] < radius_max) {
+          tau = sqrt(new_cell_vol / orig_cell_vol);
+          radius = tau * cp[ORC_R_RADIUS];
+          c_length = cp[ORC_R_CLENGTH];
+          double dtdt = 1.0 / (2.0 * sqrt(new_cell_vol)) * rV / sqrt(orig_cell_vol);
+          double drdt = dtdt * cp[ORC_R_RADIUS];
+          cp[ORC_R_DADT] = 2.0 * BMX_PI * (2.0 * radius + c_length) * drdt;
+        } else {
+          tau = new_cell_vol / orig_cell_vol;
+          radius = cp[ORC_R_RADIUS];
+          c_length = tau * cp[ORC_R_CLENGTH];
+          double dldt = rV * cp[ORC_R_CLENGTH] / orig_cell_vol;
+          cp[ORC_R_DADT] = 2.0 * BMX_PI * radius * dldt;
+        }
+        cp[ORC_R_RADIUS] = radius;
+        cp[ORC_R_CLENGTH] = c_length;
+      } else if (cp[ORC_R_RADIUS] < radius_max) {
+        cp[ORC_R_DVDT] = rV;
+        new_cell_vol += dV;
+        cp[ORC_R_VOL] = new_cell_vol;
+        double tau = sqrt(new_cell_vol / orig_cell_vol);
+        double radius = tau * cp[ORC_R_RADIUS];
+        double c_length = cp[ORC_R_CLENGTH];
+        double dtdt = 1.0 / (2.0 * sqrt(new_cell_vol)) * rV / sqrt(orig_cell_vol);
+        double drdt = dtdt * cp[ORC_R_RADIUS];
+        cp[ORC_R_RADIUS] = radius;
+        cp[ORC_R_CLENGTH] = c_length;
+        cp[ORC_R_AREA] = 2.0 * BMX_PI * radius * c_length;
+        cp[ORC_R_DADT] = 2.0 * BMX_PI * (2.0 * radius + c_length) * drdt;
+      } else {
+        cp[ORC_R_DVDT] = 0.0;
+        cp[ORC_R_VOL] = orig_cell_vol;
+        cp[ORC_R_DADT] = 0.0;
+      }
+    }
+
+    double ratio = orig_cell_vol / new_cell_vol;
+    for (int s = 0; s < 3; s++) c[s] *= ratio;
+    double d2[3];
+    d2[0] = 0.5 * dtp * new_cell_area * (k1 * f[0] - kr1 * c[0]);
+    d2[1] = 0.0;
+    d2[2] = 0.5 * dtp * new_cell_area * (k3 * f[2] - kr3 * c[2]);
+    for (int s = 0; s < 3; s++)
+      orc_uptake(&c[s], &f[s], &d2[s], new_cell_vol, fluid_vol, 1);
+    for (int s = 0; s < 3; s++)
+      if (c[s] < 0.0 && fabs(c[s]) < 1.0e-15) c[s] = 0.0;
+    for (int s = 0; s < 3; s++)
+      if (c[s] < 0.0) is_negative = 1;
+    if (f[0] < 0. || f[1] < 0. || f[2] < 0.) return err | 4;
+    for (int s = 0; s < 3; s++) pv[6 + s] += -(d1[s] + d2[s]);
+    orig_cell_vol = new_cell_vol;
+  }
+  if (is_negative) err |= 8;
+  pv[3] = c[0];
+  pv[4] = c[1];
+  pv[5] = c[2];
+  pv[0] = pv[3];
+  pv[1] = pv[4];
+  pv[2] = pv[5];
+  return err;
+}
+
+int orc_mesh_to_particle(orc_particle *p, int64_t n, const orc_geom *g,
+                         const double *X, const double *vf,
+                         const double chempar[10], double dt, int nloop,
+                         int cnc_scheme, int vf_scheme) {
+  /* bmx_calc_txfr_particle (bmx_calc_txfr.cpp:72-396) on one level:
+   * particles per cell, the gradient of species A at the particle's cell
+   * (compute_grad_X, src/bmx.cpp:171-195), interpolation of the species and
+   * of vf, then the chemistry of every particle.  X holds 3 components with
+   * one ghost layer, vf one.  Error bits as orc_chem_update, plus 16 for a
+   * particle outside the mesh. */
+  const int64_t nc = (int64_t)g->nbin[0] * g->nbin[1] * g->nbin[2];
+  const int64_t ng = (int64_t)(g->nbin[0] + 2) * (g->nbin[1] + 2) * (g->nbin[2] + 2);
+  double *npart = (double *)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+  if (!npart) return -1;
+  int err = 0;
+  if (orc_count_per_cell(p, n, g, npart) != 0) err |= 16;
+  double dx[3], dxi[3];
+  orc_mesh_dx(g, dx, dxi);
+  const double grid_vol = dx[0] * dx[1] * dx[2];
+  for (int64_t q = 0; q < n && !(err & 16); q++) {
+    int c3[3];
+    orc_mesh_cell(p[q].pos, g, dxi, c3);
+    const int i = c3[0], j = c3[1], k = c3[2];
+    double xv[3], vv;
+    if (cnc_scheme == 0)
+      for (int m = 0; m < 3; m++) xv[m] = X[m * ng + orc_ghost_lin(g, i, j, k)];
+    else
+      orc_trilinear_interp(p[q].pos, X, g, dxi, 3, xv);
+    if (vf_scheme == 0)
+      vv = vf[orc_ghost_lin(g, i, j, k)];
+    else
+      orc_trilinear_interp(p[q].pos, vf, g, dxi, 1, &vv);
+    double np = npart[orc_cell_lin(g, i, j, k)];
+    double *cp = p[q].rdata;
+    cp[ORC_R_GX] = (X[orc_ghost_lin(g, i + 1, j, k)] - X[orc_ghost_lin(g, i - 1, j, k)]) *
+                   0.5 * dxi[0];
+    cp[ORC_R_GY] = (X[orc_ghost_lin(g, i, j + 1, k)] - X[orc_ghost_lin(g, i, j - 1, k)]) *
+                   0.5 * dxi[1];
+    cp[ORC_R_GZ] = (X[orc_ghost_lin(g, i, j, k + 1)] - X[orc_ghost_lin(g, i, j, k - 1)]) *
+                   0.5 * dxi[2];
+    err |= orc_chem_update(grid_vol * vv, np, cp, xv, cp + ORC_R_FIRST_DATA, dt,
+                           nloop, chempar, p[q].idata);
+  }
+  free(npart);
+  return err;
+}
+
+void orc_interp(const double *x, const double *f, const orc_geom *g, int scheme,
+                int ncomp, double *vals) {
+  /* one_to_one_interp / trilinear_interp (bmx_interp_K.H:17-62) */
+  double dx[3], dxi[3];
+  orc_mesh_dx(g, dx, dxi);
+  if (scheme == 1) {
+    orc_trilinear_interp(x, f, g, dxi, ncomp, vals);
+    return;
+  }
+  const int64_t ng = (int64_t)(g->nbin[0] + 2) * (g->nbin[1] + 2) * (g->nbin[2] + 2);
+  int c[3];
+  for (int d = 0; d < 3; d++) c[d] = (int)floor((x[d] - g->plo[d]) * dxi[d]);
+  for (int m = 0; m < ncomp; m++) vals[m] = f[m * ng + orc_ghost_lin(g, c[0], c[1], c[2])];
+}
+
+double orc_pow(double x, double y) { return bmx_pow(x, y); }

@@ -20,6 +20,11 @@
  *   orc_bin_index                 src/interpolation/bmx_interp_K.H:55-56
  *   orc_build_neighbors           AMReX NeighborList over the +-1 bin stencil
  *                                 (un-vendored; semantics defined here)
+ *   orc_state_exchange            src/des/bmx_pc_xchng.cpp:13-257
+ *   orc_deposit                   src/deposition/bmx_deposition_K.H:64-151
+ *   orc_volume_fraction           src/des/bmx_calc_volume_fraction.cpp:14-86
+ *   orc_chem_update               src/chemistry/bmx_chem_K.H:1358-1940
+ *   orc_mesh_to_particle          src/des/bmx_calc_txfr.cpp:72-396
  *
  * Every function is compiled with -ffp-contract=off so that each floating
  * point operation is rounded exactly as written, in the reference's order.
@@ -51,7 +56,8 @@ enum {
   ORC_R_RADIUS = 0, ORC_R_CLENGTH = 1, ORC_R_THETA = 2, ORC_R_PHI = 3,
   ORC_R_AREA = 4, ORC_R_VOL = 5, ORC_R_VELX = 6, ORC_R_VELY = 7,
   ORC_R_VELZ = 8, ORC_R_WX = 9, ORC_R_WY = 10, ORC_R_WZ = 11,
-  ORC_R_BOND_SCALE = 24, ORC_R_FIRST_DATA = 25
+  ORC_R_GX = 18, ORC_R_GY = 19, ORC_R_GZ = 20, ORC_R_DADT = 21,
+  ORC_R_DVDT = 22, ORC_R_BOND_SCALE = 24, ORC_R_FIRST_DATA = 25
 };
 /* intIdx (src/chemistry/bmx_chem.H:77-118) */
 enum {
@@ -175,6 +181,28 @@ int orc_adjoint(orc_particle *p, int64_t n, const orc_geom *g,
                 double *loss, double grad[3], double *grad_r);
 /* 0: sin, 1: cos, 2: acos of the shared implementation (bmx_math.h) */
 double orc_math(int which, double x);
+/* ---- particle <-> mesh (one level; the orc_geom's nbin is the mesh) ----
+ * Output meshes: valid cells, x fastest, components slowest.  Input meshes:
+ * one ghost layer, (n+2) per axis.  Particles outside the mesh are counted
+ * (return value) and left out.  scheme 0: one-to-one, 1: trilinear. */
+int64_t orc_deposit(const orc_particle *p, int64_t n, const orc_geom *g,
+                    int scheme, int comp0, int ncomp, double *out);
+int64_t orc_volume_fraction(const orc_particle *p, int64_t n, const orc_geom *g,
+                            int scheme, double *vf);
+int64_t orc_count_per_cell(const orc_particle *p, int64_t n, const orc_geom *g,
+                           double *out);
+int orc_chem_update(double grid_vol, double npart, double *cell_par,
+                    double *mesh_vals, double *p_vals, double dt, int nloop,
+                    const double *chempar, const int32_t *cell_ipar);
+int orc_mesh_to_particle(orc_particle *p, int64_t n, const orc_geom *g,
+                         const double *X, const double *vf,
+                         const double chempar[10], double dt, int nloop,
+                         int cnc_scheme, int vf_scheme);
+/* interpolation of ncomp components of a ghosted field at x */
+void orc_interp(const double *x, const double *f, const orc_geom *g, int scheme,
+                int ncomp, double *vals);
+/* x^y of the shared implementation (bmx_math.h) */
+double orc_pow(double x, double y);
 /* OpenMP threads the force pass uses (1 without OpenMP) */
 int orc_num_threads(void);
 void orc_set_num_threads(int n);

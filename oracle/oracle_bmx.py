@@ -92,6 +92,15 @@ def lib() -> C.CDLL:
             "orc_adjoint": (C.c_int, [vp, I64, P(Geom), D, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_double, D, D, D, D]),
             "orc_set_num_threads": (None, [C.c_int]),
+            "orc_deposit": (I64, [vp, I64, P(Geom), C.c_int, C.c_int, C.c_int, D]),
+            "orc_volume_fraction": (I64, [vp, I64, P(Geom), C.c_int, D]),
+            "orc_count_per_cell": (I64, [vp, I64, P(Geom), D]),
+            "orc_chem_update": (C.c_int, [C.c_double, C.c_double, D, D, D, C.c_double,
+                                          C.c_int, D, P(I32)]),
+            "orc_mesh_to_particle": (C.c_int, [vp, I64, P(Geom), D, D, D, C.c_double,
+                                               C.c_int, C.c_int, C.c_int]),
+            "orc_pow": (C.c_double, [C.c_double, C.c_double]),
+            "orc_interp": (None, [D, D, P(Geom), C.c_int, C.c_int, D]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -290,3 +299,81 @@ def adjoint(recs, geom: Geom, fpar, mobility, dt, nsteps, rebuild_every, skin, t
     if rc != 0:
         raise RuntimeError(f"orc_adjoint failed: {rc}")
     return loss[0], grad, gr
+
+
+# ---- particle <-> mesh ------------------------------------------------------
+# Meshes are numpy arrays indexed [comp, k, j, i] (x fastest in memory).
+# Output meshes hold the valid cells; input meshes one ghost layer.
+DEPOSIT_ONE_TO_ONE, DEPOSIT_TRILINEAR = 0, 1
+
+
+def _mesh_shape(geom: Geom, ghost: int = 0):
+    return tuple(int(geom.nbin[d]) + 2 * ghost for d in (2, 1, 0))
+
+
+def deposit(recs, geom: Geom, scheme: int, comp0: int, ncomp: int):
+    """ParticleToMesh of rdata[comp0:comp0+ncomp]; (mesh, particles outside)."""
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    out = np.zeros((ncomp,) + _mesh_shape(geom))
+    k = lib().orc_deposit(recs.ctypes.data_as(C.c_void_p), len(recs), C.byref(geom),
+                          scheme, comp0, ncomp, _d(out))
+    return out, k
+
+
+def volume_fraction(recs, geom: Geom, scheme: int):
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    out = np.zeros(_mesh_shape(geom))
+    k = lib().orc_volume_fraction(recs.ctypes.data_as(C.c_void_p), len(recs),
+                                  C.byref(geom), scheme, _d(out))
+    return out, k
+
+
+def count_per_cell(recs, geom: Geom):
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    out = np.zeros(_mesh_shape(geom))
+    k = lib().orc_count_per_cell(recs.ctypes.data_as(C.c_void_p), len(recs),
+                                 C.byref(geom), _d(out))
+    return out, k
+
+
+def chem_update(grid_vol, npart, rdata, idata, mesh_vals, chempar, dt, nloop):
+    """xferMeshToParticleAndUpdateChem of one particle on copies; returns
+    (rdata, error bits)."""
+    rd = np.array(rdata, np.float64)
+    fm = np.array(mesh_vals, np.float64)
+    ip = np.ascontiguousarray(idata, np.int32)
+    cp = np.ascontiguousarray(chempar, np.float64)
+    pv = rd[25:].copy()
+    err = lib().orc_chem_update(float(grid_vol), float(npart), _d(rd), _d(fm), _d(pv),
+                                float(dt), int(nloop), _d(cp), _i(ip))
+    rd[25:] = pv
+    return rd, err
+
+
+def mesh_to_particle(recs, geom: Geom, X, vf, chempar, dt, nloop, cnc_scheme, vf_scheme):
+    """bmx_calc_txfr_particle on recs (in place); X: (3, nz+2, ny+2, nx+2),
+    vf: (nz+2, ny+2, nx+2).  Returns the error bits."""
+    assert recs.dtype == RECORD_DTYPE and recs.flags.c_contiguous
+    Xc = np.ascontiguousarray(X, np.float64)
+    vc = np.ascontiguousarray(vf, np.float64)
+    assert Xc.shape == (3,) + _mesh_shape(geom, 1) and vc.shape == _mesh_shape(geom, 1)
+    cp = np.ascontiguousarray(chempar, np.float64)
+    return lib().orc_mesh_to_particle(recs.ctypes.data_as(C.c_void_p), len(recs),
+                                      C.byref(geom), _d(Xc), _d(vc), _d(cp), float(dt),
+                                      int(nloop), int(cnc_scheme), int(vf_scheme))
+
+
+def interp(x, field, geom: Geom, scheme: int):
+    """Interpolation of a ghosted field (ncomp, nz+2, ny+2, nx+2) at x."""
+    f = np.ascontiguousarray(field, np.float64)
+    if f.ndim == 3:
+        f = f[None]
+    assert f.shape[1:] == _mesh_shape(geom, 1)
+    xv = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(f.shape[0])
+    lib().orc_interp(_d(xv), _d(f), C.byref(geom), int(scheme), f.shape[0], _d(out))
+    return out
+
+
+def pow_(x: float, y: float) -> float:
+    return lib().orc_pow(float(x), float(y))

@@ -1,0 +1,182 @@
+"""GPU particle <-> mesh coupling against the oracle.
+
+Deposition (volume fraction, fluid increments, particle counts; one-to-one
+and trilinear) sums each cell's particles in ascending particle order on both
+sides, so the meshes must match bit for bit.  The mesh -> particle update
+(interpolation, gradient, chemistry and growth over the substeps) is
+per-particle arithmetic with the shared pow, so the records must match bit
+for bit too, and the mechanics steps after it (rows rebuilt for the grown
+sizes) must give the oracle's positions.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from boltzmannmfx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+# fungi deck (exec/fungi/input_fungi): k1 k2 k3 kr1 kr2 kr3 kg kv, max_seg_radius,
+# max_seg_length
+CHEMPAR = [2.0e-5, 4.0, 2.0e-5, 2.0e-5, 6.0e-4, 2.0e-5, 4.0, 4.0e4, 2.5e-4, 35.0e-4]
+
+
+def spheres(seed=3):
+    col = W.c1_small(seed=seed)
+    col.n_cell = (16, 16, 8)
+    r = col.recs["rdata"]
+    rng = np.random.default_rng(seed)
+    r[:, 5] = 4.0 / 3.0 * math.pi * r[:, 0] ** 3
+    r[:, 4] = 4.0 * math.pi * r[:, 0] ** 2
+    r[:, 25:28] = rng.uniform(0.0, 2e-5, (len(r), 3))
+    r[:, 31:34] = rng.uniform(-1e-12, 1e-12, (len(r), 3))
+    return col
+
+
+def hyphae(n=3000, seed=17):
+    col = W.c3_hyphae(n=n, nroots=30, seed=seed)
+    rng = np.random.default_rng(seed)
+    r = col.recs["rdata"]
+    r[:, 5] = math.pi * r[:, 0] ** 2 * r[:, 1]
+    r[:, 25:28] = rng.uniform(0.0, 2e-5, (n, 3))
+    r[:, 31:34] = rng.uniform(-1e-12, 1e-12, (n, 3))
+    # a few tips at the radius limit and some interior segments at it
+    r[::97, 0] = CHEMPAR[8]
+    return col
+
+
+def setup(bmx, oracle, col, mode=0):
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    if mode == 0:
+        ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin,
+                                max_neighbors=384)
+    else:
+        ctx.set_neighbor_params(mode=bmx.NBR_STENCIL, rebuild_every=1,
+                                neighborhood=1.1 * (2 * 2e-4 + 1e-2), max_neighbors=384)
+    grecs = col.recs.copy()
+    ctx.bind(grecs)
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    return ctx, grecs, g
+
+
+def mesh_inputs(col, seed=5):
+    """X_k (3 species) and vf with one ghost layer, positive."""
+    nx, ny, nz = col.n_cell
+    rng = np.random.default_rng(seed)
+    X = np.empty((3, nz + 2, ny + 2, nx + 2))
+    X[0] = rng.uniform(1.0e-5, 3.0e-5, X[0].shape)
+    X[1] = rng.uniform(0.0, 1.0e-7, X[1].shape)
+    X[2] = rng.uniform(1.0e-6, 3.0e-6, X[2].shape)
+    vf = rng.uniform(0.2, 1.0, (nz + 2, ny + 2, nx + 2))
+    return X, vf
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("kind", ["spheres", "hyphae"])
+def test_deposition_matches_oracle(bmx, oracle, gpu, scheme, kind):
+    col = spheres() if kind == "spheres" else hyphae()
+    ctx, grecs, g = setup(bmx, oracle, col)
+    if kind == "hyphae":
+        ctx.init_bonds()
+    ctx.evolve(col.dt, 7)            # deposit at moved positions
+    vf = ctx.volume_fraction(scheme)
+    rhs = ctx.txfr_fluid(scheme)
+    cnt = ctx.count_per_cell()
+    ctx.sync()
+    ovf, out = oracle.volume_fraction(grecs, g, scheme)
+    assert out == 0
+    orhs, _ = oracle.deposit(grecs, g, scheme, 31, 3)
+    ocnt, _ = oracle.count_per_cell(grecs, g)
+    assert vf.min() < 0.999 and np.abs(rhs).max() > 0
+    assert np.array_equal(vf, ovf)
+    assert np.array_equal(rhs, orhs)
+    assert np.array_equal(cnt, ocnt)
+    ctx.close()
+
+
+def test_deposition_into_device_tensors(bmx, oracle, gpu):
+    import torch
+    col = spheres(seed=4)
+    ctx, grecs, g = setup(bmx, oracle, col)
+    nx, ny, nz = col.n_cell
+    out = torch.zeros((nz, ny, nx), dtype=torch.float64, device="cuda")
+    ctx.volume_fraction(1, out=out)
+    torch.cuda.synchronize()
+    ctx.sync()
+    ovf, _ = oracle.volume_fraction(grecs, g, 1)
+    assert np.array_equal(out.cpu().numpy(), ovf)
+    ctx.close()
+
+
+def test_particle_outside_the_mesh_is_an_error(bmx, oracle, gpu):
+    col = spheres(seed=5)
+    col.recs["pos"][17, 2] = col.prob_hi[2] + 1e-4   # above a wall-bounded top
+    ctx, _, _ = setup(bmx, oracle, col)
+    with pytest.raises(bmx.BMXError) as e:
+        ctx.volume_fraction(0)
+    assert e.value.code == 11
+    ctx.close()
+
+
+@pytest.mark.parametrize("cnc,vfs", [(0, 0), (1, 0), (1, 1)])
+@pytest.mark.parametrize("kind", ["spheres", "hyphae"])
+def test_mesh_to_particle_matches_oracle(bmx, oracle, gpu, kind, cnc, vfs):
+    col = spheres() if kind == "spheres" else hyphae()
+    mode = 0 if kind == "spheres" else 1
+    ctx, grecs, g = setup(bmx, oracle, col, mode)
+    if kind == "hyphae":
+        ctx.init_bonds()
+    ctx.evolve(col.dt, 3)
+    ctx.sync()
+    orecs = grecs.copy()
+    X, vf = mesh_inputs(col)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, cnc, vfs)
+    ctx.sync()
+    err = oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, cnc, vfs)
+    assert err == 0
+    assert np.abs(grecs["rdata"][:, 5] - col.recs["rdata"][:, 5]).max() > 0  # grew
+    assert np.array_equal(grecs["rdata"], orecs["rdata"])
+    assert np.array_equal(grecs["idata"], orecs["idata"])
+    # the mechanics go on with the new sizes
+    if kind == "spheres":
+        st = oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
+                                                  ctx.bin_dims()),
+                          col.fpar, oracle.NBR_PAIR, col.skin, 10)
+    else:
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL,
+                          1.1 * (2 * 2e-4 + 1e-2), 1)
+    ctx.evolve(col.dt, 5)
+    st.evolve(col.dt, 5)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    ctx.close()
+
+
+def test_mesh_to_particle_from_device_tensors(bmx, oracle, gpu):
+    import torch
+    col = spheres(seed=8)
+    ctx, grecs, g = setup(bmx, oracle, col)
+    orecs = col.recs.copy()
+    ctx.sync()
+    orecs[:] = grecs
+    X, vf = mesh_inputs(col, seed=9)
+    ctx.txfr_particle(CHEMPAR, torch.from_numpy(X).cuda(), torch.from_numpy(vf).cuda(),
+                      0.25, 4, 1, 1)
+    ctx.sync()
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 1) == 0
+    assert np.array_equal(grecs["rdata"], orecs["rdata"])
+    ctx.close()
+
+
+def test_negative_fluid_concentration_is_an_error(bmx, oracle, gpu):
+    col = spheres(seed=6)
+    ctx, _, _ = setup(bmx, oracle, col)
+    X, vf = mesh_inputs(col)
+    X[0, 3, 4, 5] = -1e-3
+    X[0] = -1e-3
+    with pytest.raises(bmx.BMXError) as e:
+        ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4)
+    assert e.value.code == 12
+    ctx.close()
