@@ -219,8 +219,15 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _is_device_tensor(a) -> bool:
-    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+def _device_view(a):
+    """(address, element count) of a contiguous float64 device array that
+    exposes __cuda_array_interface__, else None."""
+    cai = getattr(a, "__cuda_array_interface__", None)
+    if cai is None:
+        return None
+    assert cai["typestr"] == "<f8", "device meshes are float64"
+    assert cai.get("strides") is None, "device meshes must be contiguous"
+    return int(cai["data"][0]), int(np.prod(cai["shape"]))
 
 
 def _check(code: int) -> None:
@@ -280,61 +287,64 @@ class Context:
         _check(self.lib.bmx_set_geometry(self.h, C.byref(g)))
         self._n_cell = tuple(int(v) for v in n_cell)
 
-    # particle <-> mesh; host meshes are numpy arrays [comp, k, j, i], device
-    # meshes torch tensors on the context's device
+    # particle <-> mesh.  Host meshes are numpy arrays [comp, k, j, i];
+    # device meshes are any float64 array on the context's device exposing
+    # __cuda_array_interface__ (a torch tensor, for one), passed by address.
+    def _mesh_arg(self, a, count: int):
+        dv = _device_view(a)
+        if dv is not None:
+            assert dv[1] == count, "device mesh has the wrong size"
+            return C.c_void_p(dv[0]), 1
+        assert a.flags.c_contiguous and a.dtype == np.float64 and a.size == count
+        return a.ctypes.data_as(C.c_void_p), 0
+
+    def _cells(self, ghost: int = 0) -> int:
+        nx, ny, nz = self._n_cell
+        return (nx + 2 * ghost) * (ny + 2 * ghost) * (nz + 2 * ghost)
+
     def _mesh_out(self, ncomp: int, out):
         nx, ny, nz = self._n_cell
         if out is None:
-            out = np.zeros((ncomp, nz, ny, nx))
-        on_dev = int(_is_device_tensor(out))
-        assert (out.numel() if on_dev else out.size) == ncomp * nx * ny * nz
-        return out, on_dev
-
-    @staticmethod
-    def _ptr(a, on_dev: int):
-        if on_dev:
-            assert a.is_contiguous() and str(a.dtype) == "torch.float64"
-            return C.c_void_p(a.data_ptr())
-        assert a.flags.c_contiguous and a.dtype == np.float64
-        return a.ctypes.data_as(C.c_void_p)
+            out = np.zeros((ncomp, nz, ny, nx)) if ncomp > 1 else np.zeros((nz, ny, nx))
+        return out
 
     def volume_fraction(self, scheme: int = DEPOSIT_ONE_TO_ONE, out=None):
         """bmx_calc_volume_fraction: vf per cell, shape (nz, ny, nx)."""
-        out, dev = self._mesh_out(1, out)
-        _check(self.lib.bmx_calc_volume_fraction(self.h, int(scheme),
-                                                 self._ptr(out, dev), dev))
-        return out.reshape(out.shape[1:]) if out.ndim == 4 else out
+        out = self._mesh_out(1, out)
+        p, dev = self._mesh_arg(out, self._cells())
+        _check(self.lib.bmx_calc_volume_fraction(self.h, int(scheme), p, dev))
+        return out
 
     def txfr_fluid(self, scheme: int = DEPOSIT_ONE_TO_ONE, out=None):
         """bmx_calc_txfr_fluid: rdata[31..33] summed per cell, (3, nz, ny, nx)."""
-        out, dev = self._mesh_out(3, out)
-        _check(self.lib.bmx_calc_txfr_fluid(self.h, int(scheme), self._ptr(out, dev), dev))
+        out = self._mesh_out(3, out)
+        p, dev = self._mesh_arg(out, 3 * self._cells())
+        _check(self.lib.bmx_calc_txfr_fluid(self.h, int(scheme), p, dev))
         return out
 
     def count_per_cell(self, out=None):
-        out, dev = self._mesh_out(1, out)
-        _check(self.lib.bmx_count_particles_per_cell(self.h, self._ptr(out, dev), dev))
-        return out.reshape(out.shape[1:]) if out.ndim == 4 else out
+        out = self._mesh_out(1, out)
+        p, dev = self._mesh_arg(out, self._cells())
+        _check(self.lib.bmx_count_particles_per_cell(self.h, p, dev))
+        return out
 
     def txfr_particle(self, chempar, x_k, vf, dt: float, nloop: int = 4,
                       cnc_scheme: int = DEPOSIT_ONE_TO_ONE,
                       vf_scheme: int = DEPOSIT_ONE_TO_ONE) -> None:
         """bmx_calc_txfr_particle: x_k (3, nz+2, ny+2, nx+2) and vf
         (nz+2, ny+2, nx+2) with one ghost layer; updates the records (read
-        back by sync())."""
+        back by sync()).  Both host or both device arrays."""
         cp = np.ascontiguousarray(chempar, dtype=np.float64)
         assert cp.size == 10
-        dev = int(_is_device_tensor(x_k))
-        if not dev:
+        if _device_view(x_k) is None:
             x_k = np.ascontiguousarray(x_k, dtype=np.float64)
             vf = np.ascontiguousarray(vf, dtype=np.float64)
-        nx, ny, nz = self._n_cell
-        ng = (nx + 2) * (ny + 2) * (nz + 2)
-        assert (x_k.numel() if dev else x_k.size) == 3 * ng
-        assert (vf.numel() if dev else vf.size) == ng
+        px, dx_ = self._mesh_arg(x_k, 3 * self._cells(1))
+        pv, dv_ = self._mesh_arg(vf, self._cells(1))
+        assert dx_ == dv_, "x_k and vf must both be host or both device arrays"
         _check(self.lib.bmx_calc_txfr_particle(
-            self.h, _dptr(cp), self._ptr(x_k, dev), self._ptr(vf, dev), float(dt),
-            int(nloop), int(cnc_scheme), int(vf_scheme), dev))
+            self.h, _dptr(cp), px, pv, float(dt), int(nloop), int(cnc_scheme),
+            int(vf_scheme), dx_))
 
     def set_neighbor_params(self, mode=NBR_VERLET, rebuild_every=10, skin=0.0,
                             max_neighbors=0, bin_factor_pct=100,

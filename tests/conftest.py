@@ -39,3 +39,59 @@ def gpu(bmx):
     ctx = bmx.Context(0)
     ctx.close()
     return True
+
+
+class HipArray:
+    """float64 device memory (hipMalloc through ctypes) exposing
+    __cuda_array_interface__, for passing device meshes to the C ABI without
+    a framework."""
+
+    _hip = None
+
+    @classmethod
+    def _lib(cls):
+        import ctypes as C
+        if cls._hip is None:
+            for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+                try:
+                    cls._hip = C.CDLL(name)
+                    break
+                except OSError:
+                    continue
+        return cls._hip
+
+    def __init__(self, shape):
+        import ctypes as C
+        import numpy as np
+        self.shape = tuple(int(v) for v in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        self.ptr = C.c_void_p()
+        assert self._lib().hipMalloc(C.byref(self.ptr), C.c_size_t(self.nbytes)) == 0
+
+    @classmethod
+    def upload(cls, a):
+        import ctypes as C
+        import numpy as np
+        a = np.ascontiguousarray(a, np.float64)
+        d = cls(a.shape)
+        assert cls._lib().hipMemcpy(d.ptr, a.ctypes.data_as(C.c_void_p),
+                                    C.c_size_t(d.nbytes), 1) == 0   # host to device
+        return d
+
+    def download(self):
+        import ctypes as C
+        import numpy as np
+        out = np.empty(self.shape)
+        assert self._lib().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr,
+                                     C.c_size_t(self.nbytes), 2) == 0  # device to host
+        return out
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": self.shape, "typestr": "<f8", "data": (self.ptr.value, False),
+                "version": 2, "strides": None}
+
+    def free(self):
+        if self.ptr:
+            self._lib().hipFree(self.ptr)
+            self.ptr = None
