@@ -238,6 +238,108 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
     }
 
 
+# exec/fungi/input_fungi: k1 k2 k3 kr1 kr2 kr3 kg kv, max_seg_radius, max_seg_length
+MESH_CHEMPAR = [2.0e-5, 4.0, 2.0e-5, 2.0e-5, 6.0e-4, 2.0e-5, 4.0, 4.0e4, 2.5e-4, 35.0e-4]
+
+
+def mesh_bench(bmx, W, n: int = 1_000_000, n_cell=(64, 64, 32), calls: int = 20,
+               cpu: bool = True):
+    """Particle <-> mesh coupling of the 1,000,000-sphere colony on a
+    64 x 64 x 32 level mesh (about 7.6 spheres per cell), after 10
+    mechanics steps: the volume fraction (one-to-one), the fluid increments
+    (one-to-one) and the mesh -> particle chemistry (4 substeps, the fungi
+    deck's rates), each from and into device memory, `calls` calls each
+    after one warm-up."""
+    import ctypes as C
+    import numpy as np
+    hip = C.CDLL("libamdhip64.so")
+    col = W.c2_large(n)
+    col.n_cell = tuple(n_cell)
+    r = col.recs["rdata"]
+    rng = np.random.default_rng(11)
+    r[:, 5] = 4.0 / 3.0 * np.pi * r[:, 0] ** 3
+    r[:, 4] = 4.0 * np.pi * r[:, 0] ** 2
+    r[:, 25:28] = rng.uniform(0.0, 2e-5, (n, 3))
+    nx, ny, nz = n_cell
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    X = np.empty((3, ng))
+    X[0] = rng.uniform(1.0e-5, 3.0e-5, ng)
+    X[1] = rng.uniform(0.0, 1.0e-7, ng)
+    X[2] = rng.uniform(1.0e-6, 3.0e-6, ng)
+    vfin = rng.uniform(0.2, 1.0, ng)
+
+    class Dev:
+        def __init__(self, a):
+            a = np.ascontiguousarray(a, np.float64)
+            self.shape, self.p = a.shape, C.c_void_p()
+            assert hip.hipMalloc(C.byref(self.p), C.c_size_t(a.nbytes)) == 0
+            assert hip.hipMemcpy(self.p, a.ctypes.data_as(C.c_void_p),
+                                 C.c_size_t(a.nbytes), 1) == 0
+
+        @property
+        def __cuda_array_interface__(self):
+            return {"shape": self.shape, "typestr": "<f8", "data": (self.p.value, False),
+                    "version": 2, "strides": None}
+
+    dX, dvf = Dev(X), Dev(vfin)
+    dout = Dev(np.zeros(3 * nx * ny * nz))
+    dvfo = Dev(np.zeros(nx * ny * nz))
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    ctx.evolve(col.dt, 10)   # the particles in the order of a rebuild, as in a run
+    dt = 1.0e-3
+    res = {}
+    for name, fn in (("volume_fraction", lambda: ctx.volume_fraction(0, out=dvfo)),
+                     ("txfr_fluid", lambda: ctx.txfr_fluid(0, out=dout)),
+                     ("txfr_particle", lambda: ctx.txfr_particle(MESH_CHEMPAR, dX, dvf,
+                                                                  dt, 4))):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            fn()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / calls
+        res[name] = {"ms_per_call": ms, "particles_per_s": n / (ms * 1e-3)}
+    ctx.close()
+    for d in (dX, dvf, dout, dvfo):
+        hip.hipFree(d.p)
+    # HBM bytes a particle needs in the chemistry pass: sorted position and
+    # rod (64), type and original index (8), the record fields read (radius,
+    # length, area, vol, dadt, dvdt, species A-C, position, n_bnds: 80) and
+    # written (those sizes, the gradient and rdata 25-33: 144), new radius,
+    # length and area (24), its cell's start (8); the mesh is L2-resident
+    bytes_pp = 64 + 8 + 80 + 144 + 24 + 8
+    tp = res["txfr_particle"]
+    tp["algorithmic_bytes_per_particle"] = bytes_pp
+    tp["gbs_at_wall_time"] = bytes_pp * n / (tp["ms_per_call"] * 1e-3) / 1e9
+    out = {"workload": f"{n} spheres, level mesh {tuple(n_cell)}, one-to-one, 4 substeps",
+           **res}
+    if cpu:
+        # the oracle on the same records, one host thread, one call each
+        import oracle_bmx as orc
+        g = orc.make_geom(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+        orecs = col.recs.copy()
+        orc.lib()
+        t0 = time.perf_counter()
+        orc.volume_fraction(orecs, g, 0)
+        t_vf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.mesh_to_particle(orecs, g, X.reshape(3, nz + 2, ny + 2, nx + 2),
+                             vfin.reshape(nz + 2, ny + 2, nx + 2), MESH_CHEMPAR, dt, 4, 0, 0)
+        t_mp = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "volume_fraction_particles_per_s": n / t_vf,
+            "txfr_particle_particles_per_s": n / t_mp,
+            "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"one call of each on the same {n} records, one thread"}
+    return out
+
+
 def calibration_copy(nbytes: int) -> None:
     """One device-to-device copy of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
@@ -270,6 +372,7 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-rods", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -372,6 +475,9 @@ def main() -> int:
     if not args.no_rods:
         out["rods"] = rods_bench(bmx, W, cpu_steps=0 if args.no_cpu else 10)
         print("# rods done", file=sys.stderr, flush=True)
+    if not args.no_mesh:
+        out["mesh"] = mesh_bench(bmx, W, cpu=not args.no_cpu)
+        print("# mesh done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(col, args.cpu_budget)
         out["cpu_baseline"] = cb

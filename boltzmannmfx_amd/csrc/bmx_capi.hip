@@ -108,7 +108,8 @@ struct bmx_ctx {
   DevBuf<double4> d_chem, d_chem_inc;
   bool have_chem = false;
   // particle <-> mesh coupling: bins over the level mesh, staged fields
-  DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan, d_mperm;
+  DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan;
+  DevBuf<unsigned long long> d_mperm;
   DevBuf<double> d_mout, d_min;
   int32_t n_real = 0;
   bool have_list = false;
@@ -1607,11 +1608,11 @@ int64_t mesh_cells(const MeshGeom &g, int ghost) {
          (g.n[2] + 2 * ghost);
 }
 
-// Flushes the device state into the record copy and bins the particles.
+// Bins the particles (current sorted arrays) over the level mesh.  The
+// records are not needed: the positions are the sorted ones and the
+// deposited fields (vol, rdata[31..33]) are only changed in the records.
 bmx_status mesh_bin(bmx_ctx *c, const MeshGeom &g, int tri, int sorted) {
   (void)hipSetDevice(c->device);
-  bmx_status st = pack_records(c);
-  if (st != BMX_OK) return st;
   int64_t nb = 1;
   for (int d = 0; d < 3; d++)
     nb *= (tri && !g.periodic[d]) ? (int64_t)g.n[d] + 1 : g.n[d];
@@ -1622,10 +1623,10 @@ bmx_status mesh_bin(bmx_ctx *c, const MeshGeom &g, int tri, int sorted) {
   HIPCHK(c->d_mcount.alloc((size_t)nb + 1));
   HIPCHK(c->d_mstart.alloc((size_t)nb + 1));
   HIPCHK(c->d_mscan.alloc((size_t)scan_scratch_elems(nb) + 1));
-  HIPCHK(launch_mesh_bin(c->d_aos.p, c->lay, c->n, g, tri, c->d_mkey.p,
-                         c->d_mrank.p, c->d_mcount.p, c->d_mstart.p,
-                         c->d_mscan.p, c->d_mperm.p, sorted, c->d_err.p,
-                         c->stream));
+  HIPCHK(launch_mesh_bin(c->pos[c->cd].p, c->oid[c->cs].p, c->n, g, tri,
+                         c->d_mkey.p, c->d_mrank.p, c->d_mcount.p,
+                         c->d_mstart.p, c->d_mscan.p, c->d_mperm.p, sorted,
+                         c->d_err.p, c->stream));
   return BMX_OK;
 }
 
@@ -1643,7 +1644,8 @@ bmx_status mesh_flags(bmx_ctx *c) {
                 "more than 4096 particles");
   if (err & ERRBIT_CHEM)
     return fail(BMX_ERR_CHEM,
-                "negative concentration in the particle chemistry");
+                "negative concentration in the particle chemistry, or growth "
+                "left a negative radius or a segment length <= 0");
   if (err)
     return fail(BMX_ERR_STATE, "unexpected error bits before the mesh call");
   return BMX_OK;
@@ -1670,7 +1672,7 @@ bmx_status mesh_deposit(bmx_ctx *c, int scheme, int comp0, int ncomp, int mode,
     dst = c->d_mout.p;
   }
   HIPCHK(launch_deposit(c->d_aos.p, c->lay, g, tri, c->d_mstart.p, c->d_mperm.p,
-                        comp0, ncomp, mode, dst, c->stream));
+                        c->pos[c->cd].p, comp0, ncomp, mode, dst, c->stream));
   if (!on_device)
     HIPCHK(hipMemcpyAsync(out, dst, sizeof(double) * (size_t)(nc * ncomp),
                           hipMemcpyDeviceToHost, c->stream));
@@ -1734,26 +1736,28 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
     X = c->d_min.p;
     V = c->d_min.p + 3 * ng;
   }
-  HIPCHK(launch_mesh_to_particle(c->d_aos.p, c->lay, c->n, g, c->d_mstart.p, X,
-                                 V, cnc_scheme, vf_scheme, chempar, dt, nloop,
-                                 c->d_err.p, s));
-  // the records now hold the new sizes and species: the sorted arrays take
-  // the sizes, the species are reloaded at the next exchange, and the rows
-  // are rebuilt at the next step for the new reaches
+  // the species of the last exchange go back into the records first; the
+  // chemistry updates the records, the sorted sizes and their maxima, the
+  // species are reloaded at the next exchange, and the rows are rebuilt at
+  // the next step for the new reaches
+  if (c->have_chem)
+    HIPCHK(launch_chem_pack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+                            c->d_chem_inc.p, s));
   c->have_chem = false;
   HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
-  HIPCHK(launch_refresh_sizes(c->d_aos.p, c->lay, c->n, c->oid[c->cs].p,
-                              c->type[c->cs].p, c->pos[c->cd].p,
-                              c->rod[c->cd].p, c->d_ustats.p, s));
+  HIPCHK(launch_mesh_to_particle(c->d_aos.p, c->lay, c->n, g, c->d_mstart.p,
+                                 c->oid[c->cs].p, c->type[c->cs].p,
+                                 c->pos[c->cd].p, c->rod[c->cd].p, X, V,
+                                 cnc_scheme, vf_scheme, chempar, dt, nloop,
+                                 c->d_ustats.p, c->d_err.p, s));
   unsigned long long hs[8];
   HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
   st = mesh_flags(c);  // synchronizes the stream
-  if (st != BMX_OK) return st;
-  if (hs[4] != 0)
-    return fail(BMX_ERR_CHEM, "growth left a negative radius or a segment "
-                              "length <= 0");
+  if (st == BMX_ERR_HIP) return st;
+  // the sorted sizes have changed either way: new bins, rows rebuilt
   std::memcpy(&c->rmax, &hs[0], sizeof(double));
   std::memcpy(&c->lmax, &hs[1], sizeof(double));
   reset_graph(c);
-  return setup_boxes(c);
+  bmx_status sb = setup_boxes(c);
+  return st != BMX_OK ? st : sb;
 }

@@ -145,30 +145,32 @@ hipError_t launch_exchange(const double4 *pos, const int32_t *type,
                            double4 *inc, const double xpar[3], double dt,
                            hipStream_t s);
 
-// ---- particle <-> mesh (bmx_mesh.hip)
-// Bins the particles (original order) by mesh cell, or by trilinear base
-// cell (tri); start[0..nb] from the counts; with 'sorted', perm lists each
-// bin's particles in ascending order.
-hipError_t launch_mesh_bin(const uint8_t *aos, AosLayout lay, int64_t n,
+// ---- particle <-> mesh (bmx_mesh.hip), over the sorted arrays
+// Bins the particles by mesh cell, or by trilinear base cell (tri);
+// start[0..nb] from the counts; with 'sorted', perm lists each bin's
+// particles as (original index << 32 | sorted index), ascending.
+hipError_t launch_mesh_bin(const double4 *pos, const int32_t *oid, int64_t n,
                            const MeshGeom &g, int tri, int32_t *key,
                            int32_t *rank, int32_t *count, int32_t *start,
-                           int32_t *scan_scratch, int32_t *perm, int sorted,
-                           int32_t *err, hipStream_t s);
+                           int32_t *scan_scratch, unsigned long long *perm,
+                           int sorted, int32_t *err, hipStream_t s);
 // mode 0: sums of rdata[comp0..comp0+ncomp); 1: volume fraction of the
 // deposited volume; 2: particles per cell (one-to-one bins)
 hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
-                          int tri, const int32_t *start, const int32_t *perm,
+                          int tri, const int32_t *start,
+                          const unsigned long long *perm, const double4 *pos,
                           int comp0, int ncomp, int mode, double *out,
                           hipStream_t s);
+// interpolation + chemistry per particle; records updated in place, new
+// sizes into pos.w / rod.x / rod.w; stats[0], stats[1]: max radius and
+// segment length (bits)
 hipError_t launch_mesh_to_particle(uint8_t *aos, AosLayout lay, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
-                                   const double *X, const double *vf,
-                                   int cnc_tri, int vf_tri,
+                                   const int32_t *oid, const int32_t *type,
+                                   double4 *pos, double4 *rod, const double *X,
+                                   const double *vf, int cnc_tri, int vf_tri,
                                    const double chempar[10], double dt,
-                                   int nloop, int32_t *err, hipStream_t s);
-hipError_t launch_refresh_sizes(const uint8_t *aos, AosLayout lay, int64_t n,
-                                const int32_t *oid, const int32_t *type,
-                                double4 *pos, double4 *rod,
-                                unsigned long long *stats, hipStream_t s);
+                                   int nloop, unsigned long long *stats,
+                                   int32_t *err, hipStream_t s);
 
 }  // namespace bmx

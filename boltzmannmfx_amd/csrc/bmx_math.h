@@ -144,12 +144,15 @@ BMX_MATH_FN double bmx_cube(double c) { return c * c * c; }
  * x^y for finite x > 0, as exp(y log x) carried in double-double arithmetic
  * (a value hi + lo with |lo| <= ulp(hi)/2; Dekker/Knuth error-free sums and
  * fma products, so every step is IEEE-rounded the same way on both sides).
- * log: x = m 2^e with m in [1/sqrt2, sqrt2), s = (m-1)/(m+1),
- * log m = 2 atanh s = 2 s sum_k s^2k/(2k+1) (k <= 20, |s^2| < 0.03).
- * exp: t = k ln2 + r, |r| <= ln2/2, Taylor series to r^24.  The series
- * coefficients are 1/(2k+1) and 1/n! split exactly into hi + lo.  The result
- * is the double nearest the double-double value, so it agrees with a
- * correctly rounded pow except in rare near-halfway cases. */
+ * log: x = m 2^e with m in [1/sqrt2, sqrt2), s = (m-1)/(m+1), |s| < 0.172,
+ * log m = 2 atanh s = 2s + (2/3)s^3 + (2/5)s^5 + 2 s^7 sum_k s^2k/(2k+7):
+ * the first three terms in double-double, the rest (< 1.3e-6) in double.
+ * exp: t = k ln2 + r, |r| <= ln2/2, q = r/64; expm1 q = q + q^2/2 (double-
+ * double) + q^3 (1/3! + ... + q^5/8!) (double), then six doublings
+ * expm1 2a = expm1 a (expm1 a + 2), which keep the relative error.  The
+ * error of y log x stays below ~1e-21 and the result is the double nearest
+ * the double-double value, so it agrees with a correctly rounded pow except
+ * in rare near-halfway cases. */
 typedef struct {
   double hi, lo;
 } bmx_dd;
@@ -181,6 +184,19 @@ BMX_MATH_FN bmx_dd bmx_dd_add(bmx_dd a, bmx_dd b) {
   e = u.lo + t.lo;
   return bmx_dd_fast_sum(u.hi, e);
 }
+/* a + b for a double b */
+BMX_MATH_FN bmx_dd bmx_dd_add_d(bmx_dd a, double b) {
+  bmx_dd s = bmx_dd_two_sum(a.hi, b);
+  double e = s.lo + a.lo;
+  return bmx_dd_fast_sum(s.hi, e);
+}
+/* a * b for a double b */
+BMX_MATH_FN bmx_dd bmx_dd_mul_d(bmx_dd a, double b) {
+  double p = a.hi * b;
+  double e = fma(a.hi, b, -p);
+  e = e + a.lo * b;
+  return bmx_dd_fast_sum(p, e);
+}
 BMX_MATH_FN bmx_dd bmx_dd_mul(bmx_dd a, bmx_dd b) {
   double p = a.hi * b.hi;
   double e = fma(a.hi, b.hi, -p);
@@ -191,12 +207,12 @@ BMX_MATH_FN bmx_dd bmx_dd_mul(bmx_dd a, bmx_dd b) {
 }
 BMX_MATH_FN bmx_dd bmx_dd_div(bmx_dd a, bmx_dd b) {
   double q1 = a.hi / b.hi;
-  bmx_dd r = bmx_dd_add(a, bmx_dd_mul(b, bmx_dd_make(-q1, 0.0)));
+  bmx_dd r = bmx_dd_add(a, bmx_dd_mul_d(b, -q1));
   double q2 = r.hi / b.hi;
-  r = bmx_dd_add(r, bmx_dd_mul(b, bmx_dd_make(-q2, 0.0)));
+  r = bmx_dd_add(r, bmx_dd_mul_d(b, -q2));
   double q3 = r.hi / b.hi;
   bmx_dd q = bmx_dd_fast_sum(q1, q2);
-  return bmx_dd_add(q, bmx_dd_make(q3, 0.0));
+  return bmx_dd_add_d(q, q3);
 }
 
 #define BMX_LN2_HI 0x1.62e42fefa39efp-1
@@ -212,31 +228,27 @@ BMX_MATH_FN bmx_dd bmx_log_dd(double x) {
   }
   bmx_dd s = bmx_dd_div(bmx_dd_make(m - 1.0, 0.0), bmx_dd_two_sum(m, 1.0));
   bmx_dd z = bmx_dd_mul(s, s);
-  bmx_dd p = bmx_dd_make(0.024390243902439025, -8.46206573647223e-19);
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02564102564102564, 8.896017825522087e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02702702702702703, -1.50030138462859e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.02857142857142857, 8.921435019309293e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.030303030303030304, -8.410780489584519e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.03225806451612903, 8.953411488912552e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.034482758620689655, 4.785444071660157e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.037037037037037035, 2.05596856412066e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.04, -8.326672684688674e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.043478260869565216, 1.206764157201257e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.047619047619047616, 2.64338815386942e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.05263157894736842, 2.921639538487254e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.058823529411764705, 8.163404592832033e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.06666666666666667, 9.251858538542971e-19));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.07692307692307693, -4.270088556250602e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.09090909090909091, -2.523234146875356e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.1111111111111111, 6.1679056923619804e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.14285714285714285, 7.93016446160826e-18));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.2, -1.1102230246251566e-17));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(0.3333333333333333, 1.850371707708594e-17));
-  p = bmx_dd_add(bmx_dd_mul(p, z), bmx_dd_make(1.0, 0.0));
-  bmx_dd lm = bmx_dd_mul(bmx_dd_mul(s, p), bmx_dd_make(2.0, 0.0));
+  bmx_dd s3 = bmx_dd_mul(s, z);
+  bmx_dd s5 = bmx_dd_mul(s3, z);
+  double q = 0.037037037037037035;
+  q = q * z.hi + 0.04;
+  q = q * z.hi + 0.043478260869565216;
+  q = q * z.hi + 0.047619047619047616;
+  q = q * z.hi + 0.05263157894736842;
+  q = q * z.hi + 0.058823529411764705;
+  q = q * z.hi + 0.06666666666666667;
+  q = q * z.hi + 0.07692307692307693;
+  q = q * z.hi + 0.09090909090909091;
+  q = q * z.hi + 0.1111111111111111;
+  q = q * z.hi + 0.14285714285714285;
+  double tail = 2.0 * s5.hi * z.hi * q;
+  bmx_dd lm = bmx_dd_add(bmx_dd_make(2.0 * s.hi, 2.0 * s.lo),
+                         bmx_dd_mul(s3, bmx_dd_make(0.6666666666666666, 3.700743415417188e-17)));
+  lm = bmx_dd_add(lm, bmx_dd_mul(s5, bmx_dd_make(0.4, -2.2204460492503132e-17)));
+  lm = bmx_dd_add_d(lm, tail);
   double k = (double)ex;
   bmx_dd kl = bmx_dd_fast_sum(k * BMX_LN2_HI, fma(k, BMX_LN2_HI, -(k * BMX_LN2_HI)));
-  kl = bmx_dd_add(kl, bmx_dd_make(k * BMX_LN2_LO, 0.0));
+  kl = bmx_dd_add_d(kl, k * BMX_LN2_LO);
   return bmx_dd_add(kl, lm);
 }
 
@@ -244,40 +256,29 @@ BMX_MATH_FN bmx_dd bmx_log_dd(double x) {
 BMX_MATH_FN double bmx_exp_dd(bmx_dd t) {
   double k = floor(t.hi * 1.4426950408889634 + 0.5);
   bmx_dd kl = bmx_dd_two_sum(-(k * BMX_LN2_HI), -fma(k, BMX_LN2_HI, -(k * BMX_LN2_HI)));
-  kl = bmx_dd_add(kl, bmx_dd_make(-(k * BMX_LN2_LO), 0.0));
+  kl = bmx_dd_add_d(kl, -(k * BMX_LN2_LO));
   bmx_dd r = bmx_dd_add(t, kl);
-  bmx_dd e = bmx_dd_make(1.6117375710961184e-24, -3.6846573564509766e-41);
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(3.868170170630684e-23, -8.843177655482344e-40));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(8.896791392450574e-22, -7.911402614872376e-38));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.9572941063391263e-20, -1.3643503830087908e-36));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(4.110317623312165e-19, 1.4412973378659527e-36));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(8.22063524662433e-18, 2.2141894119604265e-34));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.5619206968586225e-16, 1.1910679660273754e-32));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.8114572543455206e-15, 1.6508842730861433e-31));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(4.779477332387385e-14, 4.399205485834081e-31));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(7.647163731819816e-13, 7.03872877733453e-30));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.1470745597729725e-11, 2.0655512752830745e-28));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.6059043836821613e-10, 1.2585294588752098e-26));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.08767569878681e-09, -1.20734505911326e-25));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.505210838544172e-08, -1.448814070935912e-24));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.755731922398589e-07, 2.3767714622250297e-23));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.7557319223985893e-06, -1.858393274046472e-22));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(2.48015873015873e-05, 2.1511947866775882e-23));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.0001984126984126984, 1.7209558293420705e-22));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.001388888888888889, -5.300543954373577e-20));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.008333333333333333, 1.1564823173178714e-19));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.041666666666666664, 2.3129646346357427e-18));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.16666666666666666, 9.25185853854297e-18));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(0.5, 0.0));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.0, 0.0));
-  e = bmx_dd_add(bmx_dd_mul(e, r), bmx_dd_make(1.0, 0.0));
+  bmx_dd q = bmx_dd_make(r.hi * 0x1p-6, r.lo * 0x1p-6);
+  double qh = q.hi;
+  double p = 2.7557319223985893e-06;
+  p = p * qh + 2.48015873015873e-05;
+  p = p * qh + 0.0001984126984126984;
+  p = p * qh + 0.001388888888888889;
+  p = p * qh + 0.008333333333333333;
+  p = p * qh + 0.041666666666666664;
+  p = p * qh + 0.16666666666666666;
+  bmx_dd q2 = bmx_dd_mul(q, q);
+  bmx_dd e = bmx_dd_add(q, bmx_dd_make(0.5 * q2.hi, 0.5 * q2.lo));
+  e = bmx_dd_add_d(e, qh * qh * qh * p);
+  for (int i = 0; i < 6; i++) e = bmx_dd_mul(e, bmx_dd_add_d(e, 2.0));
+  e = bmx_dd_add_d(e, 1.0);
   return ldexp(e.hi + e.lo, (int)k);
 }
 
 BMX_MATH_FN double bmx_pow(double x, double y) {
   if (!(x > 0.0) || x - x != 0.0) return (x - x) / (x - x); /* NaN */
   if (x == 1.0 || y == 0.0) return 1.0;
-  return bmx_exp_dd(bmx_dd_mul(bmx_dd_make(y, 0.0), bmx_log_dd(x)));
+  return bmx_exp_dd(bmx_dd_mul_d(bmx_log_dd(x), y));
 }
 
 #endif

@@ -25,11 +25,6 @@ namespace {
 
 constexpr int kMaxCellSort = 4096;  // particles one bin may hold
 
-__device__ __forceinline__ const double *rec_pos(const uint8_t *aos,
-                                                 AosLayout lay, int64_t o) {
-  return reinterpret_cast<const double *>(aos + o * (int64_t)lay.stride + lay.pos);
-}
-
 // floor((x - plo) * dxi) per axis (bmx_deposition_K.H:78-85); false when
 // the particle lies outside the mesh
 __device__ __forceinline__ bool mesh_cell(const double *x, const MeshGeom &g,
@@ -49,63 +44,123 @@ __device__ __forceinline__ int64_t ghost_lin(const MeshGeom &g, int i, int j,
   return ((int64_t)(k + 1) * (g.n[1] + 2) + (j + 1)) * (g.n[0] + 2) + (i + 1);
 }
 
-// Bin of each particle: its cell (TRI = 0) or its trilinear base cell
-// floor((x - plo) * dxi + 1/2), which is n on the high face and wraps to 0
-// there on a periodic axis.
+// Bin of each particle (sorted order t): its cell (TRI = 0) or its
+// trilinear base cell floor((x - plo) * dxi + 1/2), which is n on the high
+// face and wraps to 0 there on a periodic axis.  Neighbouring sorted
+// particles mostly share a bin, so each run of equal bins in a wave takes
+// its slots with one atomic (the run head's).
 template <int TRI>
-__global__ void k_mesh_key(const uint8_t *__restrict__ aos, AosLayout lay,
-                           int64_t n, MeshGeom g, int32_t *__restrict__ key,
-                           int32_t *__restrict__ rank,
-                           int32_t *__restrict__ count,
-                           int32_t *__restrict__ err) {
-  int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n) return;
-  const double *x = rec_pos(aos, lay, o);
-  int c[3];
-  if (!mesh_cell(x, g, c)) {
-    key[o] = -1;
-    atomicOr(err, ERRBIT_MESH);
-    return;
-  }
-  int nb0 = g.n[0], nb1 = g.n[1];
-  if (TRI) {
+__global__ __launch_bounds__(256) void k_mesh_key(
+    const double4 *__restrict__ pos, int64_t n, MeshGeom g,
+    int32_t *__restrict__ key, int32_t *__restrict__ rank,
+    int32_t *__restrict__ count, int32_t *__restrict__ err) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t k = -1;
+  if (t < n) {
+    const double4 p = pos[t];
+    const double x[3] = {p.x, p.y, p.z};
+    int c[3];
+    if (mesh_cell(x, g, c)) {
+      int nb0 = g.n[0], nb1 = g.n[1];
+      if (TRI) {
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-      int b = (int)floor((x[d] - g.plo[d]) * g.dxi[d] + 0.5);
-      if (g.periodic[d] && b == g.n[d]) b = 0;
-      c[d] = b;
+        for (int d = 0; d < 3; d++) {
+          int b = (int)floor((x[d] - g.plo[d]) * g.dxi[d] + 0.5);
+          if (g.periodic[d] && b == g.n[d]) b = 0;
+          c[d] = b;
+        }
+        nb0 = g.periodic[0] ? g.n[0] : g.n[0] + 1;
+        nb1 = g.periodic[1] ? g.n[1] : g.n[1] + 1;
+      }
+      k = (c[2] * nb1 + c[1]) * nb0 + c[0];
+    } else {
+      atomicOr(err, ERRBIT_MESH);
     }
-    nb0 = g.periodic[0] ? g.n[0] : g.n[0] + 1;
-    nb1 = g.periodic[1] ? g.n[1] : g.n[1] + 1;
+    key[t] = k;
   }
-  const int32_t k = (c[2] * nb1 + c[1]) * nb0 + c[0];
-  key[o] = k;
-  rank[o] = atomicAdd(&count[k], 1);
+  const bool active = k >= 0;
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = active && (lane == 0 || prev != k);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long act = __ballot(active);
+  // this lane's run: from the last head at or below it to the next head or
+  // the next inactive lane
+  const unsigned long long below = heads & ((2ull << lane) - 1ull);
+  const int first = 63 - __builtin_clzll(below | 1ull);
+  const unsigned long long stops = (heads | ~act) & ~((2ull << lane) - 1ull);
+  const int stop = stops ? __builtin_ctzll(stops) : 64;
+  int base = 0;
+  if (head) base = atomicAdd(&count[k], stop - lane);
+  base = __shfl(base, first, 64);
+  if (active) rank[t] = base + (lane - first);
 }
 
+// (original index << 32 | sorted index) of each binned particle at its slot
 __global__ void k_mesh_scatter(const int32_t *__restrict__ key,
                                const int32_t *__restrict__ rank,
-                               const int32_t *__restrict__ start, int64_t n,
-                               int32_t *__restrict__ perm) {
-  int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n) return;
-  const int32_t k = key[o];
-  if (k >= 0) perm[start[k] + rank[o]] = (int32_t)o;
+                               const int32_t *__restrict__ start,
+                               const int32_t *__restrict__ oid, int64_t n,
+                               unsigned long long *__restrict__ perm) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t k = key[t];
+  if (k >= 0)
+    perm[start[k] + rank[t]] =
+        ((unsigned long long)(uint32_t)oid[t] << 32) | (uint32_t)t;
 }
 
-// ascending particle order within each bin (insertion sort, bounded)
+__device__ __forceinline__ void cswap(unsigned long long &a,
+                                      unsigned long long &b) {
+  const unsigned long long lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+// Batcher's odd-even merge sort network on N register values (all indices
+// compile-time constants once unrolled).
+template <int N>
+__device__ __forceinline__ void sort_net(unsigned long long (&v)[N]) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+      for (int j = k % p; j + k < N; j += 2 * k)
+#pragma unroll
+        for (int i = 0; i < k; i++)
+          if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p))
+            cswap(v[i + j], v[i + j + k]);
+}
+
+constexpr int kNetSort = 16;
+
+// Each bin in ascending original index: up to 16 entries in registers by
+// the network, longer bins by insertion sort in place (bounded).
 __global__ void k_mesh_sort(const int32_t *__restrict__ start, int64_t nb,
-                            int32_t *__restrict__ perm,
+                            unsigned long long *__restrict__ perm,
                             int32_t *__restrict__ err) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int32_t a = start[b], e = start[b + 1];
-  if (e - a > kMaxCellSort) {
+  const int32_t m = e - a;
+  if (m <= 1) return;
+  if (m <= kNetSort) {
+    unsigned long long v[kNetSort];
+#pragma unroll
+    for (int i = 0; i < kNetSort; i++) v[i] = i < m ? perm[a + i] : ~0ull;
+    sort_net(v);
+#pragma unroll
+    for (int i = 0; i < kNetSort; i++)
+      if (i < m) perm[a + i] = v[i];
+    return;
+  }
+  if (m > kMaxCellSort) {
     atomicOr(err, ERRBIT_MESH);
     return;
   }
   for (int32_t i = a + 1; i < e; i++) {
-    const int32_t v = perm[i];
+    const unsigned long long v = perm[i];
     int32_t j = i - 1;
     while (j >= a && perm[j] > v) {
       perm[j + 1] = perm[j];
@@ -130,8 +185,9 @@ __device__ __forceinline__ double finish(double s, int mode, double inv_vol) {
 // One thread per cell: the cell's particles in ascending order.
 __global__ void k_deposit_one(const uint8_t *__restrict__ aos, AosLayout lay,
                               MeshGeom g, const int32_t *__restrict__ start,
-                              const int32_t *__restrict__ perm, int comp0,
-                              int ncomp, int mode, double *__restrict__ out) {
+                              const unsigned long long *__restrict__ perm,
+                              int comp0, int ncomp, int mode,
+                              double *__restrict__ out) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
@@ -140,21 +196,20 @@ __global__ void k_deposit_one(const uint8_t *__restrict__ aos, AosLayout lay,
     out[c] = (double)(e - a);
     return;
   }
-  for (int m = 0; m < ncomp; m++) {
-    double s = 0.0;
-    for (int32_t q = a; q < e; q++) {
-      const double *rd = reinterpret_cast<const double *>(
-          aos + (int64_t)perm[q] * lay.stride + lay.rdata);
-      s += rd[comp0 + m];
-    }
-    out[m * nc + c] = finish(s, mode, g.inv_cell_vol);
+  double sum[3] = {0.0, 0.0, 0.0};
+  for (int32_t q = a; q < e; q++) {
+    const double *rd = reinterpret_cast<const double *>(
+        aos + (int64_t)(perm[q] >> 32) * lay.stride + lay.rdata) + comp0;
+    for (int m = 0; m < ncomp; m++) sum[m] += rd[m];
   }
+  for (int m = 0; m < ncomp; m++) out[m * nc + c] = finish(sum[m], mode, g.inv_cell_vol);
 }
 
 // Weight of corner (ii, jj, kk) of a particle's trilinear stencil
 // (bmx_deposition_K.H:104-136): products normalised by their sum.
-__device__ __forceinline__ double tri_weight(const double *x, const MeshGeom &g,
+__device__ __forceinline__ double tri_weight(const double4 &p, const MeshGeom &g,
                                              int ii, int jj, int kk) {
+  const double x[3] = {p.x, p.y, p.z};
   double w[3][2];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -175,7 +230,8 @@ __device__ __forceinline__ double tri_weight(const double *x, const MeshGeom &g,
 // this cell.
 __global__ void k_deposit_tri(const uint8_t *__restrict__ aos, AosLayout lay,
                               MeshGeom g, const int32_t *__restrict__ start,
-                              const int32_t *__restrict__ perm, int comp0,
+                              const unsigned long long *__restrict__ perm,
+                              const double4 *__restrict__ pos, int comp0,
                               int ncomp, int mode, double *__restrict__ out) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,23 +253,25 @@ __global__ void k_deposit_tri(const uint8_t *__restrict__ aos, AosLayout lay,
     head[s] = start[b];
     end[s] = start[b + 1];
   }
+  // (original index, sorted index) at each list head, ~0 once a list is done
+  unsigned long long hv[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) hv[s] = head[s] < end[s] ? perm[head[s]] : ~0ull;
   double sum[3] = {0.0, 0.0, 0.0};
   for (;;) {
-    int best = -1;
-    int32_t bo = 0x7fffffff;
+    int best = 0;
 #pragma unroll
-    for (int s = 0; s < 8; s++)
-      if (head[s] < end[s] && perm[head[s]] < bo) {
-        bo = perm[head[s]];
-        best = s;
-      }
-    if (best < 0) break;
+    for (int s = 1; s < 8; s++)
+      if (hv[s] < hv[best]) best = s;
+    if (hv[best] == ~0ull) break;
+    const int64_t bo = (int64_t)(hv[best] >> 32);
+    const int64_t bt = (int64_t)(uint32_t)hv[best];
     head[best]++;
+    hv[best] = head[best] < end[best] ? perm[head[best]] : ~0ull;
     const int ii = best & 1, jj = (best >> 1) & 1, kk = best >> 2;
-    const double *x = rec_pos(aos, lay, bo);
-    const double w = tri_weight(x, g, ii, jj, kk);
+    const double w = tri_weight(pos[bt], g, ii, jj, kk);
     const double *rd =
-        reinterpret_cast<const double *>(aos + (int64_t)bo * lay.stride + lay.rdata);
+        reinterpret_cast<const double *>(aos + bo * lay.stride + lay.rdata);
     for (int m = 0; m < ncomp; m++) sum[m] += w * rd[comp0 + m];
   }
   for (int m = 0; m < ncomp; m++) out[m * nc + c] = finish(sum[m], mode, g.inv_cell_vol);
@@ -280,33 +338,36 @@ struct ChemParams {
   double k1, k2, k3, kr1, kr2, kr3, kg, kv, rmax, lmax;
 };
 
+// The size fields of a particle the chemistry updates (rdata 0, 1, 4, 5,
+// 21, 22), held in registers over the substeps.
+struct Sizes {
+  double radius, c_length, area, vol, dadt, dvdt;
+};
+
 // xferMeshToParticleAndUpdateChem with the checks of the reference's CPU
-// build.  rd: the record's rdata; returns ERRBIT_CHEM on the conditions the
-// reference aborts on.
-__device__ int chem_update(double grid_vol, double npart, double *rd,
-                           const double fm[3], int cell_type, bool is_tip,
-                           int n_bnds, double dt, int nloop,
+// build.  c: species A, B, C in, out; rel: amounts taken from the fluid.
+// Returns ERRBIT_CHEM on the conditions the reference aborts on.
+__device__ int chem_update(double grid_vol, double npart, Sizes &z, double c[3],
+                           double rel[3], const double fm[3], int cell_type,
+                           bool is_tip, int n_bnds, double dt, int nloop,
                            const ChemParams &P) {
   const double fluid_vol = grid_vol / npart;
   double kg = P.kg, kv = P.kv;
-  if (cell_type == rec::FUNGI && rd[rec::radius] >= P.rmax &&
-      rd[rec::c_length] >= P.lmax) {
+  if (cell_type == rec::FUNGI && z.radius >= P.rmax && z.c_length >= P.lmax) {
     kg = 0.0;
     kv = 0.0;
   }
-  double *pv = rd + rec::first_data;
-  double orig_vol = rd[rec::vol];
+  double orig_vol = z.vol;
   double f[3];
   for (int s = 0; s < 3; s++) {
     f[s] = fm[s];
     if (f[s] < 0.0 && fabs(f[s]) < 1.0e-15) f[s] = 0.0;
+    rel[s] = 0.0;
   }
   if (f[0] < 0. || f[1] < 0. || f[2] < 0.) return ERRBIT_CHEM;
-  double c[3] = {pv[0], pv[1], pv[2]};
   const double dtp = dt / (double)nloop;
-  double new_vol = rd[rec::vol];
-  double new_area = rd[rec::area];
-  double rel[3] = {0.0, 0.0, 0.0};  // amounts taken from the fluid
+  double new_vol = z.vol;
+  double new_area = z.area;
   bool negative = false;
   for (int it = 0; it < nloop; it++) {
     double d1[3];
@@ -370,20 +431,20 @@ __device__ int chem_update(double grid_vol, double npart, double *rd,
     negative = negative || c[0] < 0.0 || c[1] < 0.0 || c[2] < 0.0;
 
     // sizes
-    const double r0 = rd[rec::radius], l0 = rd[rec::c_length];
+    const double r0 = z.radius, l0 = z.c_length;
     if (cell_type == rec::YEAST) {
-      rd[rec::dvdt] = rV;
+      z.dvdt = rV;
       new_vol += dV;
-      rd[rec::vol] = new_vol;
+      z.vol = new_vol;
       const double r = bmx_pow((3.0 * new_vol / (4.0 * BMX_PI)), 1.0 / 3.0);
-      rd[rec::area] = 4.0 * BMX_PI * r * r;
-      new_area = rd[rec::area];
-      rd[rec::dadt] = 2.0 * rV / r;
-      rd[rec::radius] = r;
+      z.area = 4.0 * BMX_PI * r * r;
+      new_area = z.area;
+      z.dadt = 2.0 * rV / r;
+      z.radius = r;
     } else if (cell_type == rec::FUNGI && is_tip) {
-      rd[rec::dvdt] = rV;
+      z.dvdt = rV;
       new_vol += dV;
-      rd[rec::vol] = new_vol;
+      z.vol = new_vol;
       double r, l;
       if (r0 < P.rmax && l0 < P.lmax) {
         const double tau = bmx_pow(new_vol / orig_vol, 1.0 / 3.0);
@@ -392,42 +453,40 @@ __device__ int chem_update(double grid_vol, double npart, double *rd,
         const double dtdt = 1.0 / (3.0 * bmx_pow(new_vol, 2.0 / 3.0)) * rV /
                             bmx_pow(orig_vol, 1.0 / 3.0);
         const double drdt = dtdt * r0, dldt = dtdt * l0;
-        rd[rec::dadt] = 2.0 * BMX_PI * (drdt * (r + l) + r * (drdt + dldt));
-        rd[rec::area] = n_bnds == 0 ? 2.0 * BMX_PI * r * (r + l)
-                                    : BMX_PI * r * (r + 2.0 * l);
+        z.dadt = 2.0 * BMX_PI * (drdt * (r + l) + r * (drdt + dldt));
+        z.area = n_bnds == 0 ? 2.0 * BMX_PI * r * (r + l) : BMX_PI * r * (r + 2.0 * l);
       } else if (r0 < P.rmax) {
         const double tau = sqrt(new_vol / orig_vol);
         r = tau * r0;
         l = l0;
         const double dtdt = 1.0 / (2.0 * sqrt(new_vol)) * rV / sqrt(orig_vol);
         const double drdt = dtdt * r0;
-        rd[rec::dadt] = 2.0 * BMX_PI * (2.0 * r + l) * drdt;
+        z.dadt = 2.0 * BMX_PI * (2.0 * r + l) * drdt;
       } else {
         const double tau = new_vol / orig_vol;
         r = r0;
         l = tau * l0;
         const double dldt = rV * l0 / orig_vol;
-        rd[rec::dadt] = 2.0 * BMX_PI * r * dldt;
+        z.dadt = 2.0 * BMX_PI * r * dldt;
       }
-      rd[rec::radius] = r;
-      rd[rec::c_length] = l;
+      z.radius = r;
+      z.c_length = l;
     } else if (cell_type == rec::FUNGI) {
       if (r0 < P.rmax) {
-        rd[rec::dvdt] = rV;
+        z.dvdt = rV;
         new_vol += dV;
-        rd[rec::vol] = new_vol;
+        z.vol = new_vol;
         const double tau = sqrt(new_vol / orig_vol);
         const double r = tau * r0;
         const double dtdt = 1.0 / (2.0 * sqrt(new_vol)) * rV / sqrt(orig_vol);
         const double drdt = dtdt * r0;
-        rd[rec::radius] = r;
-        rd[rec::c_length] = l0;
-        rd[rec::area] = 2.0 * BMX_PI * r * l0;
-        rd[rec::dadt] = 2.0 * BMX_PI * (2.0 * r + l0) * drdt;
+        z.radius = r;
+        z.area = 2.0 * BMX_PI * r * l0;
+        z.dadt = 2.0 * BMX_PI * (2.0 * r + l0) * drdt;
       } else {
-        rd[rec::dvdt] = 0.0;
-        rd[rec::vol] = orig_vol;
-        rd[rec::dadt] = 0.0;
+        z.dvdt = 0.0;
+        z.vol = orig_vol;
+        z.dadt = 0.0;
       }
     }
 
@@ -446,87 +505,105 @@ __device__ int chem_update(double grid_vol, double npart, double *rd,
     for (int s = 0; s < 3; s++) rel[s] += -(d1[s] + d2[s]);
     orig_vol = new_vol;
   }
-  for (int s = 0; s < 3; s++) {
-    pv[s] = c[s];
-    pv[3 + s] = c[s];
-    pv[6 + s] = rel[s];
-  }
   return negative ? ERRBIT_CHEM : 0;
 }
 
-__global__ void k_mesh_to_particle(uint8_t *__restrict__ aos, AosLayout lay,
-                                   int64_t n, MeshGeom g,
-                                   const int32_t *__restrict__ start,
-                                   const double *__restrict__ X,
-                                   const double *__restrict__ vf,
-                                   int cnc_tri, int vf_tri, ChemParams P,
-                                   double dt, int nloop,
-                                   int32_t *__restrict__ err) {
-  int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n) return;
-  uint8_t *r = aos + o * (int64_t)lay.stride;
-  const double *x = reinterpret_cast<const double *>(r + lay.pos);
-  double *rd = reinterpret_cast<double *>(r + lay.rdata);
-  const int32_t *id = reinterpret_cast<const int32_t *>(r + lay.idata);
-  int c[3];
-  if (!mesh_cell(x, g, c)) return;  // reported by the binning pass
-  const int64_t ng = (int64_t)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
-  const int64_t cell = ((int64_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
-  const int64_t gc = ghost_lin(g, c[0], c[1], c[2]);
-  double fm[3];
-  for (int m = 0; m < 3; m++)
-    fm[m] = cnc_tri ? trilinear(x, X + m * ng, g) : X[m * ng + gc];
-  const double v = vf_tri ? trilinear(x, vf, g) : vf[gc];
-  const double np = (double)(start[cell + 1] - start[cell]);
-  // gradient of species A at the cell, centred differences
-  rd[rec::gx] = (X[gc + 1] - X[gc - 1]) * 0.5 * g.dxi[0];
-  const int64_t sy = g.n[0] + 2, sz = sy * (g.n[1] + 2);
-  rd[rec::gy] = (X[gc + sy] - X[gc - sy]) * 0.5 * g.dxi[1];
-  rd[rec::gz] = (X[gc + sz] - X[gc - sz]) * 0.5 * g.dxi[2];
-  const int e = chem_update(g.grid_vol * v, np, rd, fm, id[rec::cell_type],
-                            id[rec::position] == rec::TIP, id[rec::n_bnds], dt,
-                            nloop, P);
-  if (e) atomicOr(err, e);
+// Largest value over the block (non-negative doubles), in thread 0.
+__device__ __forceinline__ double block_max_nonneg(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) v = fmax(v, sh[k]);
+  return v;
 }
 
-// New radius / length / area into the sorted arrays, with their maxima.
-__global__ void k_refresh_sizes(const uint8_t *__restrict__ aos, AosLayout lay,
-                                int64_t n, const int32_t *__restrict__ oid,
-                                const int32_t *__restrict__ type,
-                                double4 *__restrict__ pos,
-                                double4 *__restrict__ rod,
-                                unsigned long long *__restrict__ stats) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double *rd = reinterpret_cast<const double *>(
-      aos + (int64_t)oid[t] * lay.stride + lay.rdata);
-  const double rad = rd[rec::radius];
-  pos[t].w = rad;
-  bool bad = !(rad >= 0.0);
-  if (!bad) atomicMax(&stats[0], (unsigned long long)__double_as_longlong(rad));
-  if (rod) {
-    const double L = rd[rec::c_length];
-    rod[t].x = L;
-    rod[t].w = rd[rec::area];
-    if (type[t] == rec::FUNGI) {
-      if (!(L > 0.0))
-        bad = true;
-      else
-        atomicMax(&stats[1], (unsigned long long)__double_as_longlong(L));
+// One thread per particle in sorted order: interpolation at its position,
+// the chemistry, the record fields written back, and the new radius, length
+// and area into the sorted arrays with their maxima (one atomic per block).
+__global__ __launch_bounds__(256) void k_mesh_to_particle(
+    uint8_t *__restrict__ aos, AosLayout lay, int64_t n, MeshGeom g,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ oid,
+    const int32_t *__restrict__ type, double4 *__restrict__ pos,
+    double4 *__restrict__ rod, const double *__restrict__ X,
+    const double *__restrict__ vf, int cnc_tri, int vf_tri, ChemParams P,
+    double dt, int nloop, unsigned long long *__restrict__ stats,
+    int32_t *__restrict__ err) {
+  __shared__ double sh[2][4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rmax = 0.0, lmax = 0.0;
+  int e = 0;
+  if (t < n) {
+    double4 p = pos[t];
+    double4 rr = rod[t];
+    const int ty = type[t];
+    const double x[3] = {p.x, p.y, p.z};
+    int c[3];
+    if (mesh_cell(x, g, c)) {  // else reported by the binning pass
+      uint8_t *r = aos + (int64_t)oid[t] * lay.stride;
+      double *rd = reinterpret_cast<double *>(r + lay.rdata);
+      const int32_t *id = reinterpret_cast<const int32_t *>(r + lay.idata);
+      const int64_t ng = (int64_t)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+      const int64_t cell = ((int64_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+      const int64_t gc = ghost_lin(g, c[0], c[1], c[2]);
+      double fm[3];
+      for (int m = 0; m < 3; m++)
+        fm[m] = cnc_tri ? trilinear(x, X + m * ng, g) : X[m * ng + gc];
+      const double v = vf_tri ? trilinear(x, vf, g) : vf[gc];
+      const double np = (double)(start[cell + 1] - start[cell]);
+      // gradient of species A at the cell, centred differences
+      const int64_t sy = g.n[0] + 2, sz = sy * (g.n[1] + 2);
+      rd[rec::gx] = (X[gc + 1] - X[gc - 1]) * 0.5 * g.dxi[0];
+      rd[rec::gy] = (X[gc + sy] - X[gc - sy]) * 0.5 * g.dxi[1];
+      rd[rec::gz] = (X[gc + sz] - X[gc - sz]) * 0.5 * g.dxi[2];
+      Sizes z{rd[rec::radius], rd[rec::c_length], rd[rec::area],
+              rd[rec::vol],    rd[rec::dadt],     rd[rec::dvdt]};
+      double cs[3] = {rd[rec::first_data], rd[rec::first_data + 1],
+                      rd[rec::first_data + 2]};
+      double rel[3];
+      e = chem_update(g.grid_vol * v, np, z, cs, rel, fm, ty,
+                      id[rec::position] == rec::TIP, id[rec::n_bnds], dt, nloop, P);
+      rd[rec::radius] = z.radius;
+      rd[rec::c_length] = z.c_length;
+      rd[rec::area] = z.area;
+      rd[rec::vol] = z.vol;
+      rd[rec::dadt] = z.dadt;
+      rd[rec::dvdt] = z.dvdt;
+      for (int k = 0; k < 3; k++) {
+        rd[rec::first_data + k] = cs[k];
+        rd[rec::first_data + 3 + k] = cs[k];
+        rd[rec::first_data + 6 + k] = rel[k];
+      }
+      p.w = z.radius;
+      rr.x = z.c_length;
+      rr.w = z.area;
+      pos[t].w = z.radius;
+      rod[t] = rr;
+    }
+    if (p.w >= 0.0) rmax = p.w; else e |= ERRBIT_CHEM;
+    if (ty == rec::FUNGI) {
+      if (rr.x > 0.0) lmax = rr.x; else e |= ERRBIT_CHEM;
     }
   }
-  if (bad) atomicAdd(&stats[4], 1ull);
+  rmax = block_max_nonneg(rmax, sh[0]);
+  lmax = block_max_nonneg(lmax, sh[1]);
+  if (e) atomicOr(err, e);
+  if (threadIdx.x == 0) {
+    atomicMax(&stats[0], (unsigned long long)__double_as_longlong(rmax));
+    atomicMax(&stats[1], (unsigned long long)__double_as_longlong(lmax));
+  }
 }
 
 int grid_of(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
 
 }  // namespace
 
-hipError_t launch_mesh_bin(const uint8_t *aos, AosLayout lay, int64_t n,
+hipError_t launch_mesh_bin(const double4 *pos, const int32_t *oid, int64_t n,
                            const MeshGeom &g, int tri, int32_t *key,
                            int32_t *rank, int32_t *count, int32_t *start,
-                           int32_t *scan_scratch, int32_t *perm, int sorted,
-                           int32_t *err, hipStream_t s) {
+                           int32_t *scan_scratch, unsigned long long *perm,
+                           int sorted, int32_t *err, hipStream_t s) {
   int64_t nb = 1;
   for (int d = 0; d < 3; d++)
     nb *= (tri && !g.periodic[d]) ? g.n[d] + 1 : g.n[d];
@@ -534,29 +611,31 @@ hipError_t launch_mesh_bin(const uint8_t *aos, AosLayout lay, int64_t n,
   if (e != hipSuccess) return e;
   if (n > 0) {
     if (tri)
-      hipLaunchKernelGGL(k_mesh_key<1>, dim3(grid_of(n)), dim3(256), 0, s, aos,
-                         lay, n, g, key, rank, count, err);
+      hipLaunchKernelGGL(k_mesh_key<1>, dim3(grid_of(n)), dim3(256), 0, s, pos,
+                         n, g, key, rank, count, err);
     else
-      hipLaunchKernelGGL(k_mesh_key<0>, dim3(grid_of(n)), dim3(256), 0, s, aos,
-                         lay, n, g, key, rank, count, err);
+      hipLaunchKernelGGL(k_mesh_key<0>, dim3(grid_of(n)), dim3(256), 0, s, pos,
+                         n, g, key, rank, count, err);
   }
   e = launch_scan(count, start, nb, scan_scratch, s);
-  if (e != hipSuccess || !sorted || n <= 0) return e != hipSuccess ? e : hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (!sorted || n <= 0) return hipGetLastError();
   hipLaunchKernelGGL(k_mesh_scatter, dim3(grid_of(n)), dim3(256), 0, s, key,
-                     rank, start, n, perm);
+                     rank, start, oid, n, perm);
   hipLaunchKernelGGL(k_mesh_sort, dim3(grid_of(nb)), dim3(256), 0, s, start,
                      nb, perm, err);
   return hipGetLastError();
 }
 
 hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
-                          int tri, const int32_t *start, const int32_t *perm,
+                          int tri, const int32_t *start,
+                          const unsigned long long *perm, const double4 *pos,
                           int comp0, int ncomp, int mode, double *out,
                           hipStream_t s) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   if (tri)
     hipLaunchKernelGGL(k_deposit_tri, dim3(grid_of(nc)), dim3(256), 0, s, aos,
-                       lay, g, start, perm, comp0, ncomp, mode, out);
+                       lay, g, start, perm, pos, comp0, ncomp, mode, out);
   else
     hipLaunchKernelGGL(k_deposit_one, dim3(grid_of(nc)), dim3(256), 0, s, aos,
                        lay, g, start, perm, comp0, ncomp, mode, out);
@@ -565,25 +644,18 @@ hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
 
 hipError_t launch_mesh_to_particle(uint8_t *aos, AosLayout lay, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
-                                   const double *X, const double *vf,
-                                   int cnc_tri, int vf_tri,
+                                   const int32_t *oid, const int32_t *type,
+                                   double4 *pos, double4 *rod, const double *X,
+                                   const double *vf, int cnc_tri, int vf_tri,
                                    const double chempar[10], double dt,
-                                   int nloop, int32_t *err, hipStream_t s) {
+                                   int nloop, unsigned long long *stats,
+                                   int32_t *err, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   ChemParams P{chempar[0], chempar[1], chempar[2], chempar[3], chempar[4],
                chempar[5], chempar[6], chempar[7], chempar[8], chempar[9]};
   hipLaunchKernelGGL(k_mesh_to_particle, dim3(grid_of(n)), dim3(256), 0, s, aos,
-                     lay, n, g, start, X, vf, cnc_tri, vf_tri, P, dt, nloop, err);
-  return hipGetLastError();
-}
-
-hipError_t launch_refresh_sizes(const uint8_t *aos, AosLayout lay, int64_t n,
-                                const int32_t *oid, const int32_t *type,
-                                double4 *pos, double4 *rod,
-                                unsigned long long *stats, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_refresh_sizes, dim3(grid_of(n)), dim3(256), 0, s, aos,
-                     lay, n, oid, type, pos, rod, stats);
+                     lay, n, g, start, oid, type, pos, rod, X, vf, cnc_tri,
+                     vf_tri, P, dt, nloop, stats, err);
   return hipGetLastError();
 }
 

@@ -56,6 +56,20 @@ def test_pow_agrees_with_libm(oracle):
         assert (d > 0).mean() < 5e-3
 
 
+def test_pow_is_correctly_rounded_on_a_sample(oracle):
+    """Against x^y to 50 digits: the double-double pow rounds correctly on
+    every sampled argument (the C library's pow, < 1 ulp, need not)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 50
+    rng = np.random.default_rng(13)
+    for _ in range(1500):
+        x = float(10.0 ** rng.uniform(-30, 6) * rng.uniform(1, 2))
+        y = float(rng.choice([1 / 3, 2 / 3, 0.5, 1.7, -2.3, rng.uniform(-4, 4)]))
+        m = oracle.pow_(x, y)
+        exact = (Decimal(x).ln() * Decimal(y)).exp()
+        assert abs(Decimal(m) - exact) <= Decimal(math.ulp(m)) / 2, (x, y)
+
+
 def test_pow_exact_cases(oracle):
     rng = np.random.default_rng(6)
     x = 10.0 ** rng.uniform(-20, 5, 2000)
