@@ -181,6 +181,8 @@ def load_library() -> C.CDLL:
                                   P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(AdjointTiming)]),
         "bmx_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
+        "bmx_split_particles": (C.c_int, [vp, P(C.c_double), C.c_uint64, C.c_uint32,
+                                          C.c_int32, C.c_int32, P(C.c_int64)]),
         "bmx_calc_volume_fraction": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
         "bmx_calc_txfr_fluid": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
         "bmx_count_particles_per_cell": (C.c_int, [vp, vp, C.c_int32]),
@@ -328,6 +330,26 @@ class Context:
         _check(self.lib.bmx_count_particles_per_cell(self.h, p, dev))
         return out
 
+    def split_particles(self, params, seed: int, call: int, next_id: int,
+                        cpu: int = 0) -> np.ndarray:
+        """split_particles: divides, grows new tips and branches.  params =
+        (max_vol, max_seg_length, max_seg_radius, branching_probability,
+        splitting_probability, overlap, seg_split_length).  Returns the
+        (possibly grown) record array, now the context's bound buffer."""
+        sp = np.ascontiguousarray(params, dtype=np.float64)
+        assert sp.size == 7
+        nn = C.c_int64()
+        _check(self.lib.bmx_split_particles(self.h, _dptr(sp), int(seed), int(call),
+                                            int(next_id), int(cpu), C.byref(nn)))
+        if nn.value == 0:
+            return self._recs
+        m = int(self.lib.bmx_num_particles(self.h))
+        out = np.zeros(m, dtype=RECORD_DTYPE)
+        _check(self.lib.bmx_sync_to_aos(self.h, out.ctypes.data_as(C.c_void_p)))
+        self._recs = out
+        self.n = m
+        return out
+
     def txfr_particle(self, chempar, x_k, vf, dt: float, nloop: int = 4,
                       cnc_scheme: int = DEPOSIT_ONE_TO_ONE,
                       vf_scheme: int = DEPOSIT_ONE_TO_ONE) -> None:
@@ -380,6 +402,8 @@ class Context:
         self.n = len(recs)
 
     def sync(self, recs: Optional[np.ndarray] = None) -> None:
+        if recs is None:
+            recs = getattr(self, "_recs", None)
         ptr = recs.ctypes.data_as(C.c_void_p) if recs is not None else None
         _check(self.lib.bmx_sync_to_aos(self.h, ptr))
 

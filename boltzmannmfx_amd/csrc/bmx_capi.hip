@@ -36,6 +36,16 @@ bmx_status fail(bmx_status st, const std::string &msg) {
   } while (0)
 
 template <typename T>
+struct DevBuf;
+
+// releases a call-local device buffer on every way out of the call
+template <typename T>
+struct Releaser {
+  DevBuf<T> &b;
+  ~Releaser() { b.release(); }
+};
+
+template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
@@ -111,7 +121,7 @@ struct bmx_ctx {
   DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan;
   DevBuf<unsigned long long> d_mperm;
   DevBuf<double> d_mout, d_min;
-  int32_t n_real = 0;
+  int32_t n_real = 0, n_int = 0;
   bool have_list = false;
   int64_t step = 0, last_build = -1;
 
@@ -622,6 +632,7 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   c->wrote_step = false;
   c->have_chem = false;
   c->n_real = L.n_real;
+  c->n_int = L.n_int;
   c->bound = true;
   return BMX_OK;
 }
@@ -836,6 +847,10 @@ bmx_status bmx_sync_to_aos(bmx_ctx *c, void *aos) {
   st = pack_records(c);
   if (st != BMX_OK) return st;
   void *dst = aos ? aos : c->user_aos;
+  if (!dst)
+    return fail(BMX_ERR_STATE,
+                "the particle count has changed since bind: pass a record "
+                "buffer with room for bmx_num_particles() records");
   size_t bytes = (size_t)c->n * (size_t)c->lay.stride;
   if (dst && bytes)
     HIPCHK(hipMemcpyAsync(dst, c->d_aos.p, bytes,
@@ -1760,4 +1775,75 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
   reset_graph(c);
   bmx_status sb = setup_boxes(c);
   return st != BMX_OK ? st : sb;
+}
+
+// ------------------------------------------------------ division, branching
+extern "C" bmx_status bmx_split_particles(bmx_ctx *c, const bmx_split_params *sp,
+                                          uint64_t seed, uint32_t call,
+                                          int32_t next_id, int32_t cpu,
+                                          int64_t *n_new) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!sp || !n_new) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (c->ncol != 1)
+    return fail(BMX_ERR_STATE, "splitting needs a single-colony context");
+  if (c->n_real < rec::first_data + 9 || c->n_int < rec::cpu + 1)
+    return fail(BMX_ERR_BAD_ARG,
+                "the record layout lacks the species or topology fields");
+  const double p[7] = {sp->max_vol,    sp->max_seg_length, sp->max_seg_radius,
+                       sp->branching_probability, sp->splitting_probability,
+                       sp->overlap,    sp->seg_split_length};
+  for (double v : p)
+    if (!std::isfinite(v)) return fail(BMX_ERR_BAD_ARG, "split parameters must be finite");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  // the records take the device state first (positions, orientations,
+  // bonds, species)
+  st = pack_records(c);
+  if (st != BMX_OK) return st;
+  const int64_t n = c->n;
+  DevBuf<int32_t> cnt, start, scratch;
+  Releaser<int32_t> r1{cnt}, r2{start}, r3{scratch};
+  HIPCHK(cnt.alloc((size_t)n + 1));
+  HIPCHK(start.alloc((size_t)n + 1));
+  HIPCHK(scratch.alloc((size_t)scan_scratch_elems(n) + 1));
+  HIPCHK(launch_split_count(c->d_aos.p, c->lay, n, p, seed, call, cnt.p, s));
+  HIPCHK(launch_scan(cnt.p, start.p, n, scratch.p, s));
+  int32_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, start.p + n, sizeof total,
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_new = total;
+  if (total > 0) {
+    if ((int64_t)next_id + total > (int64_t)INT32_MAX)
+      return fail(BMX_ERR_BAD_ARG, "particle ids would overflow int32");
+    const size_t stride = (size_t)c->lay.stride;
+    DevBuf<uint8_t> grown;
+    Releaser<uint8_t> r4{grown};
+    HIPCHK(grown.alloc((size_t)(n + total) * stride));
+    HIPCHK(hipMemcpyAsync(grown.p, c->d_aos.p, (size_t)n * stride,
+                          hipMemcpyDeviceToDevice, s));
+    HIPCHK(launch_split_apply(grown.p, c->lay, n, start.p, p, seed, call,
+                              next_id, cpu, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // bind the grown records (new sorted arrays, rows rebuilt at the next
+    // step); the caller's buffer no longer fits them
+    bmx_layout L;
+    L.stride_bytes = c->lay.stride;
+    L.pos_offset = c->lay.pos;
+    L.rdata_offset = c->lay.rdata;
+    L.id_offset = c->lay.id;
+    L.cpu_offset = c->lay.cpu;
+    L.idata_offset = c->lay.idata;
+    L.n_real = c->n_real;
+    L.n_int = c->n_int;
+    const bool was_on_device = c->aos_on_device;
+    const bmx_geometry g0 = c->colony_geom[0];
+    int64_t cs[2] = {0, n + total};
+    st = bind_common(c, grown.p, cs, 1, &g0, &L, 1);
+    c->user_aos = nullptr;
+    c->aos_on_device = was_on_device;
+    if (st != BMX_OK) return st;
+  }
+  return BMX_OK;
 }

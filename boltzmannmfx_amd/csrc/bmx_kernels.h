@@ -173,4 +173,15 @@ hipError_t launch_mesh_to_particle(uint8_t *aos, AosLayout lay, int64_t n,
                                    int nloop, unsigned long long *stats,
                                    int32_t *err, hipStream_t s);
 
+// ---- division and branching (bmx_split.hip); sp = {max_vol, max_len,
+// max_rad, brnch_prob, split_prob, overlap, split_len}
+hipError_t launch_split_count(uint8_t *aos, AosLayout lay, int64_t n,
+                              const double sp[7], uint64_t seed, uint32_t call,
+                              int32_t *cnt, hipStream_t s);
+// records [0, n) with room for the new ones at n + start[o]
+hipError_t launch_split_apply(uint8_t *aos, AosLayout lay, int64_t n,
+                              const int32_t *start, const double sp[7],
+                              uint64_t seed, uint32_t call, int32_t next_id,
+                              int32_t cpu, hipStream_t s);
+
 }  // namespace bmx

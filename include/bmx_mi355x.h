@@ -291,6 +291,31 @@ bmx_status bmx_calc_txfr_particle(bmx_ctx *ctx, const double chempar[10],
                                   double dt, int32_t nloop, int32_t cnc_scheme,
                                   int32_t vf_scheme, int32_t on_device);
 
+/* --- division and branching ---------------------------------------------
+ * split_particles (src/des/bmx_split_particles.cpp:16-166): yeast cells
+ * above max_vol divide (setNewCell), hyphal tips at full radius and length
+ * grow a new tip or split into two (setNewSegment / setNewSegments), and
+ * segments behind a tip branch with branching_probability.  The new
+ * particles follow the existing ones in record order with ids next_id + k
+ * and the given cpu; *n_new returns their number (the caller advances its
+ * NextID by it).  Random numbers are a pure function of (seed, call,
+ * particle id, cpu): a run is reproducible and independent of the launch,
+ * but the reference's AMReX engines cannot be matched draw for draw.  When
+ * particles were added, the records are re-bound: the next bmx_sync_to_aos
+ * needs a buffer of bmx_num_particles() records. */
+typedef struct {
+  double max_vol;               /* chem_species.max_vol */
+  double max_seg_length;        /* chem_species.max_seg_length (3.5e-3) */
+  double max_seg_radius;        /* chem_species.max_seg_radius (2.5e-4) */
+  double branching_probability; /* chem_species.branching_probability (1e-4) */
+  double splitting_probability; /* chem_species.splitting_probability (0.1) */
+  double overlap;               /* BMXChemistry::p_overlap (0.2) */
+  double seg_split_length;      /* chem_species.seg_split_length (3.0e-3) */
+} bmx_split_params;
+bmx_status bmx_split_particles(bmx_ctx *ctx, const bmx_split_params *params,
+                               uint64_t seed, uint32_t call, int32_t next_id,
+                               int32_t cpu, int64_t *n_new);
+
 bmx_status bmx_get_fc(bmx_ctx *ctx, double *out6n);
 /* positions, n*3 doubles (x,y,z per particle) */
 bmx_status bmx_get_positions(bmx_ctx *ctx, double *out3n);

@@ -1746,3 +1746,512 @@ void orc_interp(const double *x, const double *f, const orc_geom *g, int scheme,
 }
 
 double orc_pow(double x, double y) { return bmx_pow(x, y); }
+
+/* ---- division and branching (split_particles) -----------------------------
+ * src/des/bmx_split_particles.cpp:16-166 with checkSplit, setNewCell,
+ * setNewSegment and setNewSegments (src/chemistry/bmx_chem_K.H:25-69,
+ * 383-442, 457-1011, 2129-2493).  Random numbers: bmx_uniform of the
+ * particle's (id, cpu), draw 0 in the check, draws 1.. when the new
+ * particles are set (the reference's engines cannot be reproduced). */
+#include "../boltzmannmfx_amd/csrc/bmx_rng.h"
+
+typedef struct {
+  uint64_t seed;
+  uint32_t call;
+  int32_t id, cpu;
+  uint32_t draw;
+} orc_draws;
+
+static double orc_next(orc_draws *r) {
+  return bmx_uniform(r->seed, r->call, r->id, r->cpu, r->draw++);
+}
+
+/* checkSplit (bmx_chem_K.H:25-69): 0, 1 or 2 new particles */
+int orc_check_split(const double *par, const int32_t *ipar,
+                    const double sp[7], orc_draws *rng) {
+  const double max_vol = sp[0], max_len = sp[1], max_rad = sp[2];
+  const double brnch_prob = sp[3], split_prob = sp[4];
+  int ct = ipar[ORC_I_CELL_TYPE];
+  if (ct == ORC_YEAST) return par[ORC_R_VOL] > max_vol ? 1 : 0;
+  if (ct != ORC_FUNGI) return 0;
+  int position = ipar[ORC_I_POSITION];
+  if (position == ORC_TIP) {
+    if (par[ORC_R_RADIUS] >= max_rad) {
+      double x = orc_next(rng);
+      if (par[ORC_R_CLENGTH] >= max_len) {
+        if (x > split_prob || ipar[ORC_I_N_BNDS] == 0) return 1;
+        return 2;
+      }
+    }
+  } else if (position == ORC_SECOND_1 || position == ORC_SECOND_2) {
+    double x = orc_next(rng);
+    if (x < brnch_prob) return 1;
+  }
+  return 0;
+}
+
+/* the topology fields a new segment starts with */
+static void orc_fresh_links(int32_t *ip, int dltd) {
+  ip[ORC_I_FUSE_FLAG] = 0;
+  ip[ORC_I_SPLIT_FLAG] = 0;
+  ip[ORC_I_NEW_FLAG] = 0;
+  ip[23] = -1; /* fuse_id */
+  ip[24] = -1; /* fuse_cpu */
+  if (dltd)
+    for (int k = ORC_I_DLTD_SITE1; k <= ORC_I_DLTD_CPU2; k++) ip[k] = -1;
+}
+
+/* (theta, phi) of a unit vector the way setNewSegment recovers them */
+static void orc_angles(double gx, double gy, double gz, double *theta,
+                       double *phi) {
+  double t = bmx_acos(gz);
+  double st = bmx_sin(t);
+  double p = 0.0;
+  if (st != 0.0) {
+    p = bmx_acos(gx / st);
+    if (gy < 0.0) p = 2.0 * BMX_PI - p;
+  }
+  *theta = t;
+  *phi = p;
+}
+
+/* g (unit) turned by a random cone angle rtheta = 0.1 u1 about a random
+ * azimuth rphi = 2 pi u2 (bmx_chem_K.H:609-647, 874-905) */
+static void orc_perturb(double gx, double gy, double gz, orc_draws *rng,
+                        double out[3]) {
+  double gtheta = bmx_acos(gz);
+  double gst = bmx_sin(gtheta);
+  double gct = gz;
+  double gphi = 0.0;
+  if (gst != 0.0) {
+    gphi = bmx_acos(gx / gst);
+    if (gy < 0.0) gphi = 2.0 * BMX_PI - gphi;
+  }
+  double gcp = bmx_cos(gphi), gsp = bmx_sin(gphi);
+  double R[3][3] = {{gcp * gst, -gsp, -gcp * gct},
+                    {gsp * gst, gcp, -gsp * gct},
+                    {gct, 0.0, gst}};
+  double rtheta = 0.1 * orc_next(rng);
+  double rphi = 2.0 * BMX_PI * orc_next(rng);
+  double rct = bmx_cos(rtheta), rst = bmx_sin(rtheta);
+  double rcp = bmx_cos(rphi), rsp = bmx_sin(rphi);
+  double rx = rct, ry = rst * rsp, rz = rst * rcp;
+  for (int a = 0; a < 3; a++) out[a] = R[a][0] * rx + R[a][1] * ry + R[a][2] * rz;
+}
+
+static void orc_seg_volume(double *par) {
+  double clen = par[ORC_R_CLENGTH];
+  double rad = par[ORC_R_RADIUS];
+  par[ORC_R_VOL] = BMX_PI * rad * rad * clen;
+  par[ORC_R_AREA] = 2.0 * BMX_PI * rad * clen + 2.0 * BMX_PI * rad * rad;
+}
+
+/* setNewCell (bmx_chem_K.H:383-442) */
+void orc_set_new_cell(orc_particle *po, orc_particle *pn, double overlap,
+                      orc_draws *rng) {
+  int32_t id = pn->id, cpu = pn->cpu;
+  *pn = *po;
+  pn->id = id;
+  pn->cpu = cpu;
+  double *par_o = po->rdata, *par_n = pn->rdata;
+  double x = po->pos[0], y = po->pos[1], z = po->pos[2];
+  double radius = par_n[ORC_R_RADIUS];
+  double theta = 2.0 * orc_next(rng) - 1.0;
+  theta = bmx_acos(theta);
+  double phi = 2.0 * BMX_PI * orc_next(rng);
+  double nx = bmx_sin(theta) * bmx_cos(phi);
+  double ny = bmx_sin(theta) * bmx_sin(phi);
+  double nz = bmx_cos(theta);
+  double scale = 1.0 - overlap;
+  pn->pos[0] = x + scale * 0.5 * nx * radius;
+  pn->pos[1] = y + scale * 0.5 * ny * radius;
+  pn->pos[2] = z + scale * 0.5 * nz * radius;
+  po->pos[0] = x - scale * 0.5 * nx * radius;
+  po->pos[1] = y - scale * 0.5 * ny * radius;
+  po->pos[2] = z - scale * 0.5 * nz * radius;
+  double volume = par_o[ORC_R_VOL] / 2.0;
+  radius = bmx_pow((3.0 * volume / (4.0 * BMX_PI)), 1.0 / 3.0);
+  double area = 4.0 * BMX_PI * radius * radius;
+  double dvdt = par_o[ORC_R_DVDT];
+  double dadt = 2.0 * dvdt / radius;
+  par_o[ORC_R_VOL] = volume;
+  par_n[ORC_R_VOL] = volume;
+  par_o[ORC_R_AREA] = area;
+  par_n[ORC_R_AREA] = area;
+  par_o[ORC_R_DADT] = dadt;
+  par_n[ORC_R_DADT] = dadt;
+  par_o[ORC_R_RADIUS] = radius;
+  par_n[ORC_R_RADIUS] = radius;
+  par_n[ORC_R_BOND_SCALE] = 1.0;
+  pn->idata[ORC_I_FUSE_TIP] = 0;
+  for (int k = 31; k < 34; k++) {
+    par_o[k] = 0.5 * par_o[k];
+    par_n[k] = 0.5 * par_n[k];
+  }
+}
+
+/* setNewSegment (bmx_chem_K.H:457-1011); the gradient is zeroed there
+ * (`if (true)`), so the branch directions come from the axis alone */
+void orc_set_new_segment(orc_particle *po, orc_particle *pn, double split_len,
+                         orc_draws *rng) {
+  int32_t id = pn->id, cpu = pn->cpu;
+  *pn = *po;
+  pn->id = id;
+  pn->cpu = cpu;
+  double *par_o = po->rdata, *par_n = pn->rdata;
+  int32_t *ip_o = po->idata, *ip_n = pn->idata;
+  int nbnd = ip_o[ORC_I_N_BNDS];
+  int position = ip_o[ORC_I_POSITION];
+  double theta = par_o[ORC_R_THETA], phi = par_o[ORC_R_PHI];
+  double ct = bmx_cos(theta), st = bmx_sin(theta);
+  double cp = bmx_cos(phi), sp = bmx_sin(phi);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  if (nbnd == 0) {
+    double x_o = po->pos[0], y_o = po->pos[1], z_o = po->pos[2];
+    double c_length = par_o[ORC_R_CLENGTH];
+    po->pos[0] = x_o + 0.25 * c_length * nx;
+    po->pos[1] = y_o + 0.25 * c_length * ny;
+    po->pos[2] = z_o + 0.25 * c_length * nz;
+    pn->pos[0] = x_o - 0.25 * c_length * nx;
+    pn->pos[1] = y_o - 0.25 * c_length * ny;
+    pn->pos[2] = z_o - 0.25 * c_length * nz;
+    par_o[ORC_R_CLENGTH] = 0.5 * c_length;
+    par_n[ORC_R_CLENGTH] = 0.5 * c_length;
+    ip_o[ORC_I_N_BNDS] = 1;
+    ip_o[ORC_I_SEG1_ID1] = id;
+    ip_o[ORC_I_SEG1_ID2] = cpu;
+    ip_o[ORC_I_SITE1] = 1;
+    ip_o[ORC_I_POSITION] = ORC_TIP;
+    orc_fresh_links(ip_o, 1);
+    ip_n[ORC_I_N_BNDS] = 1;
+    ip_n[ORC_I_SEG1_ID1] = ip_o[ORC_I_ID];
+    ip_n[ORC_I_SEG1_ID2] = ip_o[ORC_I_CPU];
+    ip_n[ORC_I_SITE1] = 2;
+    ip_n[ORC_I_POSITION] = ORC_TIP;
+    orc_fresh_links(ip_n, 1);
+    ip_n[ORC_I_FIX_SITE] = 0;
+    for (int k = 31; k < 34; k++) {
+      par_o[k] = 0.5 * par_o[k];
+      par_n[k] = 0.5 * par_n[k];
+    }
+  } else if (position == ORC_TIP) {
+    int bsite = ip_o[ORC_I_SITE1];
+    double g[3];
+    orc_perturb(nx, ny, nz, rng, g); /* gn == 0: the axis itself */
+    double x = po->pos[0], y = po->pos[1], z = po->pos[2];
+    double c_length = par_o[ORC_R_CLENGTH];
+    double seg_len = c_length - split_len;
+    double gtheta, gphi;
+    if (bsite == 1) {
+      double x1 = x - 0.5 * c_length * nx, y1 = y - 0.5 * c_length * ny,
+             z1 = z - 0.5 * c_length * nz;
+      double x2 = x1 + split_len * nx, y2 = y1 + split_len * ny,
+             z2 = z1 + split_len * nz;
+      po->pos[0] = 0.5 * (x1 + x2);
+      po->pos[1] = 0.5 * (y1 + y2);
+      po->pos[2] = 0.5 * (z1 + z2);
+      par_o[ORC_R_CLENGTH] = split_len;
+      ip_o[nbnd == 1 ? ORC_I_SITE2 : ORC_I_SITE3] = 2;
+      ip_o[ORC_I_POSITION] = ORC_SECOND_2;
+      double ax = x2 + g[0] * seg_len, ay = y2 + g[1] * seg_len,
+             az = z2 + g[2] * seg_len;
+      pn->pos[0] = 0.5 * (x2 + ax);
+      pn->pos[1] = 0.5 * (y2 + ay);
+      pn->pos[2] = 0.5 * (z2 + az);
+      ip_n[ORC_I_SITE1] = 1;
+    } else {
+      double x2 = x + 0.5 * c_length * nx, y2 = y + 0.5 * c_length * ny,
+             z2 = z + 0.5 * c_length * nz;
+      double x1 = x2 - split_len * nx, y1 = y2 - split_len * ny,
+             z1 = z2 - split_len * nz;
+      po->pos[0] = 0.5 * (x1 + x2);
+      po->pos[1] = 0.5 * (y1 + y2);
+      po->pos[2] = 0.5 * (z1 + z2);
+      par_o[ORC_R_CLENGTH] = split_len;
+      ip_o[nbnd == 1 ? ORC_I_SITE2 : ORC_I_SITE3] = 1;
+      ip_o[ORC_I_POSITION] = ORC_SECOND_1;
+      double ax = x1 - g[0] * seg_len, ay = y1 - g[1] * seg_len,
+             az = z1 - g[2] * seg_len;
+      pn->pos[0] = 0.5 * (x1 + ax);
+      pn->pos[1] = 0.5 * (y1 + ay);
+      pn->pos[2] = 0.5 * (z1 + az);
+      ip_n[ORC_I_SITE1] = 2;
+    }
+    par_n[ORC_R_CLENGTH] = seg_len;
+    orc_angles(g[0], g[1], g[2], &gtheta, &gphi);
+    par_n[ORC_R_THETA] = gtheta;
+    par_n[ORC_R_PHI] = gphi;
+    ip_n[ORC_I_POSITION] = ORC_TIP;
+    orc_fresh_links(ip_n, bsite == 1);
+    ip_n[ORC_I_FIX_SITE] = 0;
+    if (nbnd == 1) {
+      ip_o[ORC_I_N_BNDS] = 2;
+      ip_o[ORC_I_SEG2_ID1] = id;
+      ip_o[ORC_I_SEG2_ID2] = cpu;
+    } else {
+      ip_o[ORC_I_N_BNDS] = 3;
+      ip_o[ORC_I_SEG3_ID1] = id;
+      ip_o[ORC_I_SEG3_ID2] = cpu;
+    }
+    ip_n[ORC_I_N_BNDS] = 1;
+    ip_n[ORC_I_ID] = id;
+    ip_n[ORC_I_CPU] = cpu;
+    ip_n[ORC_I_SEG1_ID1] = ip_o[ORC_I_ID];
+    ip_n[ORC_I_SEG1_ID2] = ip_o[ORC_I_CPU];
+    for (int k = 31; k < 34; k++) par_n[k] = 0.0;
+  } else if (position == ORC_SECOND_1 || position == ORC_SECOND_2) {
+    double da = orc_next(rng);
+    da = da > 0.5 ? 0.25 * BMX_PI : -0.25 * BMX_PI;
+    double tphi = phi + da;
+    double g[3];
+    orc_perturb(bmx_cos(tphi), bmx_sin(tphi), 0.0, rng, g);
+    double gntheta, gnphi;
+    orc_angles(g[0], g[1], g[2], &gntheta, &gnphi);
+    double x = po->pos[0], y = po->pos[1], z = po->pos[2];
+    double c_length = par_o[ORC_R_CLENGTH];
+    double x1 = x - 0.5 * c_length * nx, y1 = y - 0.5 * c_length * ny,
+           z1 = z - 0.5 * c_length * nz;
+    double x2 = x + 0.5 * c_length * nx, y2 = y + 0.5 * c_length * ny,
+           z2 = z + 0.5 * c_length * nz;
+    double seg_len = 0.0005;
+    if (position == ORC_SECOND_1) {
+      if (nbnd == 2) ip_o[ORC_I_SITE3] = 1;
+      else if (nbnd == 3) ip_o[ORC_I_SITE4] = 1;
+      double ax = x1 - g[0] * seg_len, ay = y1 - g[1] * seg_len,
+             az = z1 - g[2] * seg_len;
+      pn->pos[0] = 0.5 * (x1 + ax);
+      pn->pos[1] = 0.5 * (y1 + ay);
+      pn->pos[2] = 0.5 * (z1 + az);
+      ip_n[ORC_I_SITE1] = 2;
+      ip_n[ORC_I_SITE2] = 2;
+    } else {
+      if (nbnd == 2) ip_o[ORC_I_SITE3] = 2;
+      else if (nbnd == 3) ip_o[ORC_I_SITE4] = 2;
+      double ax = x2 + g[0] * seg_len, ay = y2 + g[1] * seg_len,
+             az = z2 + g[2] * seg_len;
+      pn->pos[0] = 0.5 * (x2 + ax);
+      pn->pos[1] = 0.5 * (y2 + ay);
+      pn->pos[2] = 0.5 * (z2 + az);
+      ip_n[ORC_I_SITE1] = 1;
+      ip_n[ORC_I_SITE2] = 1;
+    }
+    double radius = par_o[ORC_R_RADIUS];
+    double old_vol = c_length * radius * radius * BMX_PI;
+    par_n[ORC_R_THETA] = gntheta;
+    par_n[ORC_R_PHI] = gnphi;
+    radius = old_vol / (BMX_PI * (c_length + par_n[ORC_R_CLENGTH]));
+    radius = sqrt(radius);
+    par_o[ORC_R_RADIUS] = radius;
+    par_n[ORC_R_RADIUS] = radius;
+    par_n[ORC_R_CLENGTH] = seg_len;
+    if (nbnd == 2) {
+      ip_o[ORC_I_N_BNDS] = 3;
+      ip_o[ORC_I_SEG3_ID1] = id;
+      ip_o[ORC_I_SEG3_ID2] = cpu;
+    } else {
+      ip_o[ORC_I_N_BNDS] = 4;
+      ip_o[ORC_I_SEG4_ID1] = id;
+      ip_o[ORC_I_SEG4_ID2] = cpu;
+    }
+    ip_n[ORC_I_N_BNDS] = 2;
+    ip_n[ORC_I_SEG1_ID1] = ip_o[ORC_I_ID];
+    ip_n[ORC_I_SEG1_ID2] = ip_o[ORC_I_CPU];
+    if (nbnd == 2) {
+      ip_n[ORC_I_SEG2_ID1] = ip_o[ORC_I_SEG2_ID1];
+      ip_n[ORC_I_SEG2_ID2] = ip_o[ORC_I_SEG2_ID2];
+    } else {
+      ip_n[ORC_I_SEG2_ID1] = ip_o[ORC_I_SEG3_ID1];
+      ip_n[ORC_I_SEG2_ID2] = ip_o[ORC_I_SEG3_ID2];
+    }
+    ip_n[ORC_I_POSITION] = ORC_TIP;
+    orc_fresh_links(ip_n, 1);
+    ip_n[ORC_I_FIX_SITE] = 0;
+    ip_o[ORC_I_POSITION] = ORC_INTERIOR;
+    for (int k = 31; k < 34; k++) par_n[k] = 0.0;
+  }
+  ip_n[ORC_I_ID] = id;
+  ip_n[ORC_I_CPU] = cpu;
+  orc_seg_volume(par_o);
+  orc_seg_volume(par_n);
+  par_n[ORC_R_BOND_SCALE] = 1.0;
+  ip_n[ORC_I_FUSE_TIP] = 0;
+}
+
+/* setNewSegments (bmx_chem_K.H:2129-2493): a tip splits into two branches
+ * at +-45 degrees in the horizontal plane */
+void orc_set_new_segments(orc_particle *po, orc_particle *p1, orc_particle *p2,
+                          double split_len, orc_draws *rng) {
+  int32_t id1 = p1->id, cpu1 = p1->cpu, id2 = p2->id, cpu2 = p2->cpu;
+  *p1 = *po;
+  *p2 = *po;
+  p1->id = id1;
+  p1->cpu = cpu1;
+  p2->id = id2;
+  p2->cpu = cpu2;
+  double *par_o = po->rdata, *par_1 = p1->rdata, *par_2 = p2->rdata;
+  int32_t *ip_o = po->idata, *ip_1 = p1->idata, *ip_2 = p2->idata;
+  double theta = par_o[ORC_R_THETA], phi = par_o[ORC_R_PHI];
+  double ct = bmx_cos(theta), st = bmx_sin(theta);
+  double cp = bmx_cos(phi), sp = bmx_sin(phi);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  double da = 0.25 * BMX_PI;
+  double tphi1 = phi + da, tphi2 = phi - da;
+  double g1[3], g2[3];
+  /* both rotations are formed before the draws; the draws go 1 then 2 */
+  orc_perturb(bmx_cos(tphi1), bmx_sin(tphi1), 0.0, rng, g1);
+  orc_perturb(bmx_cos(tphi2), bmx_sin(tphi2), 0.0, rng, g2);
+  double t1, f1, t2, f2;
+  orc_angles(g1[0], g1[1], g1[2], &t1, &f1);
+  orc_angles(g2[0], g2[1], g2[2], &t2, &f2);
+  double x = po->pos[0], y = po->pos[1], z = po->pos[2];
+  double c_length = par_o[ORC_R_CLENGTH];
+  double x1 = x - 0.5 * c_length * nx, y1 = y - 0.5 * c_length * ny,
+         z1 = z - 0.5 * c_length * nz;
+  double x2 = x + 0.5 * c_length * nx, y2 = y + 0.5 * c_length * ny,
+         z2 = z + 0.5 * c_length * nz;
+  double seg_len = 0.5 * (c_length - split_len);
+  double new_len = c_length - 2.0 * seg_len;
+  int s = ip_o[ORC_I_SITE1] == 1 ? 1 : 2; /* the bonded end's site */
+  double sg = s == 1 ? 1.0 : -1.0;
+  double bx = s == 1 ? x1 : x2, by = s == 1 ? y1 : y2, bz = s == 1 ? z1 : z2;
+  double newx, newy, newz;
+  if (s == 1) {
+    newx = x1 + new_len * nx;
+    newy = y1 + new_len * ny;
+    newz = z1 + new_len * nz;
+  } else {
+    newx = x2 - new_len * nx;
+    newy = y2 - new_len * ny;
+    newz = z2 - new_len * nz;
+  }
+  po->pos[0] = 0.5 * (bx + newx);
+  po->pos[1] = 0.5 * (by + newy);
+  po->pos[2] = 0.5 * (bz + newz);
+  orc_particle *pp[2] = {p1, p2};
+  double *gg[2] = {g1, g2};
+  for (int b = 0; b < 2; b++) {
+    double ax, ay, az;
+    if (s == 1) {
+      ax = newx + gg[b][0] * seg_len;
+      ay = newy + gg[b][1] * seg_len;
+      az = newz + gg[b][2] * seg_len;
+    } else {
+      ax = newx - gg[b][0] * seg_len;
+      ay = newy - gg[b][1] * seg_len;
+      az = newz - gg[b][2] * seg_len;
+    }
+    pp[b]->pos[0] = 0.5 * (newx + ax);
+    pp[b]->pos[1] = 0.5 * (newy + ay);
+    pp[b]->pos[2] = 0.5 * (newz + az);
+  }
+  (void)sg;
+  ip_1[ORC_I_SITE1] = s;
+  ip_1[ORC_I_SEG1_ID1] = ip_o[ORC_I_ID];
+  ip_1[ORC_I_SEG1_ID2] = ip_o[ORC_I_CPU];
+  ip_1[ORC_I_SITE2] = s;
+  ip_1[ORC_I_SEG2_ID1] = id2;
+  ip_1[ORC_I_SEG2_ID2] = cpu2;
+  ip_1[ORC_I_N_BNDS] = 2;
+  ip_2[ORC_I_SITE1] = s;
+  ip_2[ORC_I_SEG1_ID1] = ip_o[ORC_I_ID];
+  ip_2[ORC_I_SEG1_ID2] = ip_o[ORC_I_CPU];
+  ip_2[ORC_I_SITE2] = s;
+  ip_2[ORC_I_SEG2_ID1] = id1;
+  ip_2[ORC_I_SEG2_ID2] = cpu1;
+  ip_2[ORC_I_N_BNDS] = 2;
+  const int far = 3 - s; /* the free end, where the branches attach */
+  if (ip_o[ORC_I_N_BNDS] == 1) {
+    ip_o[ORC_I_SITE2] = far;
+    ip_o[ORC_I_SEG2_ID1] = id1;
+    ip_o[ORC_I_SEG2_ID2] = cpu1;
+    ip_o[ORC_I_SITE3] = far;
+    ip_o[ORC_I_SEG3_ID1] = id2;
+    ip_o[ORC_I_SEG3_ID2] = cpu2;
+    ip_o[ORC_I_N_BNDS] = 3;
+  } else {
+    ip_o[ORC_I_SITE3] = far;
+    ip_o[ORC_I_SEG3_ID1] = id1;
+    ip_o[ORC_I_SEG3_ID2] = cpu1;
+    ip_o[ORC_I_SITE4] = far;
+    ip_o[ORC_I_SEG4_ID1] = id2;
+    ip_o[ORC_I_SEG4_ID2] = cpu2;
+    ip_o[ORC_I_N_BNDS] = 4;
+  }
+  par_o[ORC_R_CLENGTH] = new_len;
+  par_1[ORC_R_CLENGTH] = seg_len;
+  par_2[ORC_R_CLENGTH] = seg_len;
+  par_1[ORC_R_THETA] = t1;
+  par_1[ORC_R_PHI] = f1;
+  par_2[ORC_R_THETA] = t2;
+  par_2[ORC_R_PHI] = f2;
+  for (int b = 0; b < 2; b++) {
+    int32_t *ip = pp[b]->idata;
+    ip[ORC_I_POSITION] = ORC_TIP;
+    ip[ORC_I_FUSE_FLAG] = 0;
+    ip[ORC_I_SPLIT_FLAG] = 0;
+    ip[23] = -1;
+    ip[24] = -1;
+    for (int k = ORC_I_DLTD_SITE1; k <= ORC_I_DLTD_CPU2; k++) ip[k] = -1;
+    ip[ORC_I_FIX_SITE] = 0;
+  }
+  ip_o[ORC_I_POSITION] = ORC_INTERIOR;
+  ip_1[ORC_I_ID] = id1;
+  ip_1[ORC_I_CPU] = cpu1;
+  ip_2[ORC_I_ID] = id2;
+  ip_2[ORC_I_CPU] = cpu2;
+  orc_seg_volume(par_o);
+  orc_seg_volume(par_1);
+  par_1[ORC_R_BOND_SCALE] = 1.0;
+  ip_1[ORC_I_FUSE_TIP] = 0;
+  orc_seg_volume(par_2);
+  par_2[ORC_R_BOND_SCALE] = 1.0;
+  ip_2[ORC_I_FUSE_TIP] = 0;
+}
+
+int64_t orc_split_particles(orc_particle *p, int64_t n, int64_t capacity,
+                            const orc_geom *g, const double sp[7],
+                            uint64_t seed, uint32_t call, int32_t next_id,
+                            int32_t cpu) {
+  /* split_particles on one tile: the checks, new particles at
+   * n + (exclusive prefix of the counts), ids next_id + that offset, then
+   * the periodic wrap of Redistribute.  sp = {max_vol, max_len, max_rad,
+   * brnch_prob, split_prob, overlap, split_len}.  Returns the new count,
+   * or -1 if capacity is short. */
+  int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  if (!cnt) return -1;
+  int64_t total = 0;
+  for (int64_t i = 0; i < n; i++) {
+    orc_draws r = {seed, call, p[i].id, p[i].cpu, 0};
+    cnt[i] = orc_check_split(p[i].rdata, p[i].idata, sp, &r);
+    total += cnt[i];
+  }
+  if (n + total > capacity) {
+    free(cnt);
+    return -1;
+  }
+  int64_t off = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!cnt[i]) continue;
+    orc_draws r = {seed, call, p[i].id, p[i].cpu, 1};
+    orc_particle *a = &p[n + off];
+    a->id = next_id + (int32_t)off;
+    a->cpu = cpu;
+    if (cnt[i] == 1) {
+      if (p[i].idata[ORC_I_CELL_TYPE] == ORC_YEAST)
+        orc_set_new_cell(&p[i], a, sp[5], &r);
+      else
+        orc_set_new_segment(&p[i], a, sp[6], &r);
+      a->idata[ORC_I_ID] = a->id;
+      a->idata[ORC_I_CPU] = a->cpu;
+    } else {
+      orc_particle *b = &p[n + off + 1];
+      b->id = next_id + (int32_t)off + 1;
+      b->cpu = cpu;
+      orc_set_new_segments(&p[i], a, b, sp[6], &r);
+    }
+    off += cnt[i];
+  }
+  free(cnt);
+  orc_wrap(p, n + total, g);
+  return n + total;
+}

@@ -25,6 +25,7 @@
  *   orc_volume_fraction           src/des/bmx_calc_volume_fraction.cpp:14-86
  *   orc_chem_update               src/chemistry/bmx_chem_K.H:1358-1940
  *   orc_mesh_to_particle          src/des/bmx_calc_txfr.cpp:72-396
+ *   orc_split_particles           src/des/bmx_split_particles.cpp:16-166
  *
  * Every function is compiled with -ffp-contract=off so that each floating
  * point operation is rounded exactly as written, in the reference's order.
@@ -201,6 +202,14 @@ int orc_mesh_to_particle(orc_particle *p, int64_t n, const orc_geom *g,
 /* interpolation of ncomp components of a ghosted field at x */
 void orc_interp(const double *x, const double *f, const orc_geom *g, int scheme,
                 int ncomp, double *vals);
+/* split_particles (src/des/bmx_split_particles.cpp:16-166): records
+ * p[0..n) with room for 'capacity'; sp = {max_vol, max_len, max_rad,
+ * brnch_prob, split_prob, overlap, split_len}; random numbers from
+ * bmx_uniform(seed, call, id, cpu, draw).  Returns the new count or -1. */
+int64_t orc_split_particles(orc_particle *p, int64_t n, int64_t capacity,
+                            const orc_geom *g, const double sp[7],
+                            uint64_t seed, uint32_t call, int32_t next_id,
+                            int32_t cpu);
 /* x^y of the shared implementation (bmx_math.h) */
 double orc_pow(double x, double y);
 /* OpenMP threads the force pass uses (1 without OpenMP) */

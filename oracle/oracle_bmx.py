@@ -101,6 +101,8 @@ def lib() -> C.CDLL:
                                                C.c_int, C.c_int, C.c_int]),
             "orc_pow": (C.c_double, [C.c_double, C.c_double]),
             "orc_interp": (None, [D, D, P(Geom), C.c_int, C.c_int, D]),
+            "orc_split_particles": (I64, [vp, I64, I64, P(Geom), D, C.c_uint64,
+                                          C.c_uint32, C.c_int32, C.c_int32]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -377,3 +379,20 @@ def interp(x, field, geom: Geom, scheme: int):
 
 def pow_(x: float, y: float) -> float:
     return lib().orc_pow(float(x), float(y))
+
+
+def split_particles(recs, geom: Geom, sp, seed: int, call: int, next_id: int, cpu: int):
+    """split_particles on a copy of recs; returns the grown record array.
+    sp = (max_vol, max_len, max_rad, brnch_prob, split_prob, overlap,
+    split_len)."""
+    assert recs.dtype == RECORD_DTYPE
+    n = len(recs)
+    out = np.zeros(3 * n + 1, dtype=RECORD_DTYPE)
+    out[:n] = recs
+    spa = np.ascontiguousarray(sp, np.float64)
+    m = lib().orc_split_particles(out.ctypes.data_as(C.c_void_p), n, len(out),
+                                  C.byref(geom), _d(spa), int(seed), int(call),
+                                  int(next_id), int(cpu))
+    if m < 0:
+        raise RuntimeError("orc_split_particles failed")
+    return out[:m].copy()

@@ -1,0 +1,108 @@
+"""GPU division and branching against the oracle.
+
+Both sides draw from the same counter-based generator keyed by the parent's
+(id, cpu), place the new particles after the existing ones in record order
+and re-bind, so the grown record arrays must match bit for bit, and the
+mechanics steps after a split must give the oracle's positions.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from boltzmannmfx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HOOD = 1.1 * (2 * 2e-4 + 1e-2)
+
+
+def sp(max_vol=1.0e-9, brnch=0.3, split=0.3):
+    return [max_vol, 35e-4, 2.5e-4, brnch, split, 0.2, 30e-4]
+
+
+def test_yeast_division_matches_oracle(bmx, oracle, gpu):
+    col = W.c1_small(seed=31)
+    r = col.recs["rdata"]
+    r[:, 5] = 4.0 / 3.0 * math.pi * r[:, 0] ** 3
+    r[:, 4] = 4.0 * math.pi * r[:, 0] ** 2
+    r[:, 31:34] = np.random.default_rng(2).uniform(0, 1e-12, (len(r), 3))
+    max_vol = float(np.quantile(r[:, 5], 0.7))          # about 30 % divide
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+    ctx.bind(col.recs.copy())
+    ctx.evolve(col.dt, 3)
+    ctx.sync()
+    orecs = ctx._recs.copy()
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, (1, 1, 1))
+    nid = int(orecs["id"].max()) + 1
+    grecs = ctx.split_particles(sp(max_vol), 77, 5, nid, 0)
+    out = oracle.split_particles(orecs, g, sp(max_vol), 77, 5, nid, 0)
+    assert 500 < len(out) - len(orecs) < 700
+    assert len(grecs) == len(out)
+    assert np.array_equal(grecs, out)
+    # the mechanics go on with the daughters
+    st = oracle.State(out, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
+                                            ctx.bin_dims()),
+                      col.fpar, oracle.NBR_PAIR, col.skin, 10)
+    ctx.evolve(col.dt, 12)
+    st.evolve(col.dt, 12)
+    assert np.array_equal(ctx.positions(), out["pos"])
+    ctx.close()
+
+
+def test_hyphal_growth_rounds_match_oracle(bmx, oracle, gpu):
+    col = W.c3_hyphae(n=3000, nroots=30, seed=41)
+    r = col.recs["rdata"]
+    r[:, 0] = 2.5e-4
+    r[:, 1] = np.maximum(r[:, 1], 36e-4)
+    r[:, 5] = math.pi * r[:, 0] ** 2 * r[:, 1]
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    ctx.set_neighbor_params(mode=bmx.NBR_STENCIL, rebuild_every=1, neighborhood=HOOD,
+                            max_neighbors=384)
+    ctx.bind(col.recs.copy())
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    orecs = col.recs.copy()
+    st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, HOOD, 1)
+    ctx.init_bonds()
+    assert st.init_bonds() == 0
+    nid = int(orecs["id"].max()) + 1
+    kinds = set()
+    for call in range(3):
+        ctx.evolve(col.dt, 2)
+        st.evolve(col.dt, 2)
+        st.close()
+        grecs = ctx.split_particles(sp(), 123, call, nid, 0)
+        out = oracle.split_particles(orecs, g, sp(), 123, call, nid, 0)
+        assert len(grecs) == len(out) > len(orecs)
+        assert np.array_equal(grecs, out)
+        kinds |= set(np.unique(orecs["idata"][:, 19]).tolist())
+        nid += len(out) - len(orecs)
+        orecs = out
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, HOOD, 1)
+    assert {0, 1, 2, 3} <= kinds                # tips, both second positions, interior
+    ctx.evolve(col.dt, 3)
+    st.evolve(col.dt, 3)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    ctx.close()
+
+
+def test_no_split_leaves_the_binding(bmx, oracle, gpu):
+    col = W.c1_small(seed=32)
+    col.recs["rdata"][:, 5] = 1e-20
+    ctx = bmx.Context(0)
+    ctx.set_force_params(col.fpar)
+    ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+    recs = col.recs.copy()
+    ctx.bind(recs)
+    out = ctx.split_particles(sp(), 1, 1, 10**6, 0)
+    assert out is recs
+    ctx.evolve(col.dt, 2)
+    ctx.sync()                                   # into the bound buffer
+    assert np.abs(recs["pos"] - col.recs["pos"]).max() > 0
+    ctx.close()
