@@ -252,7 +252,7 @@ def mesh_bench(bmx, W, n: int = 1_000_000, n_cell=(64, 64, 32), calls: int = 20,
     after one warm-up."""
     import ctypes as C
     import numpy as np
-    hip = C.CDLL("libamdhip64.so")
+    hip = _hip_runtime()
     col = W.c2_large(n)
     col.n_cell = tuple(n_cell)
     r = col.recs["rdata"]
@@ -340,11 +340,21 @@ def mesh_bench(bmx, W, n: int = 1_000_000, n_cell=(64, 64, 32), calls: int = 20,
     return out
 
 
+def _hip_runtime():
+    """The HIP runtime the library links (/opt/rocm), not another copy a
+    framework may have loaded under the same name."""
+    import ctypes as C
+    try:
+        return C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    except OSError:
+        return C.CDLL("libamdhip64.so")
+
+
 def calibration_copy(nbytes: int) -> None:
     """One device-to-device copy of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    hip = _hip_runtime()
     a, b = C.c_void_p(), C.c_void_p()
     assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0

@@ -52,7 +52,8 @@ class HipArray:
     def _lib(cls):
         import ctypes as C
         if cls._hip is None:
-            for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            # the runtime the library links (torch may have loaded another)
+            for name in ("/opt/rocm/lib/libamdhip64.so", "libamdhip64.so"):
                 try:
                     cls._hip = C.CDLL(name)
                     break

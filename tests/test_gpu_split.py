@@ -54,10 +54,12 @@ def test_yeast_division_matches_oracle(bmx, oracle, gpu):
 
 
 def test_hyphal_growth_rounds_match_oracle(bmx, oracle, gpu):
+    # segments already at the limits (radius 2.5e-4, length 36e-4 >= 35e-4)
+    # so that every tip divides; chains whose ends touch, for InitBonds
     col = W.c3_hyphae(n=3000, nroots=30, seed=41)
+    col.recs = bmx.generate_hyphae(3000, 30, 36e-4, 2.5e-4, 0.05, 30.0, 60.0, 41,
+                                   col.prob_lo, col.prob_hi, 0.0)
     r = col.recs["rdata"]
-    r[:, 0] = 2.5e-4
-    r[:, 1] = np.maximum(r[:, 1], 36e-4)
     r[:, 5] = math.pi * r[:, 0] ** 2 * r[:, 1]
     ctx = bmx.Context(0)
     ctx.set_force_params(col.fpar)
@@ -84,7 +86,7 @@ def test_hyphal_growth_rounds_match_oracle(bmx, oracle, gpu):
         nid += len(out) - len(orecs)
         orecs = out
         st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, HOOD, 1)
-    assert {0, 1, 2, 3} <= kinds                # tips, both second positions, interior
+    assert {0, 3} <= kinds and kinds & {1, 2}   # tips, segments behind them, interior
     ctx.evolve(col.dt, 3)
     st.evolve(col.dt, 3)
     assert np.array_equal(ctx.positions(), orecs["pos"])
