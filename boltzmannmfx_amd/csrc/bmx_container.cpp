@@ -158,6 +158,41 @@ BMXParticleContainer::BMXParticleContainer(const Deck &deck, int device)
                            stencil_params(deck), device) {
   deck.exchange_params(xpar_);
   deck.init_conc(conc_);
+  have_sp_ = deck.split_params(sp_);
+  seed_ = (uint64_t)deck.integer_or("bmx.split_seed", 0);
+}
+
+bool Deck::split_params(bmx_split_params &p) const {
+  if (!has("chem_species.max_vol")) return false;
+  p.max_vol = real("chem_species.max_vol");
+  p.max_seg_length = real_or("chem_species.max_seg_length", 0.0035);
+  p.seg_split_length = real_or("chem_species.seg_split_length", 0.0030);
+  p.max_seg_radius = real_or("chem_species.max_seg_radius", 0.00025);
+  p.branching_probability = real_or("chem_species.branching_probability", 0.0001);
+  p.splitting_probability = real_or("chem_species.splitting_probability", 0.1);
+  p.overlap = 0.2;
+  return true;
+}
+
+void BMXParticleContainer::set_split_params(const bmx_split_params &p,
+                                            uint64_t seed) {
+  sp_ = p;
+  have_sp_ = true;
+  seed_ = seed;
+}
+
+void BMXParticleContainer::split_particles(double /*time*/) {
+  if (!have_sp_) throw Error(BMX_ERR_STATE, "no split parameters (chem_species.max_vol)");
+  int64_t added = 0;
+  check(bmx_split_particles(ctx_, &sp_, seed_, split_calls_++, (int32_t)next_id_, 0,
+                            &added));
+  if (added > 0) {
+    // NextID advances by the new particles (bmx_split_particles.cpp:72-81)
+    next_id_ += added;
+    n_ += added;
+    host_.resize((size_t)n_ * (size_t)bmx_default_layout().stride_bytes);
+    dirty_ = true;
+  }
 }
 
 void Deck::exchange_params(double xpar[3]) const {
@@ -218,6 +253,12 @@ void BMXParticleContainer::InitParticles(const void *records, int64_t n) {
   n_ = n;
   check(bmx_bind_particles_aos(ctx_, host_.data(), n, &L, 0));
   dirty_ = false;
+  next_id_ = 1;
+  for (int64_t i = 0; i < n; i++) {
+    int32_t id;
+    std::memcpy(&id, host_.data() + (size_t)i * L.stride_bytes + L.id_offset, sizeof id);
+    if ((int64_t)id + 1 > next_id_) next_id_ = (int64_t)id + 1;
+  }
 }
 
 void BMXParticleContainer::InitBonds() {

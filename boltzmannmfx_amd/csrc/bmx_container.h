@@ -4,6 +4,7 @@
 //   BMXParticleContainer::InitParticlesAscii   src/des/bmx_pc_init.cpp:17-132
 //   BMXParticleContainer::InitBonds            src/des/bmx_pc_interaction.cpp:500-668
 //   BMXParticleContainer::EvolveParticles      src/des/bmx_pc_interaction.cpp:13-495
+//   BMXParticleContainer::split_particles      src/des/bmx_split_particles.cpp:16-166
 //   BMXParticleContainer::WriteAsciiFileForInit src/des/bmx_pc.H:154
 //   Deck / read_deck                           the ParmParse keys the path reads
 //                                              (src/chemistry/bmx_cell_interaction.cpp:54-93,
@@ -53,6 +54,11 @@ struct Deck {
   void exchange_params(double xpar[3]) const;
   // fluid.init_conc_species (FLUID::init_conc), zeros where absent
   void init_conc(double conc[3]) const;
+  // chem_species.max_vol, max_seg_length, max_seg_radius,
+  // branching_probability, splitting_probability, seg_split_length with the
+  // reference's defaults (src/mods/bmx_chem_species_parms.cpp:55-66) and
+  // the overlap 0.2 (src/chemistry/bmx_chem.cpp:110); false without max_vol
+  bool split_params(bmx_split_params &p) const;
 };
 
 // Parses "key = v1 v2 ..." lines; '#' starts a comment; quotes are dropped.
@@ -82,6 +88,12 @@ class BMXParticleContainer {
   // rates of the deck (or set_exchange_params).
   void ParticleExchange(double dt);
   void set_exchange_params(const double xpar[3]);
+  // Division and branching with the deck's (or set_split_params) limits;
+  // new particles get ids from NextID (one past the largest id read) and
+  // cpu 0.  Random numbers: bmx_uniform(seed, call, id, cpu), the call
+  // counting the split steps of this container (seed: bmx.split_seed).
+  void split_particles(double time);
+  void set_split_params(const bmx_split_params &p, uint64_t seed = 0);
   void set_init_conc(const double conc[3]);
   // n steps without copying back in between.
   void EvolveSteps(double dt, int nsteps);
@@ -103,6 +115,11 @@ class BMXParticleContainer {
   bmx_geometry geom_{};
   double xpar_[3] = {0.0, 0.0, 0.0};
   double conc_[3] = {0.0, 0.0, 0.0};
+  bmx_split_params sp_{};
+  bool have_sp_ = false;
+  uint64_t seed_ = 0;
+  uint32_t split_calls_ = 0;
+  int64_t next_id_ = 1;
   std::vector<uint8_t> host_;
   int64_t n_ = 0;
   bool dirty_ = false;

@@ -3,11 +3,11 @@
 //   bmx_run <deck> [max_step] [out.dat]
 //
 // Mirrors the DEM part of the reference's time loop (src/main.cpp:313-342,
-// src/timestepping/bmx_evolve.cpp:93-97): read the deck, InitParticlesAscii
+// src/timestepping/bmx_evolve.cpp:93-99): read the deck, InitParticlesAscii
 // from particles.input_file (relative to the deck's directory), InitBonds at
-// time 0, then EvolveParticles and ParticleExchange every step of
-// bmx.fixed_dt.  The fluid, growth (split_particles) and fusion steps of
-// bmx::Evolve are not part of this path.  Prints
+// time 0, then EvolveParticles, split_particles (when the deck sets
+// chem_species.max_vol) and ParticleExchange every step of bmx.fixed_dt.
+// The fluid and fusion steps of bmx::Evolve are not part of this path.  Prints
 // one line per step with the time per step and, at the end, writes the
 // particles in the input format.
 #include <chrono>
@@ -33,6 +33,8 @@ int main(int argc, char **argv) {
     if (!input.empty() && input[0] != '/') input = dir + "/" + input;
 
     bmx::BMXParticleContainer pc(deck);
+    bmx_split_params sp;
+    const bool split = deck.split_params(sp);
     pc.InitParticlesAscii(input);
     std::printf("Read %lld particles from %s\n", (long long)pc.NumberOfParticles(),
                 input.c_str());
@@ -42,10 +44,12 @@ int main(int argc, char **argv) {
       if (time == 0.0) pc.InitBonds();
       int nsub = 0;
       pc.EvolveParticles(dt, nsub);
-      pc.ParticleExchange(dt);  // bmx_evolve.cpp:99 (split_particles before it is growth)
+      if (split) pc.split_particles(time);  // bmx_evolve.cpp:98
+      pc.ParticleExchange(dt);              // bmx_evolve.cpp:99
       auto t1 = std::chrono::steady_clock::now();
       time += dt;
-      std::printf("Step %d time %g Time per step %.6f\n", step + 1, time,
+      std::printf("Step %d time %g particles %lld Time per step %.6f\n", step + 1, time,
+                  (long long)pc.NumberOfParticles(),
                   std::chrono::duration<double>(t1 - t0).count());
     }
     if (argc > 3) pc.WriteAsciiFileForInit(argv[3]);

@@ -4,7 +4,8 @@ bmx_run reads a BMX input deck (exec/fungi, exec/wall_test,
 exec/nlev_large_test, copied under tests/golden/ref_decks), builds
 BMXParticleContainer on the finest level mesh with the reference's per-step
 stencil list, calls InitParticlesAscii, InitBonds at time 0 and
-EvolveParticles and ParticleExchange each step, and writes the particles
+EvolveParticles, split_particles and ParticleExchange each step, and writes
+the particles
 back in the input format (which holds no species: those are compared by
 test_gpu_exchange.py).  The same run on the oracle must give the same
 bits.
@@ -71,13 +72,27 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
     recs["rdata"][:, 25:25 + len(conc)] = conc
     # chem_species.mass_transfer_A/B/C (bmx_chem.cpp:100-105)
     xpar = [f("chem_species.mass_transfer_" + s, default=0.0) for s in "ABC"]
+    # split_particles limits (bmx_chem_species_parms.cpp:55-66, bmx_chem.cpp:110)
+    sp = [vol, f("chem_species.max_seg_length", default=0.0035),
+          f("chem_species.max_seg_radius", default=0.00025),
+          f("chem_species.branching_probability", default=0.0001),
+          f("chem_species.splitting_probability", default=0.1), 0.2,
+          f("chem_species.seg_split_length", default=0.0030)]
     g = oracle.make_geom(lo, hi, per, n_cell)
     st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
     st.init_bonds()
     dt = f("bmx.fixed_dt")
-    for _ in range(steps):   # EvolveParticles, then ParticleExchange
+    next_id = int(recs["id"].max()) + 1
+    for step in range(steps):  # EvolveParticles, split_particles, ParticleExchange
         st.evolve(dt, 1)
+        grown = oracle.split_particles(recs, g, sp, 0, step, next_id, 0)
+        if len(grown) > len(recs):
+            st.close()
+            next_id += len(grown) - len(recs)
+            recs = grown
+            st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
         st.exchange(xpar, dt)
     got = bmx.read_particles_ascii(out)
+    assert len(got) == len(recs)
     assert np.array_equal(got["pos"], recs["pos"])
     assert np.array_equal(got["rdata"][:, :23], recs["rdata"][:, :23])
