@@ -2255,3 +2255,396 @@ int64_t orc_split_particles(orc_particle *p, int64_t n, int64_t capacity,
   orc_wrap(p, n + total, g);
   return n + total;
 }
+
+/* ---- fusion (EvaluateTipFusion, EvaluateInteriorFusion, CleanupFusion) ---
+ * src/des/bmx_pc_fusion.cpp:16-493 with checkTipFusion, checkInteriorFusion,
+ * setSplitSegment, cleanupFusionBond and incrementBondScale
+ * (src/chemistry/bmx_chem_K.H:80-347, 1012-1250, 1252-1357, 2494-2508).
+ * fus = {fusion_probability, max_fusion_separation, bond_scaling_increment,
+ * max_seg_length}.  Neighbours are read as they stood when a pass began. */
+
+static int orc_bonds_shared(const int32_t *a, const int32_t *b) {
+  for (int i = 0; i < a[ORC_I_N_BNDS]; i++)
+    for (int j = 0; j < b[ORC_I_N_BNDS]; j++)
+      if (a[ORC_I_SEG1_ID1 + i] == b[ORC_I_SEG1_ID1 + j] &&
+          a[ORC_I_SEG1_ID2 + i] == b[ORC_I_SEG1_ID2 + j])
+        return 1;
+  return 0;
+}
+
+/* checkTipFusion (bmx_chem_K.H:80-263): tip i fuses into the side of an
+ * unbonded non-tip segment j when its free end is the closest point, lies
+ * within max_fusion_separation of j's interior, and a draw is below the
+ * fusion probability.  Returns 1 on fusion. */
+static int orc_check_tip_fusion(const double rij[3], double *ir, int32_t *ii,
+                                const double *jr, const int32_t *jj,
+                                const double fus[4], orc_draws *rng) {
+  if (!(ii[ORC_I_CELL_TYPE] == ORC_FUNGI && ii[ORC_I_POSITION] == ORC_TIP &&
+        ii[ORC_I_FUSE_FLAG] == 0 && jj[ORC_I_CELL_TYPE] == ORC_FUNGI &&
+        jj[ORC_I_POSITION] != ORC_TIP))
+    return 0;
+  if (orc_bonds_shared(ii, jj)) return 0;
+  double c1 = ir[ORC_R_CLENGTH], c2 = jr[ORC_R_CLENGTH];
+  double ct = bmx_cos(ir[ORC_R_THETA]), st = bmx_sin(ir[ORC_R_THETA]);
+  double cp = bmx_cos(ir[ORC_R_PHI]), sp = bmx_sin(ir[ORC_R_PHI]);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  double xi1 = rij[0] - 0.5 * nx * c1, yi1 = rij[1] - 0.5 * ny * c1,
+         zi1 = rij[2] - 0.5 * nz * c1;
+  double xi2 = rij[0] + 0.5 * nx * c1, yi2 = rij[1] + 0.5 * ny * c1,
+         zi2 = rij[2] + 0.5 * nz * c1;
+  ct = bmx_cos(jr[ORC_R_THETA]);
+  st = bmx_sin(jr[ORC_R_THETA]);
+  cp = bmx_cos(jr[ORC_R_PHI]);
+  sp = bmx_sin(jr[ORC_R_PHI]);
+  nx = st * cp;
+  ny = st * sp;
+  nz = ct;
+  double xj1 = -0.5 * nx * c2, yj1 = -0.5 * ny * c2, zj1 = -0.5 * nz * c2;
+  double xj2 = 0.5 * nx * c2, yj2 = 0.5 * ny * c2, zj2 = 0.5 * nz * c2;
+  double tau_i, tau_j;
+  orc_min_distance(xi1, yi1, zi1, xi2, yi2, zi2, xj1, yj1, zj1, xj2, yj2, zj2,
+                   &tau_i, &tau_j);
+  if (!(tau_j > 0.0 && tau_j < 1.0)) return 0;
+  if (!((ii[ORC_I_SITE1] == 1 && tau_i == 1.0) ||
+        (ii[ORC_I_SITE1] == 2 && tau_i == 0.0)))
+    return 0;
+  double rhoi[3] = {xi2 - xi1, yi2 - yi1, zi2 - zi1};
+  double rhoj[3] = {xj2 - xj1, yj2 - yj1, zj2 - zj1};
+  double ri[3] = {xi1 + tau_i * rhoi[0], yi1 + tau_i * rhoi[1], zi1 + tau_i * rhoi[2]};
+  double rj[3] = {xj1 + tau_j * rhoj[0], yj1 + tau_j * rhoj[1], zj1 + tau_j * rhoj[2]};
+  double rx = ri[0] - rj[0], ry = ri[1] - rj[1], rz = ri[2] - rj[2];
+  double r = sqrt(rx * rx + ry * ry + rz * rz);
+  if (!(r < fus[1])) return 0;
+  if (!(orc_next(rng) < fus[0])) return 0;
+  int i_bnds = ii[ORC_I_N_BNDS];
+  ii[ORC_I_FUSE_FLAG] = 1;
+  ii[23] = jj[ORC_I_ID]; /* fuse_id */
+  ii[24] = jj[ORC_I_CPU]; /* fuse_cpu */
+  ii[ORC_I_POSITION] = ORC_INTERIOR;
+  ir[23] = tau_j; /* tau_split */
+  int site = ii[ORC_I_SITE1] == 1 ? 2 : 1;
+  if (i_bnds == 1) {
+    ii[ORC_I_SITE2] = site;
+    ii[ORC_I_SEG2_ID1] = jj[ORC_I_ID];
+    ii[ORC_I_SEG2_ID2] = jj[ORC_I_CPU];
+  } else if (i_bnds == 2) {
+    ii[ORC_I_SITE3] = site;
+    ii[ORC_I_SEG3_ID1] = jj[ORC_I_ID];
+    ii[ORC_I_SEG3_ID2] = jj[ORC_I_CPU];
+  }
+  ii[ORC_I_N_BNDS] = i_bnds + 1;
+  ir[ORC_R_BOND_SCALE] = 0.0;
+  ii[ORC_I_FUSE_TIP] = site;
+  return 1;
+}
+
+/* checkInteriorFusion (bmx_chem_K.H:264-347): segment i is the one a tip j
+ * fused into (j's fuse_id) and they are not bonded */
+static int orc_check_interior_fusion(double *ir, int32_t *ii, const double *jr,
+                                     const int32_t *jj) {
+  if (!(ii[ORC_I_POSITION] != ORC_TIP && jj[ORC_I_FUSE_FLAG] == 1)) return 0;
+  if (!(ii[ORC_I_ID] == jj[23] && ii[ORC_I_CPU] == jj[24])) return 0;
+  if (orc_bonds_shared(ii, jj)) return 0;
+  ii[ORC_I_SPLIT_FLAG] = 1;
+  ii[23] = jj[ORC_I_ID];
+  ii[24] = jj[ORC_I_CPU];
+  ir[23] = jr[23];
+  return 1;
+}
+
+/* setSplitSegment (bmx_chem_K.H:1012-1250): segment o splits at tau_split;
+ * the far part becomes a new segment meeting o and the fused tip there */
+static void orc_set_split_segment(orc_particle *po, orc_particle *pn) {
+  int32_t id = pn->id, cpu = pn->cpu;
+  *pn = *po;
+  pn->id = id;
+  pn->cpu = cpu;
+  double *par_o = po->rdata, *par_n = pn->rdata;
+  int32_t *io = po->idata, *in_ = pn->idata;
+  int32_t i_id[4], i_cpu[4], i_site[4];
+  for (int k = 0; k < 4; k++) {
+    i_id[k] = io[ORC_I_SEG1_ID1 + k];
+    i_cpu[k] = io[ORC_I_SEG1_ID2 + k];
+    i_site[k] = io[ORC_I_SITE1 + k];
+  }
+  double ct = bmx_cos(par_o[ORC_R_THETA]), st = bmx_sin(par_o[ORC_R_THETA]);
+  double cp = bmx_cos(par_o[ORC_R_PHI]), sp = bmx_sin(par_o[ORC_R_PHI]);
+  double nx = st * cp, ny = st * sp, nz = ct;
+  double c_length = par_o[ORC_R_CLENGTH];
+  double rox1 = po->pos[0] - 0.5 * nx * c_length;
+  double roy1 = po->pos[1] - 0.5 * ny * c_length;
+  double roz1 = po->pos[2] - 0.5 * nz * c_length;
+  double rnx2 = po->pos[0] + 0.5 * nx * c_length;
+  double rny2 = po->pos[1] + 0.5 * ny * c_length;
+  double rnz2 = po->pos[2] + 0.5 * nz * c_length;
+  double tau = par_o[23];
+  double rox2 = rox1 + tau * nx * c_length;
+  double roy2 = roy1 + tau * ny * c_length;
+  double roz2 = roz1 + tau * nz * c_length;
+  po->pos[0] = 0.5 * (rox1 + rox2);
+  po->pos[1] = 0.5 * (roy1 + roy2);
+  po->pos[2] = 0.5 * (roz1 + roz2);
+  pn->pos[0] = 0.5 * (rox2 + rnx2);
+  pn->pos[1] = 0.5 * (roy2 + rny2);
+  pn->pos[2] = 0.5 * (roz2 + rnz2);
+  in_[ORC_I_SITE1] = 1;
+  in_[ORC_I_SEG1_ID1] = io[ORC_I_ID];
+  in_[ORC_I_SEG1_ID2] = io[ORC_I_CPU];
+  in_[ORC_I_SITE2] = 1;
+  in_[ORC_I_SEG2_ID1] = io[23];
+  in_[ORC_I_SEG2_ID2] = io[24];
+  int old_bnds = io[ORC_I_N_BNDS];
+  int32_t n_id[2] = {0, 0}, n_cpu[2] = {0, 0};
+  int nb = 0;
+  for (int i = 0; i < old_bnds; i++) {
+    if (i_site[i] == 2) {
+      n_id[nb] = i_id[i];
+      n_cpu[nb] = i_cpu[i];
+      nb++;
+      if (nb == 1) {
+        io[ORC_I_DLTD_SITE1] = 2;
+        io[ORC_I_DLTD_ID1] = n_id[0];
+        io[ORC_I_DLTD_CPU1] = n_cpu[0];
+      } else if (nb == 2) {
+        io[ORC_I_DLTD_SITE2] = 2;
+        io[ORC_I_DLTD_ID2] = n_id[1];
+        io[ORC_I_DLTD_CPU2] = n_cpu[1];
+      }
+      if (nb == 2) break;
+    }
+  }
+  nb += 2;
+  in_[ORC_I_SITE3] = 2;
+  in_[ORC_I_SEG3_ID1] = n_id[0];
+  in_[ORC_I_SEG3_ID2] = n_cpu[0];
+  if (nb > 3) {
+    in_[ORC_I_SITE4] = 2;
+    in_[ORC_I_SEG4_ID1] = n_id[1];
+    in_[ORC_I_SEG4_ID2] = n_cpu[1];
+  }
+  in_[ORC_I_N_BNDS] = nb;
+  nb = 0;
+  for (int i = 0; i < old_bnds; i++) {
+    if (i_site[i] == 1) {
+      n_id[nb] = i_id[i];
+      n_cpu[nb] = i_cpu[i];
+      nb++;
+      if (nb == 2) break;
+    }
+  }
+  io[ORC_I_SITE1] = 1;
+  io[ORC_I_SEG1_ID1] = n_id[0];
+  io[ORC_I_SEG1_ID2] = n_cpu[0];
+  if (nb > 1) {
+    io[ORC_I_SITE2] = 1;
+    io[ORC_I_SEG2_ID1] = n_id[1];
+    io[ORC_I_SEG2_ID2] = n_cpu[1];
+  }
+  io[ORC_I_N_BNDS] = nb + 2;
+  if (nb == 1) {
+    io[ORC_I_SITE2] = 2;
+    io[ORC_I_SEG2_ID1] = id;
+    io[ORC_I_SEG2_ID2] = cpu;
+    io[ORC_I_SITE3] = 2;
+    io[ORC_I_SEG3_ID1] = io[23];
+    io[ORC_I_SEG3_ID2] = io[24];
+  } else {
+    io[ORC_I_SITE3] = 2;
+    io[ORC_I_SEG3_ID1] = id;
+    io[ORC_I_SEG3_ID2] = cpu;
+    io[ORC_I_SITE4] = 2;
+    io[ORC_I_SEG4_ID1] = io[23];
+    io[ORC_I_SEG4_ID2] = io[24];
+  }
+  par_o[ORC_R_CLENGTH] = tau * c_length;
+  par_n[ORC_R_CLENGTH] = (1.0 - tau) * c_length;
+  io[ORC_I_POSITION] = ORC_INTERIOR;
+  in_[ORC_I_POSITION] = ORC_INTERIOR;
+  in_[ORC_I_ID] = id;
+  in_[ORC_I_CPU] = cpu;
+  io[ORC_I_FUSE_FLAG] = 0;
+  io[ORC_I_SPLIT_FLAG] = 0;
+  io[23] = -1;
+  io[24] = -1;
+  in_[ORC_I_FUSE_FLAG] = 0;
+  in_[ORC_I_SPLIT_FLAG] = 0;
+  in_[23] = -1;
+  in_[24] = -1;
+  for (int k = ORC_I_DLTD_SITE1; k <= ORC_I_DLTD_CPU2; k++) in_[k] = -1;
+  in_[ORC_I_FIX_SITE] = 2;
+  in_[ORC_I_NEW_FLAG] = 1;
+  orc_seg_volume(par_o);
+  orc_seg_volume(par_n);
+  par_n[ORC_R_BOND_SCALE] = 1.0;
+  in_[ORC_I_FUSE_TIP] = 0;
+  for (int k = 31; k < 34; k++) {
+    par_o[k] = tau * par_o[k];
+    par_n[k] = (1.0 - tau) * par_n[k];
+  }
+}
+
+/* cleanupFusionBond (bmx_chem_K.H:1252-1357): i drops its bond to j when j
+ * split and recorded i among the bonds it deleted */
+static void orc_cleanup_fusion_bond(int32_t *ip, const int32_t *jp) {
+  int i_bnds = ip[ORC_I_N_BNDS];
+  int32_t i_id = ip[ORC_I_ID], i_cpu = ip[ORC_I_CPU];
+  int32_t iid[4], icpu[4];
+  for (int k = 0; k < 4; k++) {
+    iid[k] = ip[ORC_I_SEG1_ID1 + k];
+    icpu[k] = ip[ORC_I_SEG1_ID2 + k];
+  }
+  int32_t j_id = jp[ORC_I_ID], j_cpu = jp[ORC_I_CPU];
+  for (int i = 0; i < i_bnds; i++) {
+    if (!(iid[i] == j_id && icpu[i] == j_cpu)) continue;
+    int idx = i;
+    if (!(ip[ORC_I_NEW_FLAG] == 0 && ip[ORC_I_FUSE_FLAG] == 0 &&
+          jp[ORC_I_SPLIT_FLAG] == 1))
+      continue;
+    if (!((jp[ORC_I_DLTD_ID1] == i_id && jp[ORC_I_DLTD_CPU1] == i_cpu) ||
+          (jp[ORC_I_DLTD_ID2] == i_id && jp[ORC_I_DLTD_CPU2] == i_cpu)))
+      continue;
+    int site = 0;
+    for (int k = idx; k < 3; k++) {
+      if (site == 0) site = ip[ORC_I_SITE1 + k];
+      ip[ORC_I_SEG1_ID1 + k] = ip[ORC_I_SEG1_ID1 + k + 1];
+      ip[ORC_I_SEG1_ID2 + k] = ip[ORC_I_SEG1_ID2 + k + 1];
+      ip[ORC_I_SITE1 + k] = ip[ORC_I_SITE1 + k + 1];
+    }
+    if (site == 0) site = ip[ORC_I_SITE4];
+    ip[ORC_I_SEG4_ID1] = 0;
+    ip[ORC_I_SEG4_ID2] = 0;
+    ip[ORC_I_SITE4] = 0;
+    i_bnds--;
+    ip[ORC_I_N_BNDS] = i_bnds;
+    ip[ORC_I_FIX_SITE] = site;
+  }
+}
+
+/* incrementBondScale (bmx_chem_K.H:2494-2508) */
+static void orc_increment_bond_scale(double *par, int32_t *ip, double inc) {
+  if (par[ORC_R_BOND_SCALE] < 1.0) {
+    par[ORC_R_BOND_SCALE] += inc;
+    if (par[ORC_R_BOND_SCALE] > 1.0) par[ORC_R_BOND_SCALE] = 1.0;
+  }
+  if (par[ORC_R_BOND_SCALE] >= 1.0) ip[ORC_I_FUSE_TIP] = 0;
+}
+
+int64_t orc_fusion(orc_particle *p, int64_t n, int64_t capacity,
+                   const orc_geom *g, const double fpar[9], int nbr_mode,
+                   double nbr_radius, const double fus[4], uint64_t seed,
+                   uint32_t call, int32_t next_id, int32_t cpu, int *fused) {
+  /* EvaluateTipFusion, then (if a tip fused) EvaluateInteriorFusion and
+   * CleanupFusion, as bmx::Evolve calls them (bmx_evolve.cpp:100-104).
+   * Returns the new count, -1 on allocation failure, -2 if capacity is
+   * short. */
+  const double small_number = 1.0e-15;
+  int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(capacity + 1));
+  int32_t *idx = NULL;
+  orc_particle *snap = (orc_particle *)malloc(sizeof(orc_particle) * (size_t)(n > 0 ? n : 1));
+  int32_t *cnt = (int32_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+  if (!off || !snap || !cnt) {
+    free(off);
+    free(snap);
+    free(cnt);
+    return -1;
+  }
+  g_pair_wa = fpar[ORC_F_WA];
+  *fused = 0;
+  int64_t result = n;
+  /* tip fusion: neighbours are non-tips, which this pass leaves alone */
+  if (orc_build_neighbors(p, n, g, nbr_mode, nbr_radius, off, &idx)) {
+    result = -1;
+    goto done;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    orc_draws r = {seed, call, p[i].id, p[i].cpu, 0};
+    for (int64_t k = off[i]; k < off[i + 1]; k++) {
+      const orc_particle *pj = &p[idx[k]];
+      double d[3];
+      orc_pair_separation(p[i].pos, pj->pos, g, d);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double r_lm = orc_max_interaction_distance(p[i].rdata, pj->rdata, p[i].idata,
+                                                 pj->idata, fpar[ORC_F_WA]);
+      if (r2 <= (r_lm - small_number) * (r_lm - small_number))
+        *fused |= orc_check_tip_fusion(d, p[i].rdata, p[i].idata, pj->rdata,
+                                       pj->idata, fus, &r);
+    }
+  }
+  if (!*fused) goto done;
+  /* interior fusion over the same rows (the positions have not moved) */
+  memcpy(snap, p, sizeof(orc_particle) * (size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int64_t k = off[i]; k < off[i + 1]; k++) {
+      const orc_particle *pj = &snap[idx[k]];
+      double d[3];
+      orc_pair_separation(p[i].pos, pj->pos, g, d);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double r_lm = orc_max_interaction_distance(p[i].rdata, pj->rdata, p[i].idata,
+                                                 pj->idata, fpar[ORC_F_WA]);
+      if (r2 <= (r_lm - small_number) * (r_lm - small_number))
+        cnt[i] += orc_check_interior_fusion(p[i].rdata, p[i].idata, pj->rdata,
+                                            pj->idata);
+    }
+  }
+  {
+    int64_t add = 0;
+    for (int64_t i = 0; i < n; i++) add += cnt[i] == 1;
+    if (n + add > capacity) {
+      result = -2;
+      goto done;
+    }
+    int64_t o = 0;
+    for (int64_t i = 0; i < n; i++) {
+      if (cnt[i] == 1) {
+        orc_particle *a = &p[n + o];
+        a->id = next_id + (int32_t)o;
+        a->cpu = cpu;
+        orc_set_split_segment(&p[i], a);
+        a->idata[ORC_I_ID] = a->id;
+        a->idata[ORC_I_CPU] = a->cpu;
+        p[i].idata[ORC_I_FUSE_FLAG] = 0;
+        p[i].idata[ORC_I_SPLIT_FLAG] = 1;
+        p[i].idata[ORC_I_NEW_FLAG] = 0;
+        p[i].idata[23] = -1;
+        p[i].idata[24] = -1;
+        a->idata[ORC_I_FUSE_FLAG] = 0;
+        a->idata[ORC_I_SPLIT_FLAG] = 0;
+        a->idata[ORC_I_NEW_FLAG] = 1;
+        a->idata[23] = -1;
+        a->idata[24] = -1;
+        o++;
+      } else if (cnt[i] == 0) {
+        p[i].idata[ORC_I_SPLIT_FLAG] = 0;
+      }
+    }
+    result = n + add;
+  }
+  orc_wrap(p, result, g);
+  /* cleanup over rows rebuilt with the new segments */
+  free(idx);
+  idx = NULL;
+  if (orc_build_neighbors(p, result, g, nbr_mode, nbr_radius, off, &idx)) {
+    result = -1;
+    goto done;
+  }
+  {
+    const double r_lm = 1.5 * fus[3];
+    for (int64_t i = 0; i < result; i++) {
+      orc_increment_bond_scale(p[i].rdata, p[i].idata, fus[2]);
+      for (int64_t k = off[i]; k < off[i + 1]; k++) {
+        const orc_particle *pj = &p[idx[k]];
+        double d[3];
+        orc_pair_separation(p[i].pos, pj->pos, g, d);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 <= (r_lm - small_number) * (r_lm - small_number))
+          orc_cleanup_fusion_bond(p[i].idata, pj->idata);
+      }
+    }
+  }
+done:
+  free(off);
+  free(idx);
+  free(snap);
+  free(cnt);
+  return result;
+}

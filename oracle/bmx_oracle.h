@@ -26,6 +26,7 @@
  *   orc_chem_update               src/chemistry/bmx_chem_K.H:1358-1940
  *   orc_mesh_to_particle          src/des/bmx_calc_txfr.cpp:72-396
  *   orc_split_particles           src/des/bmx_split_particles.cpp:16-166
+ *   orc_fusion                    src/des/bmx_pc_fusion.cpp:16-493
  *
  * Every function is compiled with -ffp-contract=off so that each floating
  * point operation is rounded exactly as written, in the reference's order.
@@ -210,6 +211,16 @@ int64_t orc_split_particles(orc_particle *p, int64_t n, int64_t capacity,
                             const orc_geom *g, const double sp[7],
                             uint64_t seed, uint32_t call, int32_t next_id,
                             int32_t cpu);
+/* EvaluateTipFusion / EvaluateInteriorFusion / CleanupFusion
+ * (src/des/bmx_pc_fusion.cpp:16-493) over neighbour lists built like the
+ * state's (nbr_mode, nbr_radius); fus = {fusion_probability,
+ * max_fusion_separation, bond_scaling_increment, max_seg_length}.  *fused
+ * tells whether a tip fused.  Returns the new count, -1 on failure, -2 if
+ * capacity is short. */
+int64_t orc_fusion(orc_particle *p, int64_t n, int64_t capacity,
+                   const orc_geom *g, const double fpar[9], int nbr_mode,
+                   double nbr_radius, const double fus[4], uint64_t seed,
+                   uint32_t call, int32_t next_id, int32_t cpu, int *fused);
 /* x^y of the shared implementation (bmx_math.h) */
 double orc_pow(double x, double y);
 /* OpenMP threads the force pass uses (1 without OpenMP) */

@@ -103,6 +103,9 @@ def lib() -> C.CDLL:
             "orc_interp": (None, [D, D, P(Geom), C.c_int, C.c_int, D]),
             "orc_split_particles": (I64, [vp, I64, I64, P(Geom), D, C.c_uint64,
                                           C.c_uint32, C.c_int32, C.c_int32]),
+            "orc_fusion": (I64, [vp, I64, I64, P(Geom), D, C.c_int, C.c_double, D,
+                                 C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,
+                                 P(C.c_int)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -396,3 +399,23 @@ def split_particles(recs, geom: Geom, sp, seed: int, call: int, next_id: int, cp
     if m < 0:
         raise RuntimeError("orc_split_particles failed")
     return out[:m].copy()
+
+
+def fusion(recs, geom: Geom, fpar, mode: int, radius: float, fus, seed: int, call: int,
+           next_id: int, cpu: int):
+    """Tip fusion, interior fusion and cleanup on a copy of recs; returns
+    (records, whether a tip fused).  fus = (fusion_probability,
+    max_fusion_separation, bond_scaling_increment, max_seg_length)."""
+    assert recs.dtype == RECORD_DTYPE
+    n = len(recs)
+    out = np.zeros(2 * n + 1, dtype=RECORD_DTYPE)
+    out[:n] = recs
+    fp = np.ascontiguousarray(fpar, np.float64)
+    fu = np.ascontiguousarray(fus, np.float64)
+    fused = C.c_int(0)
+    m = lib().orc_fusion(out.ctypes.data_as(C.c_void_p), n, len(out), C.byref(geom),
+                         _d(fp), int(mode), float(radius), _d(fu), int(seed), int(call),
+                         int(next_id), int(cpu), C.byref(fused))
+    if m < 0:
+        raise RuntimeError(f"orc_fusion failed: {m}")
+    return out[:m].copy(), bool(fused.value)
