@@ -183,6 +183,8 @@ def load_library() -> C.CDLL:
         "bmx_exchange": (C.c_int, [vp, P(C.c_double), C.c_double]),
         "bmx_split_particles": (C.c_int, [vp, P(C.c_double), C.c_uint64, C.c_uint32,
                                           C.c_int32, C.c_int32, P(C.c_int64)]),
+        "bmx_fusion": (C.c_int, [vp, P(C.c_double), C.c_uint64, C.c_uint32, C.c_int32,
+                                 C.c_int32, P(C.c_int32), P(C.c_int64)]),
         "bmx_calc_volume_fraction": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
         "bmx_calc_txfr_fluid": (C.c_int, [vp, C.c_int32, vp, C.c_int32]),
         "bmx_count_particles_per_cell": (C.c_int, [vp, vp, C.c_int32]),
@@ -349,6 +351,23 @@ class Context:
         self._recs = out
         self.n = m
         return out
+
+    def fusion(self, params, seed: int, call: int, next_id: int, cpu: int = 0):
+        """Tip fusion, interior fusion and cleanup.  params =
+        (fusion_probability, max_fusion_separation, bond_scaling_increment,
+        max_seg_length).  Returns (records, fused, new segments); the
+        records are the context's bound buffer afterwards."""
+        fp = np.ascontiguousarray(params, dtype=np.float64)
+        assert fp.size == 4
+        fused, nn = C.c_int32(), C.c_int64()
+        _check(self.lib.bmx_fusion(self.h, _dptr(fp), int(seed), int(call), int(next_id),
+                                   int(cpu), C.byref(fused), C.byref(nn)))
+        if nn.value > 0:
+            m = int(self.lib.bmx_num_particles(self.h))
+            self._recs = np.zeros(m, dtype=RECORD_DTYPE)
+            self.n = m
+        self.sync()
+        return self._recs, bool(fused.value), int(nn.value)
 
     def txfr_particle(self, chempar, x_k, vf, dt: float, nloop: int = 4,
                       cnc_scheme: int = DEPOSIT_ONE_TO_ONE,

@@ -23,7 +23,7 @@ LIB_NAME = "libbmx_mi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 SOURCES = ["bmx_kernels.hip", "bmx_capi.hip", "bmx_adjoint.hip", "bmx_exchange.hip",
-           "bmx_mesh.hip", "bmx_split.hip"]
+           "bmx_mesh.hip", "bmx_split.hip", "bmx_fusion.hip"]
 HOST_SOURCES = ["bmx_container.cpp", "bmx_io.cpp", "bmx_checkpoint.cpp"]
 
 

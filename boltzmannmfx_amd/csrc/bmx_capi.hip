@@ -1778,6 +1778,39 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
 }
 
 // ------------------------------------------------------ division, branching
+namespace {
+
+// Binds the device records src[0, n) again (a copy is taken first): new
+// sorted arrays and bins, the rows rebuilt at the next step.  The caller's
+// record buffer and its host/device kind are kept.
+bmx_status rebind_records(bmx_ctx *c, const uint8_t *src, int64_t n) {
+  const size_t bytes = (size_t)n * (size_t)c->lay.stride;
+  DevBuf<uint8_t> copy;
+  Releaser<uint8_t> rc{copy};
+  HIPCHK(copy.alloc(bytes));
+  if (bytes)
+    HIPCHK(hipMemcpyAsync(copy.p, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  bmx_layout L;
+  L.stride_bytes = c->lay.stride;
+  L.pos_offset = c->lay.pos;
+  L.rdata_offset = c->lay.rdata;
+  L.id_offset = c->lay.id;
+  L.cpu_offset = c->lay.cpu;
+  L.idata_offset = c->lay.idata;
+  L.n_real = c->n_real;
+  L.n_int = c->n_int;
+  const bool was_on_device = c->aos_on_device;
+  void *user = c->user_aos;
+  const bmx_geometry g0 = c->colony_geom[0];
+  int64_t cs[2] = {0, n};
+  bmx_status st = bind_common(c, copy.p, cs, 1, &g0, &L, 1);
+  c->aos_on_device = was_on_device;
+  c->user_aos = user;
+  return st;
+}
+
+}  // namespace
+
 extern "C" bmx_status bmx_split_particles(bmx_ctx *c, const bmx_split_params *sp,
                                           uint64_t seed, uint32_t call,
                                           int32_t next_id, int32_t cpu,
@@ -1826,24 +1859,104 @@ extern "C" bmx_status bmx_split_particles(bmx_ctx *c, const bmx_split_params *sp
     HIPCHK(launch_split_apply(grown.p, c->lay, n, start.p, p, seed, call,
                               next_id, cpu, s));
     HIPCHK(hipStreamSynchronize(s));
-    // bind the grown records (new sorted arrays, rows rebuilt at the next
-    // step); the caller's buffer no longer fits them
-    bmx_layout L;
-    L.stride_bytes = c->lay.stride;
-    L.pos_offset = c->lay.pos;
-    L.rdata_offset = c->lay.rdata;
-    L.id_offset = c->lay.id;
-    L.cpu_offset = c->lay.cpu;
-    L.idata_offset = c->lay.idata;
-    L.n_real = c->n_real;
-    L.n_int = c->n_int;
-    const bool was_on_device = c->aos_on_device;
-    const bmx_geometry g0 = c->colony_geom[0];
-    int64_t cs[2] = {0, n + total};
-    st = bind_common(c, grown.p, cs, 1, &g0, &L, 1);
+    // new sorted arrays, rows rebuilt at the next step; the caller's buffer
+    // no longer fits the records
+    st = rebind_records(c, grown.p, n + total);
     c->user_aos = nullptr;
-    c->aos_on_device = was_on_device;
     if (st != BMX_OK) return st;
   }
   return BMX_OK;
+}
+
+// ------------------------------------------------------------------ fusion
+extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
+                                 uint64_t seed, uint32_t call, int32_t next_id,
+                                 int32_t cpu, int32_t *fused, int64_t *n_new) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (!fp || !fused || !n_new) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (c->ncol != 1)
+    return fail(BMX_ERR_STATE, "fusion needs a single-colony context");
+  if (c->n_real < rec::first_data + 9 || c->n_int < rec::cpu + 1)
+    return fail(BMX_ERR_BAD_ARG,
+                "the record layout lacks the species or topology fields");
+  const double fus[4] = {fp->fusion_probability, fp->max_fusion_separation,
+                         fp->bond_scaling_increment, fp->max_seg_length};
+  for (double v : fus)
+    if (!std::isfinite(v)) return fail(BMX_ERR_BAD_ARG, "fusion parameters must be finite");
+  *fused = 0;
+  *n_new = 0;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  reset_graph(c);
+  // records and rows at the current positions (EvaluateTipFusion rebuilds)
+  st = pack_records(c);
+  if (st != BMX_OK) return st;
+  st = rebuild(c);
+  if (st != BMX_OK) return st;
+  st = check_flags(c);
+  if (st != BMX_OK) return st;
+  const BoxGeom g0 = c->boxes[0];
+  const double wa = c->fpar[0];
+  const int64_t n = c->n;
+  DevBuf<int32_t> flag;
+  Releaser<int32_t> rf{flag};
+  HIPCHK(flag.alloc(1));
+  HIPCHK(launch_fill_i32(flag.p, 1, 0, s));
+  HIPCHK(launch_tip_fusion(c->d_aos.p, c->lay, n, c->pos[c->cd].p,
+                           c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap, g0,
+                           wa, fus, seed, call, flag.p, s));
+  int32_t any = 0;
+  HIPCHK(hipMemcpyAsync(&any, flag.p, sizeof any, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (!any) return BMX_OK;  // no record changed
+  *fused = 1;
+  // interior fusion over the same rows, decisions then updates
+  DevBuf<int32_t> cnt, one, start, scratch;
+  DevBuf<int4> dec;
+  DevBuf<double> dtau;
+  Releaser<int32_t> r1{cnt}, r2{one}, r3{start}, r4{scratch};
+  Releaser<int4> r5{dec};
+  Releaser<double> r6{dtau};
+  HIPCHK(cnt.alloc((size_t)n + 1));
+  HIPCHK(one.alloc((size_t)n + 1));
+  HIPCHK(start.alloc((size_t)n + 1));
+  HIPCHK(scratch.alloc((size_t)scan_scratch_elems(n) + 1));
+  HIPCHK(dec.alloc((size_t)n));
+  HIPCHK(dtau.alloc((size_t)n));
+  HIPCHK(launch_interior_fusion(c->d_aos.p, c->lay, n, c->pos[c->cd].p,
+                                c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
+                                g0, wa, cnt.p, dec.p, dtau.p, one.p, s));
+  HIPCHK(launch_scan(one.p, start.p, n, scratch.p, s));
+  int32_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, start.p + n, sizeof total, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if ((int64_t)next_id + total > (int64_t)INT32_MAX)
+    return fail(BMX_ERR_BAD_ARG, "particle ids would overflow int32");
+  const size_t stride = (size_t)c->lay.stride;
+  DevBuf<uint8_t> grown;
+  Releaser<uint8_t> r7{grown};
+  HIPCHK(grown.alloc((size_t)(n + total) * stride));
+  HIPCHK(hipMemcpyAsync(grown.p, c->d_aos.p, (size_t)n * stride,
+                        hipMemcpyDeviceToDevice, s));
+  HIPCHK(launch_fusion_split(grown.p, c->lay, n, cnt.p, start.p, next_id, cpu, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_new = total;
+  st = rebind_records(c, grown.p, n + total);
+  if (total > 0) c->user_aos = nullptr;
+  if (st != BMX_OK) return st;
+  grown.release();
+  // cleanup over rows rebuilt with the new segments
+  st = rebuild(c);
+  if (st != BMX_OK) return st;
+  st = check_flags(c);
+  if (st != BMX_OK) return st;
+  HIPCHK(launch_fusion_cleanup(c->d_aos.p, c->lay, c->n, c->pos[c->cd].p,
+                               c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
+                               c->boxes[0], fus[2], 1.5 * fus[3], s));
+  HIPCHK(hipStreamSynchronize(s));
+  void *user = c->user_aos;
+  st = rebind_records(c, c->d_aos.p, c->n);
+  c->user_aos = user;
+  return st;
 }

@@ -184,4 +184,29 @@ hipError_t launch_split_apply(uint8_t *aos, AosLayout lay, int64_t n,
                               uint64_t seed, uint32_t call, int32_t next_id,
                               int32_t cpu, hipStream_t s);
 
+// ---- fusion (bmx_fusion.hip); fus = {fusion_probability,
+// max_fusion_separation, bond_scaling_increment, max_seg_length}
+hipError_t launch_tip_fusion(uint8_t *aos, AosLayout lay, int64_t n,
+                             const double4 *pos, const int32_t *oid,
+                             const int32_t *nbr, const int32_t *ncnt,
+                             int64_t ncap, const BoxGeom &g, double wa,
+                             const double fus[4], uint64_t seed, uint32_t call,
+                             int32_t *fused, hipStream_t s);
+// cnt[o]: tips that fused into o; one[o] = (cnt[o] == 1), the new segments
+hipError_t launch_interior_fusion(uint8_t *aos, AosLayout lay, int64_t n,
+                                  const double4 *pos, const int32_t *oid,
+                                  const int32_t *nbr, const int32_t *ncnt,
+                                  int64_t ncap, const BoxGeom &g, double wa,
+                                  int32_t *cnt, int4 *dec, double *dtau,
+                                  int32_t *one, hipStream_t s);
+hipError_t launch_fusion_split(uint8_t *aos, AosLayout lay, int64_t n,
+                               const int32_t *cnt, const int32_t *start,
+                               int32_t next_id, int32_t cpu, hipStream_t s);
+hipError_t launch_fusion_cleanup(uint8_t *aos, AosLayout lay, int64_t n,
+                                 const double4 *pos, const int32_t *oid,
+                                 const int32_t *nbr, const int32_t *ncnt,
+                                 int64_t ncap, const BoxGeom &g,
+                                 double scale_inc, double reach,
+                                 hipStream_t s);
+
 }  // namespace bmx

@@ -316,6 +316,28 @@ bmx_status bmx_split_particles(bmx_ctx *ctx, const bmx_split_params *params,
                                uint64_t seed, uint32_t call, int32_t next_id,
                                int32_t cpu, int64_t *n_new);
 
+/* --- fusion ----------------------------------------------------------------
+ * EvaluateTipFusion, then, if a tip fused, EvaluateInteriorFusion and
+ * CleanupFusion (src/des/bmx_pc_fusion.cpp:16-493, called by bmx::Evolve
+ * at src/timestepping/bmx_evolve.cpp:100-104) over rows rebuilt at the
+ * current positions: a tip whose free end comes within
+ * max_fusion_separation of the side of a segment fuses with
+ * fusion_probability, the segment splits there (new segments get ids
+ * next_id + k and the given cpu; *n_new returns their number), stale bonds
+ * are removed and bond scales grow by bond_scaling_increment.  *fused tells
+ * whether a tip fused.  Random numbers as bmx_split_particles.  When
+ * segments were added the next bmx_sync_to_aos needs a buffer of
+ * bmx_num_particles() records. */
+typedef struct {
+  double fusion_probability;     /* chem_species.fusion_probability (0) */
+  double max_fusion_separation;  /* chem_species.max_fusion_separation (5e-4) */
+  double bond_scaling_increment; /* chem_species.bond_scaling_increment (1e-3) */
+  double max_seg_length;         /* chem_species.max_seg_length (3.5e-3) */
+} bmx_fusion_params;
+bmx_status bmx_fusion(bmx_ctx *ctx, const bmx_fusion_params *params,
+                      uint64_t seed, uint32_t call, int32_t next_id, int32_t cpu,
+                      int32_t *fused, int64_t *n_new);
+
 bmx_status bmx_get_fc(bmx_ctx *ctx, double *out6n);
 /* positions, n*3 doubles (x,y,z per particle) */
 bmx_status bmx_get_positions(bmx_ctx *ctx, double *out3n);
