@@ -21,7 +21,7 @@ def pair(bmx, oracle, recs):
     ctx = bmx.Context(0)
     ctx.set_force_params(FPAR)
     ctx.set_geometry(LO, HI, (0, 0, 0))
-    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=5, skin=SKIN,
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=1, skin=SKIN,
                             max_neighbors=256)
     ctx.bind(recs.copy())
     g = oracle.make_geom(LO, HI, (0, 0, 0), ctx.bin_dims())
@@ -55,9 +55,9 @@ def test_fusion_matches_oracle(bmx, oracle, gpu, n, prob):
     assert added == len(out) - len(recs) > 0
     assert np.array_equal(grecs, out)
     # the mechanics go on from the fused network
-    st = oracle.State(out, g, np.array(FPAR), oracle.NBR_PAIR, SKIN, 5)
-    ctx.evolve(1e-4, 7)
-    st.evolve(1e-4, 7)
+    st = oracle.State(out, g, np.array(FPAR), oracle.NBR_PAIR, SKIN, 1)
+    ctx.evolve(0.01, 7)
+    st.evolve(0.01, 7)
     assert np.array_equal(ctx.positions(), out["pos"])
     ctx.close()
 

@@ -15,7 +15,8 @@ import math
 import numpy as np
 import pytest
 
-FPAR = [5.0e-5, 1.0, -1.0, 1e-5, 1.0, 0.0, 1e-2, 1e-5, 0.0]   # W_A = 5e-5
+# hyphal force parameters (W_A 5e-5, wall far below, no gravity, drag 200)
+FPAR = [5.0e-5, 1.0e6, -1.0, 1e-4, 1e5, 0.0, 1e-2, 5e-4, 200.0]
 FUS = [1.0, 6.0e-4, 1.0e-3, 35e-4]
 L = 30e-4
 
