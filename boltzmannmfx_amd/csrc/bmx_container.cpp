@@ -159,7 +159,32 @@ BMXParticleContainer::BMXParticleContainer(const Deck &deck, int device)
   deck.exchange_params(xpar_);
   deck.init_conc(conc_);
   have_sp_ = deck.split_params(sp_);
+  deck.fusion_params(fp_);
   seed_ = (uint64_t)deck.integer_or("bmx.split_seed", 0);
+}
+
+void Deck::fusion_params(bmx_fusion_params &p) const {
+  p.fusion_probability = real_or("chem_species.fusion_probability", 0.0);
+  p.max_fusion_separation = real_or("chem_species.max_fusion_separation", 5.0e-4);
+  p.bond_scaling_increment = real_or("chem_species.bond_scaling_increment", 0.001);
+  p.max_seg_length = real_or("chem_species.max_seg_length", 0.0035);
+}
+
+void BMXParticleContainer::set_fusion_params(const bmx_fusion_params &p) { fp_ = p; }
+
+bool BMXParticleContainer::Fusion() {
+  int32_t fused = 0;
+  int64_t added = 0;
+  // the fusion draws use calls of their own, after those of the splits
+  check(bmx_fusion(ctx_, &fp_, seed_ ^ 0x66757369ull, fusion_calls_++, (int32_t)next_id_,
+                   0, &fused, &added));
+  if (added > 0) {
+    next_id_ += added;
+    n_ += added;
+    host_.resize((size_t)n_ * (size_t)bmx_default_layout().stride_bytes);
+  }
+  if (fused) dirty_ = true;
+  return fused != 0;
 }
 
 bool Deck::split_params(bmx_split_params &p) const {

@@ -5,6 +5,8 @@
 //   BMXParticleContainer::InitBonds            src/des/bmx_pc_interaction.cpp:500-668
 //   BMXParticleContainer::EvolveParticles      src/des/bmx_pc_interaction.cpp:13-495
 //   BMXParticleContainer::split_particles      src/des/bmx_split_particles.cpp:16-166
+//   BMXParticleContainer::EvaluateTipFusion +
+//     EvaluateInteriorFusion + CleanupFusion   src/des/bmx_pc_fusion.cpp:16-493
 //   BMXParticleContainer::WriteAsciiFileForInit src/des/bmx_pc.H:154
 //   Deck / read_deck                           the ParmParse keys the path reads
 //                                              (src/chemistry/bmx_cell_interaction.cpp:54-93,
@@ -59,6 +61,10 @@ struct Deck {
   // reference's defaults (src/mods/bmx_chem_species_parms.cpp:55-66) and
   // the overlap 0.2 (src/chemistry/bmx_chem.cpp:110); false without max_vol
   bool split_params(bmx_split_params &p) const;
+  // chem_species.fusion_probability, max_fusion_separation,
+  // bond_scaling_increment, max_seg_length with the reference's defaults
+  // (src/chemistry/bmx_chem.cpp:106-112)
+  void fusion_params(bmx_fusion_params &p) const;
 };
 
 // Parses "key = v1 v2 ..." lines; '#' starts a comment; quotes are dropped.
@@ -94,6 +100,11 @@ class BMXParticleContainer {
   // counting the split steps of this container (seed: bmx.split_seed).
   void split_particles(double time);
   void set_split_params(const bmx_split_params &p, uint64_t seed = 0);
+  // The fusion steps of bmx::Evolve (src/timestepping/bmx_evolve.cpp:100-104):
+  // EvaluateTipFusion, and when a tip fused EvaluateInteriorFusion and
+  // CleanupFusion; returns whether a tip fused.
+  bool Fusion();
+  void set_fusion_params(const bmx_fusion_params &p);
   void set_init_conc(const double conc[3]);
   // n steps without copying back in between.
   void EvolveSteps(double dt, int nsteps);
@@ -117,6 +128,8 @@ class BMXParticleContainer {
   double conc_[3] = {0.0, 0.0, 0.0};
   bmx_split_params sp_{};
   bool have_sp_ = false;
+  bmx_fusion_params fp_{0.0, 5.0e-4, 1.0e-3, 3.5e-3};
+  uint32_t fusion_calls_ = 0;
   uint64_t seed_ = 0;
   uint32_t split_calls_ = 0;
   int64_t next_id_ = 1;

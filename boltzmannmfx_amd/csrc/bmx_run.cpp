@@ -6,8 +6,9 @@
 // src/timestepping/bmx_evolve.cpp:93-99): read the deck, InitParticlesAscii
 // from particles.input_file (relative to the deck's directory), InitBonds at
 // time 0, then EvolveParticles, split_particles (when the deck sets
-// chem_species.max_vol) and ParticleExchange every step of bmx.fixed_dt.
-// The fluid and fusion steps of bmx::Evolve are not part of this path.  Prints
+// chem_species.max_vol), ParticleExchange and the fusion steps every step of
+// bmx.fixed_dt.  The fluid step of bmx::Evolve is not part of this path.
+// Prints
 // one line per step with the time per step and, at the end, writes the
 // particles in the input format.
 #include <chrono>
@@ -46,6 +47,7 @@ int main(int argc, char **argv) {
       pc.EvolveParticles(dt, nsub);
       if (split) pc.split_particles(time);  // bmx_evolve.cpp:98
       pc.ParticleExchange(dt);              // bmx_evolve.cpp:99
+      pc.Fusion();                          // bmx_evolve.cpp:100-104
       auto t1 = std::chrono::steady_clock::now();
       time += dt;
       std::printf("Step %d time %g particles %lld Time per step %.6f\n", step + 1, time,

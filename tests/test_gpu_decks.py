@@ -4,8 +4,8 @@ bmx_run reads a BMX input deck (exec/fungi, exec/wall_test,
 exec/nlev_large_test, copied under tests/golden/ref_decks), builds
 BMXParticleContainer on the finest level mesh with the reference's per-step
 stencil list, calls InitParticlesAscii, InitBonds at time 0 and
-EvolveParticles, split_particles and ParticleExchange each step, and writes
-the particles
+EvolveParticles, split_particles, ParticleExchange and the fusion steps each
+step, and writes the particles
 back in the input format (which holds no species: those are compared by
 test_gpu_exchange.py).  The same run on the oracle must give the same
 bits.
@@ -78,6 +78,11 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
           f("chem_species.branching_probability", default=0.0001),
           f("chem_species.splitting_probability", default=0.1), 0.2,
           f("chem_species.seg_split_length", default=0.0030)]
+    # fusion parameters (bmx_chem.cpp:106-112)
+    fus = [f("chem_species.fusion_probability", default=0.0),
+           f("chem_species.max_fusion_separation", default=5.0e-4),
+           f("chem_species.bond_scaling_increment", default=0.001),
+           f("chem_species.max_seg_length", default=0.0035)]
     g = oracle.make_geom(lo, hi, per, n_cell)
     st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
     st.init_bonds()
@@ -92,6 +97,9 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
             recs = grown
             st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
         st.exchange(xpar, dt)
+        _, fused = oracle.fusion(recs, g, fpar, oracle.NBR_STENCIL, hood, fus,
+                                 0 ^ 0x66757369, step, next_id, 0)
+        assert not fused     # one seed particle per deck: nothing to fuse into
     got = bmx.read_particles_ascii(out)
     assert len(got) == len(recs)
     assert np.array_equal(got["pos"], recs["pos"])
