@@ -4,8 +4,8 @@
 Default workload: 1,000,000 FP64 spheres, radii uniform in [0.5e-4, 0.7e-4]
 cm, random sequential insertion at volume fraction 0.3 in a 2:2:1 box
 periodic in x/y with the growth surface at the bottom, cubic contact law
-with adhesion band 0.1 * mean radius, dt = 0.25, neighbor list rebuilt every
-10 steps.  A step is one pass of the hot path (binning + list build when due,
+with adhesion band 0.1 * mean radius, dt = 0.25, Verlet neighbor list (skin
+8e-6 cm) rebuilt every 40 steps.  A step is one pass of the hot path (binning + list build when due,
 fused force + overdamped update) over all particles, with the particles
 already resident in HBM.
 
@@ -287,7 +287,8 @@ def mesh_bench(bmx, W, n: int = 1_000_000, n_cell=(64, 64, 32), calls: int = 20,
     ctx = bmx.Context(0)
     ctx.set_force_params(col.fpar)
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
-    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=col.rebuild_every,
+                            skin=col.skin)
     recs = col.recs.copy()
     ctx.bind(recs)
     ctx.evolve(col.dt, 10)   # the particles in the order of a rebuild, as in a run
@@ -404,15 +405,15 @@ def main() -> int:
     ctx = bmx.Context(0)
     ctx.set_force_params(col.fpar)
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
-    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=col.skin,
-                            max_neighbors=64)
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=col.rebuild_every,
+                            skin=col.skin, max_neighbors=64)
     ctx.set_launch_params(args.block, args.remap)
     recs = col.recs.copy()
     ctx.bind(recs)
     ctx.evolve(col.dt, args.warmup)
 
     # events around every rebuild and one force launch in 7 (7 and the
-    # rebuild interval 10 are coprime, so every phase of the interval is
+    # rebuild interval 40 are coprime, so every phase of the interval is
     # sampled)
     ctx.set_timing(True, every=7)
     ctx.reset_timing()
@@ -452,7 +453,8 @@ def main() -> int:
         "config": {
             "workload": "1M FP64 spheres, r~U[0.5e-4,0.7e-4] cm, phi=0.3, 2:2:1 box "
                         "periodic x/y with growth surface, cubic law W_A=6e-6, dt=0.25, "
-                        f"Verlet list (skin {col.skin:g}) rebuilt every 10 steps",
+                        f"Verlet list (skin {col.skin:g}) rebuilt every "
+                        f"{col.rebuild_every} steps",
             "n": n,
             "mean_row": mean_row,
             "rebuilds": tm.rebuilds,
