@@ -60,7 +60,26 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, LaunchShape ls, hipStream_t s);
+                            int32_t *bstats, int32_t *wtot, LaunchShape ls,
+                            hipStream_t s);
+// Flat row layout: wbase = exclusive scan of the per-wave totals wtot
+// (wtot is left zeroed), then each wave's rows copied back to back into
+// cidx with the owning lane in cown.
+hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
+                              int64_t ncap, int64_t n, int32_t *wtot,
+                              int32_t *wbase, int32_t *scan_scratch,
+                              int32_t *cidx, uint8_t *cown, hipStream_t s);
+// Sphere force + update over the flat row layout; same contract as
+// launch_force_sphere.
+hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
+                                    const int32_t *cidx, const uint8_t *cown,
+                                    const int32_t *wbase, const int32_t *ncnt,
+                                    int64_t ncap, int64_t n, const int32_t *col,
+                                    const BoxGeom *boxes, BoxGeom g0, int multi,
+                                    ForceParams fp, double *fc,
+                                    const int32_t *fc_oid, double *vmax,
+                                    int32_t *err, LaunchShape ls,
+                                    hipStream_t s);
 // blocks of a row build / force launch over n particles (partial arrays)
 int32_t row_blocks(int64_t n, int block);
 // stats[0] = longest row, *total = entries, from the per-block partials

@@ -496,7 +496,7 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
     int64_t n, int mode, double crit, double wa, double skin,
     int32_t *__restrict__ nbr, int32_t *__restrict__ ncnt, int64_t ncap,
     int32_t maxn, int32_t *__restrict__ err, int32_t *__restrict__ bstats,
-    int remap) {
+    int32_t *__restrict__ wtot, int remap) {
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const int ti = (int)t;  // n < 2^31
   const double was = wa + skin;
@@ -596,6 +596,10 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   } else if (t < n) {
     ncnt[t] = 0;
   }
+  if (wtot) {  // entries kept per wave of 64 rows (flat row layout)
+    const int ws = (int)wave_sum_i64(cnt < maxn ? cnt : maxn);
+    if ((threadIdx.x & (kWave - 1)) == 0 && t < n) wtot[t / kWave] = ws;
+  }
   __shared__ int32_t shi[16];
   int m = block_max(cnt, shi);
   __syncthreads();
@@ -603,6 +607,39 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   if (threadIdx.x == 0) {
     bstats[2 * blockIdx.x] = m;
     bstats[2 * blockIdx.x + 1] = sm;
+  }
+}
+
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int w = __shfl_up(v, o, kWave);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+
+// Flat row layout.  The rows of each wave of 64 particles are stored back
+// to back from wbase[wave] (an exclusive scan of the per-wave totals), in
+// lane order and row order, with the owning lane of every entry beside it.
+// A wave then reads exactly its entries, 64 at a time, instead of every
+// column up to its longest row.
+__global__ __launch_bounds__(256) void k_row_compact(
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
+    int64_t ncap, int64_t n, const int32_t *__restrict__ wbase,
+    int32_t *__restrict__ cidx, uint8_t *__restrict__ cown) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int c = t < n ? row_len(ncnt[t]) : 0;
+  const int excl = wave_incl_scan(c) - c;
+  if (c > 0) {
+    const int64_t b = (int64_t)wbase[t / kWave] + excl;
+    for (int k = 0; k < c; k++) {
+      cidx[b + k] = nbr[t + (int64_t)k * ncap];
+      cown[b + k] = (uint8_t)lane;
+    }
   }
 }
 
@@ -732,6 +769,159 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere(
       sphere_row<1>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
     else
       sphere_row<7>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
+    // growth surface (K.H:1009-1030); vcom x/y are zero
+    a0 += 0.0;
+    a1 += 0.0;
+    a2 += wall_law(p.z, p.w, fp);
+    double vx = fp.mob * a0, vy = fp.mob * a1, vz = fp.mob * a2;
+    double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
+    x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+    y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+    z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+    pos_out[t] = make_double4(x, y, z, p.w);
+    if (fc) {  // original order, so a later rebuild does not reorder it
+      const int64_t o = fc_oid[t];
+      fc[o] = a0;
+      fc[o + ncap] = a1;
+      fc[o + 2 * ncap] = a2;
+      fc[o + 3 * ncap] = 0.0;
+      fc[o + 4 * ncap] = 0.0;
+      fc[o + 5 * ncap] = 0.0;
+    }
+    if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
+    v2 = vx * vx + vy * vy + vz * vz;
+  }
+  if (vmax) block_vmax(v2, fp.dt, vmax);
+}
+
+// ------------------------------------------------- flat sphere force ---
+//
+// The same sphere step as k_force_sphere over the flat row layout.  Lane l
+// of a wave takes list entries l and l + 64 of each window of 128: it loads
+// the neighbor index and its owner lane, gathers the neighbor, reads the
+// owner's position from LDS, and writes the pair's force term (zero when
+// the pair does not interact) to the window's LDS slots.  Each owner then
+// adds its own terms in row order, so every particle sums the same terms in
+// the same order as the row-by-row kernel.  Adding a zero term leaves a sum
+// unchanged: a sum that starts at +0.0 never becomes -0.0.
+constexpr int kFlatWin = 2 * kWave;
+constexpr size_t kFlatWaveBytes =
+    kWave * sizeof(double4) + 3 * kFlatWin * sizeof(double);
+
+// Orders the wave's LDS writes before its later LDS reads (lanes of one
+// wave only; waves of a block run different trip counts).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int M>
+__device__ __forceinline__ void flat_term(const double4 &pi, const double4 &q,
+                                          bool valid, const BoxGeom &g,
+                                          const ForceParams &fp, double &fx,
+                                          double &fy, double &fz) {
+  fx = 0.0;
+  fy = 0.0;
+  fz = 0.0;
+  double d0, d1, d2;
+  sphere_sep<M>(pi, q, g, d0, d1, d2);
+  double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  double lim = pi.w + q.w + fp.WA - 1.0e-15;  // r_lm - small_number
+  if (valid && r2 <= lim * lim)
+    sphere_pair(d0, d1, d2, r2, pi.w + q.w, fp, fx, fy, fz);
+}
+
+template <bool MULTI, int M>
+__device__ __forceinline__ void flat_rows(
+    int T, int P, int c, int64_t B, int lane,
+    const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
+    const double4 *__restrict__ pos_in, const double4 *sp, const int32_t *scol,
+    double *rx, double *ry, double *rz, const BoxGeom *__restrict__ boxes,
+    const BoxGeom &g0, const ForceParams &fp, double &a0, double &a1,
+    double &a2) {
+  for (int w0 = 0; w0 < T; w0 += kFlatWin) {
+    const int e0 = w0 + lane, e1 = e0 + kWave;
+    const bool v0 = e0 < T, v1 = e1 < T;
+    int u0 = 0, u1 = 0, o0 = 0, o1 = 0;
+    if (v0) {
+      u0 = cidx[B + e0];
+      o0 = cown[B + e0];
+    }
+    if (v1) {
+      u1 = cidx[B + e1];
+      o1 = cown[B + e1];
+    }
+    const double4 q0 = pos_in[u0], q1 = pos_in[u1];
+    const double4 p0 = sp[o0], p1 = sp[o1];
+    const BoxGeom &ga = MULTI ? boxes[scol[o0]] : g0;
+    const BoxGeom &gb = MULTI ? boxes[scol[o1]] : g0;
+    double f0x, f0y, f0z, f1x, f1y, f1z;
+    flat_term<M>(p0, q0, v0, ga, fp, f0x, f0y, f0z);
+    flat_term<M>(p1, q1, v1, gb, fp, f1x, f1y, f1z);
+    rx[lane] = f0x;
+    ry[lane] = f0y;
+    rz[lane] = f0z;
+    rx[lane + kWave] = f1x;
+    ry[lane + kWave] = f1y;
+    rz[lane + kWave] = f1z;
+    wave_lds_sync();
+    const int lo = P > w0 ? P : w0;
+    const int hi = P + c < w0 + kFlatWin ? P + c : w0 + kFlatWin;
+    for (int e = lo; e < hi; e++) {
+      a0 += rx[e - w0];
+      a1 += ry[e - w0];
+      a2 += rz[e - w0];
+    }
+    wave_lds_sync();
+  }
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
+    const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
+    const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
+    const int32_t *__restrict__ wbase, const int32_t *__restrict__ ncnt,
+    int64_t ncap, int64_t n, const int32_t *__restrict__ col,
+    const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
+    double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
+    double *__restrict__ vmax, int32_t *__restrict__ err, int remap) {
+  extern __shared__ __align__(16) unsigned char flat_lds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+  double4 *sp = reinterpret_cast<double4 *>(flat_lds) + wv * kWave;
+  double *rx = reinterpret_cast<double *>(flat_lds + nwv * kWave * sizeof(double4)) +
+               wv * 3 * kFlatWin;
+  double *ry = rx + kFlatWin, *rz = ry + kFlatWin;
+  int32_t *scol = reinterpret_cast<int32_t *>(flat_lds + nwv * kFlatWaveBytes) +
+                  wv * kWave;
+  const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+  const bool active = t < n;
+  const int32_t raw = active ? ncnt[t] : 0;
+  const int wave_img = wave_or3(row_img(raw));
+  const int c = row_len(raw);
+  const int incl = wave_incl_scan(c);
+  const int T = __shfl(incl, kWave - 1, kWave);
+  const int P = incl - c;
+  int64_t B = 0;
+  if (T > 0) B = wbase[t / kWave];  // the same wave index on every lane
+  const double4 p = active ? pos_in[t] : make_double4(0.0, 0.0, 0.0, 0.0);
+  sp[lane] = p;
+  if (MULTI) scol[lane] = active ? col[t] : 0;
+  wave_lds_sync();
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  if (wave_img == 0)
+    flat_rows<MULTI, 0>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
+                        rz, boxes, g0, fp, a0, a1, a2);
+  else if (wave_img == 1)
+    flat_rows<MULTI, 1>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
+                        rz, boxes, g0, fp, a0, a1, a2);
+  else
+    flat_rows<MULTI, 7>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
+                        rz, boxes, g0, fp, a0, a1, a2);
+  double v2 = 0.0;
+  if (active) {
+    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
     // growth surface (K.H:1009-1030); vcom x/y are zero
     a0 += 0.0;
     a1 += 0.0;
@@ -1125,13 +1315,14 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, LaunchShape ls, hipStream_t s) {
+                            int32_t *bstats, int32_t *wtot, LaunchShape ls,
+                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
 #define BMX_BUILD(M, RD)                                                      \
   hipLaunchKernelGGL((k_build_nbr<M, RD>), dim3(grid_for(n, b)), dim3(b), 0, s, \
                      pos, col, type, rod, boxes, g0, start, key, n, mode,     \
-                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats,      \
+                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats, wtot, \
                      ls.remap)
   if (multi && rods)
     BMX_BUILD(true, true);
@@ -1172,6 +1363,46 @@ hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
     hipLaunchKernelGGL(k_force_sphere<false>, dim3(grid_for(n, b)), dim3(b), 0,
                        s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
                        fp, fc, fc_oid, vmax, err, ls.remap);
+  return hipGetLastError();
+}
+
+size_t flat_lds_bytes(int block, int multi) {
+  return (size_t)(block / kWave) * (kFlatWaveBytes + (multi ? kWave * sizeof(int32_t) : 0));
+}
+
+hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
+                              int64_t ncap, int64_t n, int32_t *wtot,
+                              int32_t *wbase, int32_t *scan_scratch,
+                              int32_t *cidx, uint8_t *cown, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nw = (n + kWave - 1) / kWave;
+  hipError_t e = launch_scan(wtot, wbase, nw, scan_scratch, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_row_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, nbr,
+                     ncnt, ncap, n, wbase, cidx, cown);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
+                                    const int32_t *cidx, const uint8_t *cown,
+                                    const int32_t *wbase, const int32_t *ncnt,
+                                    int64_t ncap, int64_t n, const int32_t *col,
+                                    const BoxGeom *boxes, BoxGeom g0, int multi,
+                                    ForceParams fp, double *fc,
+                                    const int32_t *fc_oid, double *vmax,
+                                    int32_t *err, LaunchShape ls,
+                                    hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int b = ls.block;
+  const size_t lds = flat_lds_bytes(b, multi);
+  if (multi)
+    hipLaunchKernelGGL(k_force_sphere_flat<true>, dim3(grid_for(n, b)), dim3(b),
+                       lds, s, pos_in, pos_out, cidx, cown, wbase, ncnt, ncap, n,
+                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
+  else
+    hipLaunchKernelGGL(k_force_sphere_flat<false>, dim3(grid_for(n, b)), dim3(b),
+                       lds, s, pos_in, pos_out, cidx, cown, wbase, ncnt, ncap, n,
+                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
   return hipGetLastError();
 }
 
