@@ -799,14 +799,24 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere(
 // The same sphere step as k_force_sphere over the flat row layout.  Lane l
 // of a wave takes list entries l and l + 64 of each window of 128: it loads
 // the neighbor index and its owner lane, gathers the neighbor, reads the
-// owner's position from LDS, and writes the pair's force term (zero when
-// the pair does not interact) to the window's LDS slots.  Each owner then
-// adds its own terms in row order, so every particle sums the same terms in
-// the same order as the row-by-row kernel.  Adding a zero term leaves a sum
-// unchanged: a sum that starts at +0.0 never becomes -0.0.
+// owner's position from LDS and applies the interaction test.  Interacting
+// pairs are packed into LDS slots by ballot, entry 0 of every lane first,
+// so slot order is entry order; the cubic law then runs once per 64 packed
+// pairs and leaves each term in its slot.  An owner's interacting entries
+// occupy a contiguous run of slots, which it adds in row order: every
+// particle sums the same terms in the same order as the row-by-row kernel.
 constexpr int kFlatWin = 2 * kWave;
 constexpr size_t kFlatWaveBytes =
-    kWave * sizeof(double4) + 3 * kFlatWin * sizeof(double);
+    kWave * sizeof(double4) + 4 * kFlatWin * sizeof(double);
+
+// Largest step displacement dt*|vel| of the wave, one store per wave at its
+// wave index (the partial arrays hold one slot per 64 particles), with no
+// block barrier: waves that finish early do not wait for the block's slowest.
+__device__ __forceinline__ void wave_vmax(double v2, double dt, double *vpart,
+                                          int64_t t, int lane, int64_t n) {
+  const double m = wave_max_f64(v2);
+  if (lane == 0 && t < n) vpart[t / kWave] = dt * sqrt(m);
+}
 
 // Orders the wave's LDS writes before its later LDS reads (lanes of one
 // wave only; waves of a block run different trip counts).
@@ -816,20 +826,13 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int M>
-__device__ __forceinline__ void flat_term(const double4 &pi, const double4 &q,
-                                          bool valid, const BoxGeom &g,
-                                          const ForceParams &fp, double &fx,
-                                          double &fy, double &fz) {
-  fx = 0.0;
-  fy = 0.0;
-  fz = 0.0;
-  double d0, d1, d2;
-  sphere_sep<M>(pi, q, g, d0, d1, d2);
-  double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-  double lim = pi.w + q.w + fp.WA - 1.0e-15;  // r_lm - small_number
-  if (valid && r2 <= lim * lim)
-    sphere_pair(d0, d1, d2, r2, pi.w + q.w, fp, fx, fy, fz);
+// Interacting entries of the window before entry w0 + j (0 <= j <= 128).
+__device__ __forceinline__ int packed_before(int j, unsigned long long b0,
+                                             unsigned long long b1, int n0) {
+  if (j <= kWave)
+    return __popcll(j == kWave ? b0 : (b0 & ((1ull << j) - 1ull)));
+  j -= kWave;
+  return n0 + __popcll(j == kWave ? b1 : (b1 & ((1ull << j) - 1ull)));
 }
 
 template <bool MULTI, int M>
@@ -837,9 +840,10 @@ __device__ __forceinline__ void flat_rows(
     int T, int P, int c, int64_t B, int lane,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
     const double4 *__restrict__ pos_in, const double4 *sp, const int32_t *scol,
-    double *rx, double *ry, double *rz, const BoxGeom *__restrict__ boxes,
-    const BoxGeom &g0, const ForceParams &fp, double &a0, double &a1,
-    double &a2) {
+    double *q0s, double *q1s, double *q2s, double *qrs,
+    const BoxGeom *__restrict__ boxes, const BoxGeom &g0, const ForceParams &fp,
+    double &a0, double &a1, double &a2) {
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int w0 = 0; w0 < T; w0 += kFlatWin) {
     const int e0 = w0 + lane, e1 = e0 + kWave;
     const bool v0 = e0 < T, v1 = e1 < T;
@@ -856,22 +860,51 @@ __device__ __forceinline__ void flat_rows(
     const double4 p0 = sp[o0], p1 = sp[o1];
     const BoxGeom &ga = MULTI ? boxes[scol[o0]] : g0;
     const BoxGeom &gb = MULTI ? boxes[scol[o1]] : g0;
-    double f0x, f0y, f0z, f1x, f1y, f1z;
-    flat_term<M>(p0, q0, v0, ga, fp, f0x, f0y, f0z);
-    flat_term<M>(p1, q1, v1, gb, fp, f1x, f1y, f1z);
-    rx[lane] = f0x;
-    ry[lane] = f0y;
-    rz[lane] = f0z;
-    rx[lane + kWave] = f1x;
-    ry[lane + kWave] = f1y;
-    rz[lane + kWave] = f1z;
+    double x0, y0, z0, x1, y1, z1;
+    sphere_sep<M>(p0, q0, ga, x0, y0, z0);
+    sphere_sep<M>(p1, q1, gb, x1, y1, z1);
+    const double r20 = x0 * x0 + y0 * y0 + z0 * z0;
+    const double r21 = x1 * x1 + y1 * y1 + z1 * z1;
+    const double lim0 = p0.w + q0.w + fp.WA - 1.0e-15;  // r_lm - small_number
+    const double lim1 = p1.w + q1.w + fp.WA - 1.0e-15;
+    const bool i0 = v0 && r20 <= lim0 * lim0;
+    const bool i1 = v1 && r21 <= lim1 * lim1;
+    const unsigned long long b0 = __ballot(i0), b1 = __ballot(i1);
+    const int n0 = __popcll(b0), ni = n0 + __popcll(b1);
+    if (i0) {
+      const int k = __popcll(b0 & below);
+      q0s[k] = x0;
+      q1s[k] = y0;
+      q2s[k] = z0;
+      qrs[k] = p0.w + q0.w;
+    }
+    if (i1) {
+      const int k = n0 + __popcll(b1 & below);
+      q0s[k] = x1;
+      q1s[k] = y1;
+      q2s[k] = z1;
+      qrs[k] = p1.w + q1.w;
+    }
+    wave_lds_sync();
+    for (int k = lane; k < ni; k += kWave) {  // the cubic law, packed
+      const double d0 = q0s[k], d1 = q1s[k], d2 = q2s[k];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      sphere_pair(d0, d1, d2, r2, qrs[k], fp, fx, fy, fz);
+      q0s[k] = fx;
+      q1s[k] = fy;
+      q2s[k] = fz;
+    }
     wave_lds_sync();
     const int lo = P > w0 ? P : w0;
     const int hi = P + c < w0 + kFlatWin ? P + c : w0 + kFlatWin;
-    for (int e = lo; e < hi; e++) {
-      a0 += rx[e - w0];
-      a1 += ry[e - w0];
-      a2 += rz[e - w0];
+    if (lo < hi) {
+      const int k1 = packed_before(hi - w0, b0, b1, n0);
+      for (int k = packed_before(lo - w0, b0, b1, n0); k < k1; k++) {
+        a0 += q0s[k];
+        a1 += q1s[k];
+        a2 += q2s[k];
+      }
     }
     wave_lds_sync();
   }
@@ -890,35 +923,40 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave, nwv = blockDim.x / kWave;
   double4 *sp = reinterpret_cast<double4 *>(flat_lds) + wv * kWave;
-  double *rx = reinterpret_cast<double *>(flat_lds + nwv * kWave * sizeof(double4)) +
-               wv * 3 * kFlatWin;
-  double *ry = rx + kFlatWin, *rz = ry + kFlatWin;
+  double *q0s = reinterpret_cast<double *>(flat_lds + nwv * kWave * sizeof(double4)) +
+                wv * 4 * kFlatWin;
+  double *q1s = q0s + kFlatWin, *q2s = q1s + kFlatWin, *qrs = q2s + kFlatWin;
   int32_t *scol = reinterpret_cast<int32_t *>(flat_lds + nwv * kFlatWaveBytes) +
                   wv * kWave;
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const bool active = t < n;
+  // The wave's entries are [wbase[w], wbase[w + 1]); reading both bounds
+  // up front lets the entry loads start without waiting for the row lengths.
+  int64_t B = 0;
+  int T = 0;
+  if (t - lane < n) {  // the wave holds at least one particle
+    const int32_t b0 = wbase[t / kWave], b1 = wbase[t / kWave + 1];
+    B = b0;
+    T = b1 - b0;
+  }
   const int32_t raw = active ? ncnt[t] : 0;
   const int wave_img = wave_or3(row_img(raw));
   const int c = row_len(raw);
-  const int incl = wave_incl_scan(c);
-  const int T = __shfl(incl, kWave - 1, kWave);
-  const int P = incl - c;
-  int64_t B = 0;
-  if (T > 0) B = wbase[t / kWave];  // the same wave index on every lane
+  const int P = wave_incl_scan(c) - c;
   const double4 p = active ? pos_in[t] : make_double4(0.0, 0.0, 0.0, 0.0);
   sp[lane] = p;
   if (MULTI) scol[lane] = active ? col[t] : 0;
   wave_lds_sync();
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   if (wave_img == 0)
-    flat_rows<MULTI, 0>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
-                        rz, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 0>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
+                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
   else if (wave_img == 1)
-    flat_rows<MULTI, 1>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
-                        rz, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 1>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
+                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
   else
-    flat_rows<MULTI, 7>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, rx, ry,
-                        rz, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 7>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
+                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
   double v2 = 0.0;
   if (active) {
     const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
@@ -944,7 +982,7 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
     if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
     v2 = vx * vx + vy * vy + vz * vz;
   }
-  if (vmax) block_vmax(v2, fp.dt, vmax);
+  if (vmax) wave_vmax(v2, fp.dt, vmax, t, lane, n);
 }
 
 template <bool MULTI>
