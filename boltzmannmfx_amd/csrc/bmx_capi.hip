@@ -116,7 +116,8 @@ struct bmx_ctx {
   // flat row layout for the sphere force pass (spheres only): per-wave
   // entry totals and bases, entries back to back, owning lane per entry
   bool flat = false;
-  DevBuf<int32_t> d_wtot, d_wbase, d_wscan, d_cidx;
+  DevBuf<int32_t> d_wtot, d_cidx;
+  DevBuf<int64_t> d_wbase;
   DevBuf<uint8_t> d_cown;
   // species A, B, C and volume, and their exchange increments, in original
   // order (loaded from the records at the first exchange)
@@ -317,13 +318,11 @@ bmx_status ensure_rows(bmx_ctx *c) {
   size_t need = (size_t)c->maxn * (size_t)c->ncap;
   HIPCHK(c->d_nbr.alloc(need));
   HIPCHK(c->d_ncnt.alloc(c->ncap));
-  // The flat layout indexes entries with int32 offsets.
-  c->flat = !c->any_fungi && need < (size_t)INT32_MAX;
+  c->flat = !c->any_fungi;
   if (c->flat) {
     const int64_t nw = (c->ncap + 63) / 64;
     HIPCHK(c->d_wtot.alloc(nw + 1));
     HIPCHK(c->d_wbase.alloc(nw + 1));
-    HIPCHK(c->d_wscan.alloc(scan_scratch_elems(nw) + 1));
     HIPCHK(c->d_cidx.alloc(need));
     HIPCHK(c->d_cown.alloc(need));
   }
@@ -382,8 +381,8 @@ bmx_status rebuild(bmx_ctx *c) {
                           shape(c), s));
   if (c->flat)
     HIPCHK(launch_row_compact(c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
-                              c->d_wtot.p, c->d_wbase.p, c->d_wscan.p,
-                              c->d_cidx.p, c->d_cown.p, s));
+                              c->d_wtot.p, c->d_wbase.p, c->d_cidx.p,
+                              c->d_cown.p, s));
   c->row_nblk = row_blocks(c->n, c->lp.block_size);
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, s));
@@ -431,12 +430,6 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
         c->d_wbase.p, c->d_ncnt.p, c->ncap, c->n,
         multi ? c->col[c->cs].p : nullptr, c->d_boxes.p, c->boxes[0], multi,
         fp, fc, c->oid[c->cs].p, vm, c->d_err.p, shape(c), s));
-  } else if (!c->any_fungi) {
-    HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p,
-                               c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
-                               multi ? c->col[c->cs].p : nullptr,
-                               c->d_boxes.p, c->boxes[0], multi, fp, fc,
-                               c->oid[c->cs].p, vm, c->d_err.p, shape(c), s));
   } else {
     HIPCHK(launch_force_general(dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd),
                                 static_arrays(c, c->cs), c->d_nbr.p,
@@ -462,17 +455,11 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
   double *vm = (slot >= 0 && verlet_checked(c))
                    ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
                    : nullptr;
-  if (c->flat)
-    HIPCHK(launch_force_sphere_flat(
-        c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
-        c->d_wbase.p, c->d_ncnt.p, c->ncap, c->n, nullptr, c->d_boxes.p,
-        c->boxes[0], 0, fp, nullptr, nullptr, vm, c->d_err.p, shape(c),
-        c->stream));
-  else
-    HIPCHK(launch_force_sphere(c->pos[c->cd].p, c->pos[1 - c->cd].p,
-                               c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n, nullptr,
-                               c->d_boxes.p, c->boxes[0], 0, fp, nullptr,
-                               nullptr, vm, c->d_err.p, shape(c), c->stream));
+  HIPCHK(launch_force_sphere_flat(
+      c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
+      c->d_wbase.p, c->d_ncnt.p, c->ncap, c->n, nullptr, c->d_boxes.p,
+      c->boxes[0], 0, fp, nullptr, nullptr, vm, c->d_err.p, shape(c),
+      c->stream));
   c->cd = 1 - c->cd;
   c->step++;
   return BMX_OK;
@@ -758,7 +745,6 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_ncnt.release();
   c->d_wtot.release();
   c->d_wbase.release();
-  c->d_wscan.release();
   c->d_cidx.release();
   c->d_cown.release();
   c->d_fc.release();

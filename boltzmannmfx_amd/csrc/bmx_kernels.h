@@ -62,18 +62,20 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t ncap, int32_t maxn, int32_t *err,
                             int32_t *bstats, int32_t *wtot, LaunchShape ls,
                             hipStream_t s);
-// Flat row layout: wbase = exclusive scan of the per-wave totals wtot
-// (wtot is left zeroed), then each wave's rows copied back to back into
+// Flat row layout: wbase[0..nw] = exclusive scan of the per-wave totals
+// wtot (nw = ceil(n / 64)), then each wave's rows copied back to back into
 // cidx with the owning lane in cown.
 hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
-                              int64_t ncap, int64_t n, int32_t *wtot,
-                              int32_t *wbase, int32_t *scan_scratch,
-                              int32_t *cidx, uint8_t *cown, hipStream_t s);
-// Sphere force + update over the flat row layout; same contract as
-// launch_force_sphere.
+                              int64_t ncap, int64_t n, const int32_t *wtot,
+                              int64_t *wbase, int32_t *cidx, uint8_t *cown,
+                              hipStream_t s);
+// Sphere force + overdamped update, fused, over the flat row layout.  fc
+// (6*ncap, original order: fc[oid + c*ncap]) is written when non-null,
+// with fc_oid the original index of each sorted row; vmax[wave] receives
+// the wave's max dt*|vel| when non-null.
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
-                                    const int32_t *wbase, const int32_t *ncnt,
+                                    const int64_t *wbase, const int32_t *ncnt,
                                     int64_t ncap, int64_t n, const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,
@@ -87,17 +89,8 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
                                   int32_t *stats, unsigned long long *total,
                                   hipStream_t s);
 
-// ---- force + overdamped update, fused.  fc (6*ncap, original order:
-// fc[oid + c*ncap]) is written when non-null, with fc_oid the original
-// index of each sorted row; vmax[block] receives the block's max dt*|vel|
-// when non-null.
-hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
-                               const int32_t *nbr, const int32_t *ncnt,
-                               int64_t ncap, int64_t n, const int32_t *col,
-                               const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc,
-                               const int32_t *fc_oid, double *vmax,
-                               int32_t *err, LaunchShape ls, hipStream_t s);
+// ---- segment (and mixed) force + overdamped update, fused: fc and vmax
+// as for launch_force_sphere_flat, vmax per block.
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *nbr, const int32_t *ncnt,
                                 int64_t ncap, int64_t n, const BoxGeom *boxes,

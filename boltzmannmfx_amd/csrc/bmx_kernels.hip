@@ -5,7 +5,8 @@
 // bins with atomics, scans the counts, scatters (original index, slot)
 // pairs, orders each bin by original index, gathers every per-particle array
 // into the new order and builds column-major neighbor rows over the +-1 bin
-// stencil.  Rows are visited in a fixed order (stencil offsets dz, dy, dx
+// stencil (spheres: then copied into a flat layout, each wave's rows back to
+// back).  Rows are visited in a fixed order (stencil offsets dz, dy, dx
 // lexicographic, ascending original index inside a bin), so the force pass
 // sums each particle's pair terms in an order that does not depend on
 // thread scheduling.  The force kernel is a gather: particle i reads its row
@@ -267,18 +268,18 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ __forceinline__ int32_t block_excl_scan(int32_t s, int32_t *wsum,
-                                                   int32_t &total) {
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T s, T *wsum, T &total) {
   int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  int32_t incl = s;
+  T incl = s;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
-    int32_t y = __shfl_up(incl, o, kWave);
+    T y = __shfl_up(incl, o, kWave);
     if (lane >= o) incl += y;
   }
   if (lane == kWave - 1) wsum[wid] = incl;
   __syncthreads();
-  int32_t pre = 0, tot = 0;
+  T pre = 0, tot = 0;
   int nw = blockDim.x / kWave;
   for (int w = 0; w < nw; w++) {
     if (w < wid) pre += wsum[w];
@@ -626,16 +627,53 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // lane order and row order, with the owning lane of every entry beside it.
 // A wave then reads exactly its entries, 64 at a time, instead of every
 // column up to its longest row.
+// One block: wbase[w] = sum of wtot[0..w), wbase[nw] = the total.  Tiles
+// of 8192 totals are loaded coalesced into LDS, each thread scans 8
+// consecutive ones, and a 64-bit carry runs across tiles.
+constexpr int kBaseItems = 8;
+constexpr int kBaseTile = 1024 * kBaseItems;
+__global__ __launch_bounds__(1024) void k_wave_bases(
+    const int32_t *__restrict__ wtot, int64_t nw, int64_t *__restrict__ wbase) {
+  __shared__ int32_t tile[kBaseTile];
+  __shared__ long long wsum[1024 / kWave];
+  long long carry = 0;
+  for (int64_t t0 = 0; t0 < nw; t0 += kBaseTile) {
+#pragma unroll
+    for (int k = 0; k < kBaseItems; k++) {
+      const int64_t i = t0 + k * 1024 + threadIdx.x;
+      tile[k * 1024 + threadIdx.x] = i < nw ? wtot[i] : 0;
+    }
+    __syncthreads();
+    long long v[kBaseItems];
+    long long s = 0;
+#pragma unroll
+    for (int k = 0; k < kBaseItems; k++) {
+      v[k] = s;
+      s += tile[threadIdx.x * kBaseItems + k];
+    }
+    long long total;
+    const long long ex = block_excl_scan(s, wsum, total);
+#pragma unroll
+    for (int k = 0; k < kBaseItems; k++) {
+      const int64_t i = t0 + threadIdx.x * kBaseItems + k;
+      if (i < nw) wbase[i] = carry + ex + v[k];
+    }
+    carry += total;
+    __syncthreads();  // tile and wsum are rewritten by the next tile
+  }
+  if (threadIdx.x == 0) wbase[nw] = carry;
+}
+
 __global__ __launch_bounds__(256) void k_row_compact(
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, const int32_t *__restrict__ wbase,
+    int64_t ncap, int64_t n, const int64_t *__restrict__ wbase,
     int32_t *__restrict__ cidx, uint8_t *__restrict__ cown) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const int c = t < n ? row_len(ncnt[t]) : 0;
   const int excl = wave_incl_scan(c) - c;
   if (c > 0) {
-    const int64_t b = (int64_t)wbase[t / kWave] + excl;
+    const int64_t b = wbase[t / kWave] + excl;
     for (int k = 0; k < c; k++) {
       cidx[b + k] = nbr[t + (int64_t)k * ncap];
       cown[b + k] = (uint8_t)lane;
@@ -674,28 +712,7 @@ __device__ __forceinline__ void block_vmax(double v2, double dt, double *vpart) 
   if (threadIdx.x == 0) vpart[blockIdx.x] = dt * sqrt(m);
 }
 
-// Sphere row in chunks of kRowChunk neighbors.  All indices of a chunk are
-// loaded, then all their coordinates, so a row of up to kRowChunk entries
-// waits for memory twice.  The cheap interaction test (K.H:23-40 with
-// PCI:212) marks the neighbors that interact; the cubic law (K.H:243-270)
-// then runs only for the marked ones, in row order, on the coordinates still
-// held in registers.  The separation is recomputed with the same operations,
-// so the terms and their order are those of a single pass over the row.
-constexpr int kRowChunk = 4;
-
-__device__ __forceinline__ double4 pick4(const double4 (&o)[kRowChunk], int k) {
-  double x = o[0].x, y = o[0].y, z = o[0].z, w = o[0].w;
-#pragma unroll
-  for (int j = 1; j < kRowChunk; j++) {
-    const bool h = k == j;
-    x = h ? o[j].x : x;
-    y = h ? o[j].y : y;
-    z = h ? o[j].z : z;
-    w = h ? o[j].w : w;
-  }
-  return make_double4(x, y, z, w);
-}
-
+// Separation of two spheres along the axes, with the image test of sep_m.
 template <int M>
 __device__ __forceinline__ void sphere_sep(const double4 &p, const double4 &o,
                                            const BoxGeom &g, double &d0,
@@ -705,106 +722,19 @@ __device__ __forceinline__ void sphere_sep(const double4 &p, const double4 &o,
   d2 = sep_m<M, 2>(p.z, o.z, g);
 }
 
-template <int M>
-__device__ __forceinline__ void sphere_row(const double4 &p, int c,
-                                           const int32_t *__restrict__ row,
-                                           int64_t ncap, int64_t t,
-                                           const double4 *__restrict__ pos_in,
-                                           const BoxGeom &g,
-                                           const ForceParams &fp, double &a0,
-                                           double &a1, double &a2) {
-  const int32_t *rp = row;
-  for (int base = 0; base < c; base += kRowChunk) {
-    int u[kRowChunk];
-    double4 o[kRowChunk];
-#pragma unroll
-    for (int j = 0; j < kRowChunk; j++) {
-      u[j] = (base + j < c) ? *rp : (int)t;
-      rp += ncap;
-    }
-#pragma unroll
-    for (int j = 0; j < kRowChunk; j++) o[j] = pos_in[u[j]];
-    unsigned mask = 0u;
-#pragma unroll
-    for (int j = 0; j < kRowChunk; j++) {
-      double d0, d1, d2;
-      sphere_sep<M>(p, o[j], g, d0, d1, d2);
-      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      double lim = p.w + o[j].w + fp.WA - 1.0e-15;  // r_lm - small_number
-      if (base + j < c && r2 <= lim * lim) mask |= 1u << j;
-    }
-    while (mask) {
-      const int k = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      const double4 q = pick4(o, k);
-      double d0, d1, d2;
-      sphere_sep<M>(p, q, g, d0, d1, d2);
-      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      sphere_pair(d0, d1, d2, r2, p.w + q.w, fp, a0, a1, a2);
-    }
-  }
-}
-
-template <bool MULTI>
-__global__ __launch_bounds__(512, 6) void k_force_sphere(
-    const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, const int32_t *__restrict__ col,
-    const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
-    double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
-    double *__restrict__ vmax, int32_t *__restrict__ err, int remap) {
-  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
-  double v2 = 0.0;
-  const bool active = t < n;
-  const int32_t raw = active ? ncnt[t] : 0;
-  const int wave_img = wave_or3(row_img(raw));
-  if (active) {
-    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
-    double4 p = pos_in[t];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const int c = row_len(raw);
-    if (wave_img == 0)
-      sphere_row<0>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
-    else if (wave_img == 1)
-      sphere_row<1>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
-    else
-      sphere_row<7>(p, c, nbr + t, ncap, t, pos_in, g, fp, a0, a1, a2);
-    // growth surface (K.H:1009-1030); vcom x/y are zero
-    a0 += 0.0;
-    a1 += 0.0;
-    a2 += wall_law(p.z, p.w, fp);
-    double vx = fp.mob * a0, vy = fp.mob * a1, vz = fp.mob * a2;
-    double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
-    x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
-    y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
-    z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
-    pos_out[t] = make_double4(x, y, z, p.w);
-    if (fc) {  // original order, so a later rebuild does not reorder it
-      const int64_t o = fc_oid[t];
-      fc[o] = a0;
-      fc[o + ncap] = a1;
-      fc[o + 2 * ncap] = a2;
-      fc[o + 3 * ncap] = 0.0;
-      fc[o + 4 * ncap] = 0.0;
-      fc[o + 5 * ncap] = 0.0;
-    }
-    if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
-    v2 = vx * vx + vy * vy + vz * vz;
-  }
-  if (vmax) block_vmax(v2, fp.dt, vmax);
-}
-
 // ------------------------------------------------- flat sphere force ---
 //
-// The same sphere step as k_force_sphere over the flat row layout.  Lane l
-// of a wave takes list entries l and l + 64 of each window of 128: it loads
-// the neighbor index and its owner lane, gathers the neighbor, reads the
-// owner's position from LDS and applies the interaction test.  Interacting
-// pairs are packed into LDS slots by ballot, entry 0 of every lane first,
-// so slot order is entry order; the cubic law then runs once per 64 packed
-// pairs and leaves each term in its slot.  An owner's interacting entries
-// occupy a contiguous run of slots, which it adds in row order: every
-// particle sums the same terms in the same order as the row-by-row kernel.
+// The sphere step (force lambda PCI:170-279 with the sphere law K.H:243-276
+// and the wall K.H:1009-1030, then the update PCI:311-449) over the flat row
+// layout.  Lane l of a wave takes list entries l and l + 64 of each window
+// of 128: it loads the neighbor index and its owner lane, gathers the
+// neighbor, reads the owner's position from LDS and applies the interaction
+// test (K.H:23-40 with PCI:212).  Interacting pairs are packed into LDS
+// slots by ballot, entry 0 of every lane first, so slot order is entry
+// order; the cubic law then runs once per 64 packed pairs and leaves each
+// term in its slot.  An owner's interacting entries occupy a contiguous run
+// of slots, which it adds in row order: every particle sums its pair terms
+// in its row's order, as a loop over the row would.
 constexpr int kFlatWin = 2 * kWave;
 constexpr size_t kFlatWaveBytes =
     kWave * sizeof(double4) + 4 * kFlatWin * sizeof(double);
@@ -914,7 +844,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
     const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
-    const int32_t *__restrict__ wbase, const int32_t *__restrict__ ncnt,
+    const int64_t *__restrict__ wbase, const int32_t *__restrict__ ncnt,
     int64_t ncap, int64_t n, const int32_t *__restrict__ col,
     const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
     double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
@@ -935,9 +865,9 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   int64_t B = 0;
   int T = 0;
   if (t - lane < n) {  // the wave holds at least one particle
-    const int32_t b0 = wbase[t / kWave], b1 = wbase[t / kWave + 1];
+    const int64_t b0 = wbase[t / kWave], b1 = wbase[t / kWave + 1];
     B = b0;
-    T = b1 - b0;
+    T = (int)(b1 - b0);
   }
   const int32_t raw = active ? ncnt[t] : 0;
   const int wave_img = wave_or3(row_img(raw));
@@ -1384,38 +1314,17 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
   return hipGetLastError();
 }
 
-hipError_t launch_force_sphere(const double4 *pos_in, double4 *pos_out,
-                               const int32_t *nbr, const int32_t *ncnt,
-                               int64_t ncap, int64_t n, const int32_t *col,
-                               const BoxGeom *boxes, BoxGeom g0, int multi,
-                               ForceParams fp, double *fc,
-                               const int32_t *fc_oid, double *vmax,
-                               int32_t *err, LaunchShape ls, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  int b = ls.block;
-  if (multi)
-    hipLaunchKernelGGL(k_force_sphere<true>, dim3(grid_for(n, b)), dim3(b), 0,
-                       s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
-                       fp, fc, fc_oid, vmax, err, ls.remap);
-  else
-    hipLaunchKernelGGL(k_force_sphere<false>, dim3(grid_for(n, b)), dim3(b), 0,
-                       s, pos_in, pos_out, nbr, ncnt, ncap, n, col, boxes, g0,
-                       fp, fc, fc_oid, vmax, err, ls.remap);
-  return hipGetLastError();
-}
-
 size_t flat_lds_bytes(int block, int multi) {
   return (size_t)(block / kWave) * (kFlatWaveBytes + (multi ? kWave * sizeof(int32_t) : 0));
 }
 
 hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
-                              int64_t ncap, int64_t n, int32_t *wtot,
-                              int32_t *wbase, int32_t *scan_scratch,
-                              int32_t *cidx, uint8_t *cown, hipStream_t s) {
+                              int64_t ncap, int64_t n, const int32_t *wtot,
+                              int64_t *wbase, int32_t *cidx, uint8_t *cown,
+                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int64_t nw = (n + kWave - 1) / kWave;
-  hipError_t e = launch_scan(wtot, wbase, nw, scan_scratch, s);
-  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wave_bases, dim3(1), dim3(1024), 0, s, wtot, nw, wbase);
   hipLaunchKernelGGL(k_row_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, nbr,
                      ncnt, ncap, n, wbase, cidx, cown);
   return hipGetLastError();
@@ -1423,7 +1332,7 @@ hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
 
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
-                                    const int32_t *wbase, const int32_t *ncnt,
+                                    const int64_t *wbase, const int32_t *ncnt,
                                     int64_t ncap, int64_t n, const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,

@@ -176,6 +176,30 @@ def test_edge_cases(bmx, oracle, gpu, c1):
     assert ctx.positions().shape == (0, 3)
 
 
+def test_dense_rows_bit_exact(bmx, oracle, gpu):
+    """Long rows where nearly every entry interacts (W_A = 1e-4 cm, skin
+    1e-9 cm, lists rebuilt every step): a wave's entries span many windows
+    of 128 and a window packs more than 64 interacting pairs."""
+    col = W.sphere_colony(4000, 0.3, 11, W_A=1.0e-4, skin=1.0e-9)
+    ctx, _ = gpu_ctx(bmx, col, K=1)          # lists rebuilt every step
+    ctx.evolve(W.DT, 3)
+    st, orecs = oracle_state(oracle, col, ctx.bin_dims(), K=1)
+    fo = st.evolve(W.DT, 3)
+    assert np.array_equal(ctx.fc(), fo)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+    off, idx = ctx.neighbors()
+    assert (off[-1] / len(col.recs)) > 8
+    p, r = orecs["pos"], orecs["rdata"][:, 0]
+    i = np.repeat(np.arange(len(r)), np.diff(off))
+    d = p[i] - p[idx]
+    L = col.prob_hi - col.prob_lo
+    for a in range(3):
+        if col.periodic[a]:
+            d[:, a] -= L[a] * np.round(d[:, a] / L[a])
+    inter = np.sqrt((d * d).sum(1)) <= r[i] + r[idx] + col.fpar[0]
+    assert inter.mean() > 0.6
+
+
 def test_errors_are_reported(bmx, gpu, c1):
     col = W.c1_small()
     col.recs["pos"][1] = col.recs["pos"][0]          # coincident pair
