@@ -232,7 +232,7 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
         "cpu_baseline": cpu,
         "particle_steps_per_s": n * steps / secs,
         "ms_per_step": secs * 1e3 / steps,
-        "force_kernel": "k_force_general",
+        "force_kernel": "k_force_general_flat",
         "force_kernel_avg_ms": tm.force_ms / max(tm.force_launches, 1),
         "rebuild_avg_ms": tm.rebuild_ms / max(tm.rebuilds, 1),
         "mean_row": tm.neighbor_entries / n,

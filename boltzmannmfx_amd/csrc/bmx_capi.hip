@@ -113,11 +113,10 @@ struct bmx_ctx {
   DevBuf<unsigned long long> d_permk;
   int32_t maxn = 96;
   DevBuf<int32_t> d_nbr, d_ncnt;
-  // flat row layout for the sphere force pass (spheres only): per-wave
-  // entry totals and bases, entries back to back, owning lane per entry
-  bool flat = false;
+  // flat row layout of the force pass: per-tile entry totals and bases,
+  // entries back to back, the particle's place in its tile per entry
+  int flat_tile = 64;
   DevBuf<int32_t> d_wtot, d_cidx;
-  DevBuf<int64_t> d_wbase;
   DevBuf<uint8_t> d_cown;
   // species A, B, C and volume, and their exchange increments, in original
   // order (loaded from the records at the first exchange)
@@ -318,14 +317,12 @@ bmx_status ensure_rows(bmx_ctx *c) {
   size_t need = (size_t)c->maxn * (size_t)c->ncap;
   HIPCHK(c->d_nbr.alloc(need));
   HIPCHK(c->d_ncnt.alloc(c->ncap));
-  c->flat = !c->any_fungi;
-  if (c->flat) {
-    const int64_t nw = (c->ncap + 63) / 64;
-    HIPCHK(c->d_wtot.alloc(nw + 1));
-    HIPCHK(c->d_wbase.alloc(nw + 1));
-    HIPCHK(c->d_cidx.alloc(need));
-    HIPCHK(c->d_cown.alloc(need));
-  }
+  c->flat_tile = flat_tile(c->any_fungi ? 1 : 0);
+  const int64_t nw = (c->ncap + c->flat_tile - 1) / c->flat_tile;
+  const int64_t nflat = flat_entries(c->ncap, c->flat_tile, c->maxn);
+  HIPCHK(c->d_wtot.alloc(nw + 1));
+  HIPCHK(c->d_cidx.alloc(nflat));
+  HIPCHK(c->d_cown.alloc(nflat));
   return BMX_OK;
 }
 
@@ -341,6 +338,10 @@ bmx_status alloc_partials(bmx_ctx *c) {
   HIPCHK(c->d_slotmax.alloc(c->vmax_slots));
   HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, c->stream));
   return BMX_OK;
+}
+
+FlatRows flat_rows(bmx_ctx *c) {
+  return FlatRows{c->flat_tile, c->d_cidx.p, c->d_cown.p, c->d_wtot.p};
 }
 
 // Binning + gather into the other buffers + list build.
@@ -377,12 +378,7 @@ bmx_status rebuild(bmx_ctx *c) {
                           c->n, c->nbp.mode == BMX_NBR_STENCIL ? 1 : 0,
                           c->crit, c->fpar[0], c->nbp.skin, c->d_nbr.p,
                           c->d_ncnt.p, c->ncap, c->maxn, c->d_err.p,
-                          c->d_bstats.p, c->flat ? c->d_wtot.p : nullptr,
-                          shape(c), s));
-  if (c->flat)
-    HIPCHK(launch_row_compact(c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
-                              c->d_wtot.p, c->d_wbase.p, c->d_cidx.p,
-                              c->d_cown.p, s));
+                          c->d_bstats.p, flat_rows(c), shape(c), s));
   c->row_nblk = row_blocks(c->n, c->lp.block_size);
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, s));
@@ -424,18 +420,18 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   ForceParams fp = force_params(c, dt);
   int multi = c->ncol > 1;
   double *fc = write_fc ? c->d_fc.p : nullptr;
-  if (c->flat) {
+  if (!c->any_fungi) {
     HIPCHK(launch_force_sphere_flat(
         c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
-        c->d_wbase.p, c->d_ncnt.p, c->ncap, c->n,
+        c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n, c->maxn,
         multi ? c->col[c->cs].p : nullptr, c->d_boxes.p, c->boxes[0], multi,
         fp, fc, c->oid[c->cs].p, vm, c->d_err.p, shape(c), s));
   } else {
-    HIPCHK(launch_force_general(dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd),
-                                static_arrays(c, c->cs), c->d_nbr.p,
-                                c->d_ncnt.p, c->ncap, c->n, c->d_boxes.p,
-                                c->boxes[0], multi, fp, fc, vm, c->d_err.p,
-                                shape(c), s));
+    HIPCHK(launch_force_general(
+        dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd), static_arrays(c, c->cs),
+        c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_nbr.p, c->d_ncnt.p,
+        c->ncap, c->n, c->maxn, c->d_boxes.p, c->boxes[0], multi, fp, fc, vm,
+        c->d_err.p, shape(c), s));
   }
   if (timed) {
     HIPCHK(hipEventRecord(e1, s));
@@ -457,7 +453,7 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
                    : nullptr;
   HIPCHK(launch_force_sphere_flat(
       c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
-      c->d_wbase.p, c->d_ncnt.p, c->ncap, c->n, nullptr, c->d_boxes.p,
+      c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n, c->maxn, nullptr, c->d_boxes.p,
       c->boxes[0], 0, fp, nullptr, nullptr, vm, c->d_err.p, shape(c),
       c->stream));
   c->cd = 1 - c->cd;
@@ -744,7 +740,6 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_nbr.release();
   c->d_ncnt.release();
   c->d_wtot.release();
-  c->d_wbase.release();
   c->d_cidx.release();
   c->d_cown.release();
   c->d_fc.release();

@@ -337,38 +337,51 @@ __device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
   int iid = ti.self_id, icpu = ti.self_cpu;
   int jid = tj.self_id, jcpu = tj.self_cpu;
 
+  // Bond tables hold at most 4 entries (n_bnds is checked to 0..4 at bind
+  // and a repair stops at 4).  Every lookup below runs over the 4 slots
+  // with constant indices, so the tables stay in registers; a search for
+  // the first match keeps the lowest matching slot.
   // K.H:338-362
   if (ti.position == rec::SECOND_1 || ti.position == rec::SECOND_2) {
     int site = (ti.position == rec::SECOND_1) ? 1 : 2;
-    int ib;
-    for (ib = 0; ib < ti.n_bnds; ib++)
-      if (ti.seg_id[ib & 3] == jid && ti.seg_cpu[ib & 3] == jcpu &&
-          ti.site[ib & 3] == site)
-        break;
-    if (ib < ti.n_bnds && tj.position != rec::TIP) ti.position = rec::INTERIOR;
+    bool hit = false;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      hit = hit || (b < ti.n_bnds && ti.seg_id[b] == jid &&
+                    ti.seg_cpu[b] == jcpu && ti.site[b] == site);
+    if (hit && tj.position != rec::TIP) ti.position = rec::INTERIOR;
   }
   ti.dltd = 1;  // K.H:364-369
   // K.H:372-545
   bool i_to_j = false, j_to_i = false;
-  for (int b = 0; b < ti.n_bnds; b++)
-    if (ti.seg_id[b & 3] == jid && ti.seg_cpu[b & 3] == jcpu) { i_to_j = true; break; }
-  for (int b = 0; b < j_bnds; b++)
-    if (j_id[b & 3] == iid && j_cpu[b & 3] == icpu) { j_to_i = true; break; }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    i_to_j = i_to_j || (b < ti.n_bnds && ti.seg_id[b] == jid && ti.seg_cpu[b] == jcpu);
+    j_to_i = j_to_i || (b < j_bnds && j_id[b] == iid && j_cpu[b] == icpu);
+  }
   if (!i_to_j && j_to_i) {
     int isite = (ti.fix_site == 0) ? ((ti.site[0] == 1) ? 2 : 1) : ti.fix_site;
     int nb = ti.n_bnds;
     if (nb >= 0 && nb <= 3) {
-      ti.seg_id[nb] = jid;
-      ti.seg_cpu[nb] = jcpu;
-      ti.site[nb] = isite;
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if (b == nb) {
+          ti.seg_id[b] = jid;
+          ti.seg_cpu[b] = jcpu;
+          ti.site[b] = isite;
+        }
       ti.n_bnds = nb + 1;
     }
   } else if (i_to_j && !j_to_i) {
     int jsite = (tj.fix_site == 0) ? ((tj.site[0] == 1) ? 2 : 1) : tj.fix_site;
     if (j_bnds >= 0 && j_bnds <= 3) {
-      j_id[j_bnds] = iid;
-      j_cpu[j_bnds] = icpu;
-      j_site[j_bnds] = jsite;
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+        if (b == j_bnds) {
+          j_id[b] = iid;
+          j_cpu[b] = icpu;
+          j_site[b] = jsite;
+        }
       j_bnds++;
     }
   }
@@ -385,13 +398,20 @@ __device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
   double f1[3] = {0.0, 0.0, 0.0}, f2[3] = {0.0, 0.0, 0.0};
   bool bonded = false;
   // K.H:615-698
-  int ib, jb;
-  for (ib = 0; ib < ti.n_bnds; ib++)
-    if (ti.seg_id[ib & 3] == jid && ti.seg_cpu[ib & 3] == jcpu) break;
-  for (jb = 0; jb < j_bnds; jb++)
-    if (j_id[jb & 3] == iid && j_cpu[jb & 3] == icpu) break;
-  if (ib < ti.n_bnds && jb < j_bnds) {
-    int si = ti.site[ib & 3], sj = j_site[jb & 3];
+  bool ihit = false, jhit = false;
+  int si = 0, sj = 0;
+#pragma unroll
+  for (int b = 3; b >= 0; b--) {  // descending: the lowest match is kept
+    if (b < ti.n_bnds && ti.seg_id[b] == jid && ti.seg_cpu[b] == jcpu) {
+      ihit = true;
+      si = ti.site[b];
+    }
+    if (b < j_bnds && j_id[b] == iid && j_cpu[b] == icpu) {
+      jhit = true;
+      sj = j_site[b];
+    }
+  }
+  if (ihit && jhit) {
     int itip = ti.fuse_tip, jtip = tj.fuse_tip;
     if ((itip > 0 && itip == si) || (jtip > 0 && jtip == sj)) kb = scale * kb;
     if ((si == 1 || si == 2) && (sj == 1 || sj == 2)) {

@@ -10,6 +10,20 @@ struct LaunchShape {
   int remap;  // XCD-contiguous tile mapping
 };
 
+// Flat row layout written by the list build: each tile of 'tile' particles
+// (64 for spheres, 32 for segment contexts) has its rows back to back in
+// cidx from the start of its own region (tile index * tile * maxn), tot[tile]
+// entries, with the particle's place in the tile beside every entry in
+// cown.  cidx == nullptr builds no flat rows.
+struct FlatRows {
+  int tile;
+  int32_t *cidx;
+  uint8_t *cown;
+  int32_t *tot;
+};
+// Entries to allocate for the flat rows of ncap particles.
+int64_t flat_entries(int64_t ncap, int tile, int32_t maxn);
+
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, int64_t n,
                            unsigned long long v, hipStream_t s);
@@ -60,23 +74,20 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, int32_t *wtot, LaunchShape ls,
+                            int32_t *bstats, FlatRows flat, LaunchShape ls,
                             hipStream_t s);
-// Flat row layout: wbase[0..nw] = exclusive scan of the per-wave totals
-// wtot (nw = ceil(n / 64)), then each wave's rows copied back to back into
-// cidx with the owning lane in cown.
-hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
-                              int64_t ncap, int64_t n, const int32_t *wtot,
-                              int64_t *wbase, int32_t *cidx, uint8_t *cown,
-                              hipStream_t s);
+// Particles per tile of the flat row layout: 64 for spheres, 32 for
+// segment (and mixed) contexts.
+int flat_tile(int segments);
 // Sphere force + overdamped update, fused, over the flat row layout.  fc
 // (6*ncap, original order: fc[oid + c*ncap]) is written when non-null,
 // with fc_oid the original index of each sorted row; vmax[wave] receives
 // the wave's max dt*|vel| when non-null.
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
-                                    const int64_t *wbase, const int32_t *ncnt,
-                                    int64_t ncap, int64_t n, const int32_t *col,
+                                    const int32_t *wtot, const int32_t *ncnt,
+                                    int64_t ncap, int64_t n, int32_t maxn,
+                                    const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,
                                     const int32_t *fc_oid, double *vmax,
@@ -92,11 +103,13 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
 // ---- segment (and mixed) force + overdamped update, fused: fc and vmax
 // as for launch_force_sphere_flat, vmax per block.
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
-                                const int32_t *nbr, const int32_t *ncnt,
-                                int64_t ncap, int64_t n, const BoxGeom *boxes,
-                                BoxGeom g0, int multi, ForceParams fp,
-                                double *fc, double *vmax, int32_t *err,
-                                LaunchShape ls, hipStream_t s);
+                                const int32_t *cidx, const uint8_t *cown,
+                                const int32_t *wtot, const int32_t *nbr,
+                                const int32_t *ncnt, int64_t ncap, int64_t n,
+                                int32_t maxn, const BoxGeom *boxes, BoxGeom g0,
+                                int multi, ForceParams fp, double *fc,
+                                double *vmax, int32_t *err, LaunchShape ls,
+                                hipStream_t s);
 hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
                              const int32_t *ncnt, int64_t ncap, int64_t n,
                              const BoxGeom *boxes, BoxGeom g0, int multi,

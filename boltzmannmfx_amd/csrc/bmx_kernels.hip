@@ -488,6 +488,17 @@ __device__ __forceinline__ int wave_or3(int m) {
   return (__any(m & 1) ? 1 : 0) | (__any(m & 2) ? 2 : 0) | (__any(m & 4) ? 4 : 0);
 }
 
+// Inclusive prefix sum over segments of 'width' lanes (a power of two up
+// to the wave size).
+__device__ __forceinline__ int wave_incl_scan(int v, int width = kWave) {
+  const int lane = threadIdx.x & (width - 1);
+  for (int o = 1; o < width; o <<= 1) {
+    const int w = __shfl_up(v, o, width);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+
 template <bool MULTI, bool RODS>
 __global__ __launch_bounds__(512, 8) void k_build_nbr(
     const double4 *__restrict__ pos, const int32_t *__restrict__ col,
@@ -497,7 +508,7 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
     int64_t n, int mode, double crit, double wa, double skin,
     int32_t *__restrict__ nbr, int32_t *__restrict__ ncnt, int64_t ncap,
     int32_t maxn, int32_t *__restrict__ err, int32_t *__restrict__ bstats,
-    int32_t *__restrict__ wtot, int remap) {
+    FlatRows flat, int remap) {
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const int ti = (int)t;  // n < 2^31
   const double was = wa + skin;
@@ -597,9 +608,19 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   } else if (t < n) {
     ncnt[t] = 0;
   }
-  if (wtot) {  // entries kept per wave of 64 rows (flat row layout)
-    const int ws = (int)wave_sum_i64(cnt < maxn ? cnt : maxn);
-    if ((threadIdx.x & (kWave - 1)) == 0 && t < n) wtot[t / kWave] = ws;
+  if (flat.cidx) {
+    // Flat row layout: the tile's rows back to back from the start of the
+    // tile's own region (tile * maxn entries per tile).
+    const int tile = flat.tile, place = threadIdx.x & (tile - 1);
+    const int kept = cnt < maxn ? cnt : maxn;
+    const int incl = wave_incl_scan(kept, tile);
+    const int tot = __shfl(incl, tile - 1, tile);
+    if (place == 0 && t < n) flat.tot[t / tile] = tot;
+    const int64_t b = (t / tile) * (int64_t)tile * maxn + (incl - kept);
+    for (int k = 0; k < kept; k++) {
+      flat.cidx[b + k] = nbr[t + (int64_t)k * ncap];
+      flat.cown[b + k] = (uint8_t)place;
+    }
   }
   __shared__ int32_t shi[16];
   int m = block_max(cnt, shi);
@@ -611,75 +632,7 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   }
 }
 
-// Inclusive prefix sum over the wave.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const int w = __shfl_up(v, o, kWave);
-    if (lane >= o) v += w;
-  }
-  return v;
-}
 
-// Flat row layout.  The rows of each wave of 64 particles are stored back
-// to back from wbase[wave] (an exclusive scan of the per-wave totals), in
-// lane order and row order, with the owning lane of every entry beside it.
-// A wave then reads exactly its entries, 64 at a time, instead of every
-// column up to its longest row.
-// One block: wbase[w] = sum of wtot[0..w), wbase[nw] = the total.  Tiles
-// of 8192 totals are loaded coalesced into LDS, each thread scans 8
-// consecutive ones, and a 64-bit carry runs across tiles.
-constexpr int kBaseItems = 8;
-constexpr int kBaseTile = 1024 * kBaseItems;
-__global__ __launch_bounds__(1024) void k_wave_bases(
-    const int32_t *__restrict__ wtot, int64_t nw, int64_t *__restrict__ wbase) {
-  __shared__ int32_t tile[kBaseTile];
-  __shared__ long long wsum[1024 / kWave];
-  long long carry = 0;
-  for (int64_t t0 = 0; t0 < nw; t0 += kBaseTile) {
-#pragma unroll
-    for (int k = 0; k < kBaseItems; k++) {
-      const int64_t i = t0 + k * 1024 + threadIdx.x;
-      tile[k * 1024 + threadIdx.x] = i < nw ? wtot[i] : 0;
-    }
-    __syncthreads();
-    long long v[kBaseItems];
-    long long s = 0;
-#pragma unroll
-    for (int k = 0; k < kBaseItems; k++) {
-      v[k] = s;
-      s += tile[threadIdx.x * kBaseItems + k];
-    }
-    long long total;
-    const long long ex = block_excl_scan(s, wsum, total);
-#pragma unroll
-    for (int k = 0; k < kBaseItems; k++) {
-      const int64_t i = t0 + threadIdx.x * kBaseItems + k;
-      if (i < nw) wbase[i] = carry + ex + v[k];
-    }
-    carry += total;
-    __syncthreads();  // tile and wsum are rewritten by the next tile
-  }
-  if (threadIdx.x == 0) wbase[nw] = carry;
-}
-
-__global__ __launch_bounds__(256) void k_row_compact(
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, const int64_t *__restrict__ wbase,
-    int32_t *__restrict__ cidx, uint8_t *__restrict__ cown) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int c = t < n ? row_len(ncnt[t]) : 0;
-  const int excl = wave_incl_scan(c) - c;
-  if (c > 0) {
-    const int64_t b = wbase[t / kWave] + excl;
-    for (int k = 0; k < c; k++) {
-      cidx[b + k] = nbr[t + (int64_t)k * ncap];
-      cown[b + k] = (uint8_t)lane;
-    }
-  }
-}
 
 // Reduces the per-block row statistics of the last build.
 __global__ __launch_bounds__(1024) void k_reduce_rowstats(
@@ -844,8 +797,8 @@ template <bool MULTI>
 __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
     const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
-    const int64_t *__restrict__ wbase, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ wtot, const int32_t *__restrict__ ncnt,
+    int64_t ncap, int64_t n, int32_t maxn, const int32_t *__restrict__ col,
     const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
     double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
     double *__restrict__ vmax, int32_t *__restrict__ err, int remap) {
@@ -860,14 +813,14 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
                   wv * kWave;
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const bool active = t < n;
-  // The wave's entries are [wbase[w], wbase[w + 1]); reading both bounds
-  // up front lets the entry loads start without waiting for the row lengths.
+  // The wave's entries start at its region (64 * maxn entries per wave);
+  // reading their count up front lets the entry loads start without waiting
+  // for the row lengths.
   int64_t B = 0;
   int T = 0;
   if (t - lane < n) {  // the wave holds at least one particle
-    const int64_t b0 = wbase[t / kWave], b1 = wbase[t / kWave + 1];
-    B = b0;
-    T = (int)(b1 - b0);
+    B = (t / kWave) * (int64_t)kWave * maxn;
+    T = wtot[t / kWave];
   }
   const int32_t raw = active ? ncnt[t] : 0;
   const int wave_img = wave_or3(row_img(raw));
@@ -915,104 +868,278 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   if (vmax) wave_vmax(v2, fp.dt, vmax, t, lane, n);
 }
 
+// Serial pass over one particle's column-major row (the reference's
+// per-particle neighbor loop): sphere pairs for yeast, segment pairs for
+// hyphae, summed into acc in row order; ti is the particle's evolving
+// record (rod_pair_i may repair a bond or promote a second segment).
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_force_general(
+__device__ void general_rows(int64_t t, const BoxGeom &g, const DynArrays &in,
+                             const StaticArrays &st,
+                             const int32_t *__restrict__ nbr,
+                             const int32_t *__restrict__ ncnt, int64_t ncap,
+                             const ForceParams &fp, double acc[6], Topo &ti) {
+  const int ty = st.type[t];
+  const double4 p = in.pos[t];
+  const double4 rod_i = in.rod[t];
+  const int c = row_len(ncnt[t]);
+  const int32_t *row = nbr + t;
+  if (ty == rec::YEAST) {
+    for (int k = 0; k < c; k++) {
+      int u = row[(int64_t)k * ncap];
+      if (st.type[u] != rec::YEAST) continue;  // r_lm = 0 (K.H:39)
+      double4 o = in.pos[u];
+      double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+      double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+      double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+      double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      double rS = p.w + o.w;
+      double lim = rS + fp.WA - 1.0e-15;
+      if (r2 <= lim * lim)
+        sphere_pair(d0, d1, d2, r2, rS, fp, acc[0], acc[1], acc[2]);
+    }
+  } else if (ty == rec::FUNGI) {
+    RodSelf S;
+    rod_self_init(S, rod_i, in.trig[t], p.w, st.bscale[t], fp.visc);
+    for (int k = 0; k < c; k++) {
+      int u = row[(int64_t)k * ncap];
+      if (st.type[u] != rec::FUNGI) continue;
+      double4 o = in.pos[u];
+      double4 rod_j = in.rod[u];
+      double d[3];
+      d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+      d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+      d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double rlm = 1.5 * (rod_i.x + rod_j.x) + fp.WA;  // K.H:33-37
+      double lim = rlm - 1.0e-15;
+      if (r2 <= lim * lim) {
+        Topo tj = in.topo[u];
+        rod_pair_i(d, S, ti, tj, rod_j.x, o.w, st.bscale[u], in.trig[u], fp,
+                   acc);
+      }
+    }
+  }
+}
+
+// Surface term, overdamped update and outputs of one particle after its
+// pair terms (K.H:1009-1282, PCI:311-449); returns |vel|^2.
+__device__ double general_finish(int64_t t, const BoxGeom &g,
+                                 const DynArrays &in, const DynArrays &out,
+                                 const StaticArrays &st, const ForceParams &fp,
+                                 double acc[6], Topo &ti,
+                                 double *__restrict__ fc, int64_t ncap,
+                                 int32_t *__restrict__ err) {
+  const int ty = st.type[t];
+  const double4 p = in.pos[t];
+  const double4 rod_i = in.rod[t];
+  const double4 tr_i = in.trig[t];
+  if (ty == rec::YEAST) {
+    acc[0] += 0.0;
+    acc[1] += 0.0;
+    acc[2] += wall_law(p.z, p.w, fp);
+    acc[3] += 0.0;
+    acc[4] += 0.0;
+    acc[5] += 0.0;
+  } else if (ty == rec::FUNGI) {
+    RodSelf S;
+    rod_self_init(S, rod_i, tr_i, p.w, st.bscale[t], fp.visc);
+    double vs[3], ws[3];
+    rod_surface(p, S, fp, vs, ws);
+    acc[0] += vs[0];
+    acc[1] += vs[1];
+    acc[2] += vs[2];
+    acc[3] += ws[0];
+    acc[4] += ws[1];
+    acc[5] += ws[2];
+    ti.fix_site = 0;  // K.H:1060
+  }
+  double vx = fp.mob * acc[0], vy = fp.mob * acc[1], vz = fp.mob * acc[2];
+  double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
+  x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+  y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+  z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+  out.pos[t] = make_double4(x, y, z, p.w);
+  if (ty == rec::FUNGI) {
+    double th = rod_i.y, ph = rod_i.z;
+    double w[3] = {acc[3], acc[4], acc[5]};
+    rod_rotate(th, ph, tr_i, rod_i.x, w, fp.dt);
+    out.rod[t] = make_double4(rod_i.x, th, ph, rod_i.w);
+    out.trig[t] = make_double4(bmx_cos(th), bmx_sin(th), bmx_cos(ph), bmx_sin(ph));
+  } else {
+    out.rod[t] = rod_i;
+    out.trig[t] = tr_i;
+  }
+  out.topo[t] = ti;
+  if (fc) {  // original order
+    const int64_t o = st.oid[t];
+#pragma unroll
+    for (int k = 0; k < 6; k++) fc[o + k * ncap] = acc[k];
+  }
+  bool fin = true;
+#pragma unroll
+  for (int k = 0; k < 6; k++) fin = fin && isfinite(acc[k]);
+  if (!fin) atomicOr(err, ERRBIT_NAN);
+  return vx * vx + vy * vy + vz * vz;
+}
+
+// ------------------------------------------------ flat general force ---
+//
+// Segments (and mixed colonies) over the flat row layout with tiles of
+// kGTile particles, one tile per wave.  Per window of 128 entries a light
+// pass (neighbor position, length and type) applies the interaction test;
+// the interacting entries are packed by ballot, in entry order, and only
+// they load the heavy data and run the pair law (rod_pair_i, sphere_pair),
+// each term into its LDS slot.  The tile's particles (lanes 0..kGTile-1)
+// then add their runs of terms in row order.  The pair law is evaluated
+// with the particle's record as it stands before its row; that is what the
+// row loop sees unless an earlier pair of the row repaired a bond.  A tile
+// where some pair repairs a bond is recomputed row by row (general_rows),
+// so every result is the row loop's.  Promotion of a second segment and
+// the dltd reset do not depend on the order of the pairs.
+constexpr int kGTile = 32;
+constexpr int kGWin = 2 * kWave;
+constexpr size_t kGenWaveBytes =
+    6 * kGWin * sizeof(double) + 3 * kGWin * sizeof(int32_t);
+
+template <bool MULTI>
+__device__ __forceinline__ bool general_test(
+    int64_t ti_, int64_t u, bool valid, const DynArrays &in,
+    const StaticArrays &st, const BoxGeom *__restrict__ boxes,
+    const BoxGeom &g0, const ForceParams &fp) {
+  if (!valid) return false;
+  const int ty = st.type[ti_], tyj = st.type[u];
+  if (ty != tyj || (ty != rec::YEAST && ty != rec::FUNGI)) return false;
+  const BoxGeom &g = MULTI ? boxes[st.col[ti_]] : g0;
+  const double4 p = in.pos[ti_], o = in.pos[u];
+  const double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+  const double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+  const double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+  const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+  double lim;
+  if (ty == rec::YEAST) {
+    lim = p.w + o.w + fp.WA - 1.0e-15;
+  } else {
+    const double rlm = 1.5 * (in.rod[ti_].x + in.rod[u].x) + fp.WA;  // K.H:33-37
+    lim = rlm - 1.0e-15;
+  }
+  return r2 <= lim * lim;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_force_general_flat(
     DynArrays in, DynArrays out, StaticArrays st,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, const BoxGeom *__restrict__ boxes, BoxGeom g0,
-    ForceParams fp, double *__restrict__ fc,
-    double *__restrict__ vmax, int32_t *__restrict__ err,
-    int remap) {
-  int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
+    const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
+    const int32_t *__restrict__ wtot, const int32_t *__restrict__ nbr,
+    const int32_t *__restrict__ ncnt, int64_t ncap, int64_t n, int32_t maxn,
+    const BoxGeom *__restrict__ boxes, BoxGeom g0,
+    ForceParams fp, double *__restrict__ fc, double *__restrict__ vmax,
+    int32_t *__restrict__ err, int remap) {
+  extern __shared__ __align__(16) unsigned char gen_lds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+  double *res = reinterpret_cast<double *>(gen_lds + wv * kGenWaveBytes);
+  int32_t *pu = reinterpret_cast<int32_t *>(res + 6 * kGWin);
+  int32_t *po = pu + kGWin;
+  int32_t *pf = po + kGWin;
+  const int64_t tile = tile_of(remap) * nwv + wv;
+  const int64_t t0 = tile * kGTile;
+  const int64_t t = t0 + lane;
+  const bool active = lane < kGTile && t < n;
+  int64_t B = 0;
+  int T = 0;
+  if (t0 < n) {
+    B = tile * (int64_t)kGTile * maxn;
+    T = wtot[tile];
+  }
+  const int c = active ? row_len(ncnt[t]) : 0;
+  const int P = wave_incl_scan(c) - c;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  bool paired = false, promoted = false, repair = false;
+  for (int w0 = 0; w0 < T; w0 += kGWin) {
+    const int e0 = w0 + lane, e1 = e0 + kWave;
+    const bool v0 = e0 < T, v1 = e1 < T;
+    int u0 = 0, u1 = 0, o0 = 0, o1 = 0;
+    if (v0) {
+      u0 = cidx[B + e0];
+      o0 = cown[B + e0];
+    }
+    if (v1) {
+      u1 = cidx[B + e1];
+      o1 = cown[B + e1];
+    }
+    const bool i0 = general_test<MULTI>(t0 + o0, u0, v0, in, st, boxes, g0, fp);
+    const bool i1 = general_test<MULTI>(t0 + o1, u1, v1, in, st, boxes, g0, fp);
+    const unsigned long long b0 = __ballot(i0), b1 = __ballot(i1);
+    const int n0 = __popcll(b0), ni = n0 + __popcll(b1);
+    if (i0) {
+      const int k = __popcll(b0 & below);
+      pu[k] = u0;
+      po[k] = o0;
+    }
+    if (i1) {
+      const int k = n0 + __popcll(b1 & below);
+      pu[k] = u1;
+      po[k] = o1;
+    }
+    wave_lds_sync();
+    for (int k = lane; k < ni; k += kWave) {  // pair laws, packed
+      const int64_t u = pu[k], ti_ = t0 + po[k];
+      const BoxGeom &g = MULTI ? boxes[st.col[ti_]] : g0;
+      const double4 p = in.pos[ti_], o = in.pos[u];
+      double d[3];
+      d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+      d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+      d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+      double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int flag = 0;
+      if (st.type[ti_] == rec::YEAST) {
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        sphere_pair(d[0], d[1], d[2], r2, p.w + o.w, fp, f[0], f[1], f[2]);
+      } else {
+        RodSelf S;
+        rod_self_init(S, in.rod[ti_], in.trig[ti_], p.w, st.bscale[ti_],
+                      fp.visc);
+        Topo tio = in.topo[ti_];
+        const int pos0 = tio.position, nb0 = tio.n_bnds;
+        const Topo tj = in.topo[u];
+        rod_pair_i(d, S, tio, tj, in.rod[u].x, o.w, st.bscale[u], in.trig[u],
+                   fp, f);
+        if (tio.n_bnds != nb0) repair = true;
+        if (tio.position != pos0) flag = 1;
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) res[q * kGWin + k] = f[q];
+      pf[k] = flag;
+    }
+    wave_lds_sync();
+    const int lo = P > w0 ? P : w0;
+    const int hi = P + c < w0 + kGWin ? P + c : w0 + kGWin;
+    if (lo < hi) {
+      const int k1 = packed_before(hi - w0, b0, b1, n0);
+      for (int k = packed_before(lo - w0, b0, b1, n0); k < k1; k++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += res[q * kGWin + k];
+        promoted = promoted || pf[k] != 0;
+        paired = true;
+      }
+    }
+    wave_lds_sync();
+  }
+  const bool redo = __any(repair);
   double v2 = 0.0;
-  if (t < n) {
+  if (active) {
     const BoxGeom &g = MULTI ? boxes[st.col[t]] : g0;
-    const int ty = st.type[t];
-    const double4 p = in.pos[t];
-    const double4 rod_i = in.rod[t];
-    const double4 tr_i = in.trig[t];
     Topo ti = in.topo[t];
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int c = row_len(ncnt[t]);
-    const int32_t *row = nbr + t;
-    if (ty == rec::YEAST) {
-      for (int k = 0; k < c; k++) {
-        int u = row[(int64_t)k * ncap];
-        if (st.type[u] != rec::YEAST) continue;  // r_lm = 0 (K.H:39)
-        double4 o = in.pos[u];
-        double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-        double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-        double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        double rS = p.w + o.w;
-        double lim = rS + fp.WA - 1.0e-15;
-        if (r2 <= lim * lim)
-          sphere_pair(d0, d1, d2, r2, rS, fp, acc[0], acc[1], acc[2]);
-      }
-      acc[0] += 0.0;
-      acc[1] += 0.0;
-      acc[2] += wall_law(p.z, p.w, fp);
-      acc[3] += 0.0;
-      acc[4] += 0.0;
-      acc[5] += 0.0;
-    } else if (ty == rec::FUNGI) {
-      RodSelf S;
-      rod_self_init(S, rod_i, tr_i, p.w, st.bscale[t], fp.visc);
-      for (int k = 0; k < c; k++) {
-        int u = row[(int64_t)k * ncap];
-        if (st.type[u] != rec::FUNGI) continue;
-        double4 o = in.pos[u];
-        double4 rod_j = in.rod[u];
-        double d[3];
-        d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-        d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-        d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double rlm = 1.5 * (rod_i.x + rod_j.x) + fp.WA;  // K.H:33-37
-        double lim = rlm - 1.0e-15;
-        if (r2 <= lim * lim) {
-          Topo tj = in.topo[u];
-          rod_pair_i(d, S, ti, tj, rod_j.x, o.w, st.bscale[u], in.trig[u], fp,
-                     acc);
-        }
-      }
-      double vs[3], ws[3];
-      rod_surface(p, S, fp, vs, ws);
-      acc[0] += vs[0];
-      acc[1] += vs[1];
-      acc[2] += vs[2];
-      acc[3] += ws[0];
-      acc[4] += ws[1];
-      acc[5] += ws[2];
-      ti.fix_site = 0;  // K.H:1060
-    }
-    double vx = fp.mob * acc[0], vy = fp.mob * acc[1], vz = fp.mob * acc[2];
-    double x = p.x + fp.dt * vx, y = p.y + fp.dt * vy, z = p.z + fp.dt * vz;
-    x = wrap_axis(x, g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
-    y = wrap_axis(y, g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
-    z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
-    out.pos[t] = make_double4(x, y, z, p.w);
-    if (ty == rec::FUNGI) {
-      double th = rod_i.y, ph = rod_i.z;
-      double w[3] = {acc[3], acc[4], acc[5]};
-      rod_rotate(th, ph, tr_i, rod_i.x, w, fp.dt);
-      out.rod[t] = make_double4(rod_i.x, th, ph, rod_i.w);
-      out.trig[t] = make_double4(bmx_cos(th), bmx_sin(th), bmx_cos(ph), bmx_sin(ph));
-    } else {
-      out.rod[t] = rod_i;
-      out.trig[t] = tr_i;
-    }
-    out.topo[t] = ti;
-    if (fc) {  // original order
-      const int64_t o = st.oid[t];
+    if (redo) {
 #pragma unroll
-      for (int k = 0; k < 6; k++) fc[o + k * ncap] = acc[k];
+      for (int q = 0; q < 6; q++) acc[q] = 0.0;
+      general_rows<MULTI>(t, g, in, st, nbr, ncnt, ncap, fp, acc, ti);
+    } else if (st.type[t] == rec::FUNGI && paired) {
+      ti.dltd = 1;  // K.H:364-369
+      if (promoted) ti.position = rec::INTERIOR;
     }
-    bool fin = true;
-#pragma unroll
-    for (int k = 0; k < 6; k++) fin = fin && isfinite(acc[k]);
-    if (!fin) atomicOr(err, ERRBIT_NAN);
-    v2 = vx * vx + vy * vy + vz * vz;
+    v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
   }
   if (vmax) block_vmax(v2, fp.dt, vmax);
 }
@@ -1283,14 +1410,14 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, int32_t *wtot, LaunchShape ls,
+                            int32_t *bstats, FlatRows flat, LaunchShape ls,
                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
 #define BMX_BUILD(M, RD)                                                      \
   hipLaunchKernelGGL((k_build_nbr<M, RD>), dim3(grid_for(n, b)), dim3(b), 0, s, \
                      pos, col, type, rod, boxes, g0, start, key, n, mode,     \
-                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats, wtot, \
+                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats, flat, \
                      ls.remap)
   if (multi && rods)
     BMX_BUILD(true, true);
@@ -1318,22 +1445,11 @@ size_t flat_lds_bytes(int block, int multi) {
   return (size_t)(block / kWave) * (kFlatWaveBytes + (multi ? kWave * sizeof(int32_t) : 0));
 }
 
-hipError_t launch_row_compact(const int32_t *nbr, const int32_t *ncnt,
-                              int64_t ncap, int64_t n, const int32_t *wtot,
-                              int64_t *wbase, int32_t *cidx, uint8_t *cown,
-                              hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int64_t nw = (n + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_wave_bases, dim3(1), dim3(1024), 0, s, wtot, nw, wbase);
-  hipLaunchKernelGGL(k_row_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, nbr,
-                     ncnt, ncap, n, wbase, cidx, cown);
-  return hipGetLastError();
-}
-
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
-                                    const int64_t *wbase, const int32_t *ncnt,
-                                    int64_t ncap, int64_t n, const int32_t *col,
+                                    const int32_t *wtot, const int32_t *ncnt,
+                                    int64_t ncap, int64_t n, int32_t maxn,
+                                    const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,
                                     const int32_t *fc_oid, double *vmax,
@@ -1344,32 +1460,44 @@ hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
   const size_t lds = flat_lds_bytes(b, multi);
   if (multi)
     hipLaunchKernelGGL(k_force_sphere_flat<true>, dim3(grid_for(n, b)), dim3(b),
-                       lds, s, pos_in, pos_out, cidx, cown, wbase, ncnt, ncap, n,
-                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
+                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, ncap, n,
+                       maxn, col, boxes, g0, fp, fc, fc_oid, vmax, err,
+                       ls.remap);
   else
     hipLaunchKernelGGL(k_force_sphere_flat<false>, dim3(grid_for(n, b)), dim3(b),
-                       lds, s, pos_in, pos_out, cidx, cown, wbase, ncnt, ncap, n,
-                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
+                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, ncap, n,
+                       maxn, col, boxes, g0, fp, fc, fc_oid, vmax, err,
+                       ls.remap);
   return hipGetLastError();
 }
 
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
-                                const int32_t *nbr, const int32_t *ncnt,
-                                int64_t ncap, int64_t n, const BoxGeom *boxes,
-                                BoxGeom g0, int multi, ForceParams fp,
-                                double *fc, double *vmax, int32_t *err,
-                                LaunchShape ls, hipStream_t s) {
+                                const int32_t *cidx, const uint8_t *cown,
+                                const int32_t *wtot, const int32_t *nbr,
+                                const int32_t *ncnt, int64_t ncap, int64_t n,
+                                int32_t maxn, const BoxGeom *boxes, BoxGeom g0,
+                                int multi, ForceParams fp, double *fc,
+                                double *vmax, int32_t *err, LaunchShape ls,
+                                hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  int b = ls.block > 256 ? 256 : ls.block;
+  const int b = ls.block > 256 ? 256 : ls.block;
+  const int64_t tiles = (n + kGTile - 1) / kGTile;
+  const int grid = grid_for(tiles, b / kWave);
+  const size_t lds = (size_t)(b / kWave) * kGenWaveBytes;
   if (multi)
-    hipLaunchKernelGGL(k_force_general<true>, dim3(grid_for(n, b)), dim3(b), 0,
-                       s, in, out, st, nbr, ncnt, ncap, n, boxes, g0, fp, fc,
-                       vmax, err, ls.remap);
+    hipLaunchKernelGGL(k_force_general_flat<true>, dim3(grid), dim3(b), lds, s,
+                       in, out, st, cidx, cown, wtot, nbr, ncnt, ncap, n, maxn,
+                       boxes, g0, fp, fc, vmax, err, ls.remap);
   else
-    hipLaunchKernelGGL(k_force_general<false>, dim3(grid_for(n, b)), dim3(b), 0,
-                       s, in, out, st, nbr, ncnt, ncap, n, boxes, g0, fp, fc,
-                       vmax, err, ls.remap);
+    hipLaunchKernelGGL(k_force_general_flat<false>, dim3(grid), dim3(b), lds, s,
+                       in, out, st, cidx, cown, wtot, nbr, ncnt, ncap, n, maxn,
+                       boxes, g0, fp, fc, vmax, err, ls.remap);
   return hipGetLastError();
+}
+
+int flat_tile(int segments) { return segments ? kGTile : kWave; }
+int64_t flat_entries(int64_t ncap, int tile, int32_t maxn) {
+  return (ncap + tile - 1) / tile * (int64_t)tile * maxn;
 }
 
 hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
