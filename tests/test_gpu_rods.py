@@ -117,6 +117,40 @@ def test_mixed_spheres_and_segments(bmx, oracle, gpu):
     assert np.array_equal(ctx.int_fields(), orecs["idata"])
 
 
+def test_bond_repair_and_promotion(bmx, oracle, gpu):
+    """One-sided bond tables (a bond dropped from one end, as division and
+    fusion leave them) and second segments: the force pass repairs the
+    tables and promotes the segments as the row loop does (on the GPU a
+    tile with a repair is recomputed row by row), bit for bit."""
+    col = small_hyphae()
+    hood = 1.1 * (2 * 5.47e-4 + 1e-2)
+    recs = col.recs.copy()
+    g = oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+    st0 = oracle.State(recs, g, col.fpar, oracle.NBR_STENCIL, hood, 1)
+    assert st0.init_bonds() == 0
+    ip = recs["idata"]
+    rng = np.random.default_rng(5)
+    bonded = np.flatnonzero(ip[:, 6] >= 1)
+    drop = rng.choice(bonded, size=len(bonded) // 10, replace=False)
+    ip[drop, 6] -= 1                        # the last bond forgotten on one side
+    inner = np.flatnonzero(ip[:, 19] == 3)
+    sec = rng.choice(inner, size=60, replace=False)
+    ip[sec, 19] = rng.integers(1, 3, size=60)       # SECOND_1 / SECOND_2
+    col.recs = recs.copy()
+    before = col.recs["idata"].copy()
+    ctx, grecs, st, orecs = make_pair(bmx, oracle, col, mode=1, K=1, hood=hood,
+                                      maxn=256)
+    for _ in range(3):
+        ctx.evolve(col.dt, 1)
+        fo = st.evolve(col.dt, 1)
+        close_fc(ctx.fc(), fo)
+    close_traj(ctx, orecs, (col.prob_hi - col.prob_lo).max())
+    gi = ctx.int_fields()
+    assert np.array_equal(gi, orecs["idata"])
+    assert (gi[drop, 6] > before[drop, 6]).sum() > len(drop) // 2   # repaired
+    assert (gi[sec, 19] == 3).sum() > 10                             # promoted
+
+
 @pytest.mark.parametrize("fname", ["fungi_init_cfg.dat", "wall_test_fungi_init_cfg.dat"])
 def test_reference_single_segment_decks(bmx, oracle, gpu, fname):
     """The reference's own fungi and wall_test inputs: one segment settling on
