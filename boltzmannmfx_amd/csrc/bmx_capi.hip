@@ -118,6 +118,7 @@ struct bmx_ctx {
   int flat_tile = 64;
   DevBuf<int32_t> d_wtot, d_cidx;
   DevBuf<uint8_t> d_cown;
+  DevBuf<int8_t> d_redo;  // segment tiles left to the row loop this step
   // species A, B, C and volume, and their exchange increments, in original
   // order (loaded from the records at the first exchange)
   DevBuf<double4> d_chem, d_chem_inc;
@@ -321,6 +322,7 @@ bmx_status ensure_rows(bmx_ctx *c) {
   const int64_t nw = (c->ncap + c->flat_tile - 1) / c->flat_tile;
   const int64_t nflat = flat_entries(c->ncap, c->flat_tile, c->maxn);
   HIPCHK(c->d_wtot.alloc(nw + 1));
+  HIPCHK(c->d_redo.alloc(nw + 1));
   HIPCHK(c->d_cidx.alloc(nflat));
   HIPCHK(c->d_cown.alloc(nflat));
   return BMX_OK;
@@ -429,9 +431,9 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   } else {
     HIPCHK(launch_force_general(
         dyn_arrays(c, c->cd), dyn_arrays(c, 1 - c->cd), static_arrays(c, c->cs),
-        c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_nbr.p, c->d_ncnt.p,
-        c->ncap, c->n, c->maxn, c->d_boxes.p, c->boxes[0], multi, fp, fc, vm,
-        c->d_err.p, shape(c), s));
+        c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_redo.p, c->d_nbr.p,
+        c->d_ncnt.p, c->ncap, c->n, c->maxn, c->d_boxes.p, c->boxes[0], multi,
+        fp, fc, vm, c->d_err.p, shape(c), s));
   }
   if (timed) {
     HIPCHK(hipEventRecord(e1, s));
@@ -740,6 +742,7 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_nbr.release();
   c->d_ncnt.release();
   c->d_wtot.release();
+  c->d_redo.release();
   c->d_cidx.release();
   c->d_cown.release();
   c->d_fc.release();

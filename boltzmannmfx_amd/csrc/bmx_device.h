@@ -285,9 +285,9 @@ __device__ inline void rod_mobility(double area, double c_length, double visc,
     }
 }
 
-__device__ inline void rod_self_init(RodSelf &S, const double4 &rod,
-                                     const double4 &tr, double rad, double bs,
-                                     double visc) {
+// The geometric part of RodSelf (all rod_pair_forces needs).
+__device__ inline void rod_self_geom(RodSelf &S, const double4 &rod,
+                                     const double4 &tr, double rad, double bs) {
   S.L = rod.x;
   S.area = rod.w;
   S.rad = rad;
@@ -295,9 +295,21 @@ __device__ inline void rod_self_init(RodSelf &S, const double4 &rod,
   S.nx = tr.y * tr.z;
   S.ny = tr.y * tr.w;
   S.nz = tr.x;
+}
+
+// The mobility part of RodSelf (after rod_self_geom).
+__device__ inline void rod_self_mob(RodSelf &S, const double4 &tr,
+                                    double visc) {
   double R[3][3];
   rod_rotation(tr, R);
-  rod_mobility(rod.w, rod.x, visc, R, S.dii, S.dij);
+  rod_mobility(S.area, S.L, visc, R, S.dii, S.dij);
+}
+
+__device__ inline void rod_self_init(RodSelf &S, const double4 &rod,
+                                     const double4 &tr, double rad, double bs,
+                                     double visc) {
+  rod_self_geom(S, rod, tr, rad, bs);
+  rod_self_mob(S, tr, visc);
 }
 
 __device__ __forceinline__ void remove_axial(double f1[3], double f2[3],
@@ -318,10 +330,16 @@ __device__ __forceinline__ void remove_axial(double f1[3], double f2[3],
 // reference discards the j-side outputs (PCI:251-261 are disabled) and its
 // writes to the neighbor's record land in a per-pair copy (PCI:188).
 // ti is i's own evolving record; tj is the neighbor's record at step start.
-__device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
-                                  Topo &ti, const Topo &tj, double Lj,
-                                  double radj, double bsj, const double4 &trj,
-                                  const ForceParams &fp, double acc[6]) {
+// rod_pair_forces forms the site forces f1, f2 (after the axial removal)
+// and the i axis r_i; it reads only the geometric part of S.
+// rod_pair_mobility applies the two-site mobility and adds the pair's
+// velocity and angular terms to acc.  rod_pair_i is the two in sequence.
+__device__ inline void rod_pair_forces(const double d[3], const RodSelf &S,
+                                       Topo &ti, const Topo &tj, double Lj,
+                                       double radj, double bsj,
+                                       const double4 &trj,
+                                       const ForceParams &fp, double f1[3],
+                                       double f2[3], double ri[3]) {
   double rS = S.rad + radj;
   double rA = rS + fp.WA;
   double scale = S.bs * bsj;
@@ -395,7 +413,11 @@ __device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
   double xj1 = -0.5 * nxj * Lj, yj1 = -0.5 * nyj * Lj, zj1 = -0.5 * nzj * Lj;
   double xj2 = 0.5 * nxj * Lj, yj2 = 0.5 * nyj * Lj, zj2 = 0.5 * nzj * Lj;
 
-  double f1[3] = {0.0, 0.0, 0.0}, f2[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    f1[a] = 0.0;
+    f2[a] = 0.0;
+  }
   bool bonded = false;
   // K.H:615-698
   bool ihit = false, jhit = false;
@@ -459,8 +481,16 @@ __device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
     }
   }
   // K.H:776-792
-  double rxi = xi1 - xi2, ryi = yi1 - yi2, rzi = zi1 - zi2;
-  remove_axial(f1, f2, rxi, ryi, rzi);
+  ri[0] = xi1 - xi2;
+  ri[1] = yi1 - yi2;
+  ri[2] = zi1 - zi2;
+  remove_axial(f1, f2, ri[0], ri[1], ri[2]);
+}
+
+__device__ inline void rod_pair_mobility(const RodSelf &S, const double f1[3],
+                                         const double f2[3],
+                                         const double ri[3], double acc[6]) {
+  const double rxi = ri[0], ryi = ri[1], rzi = ri[2];
   // K.H:884-893
   double v1[3], v2[3];
 #pragma unroll
@@ -487,6 +517,15 @@ __device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
   acc[3] += w0;
   acc[4] += w1;
   acc[5] += w2;
+}
+
+__device__ inline void rod_pair_i(const double d[3], const RodSelf &S,
+                                  Topo &ti, const Topo &tj, double Lj,
+                                  double radj, double bsj, const double4 &trj,
+                                  const ForceParams &fp, double acc[6]) {
+  double f1[3], f2[3], ri[3];
+  rod_pair_forces(d, S, ti, tj, Lj, radj, bsj, trj, fp, f1, f2, ri);
+  rod_pair_mobility(S, f1, f2, ri, acc);
 }
 
 // Rod growth-surface term (K.H:1031-1282; the mobility velocities of

@@ -104,12 +104,12 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
 // as for launch_force_sphere_flat, vmax per block.
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *cidx, const uint8_t *cown,
-                                const int32_t *wtot, const int32_t *nbr,
-                                const int32_t *ncnt, int64_t ncap, int64_t n,
-                                int32_t maxn, const BoxGeom *boxes, BoxGeom g0,
-                                int multi, ForceParams fp, double *fc,
-                                double *vmax, int32_t *err, LaunchShape ls,
-                                hipStream_t s);
+                                const int32_t *wtot, int8_t *redo_tile,
+                                const int32_t *nbr, const int32_t *ncnt,
+                                int64_t ncap, int64_t n, int32_t maxn,
+                                const BoxGeom *boxes, BoxGeom g0, int multi,
+                                ForceParams fp, double *fc, double *vmax,
+                                int32_t *err, LaunchShape ls, hipStream_t s);
 hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
                              const int32_t *ncnt, int64_t ncap, int64_t n,
                              const BoxGeom *boxes, BoxGeom g0, int multi,

@@ -993,8 +993,9 @@ __device__ double general_finish(int64_t t, const BoxGeom &g,
 // then add their runs of terms in row order.  The pair law is evaluated
 // with the particle's record as it stands before its row; that is what the
 // row loop sees unless an earlier pair of the row repaired a bond.  A tile
-// where some pair repairs a bond is recomputed row by row (general_rows),
-// so every result is the row loop's.  Promotion of a second segment and
+// where some pair repairs a bond is marked and recomputed row by row by
+// k_force_general_redo (general_rows), so every result is the row loop's;
+// keeping the row loop out of this kernel keeps its register budget low.  Promotion of a second segment and
 // the dltd reset do not depend on the order of the pairs.
 constexpr int kGTile = 32;
 constexpr int kGWin = 2 * kWave;
@@ -1026,10 +1027,10 @@ __device__ __forceinline__ bool general_test(
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_force_general_flat(
+__global__ __launch_bounds__(256, 4) void k_force_general_flat(
     DynArrays in, DynArrays out, StaticArrays st,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
-    const int32_t *__restrict__ wtot, const int32_t *__restrict__ nbr,
+    const int32_t *__restrict__ wtot, int8_t *__restrict__ redo_tile,
     const int32_t *__restrict__ ncnt, int64_t ncap, int64_t n, int32_t maxn,
     const BoxGeom *__restrict__ boxes, BoxGeom g0,
     ForceParams fp, double *__restrict__ fc, double *__restrict__ vmax,
@@ -1097,16 +1098,22 @@ __global__ __launch_bounds__(256) void k_force_general_flat(
         const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         sphere_pair(d[0], d[1], d[2], r2, p.w + o.w, fp, f[0], f[1], f[2]);
       } else {
+        // site forces first, the mobility after: its matrices are not live
+        // during the closest-point computation
         RodSelf S;
-        rod_self_init(S, in.rod[ti_], in.trig[ti_], p.w, st.bscale[ti_],
-                      fp.visc);
+        rod_self_geom(S, in.rod[ti_], in.trig[ti_], p.w, st.bscale[ti_]);
         Topo tio = in.topo[ti_];
         const int pos0 = tio.position, nb0 = tio.n_bnds;
-        const Topo tj = in.topo[u];
-        rod_pair_i(d, S, tio, tj, in.rod[u].x, o.w, st.bscale[u], in.trig[u],
-                   fp, f);
+        double f1[3], f2[3], ri[3];
+        {
+          const Topo tj = in.topo[u];
+          rod_pair_forces(d, S, tio, tj, in.rod[u].x, o.w, st.bscale[u],
+                          in.trig[u], fp, f1, f2, ri);
+        }
         if (tio.n_bnds != nb0) repair = true;
         if (tio.position != pos0) flag = 1;
+        rod_self_mob(S, in.trig[ti_], fp.visc);
+        rod_pair_mobility(S, f1, f2, ri, f);
       }
 #pragma unroll
       for (int q = 0; q < 6; q++) res[q * kGWin + k] = f[q];
@@ -1126,22 +1133,43 @@ __global__ __launch_bounds__(256) void k_force_general_flat(
     }
     wave_lds_sync();
   }
+  // A tile with a bond repair is left to k_force_general_redo.
   const bool redo = __any(repair);
+  if (lane == 0 && t0 < n) redo_tile[tile] = redo ? 1 : 0;
   double v2 = 0.0;
-  if (active) {
+  if (active && !redo) {
     const BoxGeom &g = MULTI ? boxes[st.col[t]] : g0;
     Topo ti = in.topo[t];
-    if (redo) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) acc[q] = 0.0;
-      general_rows<MULTI>(t, g, in, st, nbr, ncnt, ncap, fp, acc, ti);
-    } else if (st.type[t] == rec::FUNGI && paired) {
+    if (st.type[t] == rec::FUNGI && paired) {
       ti.dltd = 1;  // K.H:364-369
       if (promoted) ti.position = rec::INTERIOR;
     }
     v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
   }
   if (vmax) block_vmax(v2, fp.dt, vmax);
+}
+
+// The row loop for the particles of the tiles that the flat pass marked
+// (a pair of the tile repairs a bond).  Runs after the flat pass on the
+// same stream; its displacement maxima join the step's partials through an
+// atomic max (non-negative doubles order as their bit patterns).
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_force_general_redo(
+    DynArrays in, DynArrays out, StaticArrays st,
+    const int8_t *__restrict__ redo_tile, const int32_t *__restrict__ nbr,
+    const int32_t *__restrict__ ncnt, int64_t ncap, int64_t n,
+    const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
+    double *__restrict__ fc, double *__restrict__ vmax,
+    int32_t *__restrict__ err) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !redo_tile[t / kGTile]) return;
+  const BoxGeom &g = MULTI ? boxes[st.col[t]] : g0;
+  Topo ti = in.topo[t];
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  general_rows<MULTI>(t, g, in, st, nbr, ncnt, ncap, fp, acc, ti);
+  const double v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
+  if (vmax)
+    atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(fp.dt * sqrt(v2)));
 }
 
 // InitBonds: per segment, scan the row in order and append a bond for the
@@ -1473,25 +1501,32 @@ hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
 
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *cidx, const uint8_t *cown,
-                                const int32_t *wtot, const int32_t *nbr,
-                                const int32_t *ncnt, int64_t ncap, int64_t n,
-                                int32_t maxn, const BoxGeom *boxes, BoxGeom g0,
-                                int multi, ForceParams fp, double *fc,
-                                double *vmax, int32_t *err, LaunchShape ls,
-                                hipStream_t s) {
+                                const int32_t *wtot, int8_t *redo_tile,
+                                const int32_t *nbr, const int32_t *ncnt,
+                                int64_t ncap, int64_t n, int32_t maxn,
+                                const BoxGeom *boxes, BoxGeom g0, int multi,
+                                ForceParams fp, double *fc, double *vmax,
+                                int32_t *err, LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int b = ls.block > 256 ? 256 : ls.block;
   const int64_t tiles = (n + kGTile - 1) / kGTile;
   const int grid = grid_for(tiles, b / kWave);
   const size_t lds = (size_t)(b / kWave) * kGenWaveBytes;
-  if (multi)
+  if (multi) {
     hipLaunchKernelGGL(k_force_general_flat<true>, dim3(grid), dim3(b), lds, s,
-                       in, out, st, cidx, cown, wtot, nbr, ncnt, ncap, n, maxn,
-                       boxes, g0, fp, fc, vmax, err, ls.remap);
-  else
+                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, ncap, n,
+                       maxn, boxes, g0, fp, fc, vmax, err, ls.remap);
+    hipLaunchKernelGGL(k_force_general_redo<true>, dim3(grid_for(n, 256)),
+                       dim3(256), 0, s, in, out, st, redo_tile, nbr, ncnt,
+                       ncap, n, boxes, g0, fp, fc, vmax, err);
+  } else {
     hipLaunchKernelGGL(k_force_general_flat<false>, dim3(grid), dim3(b), lds, s,
-                       in, out, st, cidx, cown, wtot, nbr, ncnt, ncap, n, maxn,
-                       boxes, g0, fp, fc, vmax, err, ls.remap);
+                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, ncap, n,
+                       maxn, boxes, g0, fp, fc, vmax, err, ls.remap);
+    hipLaunchKernelGGL(k_force_general_redo<false>, dim3(grid_for(n, 256)),
+                       dim3(256), 0, s, in, out, st, redo_tile, nbr, ncnt,
+                       ncap, n, boxes, g0, fp, fc, vmax, err);
+  }
   return hipGetLastError();
 }
 
