@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "particle-steps/s at 1M FP64 spheres; batched ensemble speedup; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # MI355X FP64 vector
 
 
 def algorithmic_bytes_per_particle(mean_row: float) -> float:
@@ -226,6 +227,17 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
         cpu = {"value": n * cpu_steps / el, "unit": "particle-steps/s", "cores": nt,
                "kind": "port", "sample": f"{cpu_steps} steps of the {n}-segment workload "
                                          f"on {nt} OpenMP threads"}
+    force_ms = tm.force_ms / max(tm.force_launches, 1)
+    roof = None
+    pmc_path = os.path.join(ROOT, "profiles", "force_general_200k_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            flop = json.load(f).get("fp64_flop_per_launch")
+        if flop:
+            ach = flop / (force_ms * 1e-3) / 1e12
+            roof = {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFS,
+                    "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFS,
+                    "flop_per_launch": flop}
     return {
         "workload": f"{n} hyphal segments (2,000 roots), mesh bins {col.n_cell}, "
                     f"reference stencil list every step, {steps} steps",
@@ -233,10 +245,11 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
         "particle_steps_per_s": n * steps / secs,
         "ms_per_step": secs * 1e3 / steps,
         "force_kernel": "k_force_general_flat",
-        "force_kernel_avg_ms": tm.force_ms / max(tm.force_launches, 1),
+        "force_kernel_avg_ms": force_ms,
         "rebuild_avg_ms": tm.rebuild_ms / max(tm.rebuilds, 1),
         "mean_row": tm.neighbor_entries / n,
         "bound": "fp64-valu",
+        "roofline": roof,
     }
 
 
@@ -385,6 +398,10 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-rods", action="store_true")
+    ap.add_argument("--rods-steps", type=int, default=500)
+    ap.add_argument("--rods-only", action="store_true",
+                    help="run only the 200,000-segment workload and print its "
+                         "object (for profiling that kernel)")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the "
@@ -392,8 +409,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the workload would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path",
+              file=sys.stderr)
         return 2
 
     import numpy as np
@@ -402,6 +419,10 @@ def main() -> int:
 
     if args.calibrate_copy > 0:
         calibration_copy(args.calibrate_copy << 20)
+    if args.rods_only:
+        print(json.dumps(rods_bench(bmx, W, steps=args.rods_steps,
+                                    cpu_steps=0 if args.no_cpu else 10)))
+        return 0
 
     col = W.c2_large(args.n)
     ctx = bmx.Context(0)
@@ -487,7 +508,8 @@ def main() -> int:
         out["adjoint"] = adjoint_bench(bmx, W)
         print("# adjoint done", file=sys.stderr, flush=True)
     if not args.no_rods:
-        out["rods"] = rods_bench(bmx, W, cpu_steps=0 if args.no_cpu else 10)
+        out["rods"] = rods_bench(bmx, W, steps=args.rods_steps,
+                                 cpu_steps=0 if args.no_cpu else 10)
         print("# rods done", file=sys.stderr, flush=True)
     if not args.no_mesh:
         out["mesh"] = mesh_bench(bmx, W, cpu=not args.no_cpu)
