@@ -727,17 +727,35 @@ __device__ __forceinline__ void flat_rows(
     const BoxGeom *__restrict__ boxes, const BoxGeom &g0, const ForceParams &fp,
     double &a0, double &a1, double &a2) {
   const unsigned long long below = (1ull << lane) - 1ull;
+  // One box: the first window is loaded before the entry count is known
+  // (a tile's region, and the padding after the last one, is always
+  // readable; entries past the count are masked below).  Several boxes: the
+  // extra live registers would spill, so every window waits for the count.
+  int f0u = 0, f1u = 0, f0o = 0, f1o = 0;
+  if (!MULTI) {
+    f0u = cidx[B + lane];
+    f1u = cidx[B + kWave + lane];
+    f0o = cown[B + lane];
+    f1o = cown[B + kWave + lane];
+  }
   for (int w0 = 0; w0 < T; w0 += kFlatWin) {
     const int e0 = w0 + lane, e1 = e0 + kWave;
     const bool v0 = e0 < T, v1 = e1 < T;
     int u0 = 0, u1 = 0, o0 = 0, o1 = 0;
-    if (v0) {
-      u0 = cidx[B + e0];
-      o0 = cown[B + e0];
-    }
-    if (v1) {
-      u1 = cidx[B + e1];
-      o1 = cown[B + e1];
+    if (!MULTI && w0 == 0) {
+      u0 = v0 ? f0u : 0;
+      u1 = v1 ? f1u : 0;
+      o0 = v0 ? f0o : 0;
+      o1 = v1 ? f1o : 0;
+    } else {
+      if (v0) {
+        u0 = cidx[B + e0];
+        o0 = cown[B + e0];
+      }
+      if (v1) {
+        u1 = cidx[B + e1];
+        o1 = cown[B + e1];
+      }
     }
     const double4 q0 = pos_in[u0], q1 = pos_in[u1];
     const double4 p0 = sp[o0], p1 = sp[o1];
@@ -1532,7 +1550,8 @@ hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
 
 int flat_tile(int segments) { return segments ? kGTile : kWave; }
 int64_t flat_entries(int64_t ncap, int tile, int32_t maxn) {
-  return (ncap + tile - 1) / tile * (int64_t)tile * maxn;
+  // + one window: the force pass reads a window before it knows the count
+  return (ncap + tile - 1) / tile * (int64_t)tile * maxn + kFlatWin;
 }
 
 hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
