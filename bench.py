@@ -93,7 +93,7 @@ def ensemble_bench(bmx, W, steps: int, seq: bool = True):
     cols = W.c4_ensemble()
     recs, starts, geoms = W.concat_colonies(cols)
     n = len(recs)
-    K = 10
+    K = cols[0].rebuild_every
     ctx = bmx.Context(0)
     ctx.set_force_params(cols[0].fpar)
     ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K, skin=cols[0].skin,

@@ -68,20 +68,15 @@ def c1_small(seed: int = 21893) -> Colony:
     return sphere_colony(2000, 0.2, seed, W_A=5.0e-5)
 
 
-# Verlet skin of the large sphere workloads.  The largest displacement over
-# a 10-step rebuild interval is about 4.2e-7 cm (a 50,000-sphere sample at
-# volume fraction 0.3 over 1,000 steps, set by the growth-surface pull), so a
-# skin of 5e-6 cm leaves the skin check (displacement sum <= skin / 2) a
-# margin of about 6; it keeps 1.2 list entries per sphere against 2.9 for
-# 2e-5 cm.
-LARGE_SKIN = 5.0e-6
-
-
-# Verlet policy of the 1M-sphere workload: lists rebuilt every 40 steps with
-# a skin of 8e-6 cm.  The largest displacement over 40 steps is 1.68e-6 cm
-# (the whole 1M colony, measured after 20 warm-up steps), against the skin
-# check's bound skin / 2 = 4e-6: a margin of 2.4.  Lists hold 1.56 entries
-# per sphere; the step costs 32 us against 44 us at (5e-6, every 10).
+# Verlet policy of the large sphere workloads (the 1M colony and the
+# 256-colony ensemble): lists rebuilt every 40 steps with a skin of 8e-6 cm.
+# The largest displacement over 40 steps is 1.68e-6 cm (the whole 1M colony,
+# measured after 20 warm-up steps), against the skin check's bound
+# skin / 2 = 4e-6: a margin of 2.4; every call checks it and reports an
+# interval too long for the skin.  Lists hold 1.56 entries per sphere at
+# volume fraction 0.3.  Swept on the GPU: 1M colony 27.7 us per step here,
+# 27.6 at (1e-5, 60), 45.5 at (5e-6, 10); ensemble 2.9e10 particle-steps/s
+# here, 2.7e10 at (1e-5, 40), 1.9e10 at (5e-6, 10).
 C2_SKIN = 8.0e-6
 C2_REBUILD_EVERY = 40
 
@@ -113,15 +108,18 @@ def c3_hyphae(n: int = 200_000, nroots: int = 2000, seed: int = 43343,
 
 
 def c4_ensemble(ncol: int = 256, seed: int = 2024, nmin: int = 500,
-                nmax: int = 50_000, skin: float = LARGE_SKIN) -> List[Colony]:
+                nmax: int = 50_000, skin: float = C2_SKIN) -> List[Colony]:
     """Colonies with cell counts log-uniform in [nmin, nmax] and volume
     fraction uniform in [0.1, 0.35], each in its own periodic x/y box."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), ncol)).astype(int)
     phis = rng.uniform(0.1, 0.35, ncol)
     seeds = rng.integers(1, 2**62, ncol)
-    return [sphere_colony(int(n), float(p), int(s), skin=skin)
+    cols = [sphere_colony(int(n), float(p), int(s), skin=skin)
             for n, p, s in zip(sizes, phis, seeds)]
+    for c in cols:
+        c.rebuild_every = C2_REBUILD_EVERY
+    return cols
 
 
 def concat_colonies(cols: List[Colony]):
