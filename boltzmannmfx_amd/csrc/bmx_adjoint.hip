@@ -4,7 +4,8 @@
 // F(r) = -mu (r - R_A)^2 (r - R_S) (K.H:243-263) and the surface law
 // (K.H:1009-1030).  With A = dF/dd = (F/r) I + (F' - F/r) u u^T, which is
 // symmetric and even in d, and interaction sets that are symmetric, the
-// transpose Jacobian is a gather over the same rows:
+// transpose Jacobian is a gather over the same rows (the flat rows of the
+// force pass):
 //   lambda_i <- lambda_i + dt m [ sum_j A_ij (lambda_i - lambda_j)
 //                                 + W'(z_i) lambda_i,z e_z ].
 // Parameter terms of one step (summed per block into fixed slots, so the
@@ -62,53 +63,142 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_loss(
   if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
 }
 
+// One adjoint step over the flat rows (tiles of 64 particles, one per
+// wave; thread i of block b owns particle 256 b + i as in a row-by-row
+// pass).  Per window of 128 entries the lanes apply the pair test
+// (r_lm - small_number, then r < R_A); the pairs that pass are packed in
+// entry order and their nine terms formed 64 at a time into LDS; every
+// owner adds its own terms in row order.  The owner sums, the surface term
+// and the per-block partials are those of a loop over the row.
+constexpr int kAdjWin = 128;
+constexpr int kAdjPass = 64;
+struct AdjWave {
+  double4 p[64];            // owners' positions
+  double4 l[64];            // owners' adjoints
+  int32_t pu[kAdjWin];      // packed pairs of the window: neighbor
+  int32_t po[kAdjWin];      //   and owner lane
+  double term[9][kAdjPass]; // terms of one pass
+};
+union AdjLds {
+  AdjWave w[kAdjBlock / 64];
+  double sh[3][kAdjBlock];
+};
+
 __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
-    const double4 *__restrict__ pos, const int32_t *__restrict__ nbr,
-    const int32_t *__restrict__ ncnt, int64_t ncap,
+    const double4 *__restrict__ pos, const int32_t *__restrict__ cidx,
+    const uint8_t *__restrict__ cown, const int32_t *__restrict__ wtot,
+    const int32_t *__restrict__ ncnt, int32_t maxn,
     const int32_t *__restrict__ oid, int64_t n, BoxGeom g, ForceParams fp,
     const double4 *__restrict__ lam_in, double4 *__restrict__ lam_out,
     double *__restrict__ grad_r, double *__restrict__ acc) {
-  __shared__ double sh[3][kAdjBlock];
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double pmu = 0.0, pwa = 0.0, pm = 0.0;
-  if (t < n) {
-    const double4 p = pos[t];
-    const double4 li = lam_in[t];
-    const double ri = p.w;
-    const int c = ncnt[t] & ((1 << 28) - 1);  // bits 28..30: image axes
-    double jl0 = 0.0, jl1 = 0.0, jl2 = 0.0;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    double dmu = 0.0, dwa = 0.0, dr = 0.0;
-    for (int k = 0; k < c; k++) {
-      int u = nbr[(int64_t)k * ncap + t];
-      double4 o = pos[u];
-      double d0 = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-      double d1 = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-      double d2 = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+  __shared__ AdjLds lds;
+  const int lane = threadIdx.x & 63;
+  AdjWave &L = lds.w[threadIdx.x / 64];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < n;
+  int64_t B = 0;
+  int T = 0;
+  if (t - lane < n) {  // the wave's tile holds at least one particle
+    B = (t / 64) * (int64_t)64 * maxn;
+    T = wtot[t / 64];
+  }
+  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double4 p = active ? pos[t] : zero;
+  const double4 li = active ? lam_in[t] : zero;
+  const int c = active ? (ncnt[t] & ((1 << 28) - 1)) : 0;  // bits 28..30: image axes
+  const int P = wave_incl_scan(c) - c;
+  L.p[lane] = p;
+  L.l[lane] = li;
+  wave_lds_sync();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double jl0 = 0.0, jl1 = 0.0, jl2 = 0.0;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  double dmu = 0.0, dwa = 0.0, dr = 0.0;
+  for (int w0 = 0; w0 < T; w0 += kAdjWin) {
+    bool in[2];
+    int uu[2], oo[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int e = w0 + h * 64 + lane;
+      const bool v = e < T;
+      uu[h] = v ? cidx[B + e] : 0;
+      oo[h] = v ? cown[B + e] : 0;
+      const double4 pi = L.p[oo[h]], o = pos[uu[h]];
+      double d0 = sep_axis(pi.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+      double d1 = sep_axis(pi.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+      double d2 = sep_axis(pi.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
       double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-      double rS = ri + o.w, rA = rS + fp.WA;
+      double rS = pi.w + o.w, rA = rS + fp.WA;
       double lim = rS + fp.WA - 1.0e-15;
-      if (!(r2 <= lim * lim)) continue;
-      double r = sqrt(r2);
-      if (!(r < rA)) continue;
-      double u0 = d0 / r, u1 = d1 / r, u2 = d2 / r;
-      double F = -fp.mu * (r - rA) * (r - rA) * (r - rS);
-      double Fp = -fp.mu * (r - rA) * (3.0 * r - 2.0 * rS - rA);
-      double4 lj = lam_in[u];
-      double l0 = li.x - lj.x, l1 = li.y - lj.y, l2 = li.z - lj.z;
-      double udl = u0 * l0 + u1 * l1 + u2 * l2;
-      double uli = u0 * li.x + u1 * li.y + u2 * li.z;
-      double a = F / r, b = (Fp - F / r) * udl;
-      jl0 += a * l0 + b * u0;
-      jl1 += a * l1 + b * u1;
-      jl2 += a * l2 + b * u2;
-      f0 += F * u0;
-      f1 += F * u1;
-      f2 += F * u2;
-      dmu += -(r - rA) * (r - rA) * (r - rS) * uli;
-      dwa += 2.0 * fp.mu * (r - rA) * (r - rS) * uli;
-      dr += -Fp * udl;
+      in[h] = v && (r2 <= lim * lim) && (sqrt(r2) < rA);
     }
+    const unsigned long long b0 = __ballot(in[0]), b1 = __ballot(in[1]);
+    const int n0 = __popcll(b0), ni = n0 + __popcll(b1);
+    if (in[0]) {
+      const int k = __popcll(b0 & below);
+      L.pu[k] = uu[0];
+      L.po[k] = oo[0];
+    }
+    if (in[1]) {
+      const int k = n0 + __popcll(b1 & below);
+      L.pu[k] = uu[1];
+      L.po[k] = oo[1];
+    }
+    wave_lds_sync();
+    const int lo = P > w0 ? P : w0;
+    const int hi = P + c < w0 + kAdjWin ? P + c : w0 + kAdjWin;
+    const int klo = lo < hi ? packed_before(lo - w0, b0, b1, n0) : 0;
+    const int khi = lo < hi ? packed_before(hi - w0, b0, b1, n0) : 0;
+    for (int k0 = 0; k0 < ni; k0 += kAdjPass) {
+      const int k = k0 + lane;
+      if (k < ni) {
+        const int u = L.pu[k];
+        const double4 pi = L.p[L.po[k]], lii = L.l[L.po[k]], o = pos[u];
+        double d0 = sep_axis(pi.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+        double d1 = sep_axis(pi.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+        double d2 = sep_axis(pi.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        double rS = pi.w + o.w, rA = rS + fp.WA;
+        double r = sqrt(r2);
+        double u0 = d0 / r, u1 = d1 / r, u2 = d2 / r;
+        double F = -fp.mu * (r - rA) * (r - rA) * (r - rS);
+        double Fp = -fp.mu * (r - rA) * (3.0 * r - 2.0 * rS - rA);
+        double4 lj = lam_in[u];
+        double l0 = lii.x - lj.x, l1 = lii.y - lj.y, l2 = lii.z - lj.z;
+        double udl = u0 * l0 + u1 * l1 + u2 * l2;
+        double uli = u0 * lii.x + u1 * lii.y + u2 * lii.z;
+        double a = F / r, b = (Fp - F / r) * udl;
+        L.term[0][lane] = a * l0 + b * u0;
+        L.term[1][lane] = a * l1 + b * u1;
+        L.term[2][lane] = a * l2 + b * u2;
+        L.term[3][lane] = F * u0;
+        L.term[4][lane] = F * u1;
+        L.term[5][lane] = F * u2;
+        L.term[6][lane] = -(r - rA) * (r - rA) * (r - rS) * uli;
+        L.term[7][lane] = 2.0 * fp.mu * (r - rA) * (r - rS) * uli;
+        L.term[8][lane] = -Fp * udl;
+      }
+      wave_lds_sync();
+      const int a0 = klo > k0 ? klo : k0;
+      const int a1 = khi < k0 + kAdjPass ? khi : k0 + kAdjPass;
+      for (int k2 = a0; k2 < a1; k2++) {
+        const int q = k2 - k0;
+        jl0 += L.term[0][q];
+        jl1 += L.term[1][q];
+        jl2 += L.term[2][q];
+        f0 += L.term[3][q];
+        f1 += L.term[4][q];
+        f2 += L.term[5][q];
+        dmu += L.term[6][q];
+        dwa += L.term[7][q];
+        dr += L.term[8][q];
+      }
+      wave_lds_sync();
+    }
+  }
+  double pmu = 0.0, pwa = 0.0, pm = 0.0;
+  if (active) {
+    const double ri = p.w;
     // growth surface
     double dz = p.z - fp.zw, rAw = ri + fp.WZ;
     double w, wp, wr;
@@ -136,19 +226,20 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
     pwa = dtm * dwa;
     pm = fp.dt * (li.x * f0 + li.y * f1 + li.z * f2);
   }
-  sh[0][threadIdx.x] = pmu;
-  sh[1][threadIdx.x] = pwa;
-  sh[2][threadIdx.x] = pm;
+  __syncthreads();  // the partials reuse the window buffers
+  lds.sh[0][threadIdx.x] = pmu;
+  lds.sh[1][threadIdx.x] = pwa;
+  lds.sh[2][threadIdx.x] = pm;
   __syncthreads();
   for (int w = kAdjBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-      sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+      lds.sh[0][threadIdx.x] += lds.sh[0][threadIdx.x + w];
+      lds.sh[1][threadIdx.x] += lds.sh[1][threadIdx.x + w];
+      lds.sh[2][threadIdx.x] += lds.sh[2][threadIdx.x + w];
     }
     __syncthreads();
   }
-  if (threadIdx.x < 3) acc[3 * blockIdx.x + threadIdx.x] += sh[threadIdx.x][0];
+  if (threadIdx.x < 3) acc[3 * blockIdx.x + threadIdx.x] += lds.sh[threadIdx.x][0];
 }
 
 // out[oid[t]] = in[t] (sorted -> original order) and its inverse.
@@ -202,16 +293,17 @@ hipError_t launch_adj_loss(const double4 *pos, const int32_t *oid,
   return hipGetLastError();
 }
 
-hipError_t launch_adj_step(const double4 *pos, const int32_t *nbr,
-                           const int32_t *ncnt, int64_t ncap,
+hipError_t launch_adj_step(const double4 *pos, const int32_t *cidx,
+                           const uint8_t *cown, const int32_t *wtot,
+                           const int32_t *ncnt, int32_t maxn,
                            const int32_t *oid, int64_t n, BoxGeom g,
                            ForceParams fp, const double4 *lam_in,
                            double4 *lam_out, double *grad_r, double *acc,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_adj_step, dim3(blocks(n, kAdjBlock)), dim3(kAdjBlock), 0,
-                     s, pos, nbr, ncnt, ncap, oid, n, g, fp, lam_in, lam_out,
-                     grad_r, acc);
+                     s, pos, cidx, cown, wtot, ncnt, maxn, oid, n, g, fp,
+                     lam_in, lam_out, grad_r, acc);
   return hipGetLastError();
 }
 

@@ -1014,15 +1014,20 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
   const int C = ckpt_every;
   const int S = (nsteps + C - 1) / C;
   const int R = (C + K - 1) / K;  // rebuilds per segment
-  const size_t rowbuf = (size_t)c->maxn * (size_t)cap;
+  // rows of each rebuild of a segment, in the flat layout of the force pass
+  const size_t rowbuf = (size_t)flat_entries(cap, c->flat_tile, c->maxn);
+  const size_t tilebuf = (size_t)((cap + c->flat_tile - 1) / c->flat_tile + 1);
   DevBuf<double> d_target, d_gr, d_acc, d_lpart, d_out;
   DevBuf<double4> d_ckpt, d_hpos, d_lam0, d_lam1, d_lamo, d_final;
-  DevBuf<int32_t> d_hnbr, d_hcnt, d_hoid;
+  DevBuf<int32_t> d_hidx, d_htot, d_hcnt, d_hoid;
+  DevBuf<uint8_t> d_hown;
   HIPCHK(d_target.alloc(3 * (size_t)std::max<int64_t>(n, 1)));
   HIPCHK(d_gr.alloc((size_t)std::max<int64_t>(n, 1)));
   HIPCHK(d_ckpt.alloc((size_t)S * cap));
   HIPCHK(d_hpos.alloc((size_t)C * cap));
-  HIPCHK(d_hnbr.alloc((size_t)R * rowbuf));
+  HIPCHK(d_hidx.alloc((size_t)R * rowbuf));
+  HIPCHK(d_hown.alloc((size_t)R * rowbuf));
+  HIPCHK(d_htot.alloc((size_t)R * tilebuf));
   HIPCHK(d_hcnt.alloc((size_t)R * cap));
   HIPCHK(d_hoid.alloc((size_t)R * cap));
   HIPCHK(d_lam0.alloc(cap));
@@ -1096,23 +1101,34 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
                              c->pos[c->cd].p, s));
     c->have_list = false;
     for (int t = 0; t < len; t++) {
+      // Each step writes the next state straight into the history; the
+      // context's buffers carry the state only through a rebuild.
       if (t % K == 0) {
+        if (t > 0)
+          HIPCHK(hipMemcpyAsync(c->pos[c->cd].p, d_hpos.p + (size_t)t * cap,
+                                sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
         st = rebuild(c);
         if (st != BMX_OK) return st;
+        HIPCHK(hipMemcpyAsync(d_hpos.p + (size_t)t * cap, c->pos[c->cd].p,
+                              sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
         const int r = t / K;
-        HIPCHK(hipMemcpyAsync(d_hnbr.p + (size_t)r * rowbuf, c->d_nbr.p,
+        HIPCHK(hipMemcpyAsync(d_hidx.p + (size_t)r * rowbuf, c->d_cidx.p,
                               sizeof(int32_t) * rowbuf, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_hown.p + (size_t)r * rowbuf, c->d_cown.p, rowbuf,
+                              hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_htot.p + (size_t)r * tilebuf, c->d_wtot.p,
+                              sizeof(int32_t) * tilebuf, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(d_hcnt.p + (size_t)r * cap, c->d_ncnt.p,
                               sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(d_hoid.p + (size_t)r * cap, c->oid[c->cs].p,
                               sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
       }
-      HIPCHK(hipMemcpyAsync(d_hpos.p + (size_t)t * cap, c->pos[c->cd].p,
-                            sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
-      if (t + 1 < len) {  // the last state of the segment is not needed
-        st = enqueue_force_only(c, dt, -1);
-        if (st != BMX_OK) return st;
-      }
+      if (t + 1 < len)  // the last state of the segment is not needed
+        HIPCHK(launch_force_sphere_flat(
+            d_hpos.p + (size_t)t * cap, d_hpos.p + (size_t)(t + 1) * cap,
+            c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n,
+            c->maxn, nullptr, c->d_boxes.p, c->boxes[0], 0, fp, nullptr,
+            nullptr, nullptr, c->d_err.p, shape(c), s));
     }
     // lambda of the segment's last step, in that step's order
     int r = (len - 1) / K;
@@ -1120,9 +1136,11 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
     for (int t = len - 1; t >= 0; t--) {
       r = t / K;
       const int32_t *oid = d_hoid.p + (size_t)r * cap;
-      HIPCHK(launch_adj_step(d_hpos.p + (size_t)t * cap, d_hnbr.p + (size_t)r * rowbuf,
-                             d_hcnt.p + (size_t)r * cap, cap, oid, n, c->boxes[0],
-                             fp, lam_in, lam_out, d_gr.p, d_acc.p, s));
+      HIPCHK(launch_adj_step(d_hpos.p + (size_t)t * cap, d_hidx.p + (size_t)r * rowbuf,
+                             d_hown.p + (size_t)r * rowbuf,
+                             d_htot.p + (size_t)r * tilebuf,
+                             d_hcnt.p + (size_t)r * cap, c->maxn, oid, n,
+                             c->boxes[0], fp, lam_in, lam_out, d_gr.p, d_acc.p, s));
       std::swap(lam_in, lam_out);
       if (t % K == 0 && t > 0) {  // back across a rebuild: reorder lambda
         HIPCHK(launch_to_orig4(lam_in, oid, n, d_lamo.p, s));

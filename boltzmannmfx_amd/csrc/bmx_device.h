@@ -606,4 +606,36 @@ __device__ inline void rod_rotate(double &theta, double &phi,
   if (nd[1] < 0.0) phi = 2.0 * kPi - phi;
 }
 
+
+// ---------------------------------------------- wave-level helpers ---
+
+// Inclusive prefix sum over segments of 'width' lanes (a power of two up
+// to the wave size).
+__device__ __forceinline__ int wave_incl_scan(int v, int width = 64) {
+  const int lane = threadIdx.x & (width - 1);
+  for (int o = 1; o < width; o <<= 1) {
+    const int w = __shfl_up(v, o, width);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+
+// Orders the wave's LDS writes before its later LDS reads (lanes of one
+// wave only; waves of a block run different trip counts).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Interacting entries of the window before entry w0 + j (0 <= j <= 128).
+__device__ __forceinline__ int packed_before(int j, unsigned long long b0,
+                                             unsigned long long b1, int n0) {
+  if (j <= 64)
+    return __popcll(j == 64 ? b0 : (b0 & ((1ull << j) - 1ull)));
+  j -= 64;
+  return n0 + __popcll(j == 64 ? b1 : (b1 & ((1ull << j) - 1ull)));
+}
+
+
 }  // namespace bmx

@@ -144,8 +144,9 @@ int32_t adj_blocks(int64_t n);
 hipError_t launch_adj_loss(const double4 *pos, const int32_t *oid,
                            const double *target, int64_t n, BoxGeom g,
                            double4 *lam, double *lpart, hipStream_t s);
-hipError_t launch_adj_step(const double4 *pos, const int32_t *nbr,
-                           const int32_t *ncnt, int64_t ncap,
+hipError_t launch_adj_step(const double4 *pos, const int32_t *cidx,
+                           const uint8_t *cown, const int32_t *wtot,
+                           const int32_t *ncnt, int32_t maxn,
                            const int32_t *oid, int64_t n, BoxGeom g,
                            ForceParams fp, const double4 *lam_in,
                            double4 *lam_out, double *grad_r, double *acc,

@@ -488,16 +488,6 @@ __device__ __forceinline__ int wave_or3(int m) {
   return (__any(m & 1) ? 1 : 0) | (__any(m & 2) ? 2 : 0) | (__any(m & 4) ? 4 : 0);
 }
 
-// Inclusive prefix sum over segments of 'width' lanes (a power of two up
-// to the wave size).
-__device__ __forceinline__ int wave_incl_scan(int v, int width = kWave) {
-  const int lane = threadIdx.x & (width - 1);
-  for (int o = 1; o < width; o <<= 1) {
-    const int w = __shfl_up(v, o, width);
-    if (lane >= o) v += w;
-  }
-  return v;
-}
 
 template <bool MULTI, bool RODS>
 __global__ __launch_bounds__(512, 8) void k_build_nbr(
@@ -701,22 +691,7 @@ __device__ __forceinline__ void wave_vmax(double v2, double dt, double *vpart,
   if (lane == 0 && t < n) vpart[t / kWave] = dt * sqrt(m);
 }
 
-// Orders the wave's LDS writes before its later LDS reads (lanes of one
-// wave only; waves of a block run different trip counts).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
-// Interacting entries of the window before entry w0 + j (0 <= j <= 128).
-__device__ __forceinline__ int packed_before(int j, unsigned long long b0,
-                                             unsigned long long b1, int n0) {
-  if (j <= kWave)
-    return __popcll(j == kWave ? b0 : (b0 & ((1ull << j) - 1ull)));
-  j -= kWave;
-  return n0 + __popcll(j == kWave ? b1 : (b1 & ((1ull << j) - 1ull)));
-}
 
 template <bool MULTI, int M>
 __device__ __forceinline__ void flat_rows(
