@@ -200,6 +200,30 @@ def test_dense_rows_bit_exact(bmx, oracle, gpu):
     assert inter.mean() > 0.6
 
 
+@pytest.mark.parametrize("n,phi", [(3000, 0.001), (2000, 0.003), (64, 0.05), (1000, 0.3)])
+def test_max_neighbors_at_the_longest_row(bmx, oracle, gpu, n, phi):
+    """max_neighbors equal to the longest row of the run: each tile's region
+    of the flat rows is as small as it can be (down to 64 entries), so the
+    force pass's first window reads into the next tile's region and, for the
+    last tile, into the padding; results stay bit-exact."""
+    col = W.sphere_colony(n, phi, 5, W_A=2e-5)
+    probe, _ = gpu_ctx(bmx, col, K=1)
+    nb = probe.bin_dims()
+    st, _ = oracle_state(oracle, col, nb, K=1)
+    longest = 1
+    for _ in range(4):                   # the list each step uses
+        st.evolve(W.DT, 1)
+        off, _ = st.neighbors()
+        longest = max(longest, int(np.diff(off).max()))
+    ctx, _ = gpu_ctx(bmx, col, K=1, maxn=longest)
+    st, orecs = oracle_state(oracle, col, nb, K=1)
+    for _ in range(4):
+        ctx.evolve(W.DT, 1)
+        fo = st.evolve(W.DT, 1)
+        assert np.array_equal(ctx.fc(), fo)
+    assert np.array_equal(ctx.positions(), orecs["pos"])
+
+
 def test_errors_are_reported(bmx, gpu, c1):
     col = W.c1_small()
     col.recs["pos"][1] = col.recs["pos"][0]          # coincident pair
