@@ -992,8 +992,13 @@ __device__ double general_finish(int64_t t, const BoxGeom &g,
 // the dltd reset do not depend on the order of the pairs.
 constexpr int kGTile = 32;
 constexpr int kGWin = 2 * kWave;
-constexpr size_t kGenWaveBytes =
-    6 * kGWin * sizeof(double) + 3 * kGWin * sizeof(int32_t);
+constexpr int kGPass = kWave;  // packed pairs per pass
+// LDS per wave: the terms of one pass (6 x 64), the tile's mobility
+// matrices (18 x 32, formed once per segment) and the window's packed
+// pairs (neighbor, owner, promotion flag: 3 x 128)
+constexpr size_t kGenWaveBytes = 6 * kGPass * sizeof(double) +
+                                 18 * kGTile * sizeof(double) +
+                                 3 * kGWin * sizeof(int32_t);
 
 template <bool MULTI>
 __device__ __forceinline__ bool general_test(
@@ -1032,7 +1037,8 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave, nwv = blockDim.x / kWave;
   double *res = reinterpret_cast<double *>(gen_lds + wv * kGenWaveBytes);
-  int32_t *pu = reinterpret_cast<int32_t *>(res + 6 * kGWin);
+  double *mob = res + 6 * kGPass;  // mob[m * kGTile + owner]: dii (m < 9), dij
+  int32_t *pu = reinterpret_cast<int32_t *>(mob + 18 * kGTile);
   int32_t *po = pu + kGWin;
   int32_t *pf = po + kGWin;
   const int64_t tile = tile_of(remap) * nwv + wv;
@@ -1047,6 +1053,20 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
   }
   const int c = active ? row_len(ncnt[t]) : 0;
   const int P = wave_incl_scan(c) - c;
+  if (active && st.type[t] == rec::FUNGI && c > 0) {
+    // the two-site mobility of the segment, once (the pairs read it)
+    RodSelf S;
+    rod_self_geom(S, in.rod[t], in.trig[t], in.pos[t].w, st.bscale[t]);
+    rod_self_mob(S, in.trig[t], fp.visc);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        mob[(3 * a + b) * kGTile + lane] = S.dii[a][b];
+        mob[(9 + 3 * a + b) * kGTile + lane] = S.dij[a][b];
+      }
+  }
+  wave_lds_sync();
   const unsigned long long below = (1ull << lane) - 1ull;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   bool paired = false, promoted = false, repair = false;
@@ -1077,52 +1097,62 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
       po[k] = o1;
     }
     wave_lds_sync();
-    for (int k = lane; k < ni; k += kWave) {  // pair laws, packed
-      const int64_t u = pu[k], ti_ = t0 + po[k];
-      const BoxGeom &g = MULTI ? boxes[st.col[ti_]] : g0;
-      const double4 p = in.pos[ti_], o = in.pos[u];
-      double d[3];
-      d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
-      d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
-      d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
-      double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      int flag = 0;
-      if (st.type[ti_] == rec::YEAST) {
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        sphere_pair(d[0], d[1], d[2], r2, p.w + o.w, fp, f[0], f[1], f[2]);
-      } else {
-        // site forces first, the mobility after: its matrices are not live
-        // during the closest-point computation
-        RodSelf S;
-        rod_self_geom(S, in.rod[ti_], in.trig[ti_], p.w, st.bscale[ti_]);
-        Topo tio = in.topo[ti_];
-        const int pos0 = tio.position, nb0 = tio.n_bnds;
-        double f1[3], f2[3], ri[3];
-        {
-          const Topo tj = in.topo[u];
-          rod_pair_forces(d, S, tio, tj, in.rod[u].x, o.w, st.bscale[u],
-                          in.trig[u], fp, f1, f2, ri);
-        }
-        if (tio.n_bnds != nb0) repair = true;
-        if (tio.position != pos0) flag = 1;
-        rod_self_mob(S, in.trig[ti_], fp.visc);
-        rod_pair_mobility(S, f1, f2, ri, f);
-      }
-#pragma unroll
-      for (int q = 0; q < 6; q++) res[q * kGWin + k] = f[q];
-      pf[k] = flag;
-    }
-    wave_lds_sync();
     const int lo = P > w0 ? P : w0;
     const int hi = P + c < w0 + kGWin ? P + c : w0 + kGWin;
-    if (lo < hi) {
-      const int k1 = packed_before(hi - w0, b0, b1, n0);
-      for (int k = packed_before(lo - w0, b0, b1, n0); k < k1; k++) {
+    const int klo = lo < hi ? packed_before(lo - w0, b0, b1, n0) : 0;
+    const int khi = lo < hi ? packed_before(hi - w0, b0, b1, n0) : 0;
+    for (int k0 = 0; k0 < ni; k0 += kGPass) {  // pair laws, packed
+      const int k = k0 + lane;
+      if (k < ni) {
+        const int64_t u = pu[k], ti_ = t0 + po[k];
+        const BoxGeom &g = MULTI ? boxes[st.col[ti_]] : g0;
+        const double4 p = in.pos[ti_], o = in.pos[u];
+        double d[3];
+        d[0] = sep_axis(p.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
+        d[1] = sep_axis(p.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
+        d[2] = sep_axis(p.z, o.z, g.L[2], g.halfL[2], g.periodic[2]);
+        double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int flag = 0;
+        if (st.type[ti_] == rec::YEAST) {
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          sphere_pair(d[0], d[1], d[2], r2, p.w + o.w, fp, f[0], f[1], f[2]);
+        } else {
+          RodSelf S;
+          rod_self_geom(S, in.rod[ti_], in.trig[ti_], p.w, st.bscale[ti_]);
+          Topo tio = in.topo[ti_];
+          const int pos0 = tio.position, nb0 = tio.n_bnds;
+          double f1[3], f2[3], ri[3];
+          {
+            const Topo tj = in.topo[u];
+            rod_pair_forces(d, S, tio, tj, in.rod[u].x, o.w, st.bscale[u],
+                            in.trig[u], fp, f1, f2, ri);
+          }
+          if (tio.n_bnds != nb0) repair = true;
+          if (tio.position != pos0) flag = 1;
+          const int ow = po[k];
 #pragma unroll
-        for (int q = 0; q < 6; q++) acc[q] += res[q * kGWin + k];
-        promoted = promoted || pf[k] != 0;
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+              S.dii[a][b] = mob[(3 * a + b) * kGTile + ow];
+              S.dij[a][b] = mob[(9 + 3 * a + b) * kGTile + ow];
+            }
+          rod_pair_mobility(S, f1, f2, ri, f);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) res[q * kGPass + lane] = f[q];
+        pf[k] = flag;
+      }
+      wave_lds_sync();
+      const int a0 = klo > k0 ? klo : k0;
+      const int a1 = khi < k0 + kGPass ? khi : k0 + kGPass;
+      for (int k2 = a0; k2 < a1; k2++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += res[q * kGPass + (k2 - k0)];
+        promoted = promoted || pf[k2] != 0;
         paired = true;
       }
+      wave_lds_sync();
     }
     wave_lds_sync();
   }
