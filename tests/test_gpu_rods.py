@@ -151,6 +151,36 @@ def test_bond_repair_and_promotion(bmx, oracle, gpu):
     assert (gi[sec, 19] == 3).sum() > 10                             # promoted
 
 
+def test_segment_colonies_match_separate_runs(bmx, gpu):
+    """Three hyphae colonies bound as one context (one box each) step
+    exactly as three separate contexts: the flat segment pass with one box
+    per particle gives the single-box results bit for bit."""
+    cols = [small_hyphae(n=1500, nroots=15, seed=s) for s in (3, 4, 5)]
+    recs, starts, geoms = W.concat_colonies(cols)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(cols[0].fpar)
+    ctx.set_neighbor_params(mode=0, rebuild_every=10, skin=cols[0].skin,
+                            max_neighbors=256)
+    ctx.bind_colonies(recs, starts, geoms)
+    ctx.init_bonds()
+    ctx.evolve(cols[0].dt, 20)
+    pos, ori, ints = ctx.positions(), ctx.orientations(), ctx.int_fields()
+    for k, col in enumerate(cols):
+        one = bmx.Context(0)
+        one.set_force_params(col.fpar)
+        one.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
+        one.set_neighbor_params(mode=0, rebuild_every=10, skin=col.skin,
+                                max_neighbors=256)
+        one.bind(col.recs.copy())
+        one.init_bonds()
+        one.evolve(col.dt, 20)
+        sl = slice(starts[k], starts[k + 1])
+        assert np.array_equal(pos[sl], one.positions()), k
+        assert np.array_equal(ori[sl], one.orientations()), k
+        assert np.array_equal(ints[sl], one.int_fields()), k
+        assert (ints[sl][:, 6] > 0).any()
+
+
 @pytest.mark.parametrize("fname", ["fungi_init_cfg.dat", "wall_test_fungi_init_cfg.dat"])
 def test_reference_single_segment_decks(bmx, oracle, gpu, fname):
     """The reference's own fungi and wall_test inputs: one segment settling on
