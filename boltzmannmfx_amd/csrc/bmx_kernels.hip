@@ -266,6 +266,7 @@ __global__ void k_bin_count(const double4 *__restrict__ pos,
 // Exclusive scan: 256 threads x 8 items per block, wave prefix by shuffles.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
+static_assert(kScanItems == 8, "k_scan_local moves a thread's counts as two int4");
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 template <typename T>
@@ -296,23 +297,45 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(
     int32_t *__restrict__ bsum, int64_t n) {
   __shared__ int32_t wsum[kScanThreads / kWave];
   int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  // a thread's 8 counts are 32 aligned bytes (the arrays are allocation
+  // bases): two 16-byte loads and stores, element by element at the tail
+  const bool whole = base + kScanItems <= n;
+  int32_t x[kScanItems];
+  if (whole) {
+    int4 *vin = reinterpret_cast<int4 *>(in + base);
+    const int4 a = vin[0], b = vin[1];
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+    x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    vin[0] = make_int4(0, 0, 0, 0);
+    vin[1] = make_int4(0, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+      x[i] = 0;
+      if (base + i < n) {
+        x[i] = in[base + i];
+        in[base + i] = 0;
+      }
+    }
+  }
   int32_t v[kScanItems];
   int32_t s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    int32_t x = 0;
-    if (base + i < n) {
-      x = in[base + i];
-      in[base + i] = 0;
-    }
     v[i] = s;
-    s += x;
+    s += x[i];
   }
   int32_t total;
   int32_t ex = block_excl_scan(s, wsum, total);
+  if (whole) {
+    int4 *vout = reinterpret_cast<int4 *>(out + base);
+    vout[0] = make_int4(v[0] + ex, v[1] + ex, v[2] + ex, v[3] + ex);
+    vout[1] = make_int4(v[4] + ex, v[5] + ex, v[6] + ex, v[7] + ex);
+  } else {
 #pragma unroll
-  for (int i = 0; i < kScanItems; i++)
-    if (base + i < n) out[base + i] = v[i] + ex;
+    for (int i = 0; i < kScanItems; i++)
+      if (base + i < n) out[base + i] = v[i] + ex;
+  }
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
