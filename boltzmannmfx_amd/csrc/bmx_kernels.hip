@@ -364,10 +364,19 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int32_t *__restrict__ bsum,
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(
     int32_t *__restrict__ out, const int32_t *__restrict__ bsum, int64_t n) {
-  int32_t add = bsum[blockIdx.x];
-  int64_t base = (int64_t)blockIdx.x * kScanTile;
-  for (int i = threadIdx.x; i < kScanTile; i += kScanThreads)
-    if (base + i < n) out[base + i] += add;
+  const int32_t add = bsum[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  if (base + kScanItems <= n) {  // 32 aligned bytes, as in k_scan_local
+    int4 *v = reinterpret_cast<int4 *>(out + base);
+    int4 a = v[0], b = v[1];
+    a.x += add; a.y += add; a.z += add; a.w += add;
+    b.x += add; b.y += add; b.z += add; b.w += add;
+    v[0] = a;
+    v[1] = b;
+  } else {
+    for (int i = 0; i < kScanItems; i++)
+      if (base + i < n) out[base + i] += add;
+  }
 }
 
 __global__ void k_bin_scatter(const int32_t *__restrict__ key,
