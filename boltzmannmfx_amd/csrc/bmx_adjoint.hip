@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
   const double4 p = active ? pos[t] : zero;
   const double4 li = active ? lam_in[t] : zero;
   const int c = active ? (ncnt[t] & ((1 << 28) - 1)) : 0;  // bits 28..30: image axes
-  const int P = wave_incl_scan(c) - c;
+  const int P = wave_incl_scan64(c) - c;
   L.p[lane] = p;
   L.l[lane] = li;
   wave_lds_sync();

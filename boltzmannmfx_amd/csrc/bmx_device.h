@@ -620,6 +620,53 @@ __device__ __forceinline__ int wave_incl_scan(int v, int width = 64) {
   return v;
 }
 
+// DPP lane moves of GFX9 (rows of 16 lanes): row_shr:n reads lane i - n of
+// the same row, row_bcast:15 gives rows 1 and 3 (row mask 0xa) the last lane
+// of the row below, row_bcast:31 gives rows 2 and 3 (row mask 0xc) lane 31.
+// A lane whose source is outside its row, or whose row is masked off,
+// receives 'old'.  These move registers directly, where a shuffle is a
+// round trip through the LDS crossbar.
+constexpr int kDppRowShr = 0x110, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+// Inclusive prefix sum over the 64 lanes of a wave (every lane active).
+__device__ __forceinline__ int wave_incl_scan64(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr + 1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr + 2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr + 4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr + 8, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppBcast15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppBcast31, 0xc, 0xf, false);
+  return v;
+}
+
+// A double moved by one DPP pattern, as its two 32-bit halves; lanes
+// without a source keep their own value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const unsigned l2 = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+  const unsigned h2 = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
+}
+
+// Largest value over the 64 lanes of a wave (every lane active), in every
+// lane: running maxima within the rows, carried to lane 63 by the two
+// broadcasts, then read from lane 63.
+__device__ __forceinline__ double wave_max_f64(double v) {
+  double w;
+  w = dpp_f64<kDppRowShr + 1, 0xf>(v); v = w > v ? w : v;
+  w = dpp_f64<kDppRowShr + 2, 0xf>(v); v = w > v ? w : v;
+  w = dpp_f64<kDppRowShr + 4, 0xf>(v); v = w > v ? w : v;
+  w = dpp_f64<kDppRowShr + 8, 0xf>(v); v = w > v ? w : v;
+  w = dpp_f64<kDppBcast15, 0xa>(v); v = w > v ? w : v;
+  w = dpp_f64<kDppBcast31, 0xc>(v); v = w > v ? w : v;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Orders the wave's LDS writes before its later LDS reads (lanes of one
 // wave only; waves of a block run different trip counts).
 __device__ __forceinline__ void wave_lds_sync() {

@@ -51,14 +51,6 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
 }
-__device__ __forceinline__ double wave_max_f64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double w = __shfl_xor(v, o, kWave);
-    v = w > v ? w : v;
-  }
-  return v;
-}
 
 // Block-wide max / sum through LDS (sh: one slot per wave); every thread
 // must call it; the result is valid in thread 0.
@@ -850,7 +842,7 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   const int32_t raw = active ? ncnt[t] : 0;
   const int wave_img = wave_or3(row_img(raw));
   const int c = row_len(raw);
-  const int P = wave_incl_scan(c) - c;
+  const int P = wave_incl_scan64(c) - c;
   const double4 p = active ? pos_in[t] : make_double4(0.0, 0.0, 0.0, 0.0);
   sp[lane] = p;
   if (MULTI) scol[lane] = active ? col[t] : 0;
@@ -1084,7 +1076,7 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
     T = wtot[tile];
   }
   const int c = active ? row_len(ncnt[t]) : 0;
-  const int P = wave_incl_scan(c) - c;
+  const int P = wave_incl_scan64(c) - c;
   if (active && st.type[t] == rec::FUNGI && c > 0) {
     // the two-site mobility of the segment, once (the pairs read it)
     RodSelf S;
