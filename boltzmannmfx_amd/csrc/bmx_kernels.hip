@@ -22,7 +22,6 @@ namespace bmx {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kUnroll = 1;  // candidates in flight per thread (list build)
 
 __device__ __forceinline__ unsigned long long dbits(double x) {
   return (unsigned long long)__double_as_longlong(x);
@@ -465,13 +464,11 @@ __device__ __forceinline__ void scan_range(
     const int32_t *__restrict__ type, const double4 *__restrict__ rod,
     int mode, double crit, double was, int32_t *&out, int64_t ncap,
     int32_t maxn, int &cnt) {
-  for (int u0 = ua; u0 < ub; u0 += kUnroll) {
-    double4 o[kUnroll];
-#pragma unroll
-    for (int j = 0; j < kUnroll; j++) o[j] = pos[min(u0 + j, ub - 1)];
-#pragma unroll
-    for (int j = 0; j < kUnroll; j++) {
-      const int u = u0 + j;
+  for (int u0 = ua; u0 < ub; u0++) {
+    double4 o[1];
+    o[0] = pos[u0];
+    {
+      const int j = 0, u = u0;
       const double d0 = sep_m<M, 0>(p.x, o[j].x, g);
       const double d1 = sep_m<M, 1>(p.y, o[j].y, g);
       const double d2 = sep_m<M, 2>(p.z, o[j].z, g);
@@ -494,6 +491,101 @@ __device__ __forceinline__ void scan_range(
         cnt++;
       }
     }
+  }
+}
+
+
+// Candidate range q of a particle's bin stencil, in the canonical order
+// (oz, then oy, then ox): with nx = 1 (the particle's bin at least one bin
+// from both x faces) one range of three x bins per (oz, oy) row, with nx = 3
+// the x bins one at a time (periodic images, or clipped at the faces).  A
+// bin outside a non-periodic box gives an empty range.
+__device__ __forceinline__ int2 stencil_range(int q, int nx, int bx, int by,
+                                              int bz, const BoxGeom &g,
+                                              const int32_t *__restrict__ start) {
+  const int r = nx == 1 ? q : q / 3;
+  const int oz = r / 3 - 1, oy = r % 3 - 1;
+  int cz = bz + oz, cy = by + oy;
+  const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
+  bool ok = true;
+  if (g.periodic[2]) cz = (cz + nbz) % nbz;
+  else ok = cz >= 0 && cz < nbz;
+  if (g.periodic[1]) cy = (cy + nby) % nby;
+  else ok = ok && cy >= 0 && cy < nby;
+  int lo = bx - 1, hi = bx + 1;
+  if (nx != 1) {
+    int cx = bx + (q - 3 * r) - 1;
+    if (g.periodic[0]) cx = (cx + nbx) % nbx;
+    else ok = ok && cx >= 0 && cx < nbx;
+    lo = hi = cx;
+  }
+  if (!ok) return make_int2(0, 0);
+  const int row = g.bin_offset + (cz * nby + cy) * nbx;
+  return make_int2(start[row + lo], start[row + hi + 1]);
+}
+
+// Scans a particle's stencil as one sequence of candidates and appends the
+// accepted ones to its row (out walks the column-major row, one stride of
+// ncap per entry).  Each lane moves on to its next range as soon as its
+// current one ends, so a wave takes as many iterations as its longest
+// stencil rather than the sum over ranges of the longest range; the next
+// range's bounds are loaded one range ahead.  Candidates visited are capped
+// at scan_cap per particle (a range that would pass it is dropped and the
+// row reported as overflowing).  Bit a of M asks for the periodic image
+// test along axis a; an axis left out is one along which no candidate of
+// the wave can need an image.  was = W_A + skin.  Sphere builds: a
+// candidate's reach is its radius.
+template <int M>
+__device__ __forceinline__ void scan_stencil(
+    int nx, int bx, int by, int bz, const BoxGeom &g,
+    const int32_t *__restrict__ start, int64_t scan_cap, int t,
+    const double4 &p, const double4 *__restrict__ pos, int mode, double crit,
+    double was, int32_t *&out, int64_t ncap, int32_t maxn, int &cnt) {
+  const int nq = 9 * nx;
+  int64_t scanned = 0;
+  int q = 0;
+  const int2 first = stencil_range(0, nx, bx, by, bz, g, start);
+  int2 next = stencil_range(1, nx, bx, by, bz, g, start);
+  int u = first.x, e = first.y;
+  scanned = e - u;
+  if (scanned > scan_cap) {
+    cnt = maxn + 1;
+    e = u;
+  }
+  for (;;) {
+    while (u >= e && q + 1 < nq) {
+      q++;
+      u = next.x;
+      e = next.y;
+      const int64_t len = e - u;
+      if (scanned + len > scan_cap) {
+        cnt = maxn + 1;  // reported as overflow
+        e = u;
+      }
+      scanned += len;
+      if (q + 1 < nq) next = stencil_range(q + 1, nx, bx, by, bz, g, start);
+    }
+    if (u >= e) break;
+    const double4 o = pos[u];
+    const double d0 = sep_m<M, 0>(p.x, o.x, g);
+    const double d1 = sep_m<M, 1>(p.y, o.y, g);
+    const double d2 = sep_m<M, 2>(p.z, o.z, g);
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    bool ok;
+    if (mode == 1) {
+      ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
+    } else {
+      const double cut = p.w + o.w + was;
+      ok = r2 <= cut * cut;
+    }
+    if (ok && u != t) {
+      if (cnt < maxn) {
+        *out = u;
+        out += ncap;
+      }
+      cnt++;
+    }
+    u++;
   }
 }
 
@@ -532,7 +624,6 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   // bins would otherwise make this quadratic); hitting the cap is reported
   // as an overflow.  After a non-finite force no rows are built.
   const int64_t scan_cap = 64 * (int64_t)maxn + 4096;
-  int64_t scanned = 0;
   bool active = t < n && !(*err & ERRBIT_NAN);
   const BoxGeom &g = MULTI ? boxes[active ? col[t] : 0] : g0;
   const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
@@ -560,62 +651,78 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   }
   const int wave_img = wave_or3(scan_img);
   if (active) {
-    for (int oz = -1; oz <= 1; oz++) {
-      int cz = bz + oz;
-      if (g.periodic[2]) {
-        cz = (cz + nbz) % nbz;
-      } else if (cz < 0 || cz >= nbz) {
-        continue;
-      }
-      for (int oy = -1; oy <= 1; oy++) {
-        int cy = by + oy;
-        if (g.periodic[1]) {
-          cy = (cy + nby) % nby;
-        } else if (cy < 0 || cy >= nby) {
+    if (RODS) {
+      // Segment builds (sparse mesh-cell bins): range by range, all lanes
+      // together, which measured faster there than the flattened scan.
+      int64_t scanned = 0;
+      for (int oz = -1; oz <= 1; oz++) {
+        int cz = bz + oz;
+        if (g.periodic[2]) {
+          cz = (cz + nbz) % nbz;
+        } else if (cz < 0 || cz >= nbz) {
           continue;
         }
-        int row = g.bin_offset + (cz * nby + cy) * nbx;
-        // up to three x-ranges; one contiguous range when no x wrap
-        int ranges[3][2];
-        int nr = 0;
-        if (bx >= 1 && bx + 1 < nbx) {
-          ranges[0][0] = start[row + bx - 1];
-          ranges[0][1] = start[row + bx + 2];
-          nr = 1;
-        } else {
-          for (int ox = -1; ox <= 1; ox++) {
-            int cx = bx + ox;
-            if (g.periodic[0]) {
-              cx = (cx + nbx) % nbx;
-            } else if (cx < 0 || cx >= nbx) {
-              continue;
+        for (int oy = -1; oy <= 1; oy++) {
+          int cy = by + oy;
+          if (g.periodic[1]) {
+            cy = (cy + nby) % nby;
+          } else if (cy < 0 || cy >= nby) {
+            continue;
+          }
+          int row = g.bin_offset + (cz * nby + cy) * nbx;
+          // up to three x-ranges; one contiguous range when no x wrap
+          int ranges[3][2];
+          int nr = 0;
+          if (bx >= 1 && bx + 1 < nbx) {
+            ranges[0][0] = start[row + bx - 1];
+            ranges[0][1] = start[row + bx + 2];
+            nr = 1;
+          } else {
+            for (int ox = -1; ox <= 1; ox++) {
+              int cx = bx + ox;
+              if (g.periodic[0]) {
+                cx = (cx + nbx) % nbx;
+              } else if (cx < 0 || cx >= nbx) {
+                continue;
+              }
+              ranges[nr][0] = start[row + cx];
+              ranges[nr][1] = start[row + cx + 1];
+              nr++;
             }
-            ranges[nr][0] = start[row + cx];
-            ranges[nr][1] = start[row + cx + 1];
-            nr++;
           }
-        }
-        for (int q = 0; q < nr; q++) {
-          int64_t len = ranges[q][1] - ranges[q][0];
-          if (scanned + len > scan_cap) {
-            cnt = maxn + 1;  // reported as overflow below
-            ranges[q][1] = ranges[q][0];
+          for (int q = 0; q < nr; q++) {
+            int64_t len = ranges[q][1] - ranges[q][0];
+            if (scanned + len > scan_cap) {
+              cnt = maxn + 1;  // reported as overflow below
+              ranges[q][1] = ranges[q][0];
+            }
+            scanned += len;
+            if (wave_img == 0)
+              scan_range<RODS, 0>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                  pos, type, rod, mode, crit, was, out, ncap,
+                                  maxn, cnt);
+            else if (wave_img == 1)
+              scan_range<RODS, 1>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                  pos, type, rod, mode, crit, was, out, ncap,
+                                  maxn, cnt);
+            else
+              scan_range<RODS, 7>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                  pos, type, rod, mode, crit, was, out, ncap,
+                                  maxn, cnt);
           }
-          scanned += len;
-          if (wave_img == 0)
-            scan_range<RODS, 0>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                pos, type, rod, mode, crit, was, out, ncap,
-                                maxn, cnt);
-          else if (wave_img == 1)
-            scan_range<RODS, 1>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                pos, type, rod, mode, crit, was, out, ncap,
-                                maxn, cnt);
-          else
-            scan_range<RODS, 7>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                pos, type, rod, mode, crit, was, out, ncap,
-                                maxn, cnt);
         }
       }
+    } else {
+      const int nx = (bx >= 1 && bx + 1 < nbx) ? 1 : 3;
+      if (wave_img == 0)
+        scan_stencil<0>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                        crit, was, out, ncap, maxn, cnt);
+      else if (wave_img == 1)
+        scan_stencil<1>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                        crit, was, out, ncap, maxn, cnt);
+      else
+        scan_stencil<7>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                        crit, was, out, ncap, maxn, cnt);
     }
     ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
     if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
