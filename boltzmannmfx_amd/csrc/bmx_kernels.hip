@@ -330,32 +330,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_local(
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Single block scans the per-tile sums in place and writes the grand total
-// to out_total.
-__global__ __launch_bounds__(1024) void k_scan_tiles(int32_t *__restrict__ bsum,
-                                                     int64_t nt,
-                                                     int32_t *__restrict__ out_total) {
-  __shared__ int32_t wsum[1024 / kWave];
-  int64_t per = (nt + blockDim.x - 1) / blockDim.x;
-  int64_t base = threadIdx.x * per;
-  int32_t s = 0;
-  for (int64_t i = 0; i < per; i++)
-    if (base + i < nt) s += bsum[base + i];
-  int32_t total;
-  int32_t ex = block_excl_scan(s, wsum, total);
-  int32_t run = ex;
-  for (int64_t i = 0; i < per; i++)
-    if (base + i < nt) {
-      int32_t x = bsum[base + i];
-      bsum[base + i] = run;
-      run += x;
-    }
-  if (threadIdx.x == 0) *out_total = total;
-}
-
+// Adds each tile's offset, the sum of the tile sums before it, which every
+// block forms itself from the (L2-resident) tile sums, so no separate pass
+// scans them; the last block also writes the grand total to out[n].
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(
-    int32_t *__restrict__ out, const int32_t *__restrict__ bsum, int64_t n) {
-  const int32_t add = bsum[blockIdx.x];
+    int32_t *__restrict__ out, const int32_t *__restrict__ bsum, int64_t nt,
+    int64_t n) {
+  __shared__ int32_t wsum[kScanThreads / kWave];
+  int32_t part = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += kScanThreads)
+    part += bsum[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = part;
+  __syncthreads();
+  int32_t add = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / kWave; w++) add += wsum[w];
+  if (threadIdx.x == 0 && (int64_t)blockIdx.x == nt - 1) out[n] = add + bsum[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   if (base + kScanItems <= n) {  // 32 aligned bytes, as in k_scan_local
     int4 *v = reinterpret_cast<int4 *>(out + base);
@@ -1551,10 +1543,8 @@ hipError_t launch_scan(int32_t *count, int32_t *start, int64_t nb,
   if (nt > 1024 * 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nt), dim3(kScanThreads), 0,
                      s, count, start, scratch, nb);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, scratch, nt,
-                     start + nb);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(kScanThreads), 0, s,
-                     start, scratch, nb);
+                     start, scratch, nt, nb);
   return hipGetLastError();
 }
 
