@@ -818,24 +818,15 @@ __device__ __forceinline__ void wave_vmax(double v2, double dt, double *vpart,
 
 template <bool MULTI, int M>
 __device__ __forceinline__ void flat_rows(
-    int T, int P, int c, int64_t B, int lane,
+    int T, int P, int c, int64_t B, int lane, int f0u, int f1u, int f0o, int f1o,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
     const double4 *__restrict__ pos_in, const double4 *sp, const int32_t *scol,
     double *q0s, double *q1s, double *q2s, double *qrs,
     const BoxGeom *__restrict__ boxes, const BoxGeom &g0, const ForceParams &fp,
     double &a0, double &a1, double &a2) {
   const unsigned long long below = (1ull << lane) - 1ull;
-  // One box: the first window is loaded before the entry count is known
-  // (a tile's region, and the padding after the last one, is always
-  // readable; entries past the count are masked below).  Several boxes: the
-  // extra live registers would spill, so every window waits for the count.
-  int f0u = 0, f1u = 0, f0o = 0, f1o = 0;
-  if (!MULTI) {
-    f0u = cidx[B + lane];
-    f1u = cidx[B + kWave + lane];
-    f0o = cown[B + lane];
-    f1o = cown[B + kWave + lane];
-  }
+  // f0u, f1u, f0o, f1o: with one box, the first window's entries, loaded
+  // by the caller (entries past the count are masked below)
   for (int w0 = 0; w0 < T; w0 += kFlatWin) {
     const int e0 = w0 + lane, e1 = e0 + kWave;
     const bool v0 = e0 < T, v1 = e1 < T;
@@ -929,33 +920,49 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
                   wv * kWave;
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const bool active = t < n;
-  // The wave's entries start at its region (64 * maxn entries per wave);
-  // reading their count up front lets the entry loads start without waiting
-  // for the row lengths.
-  int64_t B = 0;
-  int T = 0;
-  if (t - lane < n) {  // the wave holds at least one particle
-    B = (t / kWave) * (int64_t)kWave * maxn;
-    T = wtot[t / kWave];
+  // The wave's entries start at its region (64 * maxn entries per wave).
+  // Every load that does not depend on another is issued here, before
+  // anything waits, from indexes valid for every lane (the last particle
+  // for lanes past n, tile 0 for a wave past n): the row length, the wave's
+  // entry count, the particle and, with one box, the first window of
+  // entries (a tile's region, and the padding after the last one, is
+  // always readable).  With several boxes the extra live registers would
+  // spill, so the windows wait for the count.
+  const bool live = t - lane < n;  // the wave holds at least one particle
+  const int64_t B = live ? (t / kWave) * (int64_t)kWave * maxn : 0;
+  const int64_t tc = active ? t : n - 1;
+  const int32_t raw_l = ncnt[tc];
+  const int32_t T_l = wtot[live ? t / kWave : 0];
+  const double4 p_l = pos_in[tc];
+  int f0u = 0, f1u = 0, f0o = 0, f1o = 0;
+  if (!MULTI) {
+    f0u = cidx[B + lane];
+    f1u = cidx[B + kWave + lane];
+    f0o = cown[B + lane];
+    f1o = cown[B + kWave + lane];
   }
-  const int32_t raw = active ? ncnt[t] : 0;
+  const int T = live ? T_l : 0;
+  const int32_t raw = active ? raw_l : 0;
   const int wave_img = wave_or3(row_img(raw));
   const int c = row_len(raw);
   const int P = wave_incl_scan64(c) - c;
-  const double4 p = active ? pos_in[t] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const double4 p = active ? p_l : make_double4(0.0, 0.0, 0.0, 0.0);
   sp[lane] = p;
   if (MULTI) scol[lane] = active ? col[t] : 0;
   wave_lds_sync();
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   if (wave_img == 0)
-    flat_rows<MULTI, 0>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
-                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 0>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                        a0, a1, a2);
   else if (wave_img == 1)
-    flat_rows<MULTI, 1>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
-                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 1>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                        a0, a1, a2);
   else
-    flat_rows<MULTI, 7>(T, P, c, B, lane, cidx, cown, pos_in, sp, scol, q0s,
-                        q1s, q2s, qrs, boxes, g0, fp, a0, a1, a2);
+    flat_rows<MULTI, 7>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                        a0, a1, a2);
   double v2 = 0.0;
   if (active) {
     const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
