@@ -96,16 +96,33 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
   AdjWave &L = lds.w[threadIdx.x / 64];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = t < n;
-  int64_t B = 0;
-  int T = 0;
-  if (t - lane < n) {  // the wave's tile holds at least one particle
-    B = (t / 64) * (int64_t)64 * maxn;
-    T = wtot[t / 64];
+  // Every load that does not depend on another is issued here, before
+  // anything waits, from indexes valid for every lane (the last particle
+  // for lanes past n, tile 0 for a wave past n): the row length, the wave's
+  // entry count, the particle, its adjoint and the first window of entries
+  // (a tile's region, and the padding after the last one, is always
+  // readable; entries past the count are masked below).
+  const bool live = t - lane < n;  // the wave's tile holds at least one particle
+  const int64_t B = live ? (t / 64) * (int64_t)64 * maxn : 0;
+  const int64_t tc = active ? t : n - 1;
+  const int32_t raw = ncnt[tc];
+  const int32_t T_l = wtot[live ? t / 64 : 0];
+  const double4 p_l = pos[tc];
+  const double4 li_l = lam_in[tc];
+  int fu[2], fo[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    fu[h] = cidx[B + h * 64 + lane];
+    fo[h] = cown[B + h * 64 + lane];
   }
-  const double4 zero = make_double4(0.0, 0.0, 0.0, 0.0);
-  const double4 p = active ? pos[t] : zero;
-  const double4 li = active ? lam_in[t] : zero;
-  const int c = active ? (ncnt[t] & ((1 << 28) - 1)) : 0;  // bits 28..30: image axes
+  const int T = live ? T_l : 0;
+  // component by component: a select of whole vectors became a select of
+  // stack addresses
+  const double4 p = make_double4(active ? p_l.x : 0.0, active ? p_l.y : 0.0,
+                                 active ? p_l.z : 0.0, active ? p_l.w : 0.0);
+  const double4 li = make_double4(active ? li_l.x : 0.0, active ? li_l.y : 0.0,
+                                  active ? li_l.z : 0.0, active ? li_l.w : 0.0);
+  const int c = active ? (raw & ((1 << 28) - 1)) : 0;  // bits 28..30: image axes
   const int P = wave_incl_scan64(c) - c;
   L.p[lane] = p;
   L.l[lane] = li;
@@ -114,15 +131,23 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_step(
   double jl0 = 0.0, jl1 = 0.0, jl2 = 0.0;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   double dmu = 0.0, dwa = 0.0, dr = 0.0;
-  for (int w0 = 0; w0 < T; w0 += kAdjWin) {
+  // The first window is entered even when T is 0 (then nothing passes the
+  // test and nothing is added): with no branch between them and the loads
+  // above, the compiler keeps those loads at the top.
+  for (int w0 = 0; w0 == 0 || w0 < T; w0 += kAdjWin) {
     bool in[2];
     int uu[2], oo[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int e = w0 + h * 64 + lane;
       const bool v = e < T;
-      uu[h] = v ? cidx[B + e] : 0;
-      oo[h] = v ? cown[B + e] : 0;
+      int ue = fu[h], oe = fo[h];
+      if (w0 > 0 && v) {
+        ue = cidx[B + e];
+        oe = cown[B + e];
+      }
+      uu[h] = v ? ue : 0;
+      oo[h] = v ? oe : 0;
       const double4 pi = L.p[oo[h]], o = pos[uu[h]];
       double d0 = sep_axis(pi.x, o.x, g.L[0], g.halfL[0], g.periodic[0]);
       double d1 = sep_axis(pi.y, o.y, g.L[1], g.halfL[1], g.periodic[1]);
