@@ -924,23 +924,19 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   // Every load that does not depend on another is issued here, before
   // anything waits, from indexes valid for every lane (the last particle
   // for lanes past n, tile 0 for a wave past n): the row length, the wave's
-  // entry count, the particle and, with one box, the first window of
-  // entries (a tile's region, and the padding after the last one, is
-  // always readable).  With several boxes the extra live registers would
-  // spill, so the windows wait for the count.
+  // entry count, the particle, its box and the first window of entries (a
+  // tile's region, and the padding after the last one, is always
+  // readable).  The one-box path takes the first window from these; the
+  // per-entry-box path, which would spill with them live, reloads it.
   const bool live = t - lane < n;  // the wave holds at least one particle
   const int64_t B = live ? (t / kWave) * (int64_t)kWave * maxn : 0;
   const int64_t tc = active ? t : n - 1;
   const int32_t raw_l = ncnt[tc];
   const int32_t T_l = wtot[live ? t / kWave : 0];
   const double4 p_l = pos_in[tc];
-  int f0u = 0, f1u = 0, f0o = 0, f1o = 0;
-  if (!MULTI) {
-    f0u = cidx[B + lane];
-    f1u = cidx[B + kWave + lane];
-    f0o = cown[B + lane];
-    f1o = cown[B + kWave + lane];
-  }
+  const int32_t col_l = MULTI ? col[tc] : 0;
+  const int f0u = cidx[B + lane], f1u = cidx[B + kWave + lane];
+  const int f0o = cown[B + lane], f1o = cown[B + kWave + lane];
   const int T = live ? T_l : 0;
   const int32_t raw = active ? raw_l : 0;
   const int wave_img = wave_or3(row_img(raw));
@@ -948,24 +944,50 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
   const int P = wave_incl_scan64(c) - c;
   const double4 p = active ? p_l : make_double4(0.0, 0.0, 0.0, 0.0);
   sp[lane] = p;
-  if (MULTI) scol[lane] = active ? col[t] : 0;
+  if (MULTI) scol[lane] = active ? col_l : 0;
+  // Several boxes: a wave whose particles all lie in one box (all but the
+  // waves at colony boundaries) takes the one-box path with that box's
+  // geometry, read once into scalar registers; the others read each
+  // owner's box per entry.
+  bool one_box = true;
+  BoxGeom gw = g0;
+  if (MULTI) {
+    const int cw = __builtin_amdgcn_readfirstlane(col_l);  // lane 0 is active
+    one_box = __all(!active || col_l == cw);
+    if (one_box) gw = boxes[cw];
+  }
   wave_lds_sync();
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  if (wave_img == 0)
-    flat_rows<MULTI, 0>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
-                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
-                        a0, a1, a2);
-  else if (wave_img == 1)
-    flat_rows<MULTI, 1>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
-                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
-                        a0, a1, a2);
-  else
-    flat_rows<MULTI, 7>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
-                        pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
-                        a0, a1, a2);
+  if (one_box) {
+    if (wave_img == 0)
+      flat_rows<false, 0>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                          pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, gw, fp,
+                          a0, a1, a2);
+    else if (wave_img == 1)
+      flat_rows<false, 1>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                          pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, gw, fp,
+                          a0, a1, a2);
+    else
+      flat_rows<false, 7>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                          pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, gw, fp,
+                          a0, a1, a2);
+  } else if (MULTI) {
+    if (wave_img == 0)
+      flat_rows<true, 0>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                         pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                         a0, a1, a2);
+    else if (wave_img == 1)
+      flat_rows<true, 1>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                         pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                         a0, a1, a2);
+    else
+      flat_rows<true, 7>(T, P, c, B, lane, f0u, f1u, f0o, f1o, cidx, cown,
+                         pos_in, sp, scol, q0s, q1s, q2s, qrs, boxes, g0, fp,
+                         a0, a1, a2);
+  }
   double v2 = 0.0;
   if (active) {
-    const BoxGeom &g = MULTI ? boxes[col[t]] : g0;
+    const BoxGeom &g = MULTI ? boxes[col_l] : g0;
     // growth surface (K.H:1009-1030); vcom x/y are zero
     a0 += 0.0;
     a1 += 0.0;
