@@ -616,110 +616,126 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
   // bins would otherwise make this quadratic); hitting the cap is reported
   // as an overflow.  After a non-finite force no rows are built.
   const int64_t scan_cap = 64 * (int64_t)maxn + 4096;
-  bool active = t < n && !(*err & ERRBIT_NAN);
-  const BoxGeom &g = MULTI ? boxes[active ? col[t] : 0] : g0;
-  const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
-  int bx = 0, by = 0, bz = 0;
-  double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
-  double reach_i = 0.0;
-  // axes along which a candidate of this list build (scan) or a neighbor
-  // before the next rebuild (force) can need a periodic image
-  int scan_img = 0, force_img = 0;
-  if (active) {
-    p = pos[t];
-    reach_i = RODS ? reach_of<true>(p, type[t], rod, t) : p.w;
-    int lb = key[t] - g.bin_offset;
-    bx = lb % nbx;
-    int rr = lb / nbx;
-    by = rr % nby;
-    bz = rr / nby;
-    const int b3[3] = {bx, by, bz};
-    for (int a = 0; a < 3; a++) {
-      if (!g.periodic[a]) continue;
-      int nb = g.nbin[a];
-      if (!(nb >= 4 && b3[a] >= 1 && b3[a] + 1 < nb)) scan_img |= 1 << a;
-      if (!(nb >= 6 && b3[a] >= 2 && b3[a] + 2 < nb)) force_img |= 1 << a;
+  const bool active = t < n && !(*err & ERRBIT_NAN);
+  // The row build over box g.  Several boxes: a wave whose particles all
+  // lie in one box (all but the waves at colony boundaries) reads that box
+  // once into scalar registers; the others read each particle's box.
+  auto build = [&](const BoxGeom &g) {
+    const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
+    int bx = 0, by = 0, bz = 0;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    double reach_i = 0.0;
+    // axes along which a candidate of this list build (scan) or a neighbor
+    // before the next rebuild (force) can need a periodic image
+    int scan_img = 0, force_img = 0;
+    if (active) {
+      p = pos[t];
+      reach_i = RODS ? reach_of<true>(p, type[t], rod, t) : p.w;
+      int lb = key[t] - g.bin_offset;
+      bx = lb % nbx;
+      int rr = lb / nbx;
+      by = rr % nby;
+      bz = rr / nby;
+      const int b3[3] = {bx, by, bz};
+      for (int a = 0; a < 3; a++) {
+        if (!g.periodic[a]) continue;
+        int nb = g.nbin[a];
+        if (!(nb >= 4 && b3[a] >= 1 && b3[a] + 1 < nb)) scan_img |= 1 << a;
+        if (!(nb >= 6 && b3[a] >= 2 && b3[a] + 2 < nb)) force_img |= 1 << a;
+      }
     }
-  }
-  const int wave_img = wave_or3(scan_img);
-  if (active) {
-    if (RODS) {
-      // Segment builds (sparse mesh-cell bins): range by range, all lanes
-      // together, which measured faster there than the flattened scan.
-      int64_t scanned = 0;
-      for (int oz = -1; oz <= 1; oz++) {
-        int cz = bz + oz;
-        if (g.periodic[2]) {
-          cz = (cz + nbz) % nbz;
-        } else if (cz < 0 || cz >= nbz) {
-          continue;
-        }
-        for (int oy = -1; oy <= 1; oy++) {
-          int cy = by + oy;
-          if (g.periodic[1]) {
-            cy = (cy + nby) % nby;
-          } else if (cy < 0 || cy >= nby) {
+    const int wave_img = wave_or3(scan_img);
+    if (active) {
+      if (RODS) {
+        // Segment builds (sparse mesh-cell bins): range by range, all lanes
+        // together, which measured faster there than the flattened scan.
+        int64_t scanned = 0;
+        for (int oz = -1; oz <= 1; oz++) {
+          int cz = bz + oz;
+          if (g.periodic[2]) {
+            cz = (cz + nbz) % nbz;
+          } else if (cz < 0 || cz >= nbz) {
             continue;
           }
-          int row = g.bin_offset + (cz * nby + cy) * nbx;
-          // up to three x-ranges; one contiguous range when no x wrap
-          int ranges[3][2];
-          int nr = 0;
-          if (bx >= 1 && bx + 1 < nbx) {
-            ranges[0][0] = start[row + bx - 1];
-            ranges[0][1] = start[row + bx + 2];
-            nr = 1;
-          } else {
-            for (int ox = -1; ox <= 1; ox++) {
-              int cx = bx + ox;
-              if (g.periodic[0]) {
-                cx = (cx + nbx) % nbx;
-              } else if (cx < 0 || cx >= nbx) {
-                continue;
+          for (int oy = -1; oy <= 1; oy++) {
+            int cy = by + oy;
+            if (g.periodic[1]) {
+              cy = (cy + nby) % nby;
+            } else if (cy < 0 || cy >= nby) {
+              continue;
+            }
+            int row = g.bin_offset + (cz * nby + cy) * nbx;
+            // up to three x-ranges; one contiguous range when no x wrap
+            int ranges[3][2];
+            int nr = 0;
+            if (bx >= 1 && bx + 1 < nbx) {
+              ranges[0][0] = start[row + bx - 1];
+              ranges[0][1] = start[row + bx + 2];
+              nr = 1;
+            } else {
+              for (int ox = -1; ox <= 1; ox++) {
+                int cx = bx + ox;
+                if (g.periodic[0]) {
+                  cx = (cx + nbx) % nbx;
+                } else if (cx < 0 || cx >= nbx) {
+                  continue;
+                }
+                ranges[nr][0] = start[row + cx];
+                ranges[nr][1] = start[row + cx + 1];
+                nr++;
               }
-              ranges[nr][0] = start[row + cx];
-              ranges[nr][1] = start[row + cx + 1];
-              nr++;
             }
-          }
-          for (int q = 0; q < nr; q++) {
-            int64_t len = ranges[q][1] - ranges[q][0];
-            if (scanned + len > scan_cap) {
-              cnt = maxn + 1;  // reported as overflow below
-              ranges[q][1] = ranges[q][0];
+            for (int q = 0; q < nr; q++) {
+              int64_t len = ranges[q][1] - ranges[q][0];
+              if (scanned + len > scan_cap) {
+                cnt = maxn + 1;  // reported as overflow below
+                ranges[q][1] = ranges[q][0];
+              }
+              scanned += len;
+              if (wave_img == 0)
+                scan_range<RODS, 0>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                    pos, type, rod, mode, crit, was, out, ncap,
+                                    maxn, cnt);
+              else if (wave_img == 1)
+                scan_range<RODS, 1>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                    pos, type, rod, mode, crit, was, out, ncap,
+                                    maxn, cnt);
+              else
+                scan_range<RODS, 7>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
+                                    pos, type, rod, mode, crit, was, out, ncap,
+                                    maxn, cnt);
             }
-            scanned += len;
-            if (wave_img == 0)
-              scan_range<RODS, 0>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                  pos, type, rod, mode, crit, was, out, ncap,
-                                  maxn, cnt);
-            else if (wave_img == 1)
-              scan_range<RODS, 1>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                  pos, type, rod, mode, crit, was, out, ncap,
-                                  maxn, cnt);
-            else
-              scan_range<RODS, 7>(ranges[q][0], ranges[q][1], ti, p, reach_i, g,
-                                  pos, type, rod, mode, crit, was, out, ncap,
-                                  maxn, cnt);
           }
         }
+      } else {
+        const int nx = (bx >= 1 && bx + 1 < nbx) ? 1 : 3;
+        if (wave_img == 0)
+          scan_stencil<0>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                          crit, was, out, ncap, maxn, cnt);
+        else if (wave_img == 1)
+          scan_stencil<1>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                          crit, was, out, ncap, maxn, cnt);
+        else
+          scan_stencil<7>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
+                          crit, was, out, ncap, maxn, cnt);
       }
-    } else {
-      const int nx = (bx >= 1 && bx + 1 < nbx) ? 1 : 3;
-      if (wave_img == 0)
-        scan_stencil<0>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                        crit, was, out, ncap, maxn, cnt);
-      else if (wave_img == 1)
-        scan_stencil<1>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                        crit, was, out, ncap, maxn, cnt);
-      else
-        scan_stencil<7>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                        crit, was, out, ncap, maxn, cnt);
+      ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
+      if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
+    } else if (t < n) {
+      ncnt[t] = 0;
     }
-    ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
-    if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
-  } else if (t < n) {
-    ncnt[t] = 0;
+  };
+  if (MULTI) {
+    const int32_t cl = col[t < n ? t : n - 1];
+    const int cw = __builtin_amdgcn_readfirstlane(cl);
+    if (__all(t >= n || cl == cw)) {
+      const BoxGeom gw = boxes[cw];
+      build(gw);
+    } else {
+      build(boxes[active ? col[t] : 0]);
+    }
+  } else {
+    build(g0);
   }
   if (flat.cidx) {
     // Flat row layout: the tile's rows back to back from the start of the
