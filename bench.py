@@ -99,10 +99,12 @@ def ensemble_bench(bmx, W, steps: int, seq: bool = True):
     ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K, skin=cols[0].skin,
                             max_neighbors=64)
     ctx.bind_colonies(recs, starts, geoms)
-    t0 = time.perf_counter()
-    tune = ctx.autotune(W.DT, 2 * K)
-    tune["seconds"] = time.perf_counter() - t0
     ctx.evolve(W.DT, K)                      # into the steady rebuild phase
+    # trials of 4 rebuild intervals from that state (2 intervals from the
+    # freshly bound records picked launch shapes that ran slower later)
+    t0 = time.perf_counter()
+    tune = ctx.autotune(W.DT, 4 * K)
+    tune["seconds"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     ctx.evolve_graph(W.DT, steps)            # capture + first replay
     capture_s = time.perf_counter() - t0
