@@ -462,7 +462,9 @@ class Context:
     def autotune(self, dt: float, trial_steps: int = 20,
                  cache: Optional[str] = None) -> dict:
         """Times the launch shapes and bin edges on the bound particles,
-        applies the fastest and returns it; the state is left unchanged."""
+        applies the fastest and returns it; the state is left unchanged.
+        Trials of several rebuild periods, after the first rebuild, rank the
+        shapes as a long run does."""
         t = Tuning()
         _check(self.lib.bmx_autotune(self.h, float(dt), int(trial_steps),
                                      cache.encode() if cache else None, C.byref(t)))

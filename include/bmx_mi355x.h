@@ -148,7 +148,10 @@ typedef struct {
  * run; the next step rebuilds the lists.  When cache_path is not NULL the
  * choice is looked up in / appended to that file (one JSON object per line,
  * keyed by architecture, policy, rebuild interval, colony count, particle
- * count and packing).  trial_steps is rounded up to whole rebuild periods. */
+ * count and packing).  trial_steps is rounded up to whole rebuild periods;
+ * trials of several periods, taken after the run has passed its first
+ * rebuild, rank the shapes as a long run does (two-period trials from
+ * freshly bound records have favoured shapes that later ran slower). */
 bmx_status bmx_autotune(bmx_ctx *ctx, double dt, int32_t trial_steps,
                         const char *cache_path, bmx_tuning *out);
 bmx_layout bmx_default_layout(void);
