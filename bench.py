@@ -5,13 +5,15 @@ Default workload: 1,000,000 FP64 spheres, radii uniform in [0.5e-4, 0.7e-4]
 cm, random sequential insertion at volume fraction 0.3 in a 2:2:1 box
 periodic in x/y with the growth surface at the bottom, cubic contact law
 with adhesion band 0.1 * mean radius, dt = 0.25, Verlet neighbor list (skin
-8e-6 cm) rebuilt every 40 steps.  A step is one pass of the hot path (binning + list build when due,
-fused force + overdamped update) over all particles, with the particles
-already resident in HBM.
+4e-6 cm) rebuilt every 10 steps.  A step is one pass of the hot path
+(binning + list build when due, fused force + overdamped update) over all
+particles, with the particles already resident in HBM.  The timed window
+starts at a rebuild, so `--steps K` holds ceil(K / 10) list builds.
 
 Prints one JSON line with the throughput (particle-steps/s), the roofline of
-the dominant kernel (fused force + update, HIP events on the library's own
-stream over the timed region) and the CPU baseline: the oracle (the C
+the dominant kernel (fused force + update: HIP events on the library's own
+stream around each run of force launches between two rebuilds, in a second
+window of the same length) and the CPU baseline: the oracle (the C
 restatement of the reference's CPU path, 448-byte AoS records, stencil list
 rebuilt every step) timed on a bounded sample of the same workload.
 """
@@ -41,11 +43,37 @@ def algorithmic_bytes_per_particle(mean_row: float) -> float:
     return 32.0 + 4.0 + 4.0 * mean_row + 32.0
 
 
-def cpu_baseline(col, budget_s: float = 12.0):
+def survey_bytes_per_particle_step(mean_row: float, K: int) -> float:
+    """Compulsory bytes per particle-step of a sphere step with rebuilds every
+    K steps, as the workload's accounting counts them: step 84 + 4(L+1)
+    (x, y, z, r and type read; x, y, z and velocity written; row offset and
+    L indices read), rebuild (112 + 4(L+1)) / K."""
+    return 84.0 + 4.0 * (mean_row + 1.0) + (112.0 + 4.0 * (mean_row + 1.0)) / K
+
+
+def host_cpu() -> dict:
+    """Model name (lscpu / /proc/cpuinfo) and logical CPUs of the host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(col, budget_s: float = 12.0, min_steps_1: int = 3):
     """The oracle in the reference's configuration (AoS records, per-step
     stencil list over mesh-cell bins, per-particle loop), timed on the host:
-    as many whole steps of the workload as fit in budget_s, one thread and
-    all threads."""
+    as many whole steps of the workload as fit in budget_s / 2 (at least
+    min_steps_1 on one thread), on one thread and on all threads."""
     import numpy as np
     import oracle_bmx as orc
 
@@ -67,7 +95,7 @@ def cpu_baseline(col, budget_s: float = 12.0):
             st.evolve(col.dt, 1)
             steps += 1
             el = time.perf_counter() - t0
-            if el > budget_s / 2 or steps >= 50:
+            if (el > budget_s / 2 and (nt > 1 or steps >= min_steps_1)) or steps >= 50:
                 break
         results[label] = (len(recs) * steps / el, steps, el, nt)
         st.close()
@@ -81,6 +109,8 @@ def cpu_baseline(col, budget_s: float = 12.0):
                   f"{va[3]} OpenMP threads; 1 thread: {v1[0]:.4g} particle-steps/s "
                   f"over {v1[1]} steps",
         "value_1core": v1[0],
+        "steps_1core": v1[1],
+        "host": host_cpu(),
     }
 
 
@@ -203,7 +233,7 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
     ctx.bind(col.recs.copy())
     ctx.init_bonds()
     ctx.evolve(col.dt, 10)
-    ctx.set_timing(True, every=7)
+    ctx.set_timing(True)
     ctx.reset_timing()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -390,6 +420,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="Verlet skin in cm (default: the workload's)")
+    ap.add_argument("--rebuild-every", type=int, default=0,
+                    help="steps between list builds (default: the workload's, 10)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--remap", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -427,34 +461,49 @@ def main() -> int:
         return 0
 
     col = W.c2_large(args.n)
+    if args.skin > 0:
+        col.skin = args.skin
+    if args.rebuild_every > 0:
+        col.rebuild_every = args.rebuild_every
+    K = col.rebuild_every
     ctx = bmx.Context(0)
     ctx.set_force_params(col.fpar)
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
-    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=col.rebuild_every,
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K,
                             skin=col.skin, max_neighbors=64)
     ctx.set_launch_params(args.block, args.remap)
     recs = col.recs.copy()
     ctx.bind(recs)
-    ctx.evolve(col.dt, args.warmup)
-
-    # events around every rebuild and one force launch in 29 (29 and the
-    # rebuild interval 40 are coprime, so the samples spread over the phases
-    # of the interval; each timed launch costs a few microseconds of queue
-    # time, so sparse sampling keeps the throughput figure clean)
-    ctx.set_timing(True, every=29)
-    ctx.reset_timing()
+    # Warm-up: `warmup` steps, then on to the next rebuild (the first step
+    # after a bind rebuilds, then every K steps), so the timed window starts
+    # with a rebuild and holds ceil(steps / K) of them.
+    warm = -(-args.warmup // K) * K
+    ctx.evolve(col.dt, warm)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.evolve(col.dt, args.steps)  # synchronizes and checks skin/overflow/NaN
     t1 = time.perf_counter()
+    secs = t1 - t0
+    n = len(recs)
+    value = n * args.steps / secs
+    ms_per_step = secs * 1e3 / args.steps
+
+    # Kernel split: the same window again (it starts at a rebuild too) with
+    # HIP events on the library's stream around every rebuild and around
+    # every run of force launches between two rebuilds, so the force time per
+    # launch carries no per-launch event overhead and cannot exceed a step.
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.evolve(col.dt, args.steps)
+    secs_timed = time.perf_counter() - t0
     tm = ctx.timing()
     ctx.set_timing(False)
 
-    n = len(recs)
-    secs = t1 - t0
-    value = n * args.steps / secs
     mean_row = tm.neighbor_entries / max(n, 1)
     per_launch_ms = tm.force_ms / max(tm.force_launches, 1)
+    rebuild_ms = tm.rebuild_ms / max(tm.rebuilds, 1)
     alg_bytes = algorithmic_bytes_per_particle(mean_row) * n
     achieved = alg_bytes / (per_launch_ms * 1e-3) / 1e9
     traffic = None
@@ -470,7 +519,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": secs * 1e3 / args.steps,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -479,10 +528,12 @@ def main() -> int:
         "config": {
             "workload": "1M FP64 spheres, r~U[0.5e-4,0.7e-4] cm, phi=0.3, 2:2:1 box "
                         "periodic x/y with growth surface, cubic law W_A=6e-6, dt=0.25, "
-                        f"Verlet list (skin {col.skin:g}) rebuilt every "
-                        f"{col.rebuild_every} steps",
+                        f"Verlet list (skin {col.skin:g}) rebuilt every {K} steps",
             "n": n,
+            "rebuild_every": K,
+            "skin": col.skin,
             "mean_row": mean_row,
+            "warmup_steps_run": warm,
             "rebuilds": tm.rebuilds,
             "block": args.block,
             "xcd_remap": args.remap,
@@ -496,9 +547,14 @@ def main() -> int:
             "traffic": traffic,
             "kernel": "k_force_sphere_flat",
             "kernel_avg_ms": per_launch_ms,
-            "kernel_share_of_step": per_launch_ms / (secs * 1e3 / args.steps),
-            "rebuild_avg_ms": tm.rebuild_ms / max(tm.rebuilds, 1),
+            "kernel_launches_timed": tm.force_launches,
+            "kernel_share_of_step": per_launch_ms / ms_per_step,
+            "rebuild_avg_ms": rebuild_ms,
+            "events_window_ms_per_step": secs_timed * 1e3 / args.steps,
+            "device_ms_per_step": (tm.force_ms + tm.rebuild_ms) / args.steps,
+            "algorithmic_bytes_per_particle": algorithmic_bytes_per_particle(mean_row),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "survey_bytes_per_particle_step": survey_bytes_per_particle_step(mean_row, K),
         },
     }
     ctx.close()

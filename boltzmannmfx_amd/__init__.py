@@ -513,9 +513,10 @@ class Context:
         _check(self.lib.bmx_get_bin_dims(self.h, int(colony), out))
         return tuple(int(v) for v in out)
 
-    def set_timing(self, on, every: int = 1) -> None:
-        """Events around every rebuild and one force launch in `every`."""
-        _check(self.lib.bmx_set_timing(self.h, int(every) if on else 0))
+    def set_timing(self, on) -> None:
+        """Events around every rebuild and every run of force launches
+        between rebuilds."""
+        _check(self.lib.bmx_set_timing(self.h, 1 if on else 0))
 
     def timing(self) -> Timing:
         t = Timing()

@@ -136,18 +136,23 @@ struct bmx_ctx {
   bool wrote_step = false;
   DevBuf<int32_t> d_err, d_stats, d_bstats;
   DevBuf<unsigned long long> d_total, d_ustats;
-  DevBuf<double> d_vpart;      // [slot][block] max dt*|vel| per force block
-  DevBuf<unsigned long long> d_slotmax;  // per-slot maxima (skin check)
+  DevBuf<double> d_vpart;      // [slot][kVmaxWords] max dt*|vel| per step
   int32_t vmax_slots = 1;
-  int32_t vstride = 1;         // blocks per slot (smallest block size)
+  int32_t bstride = 1;         // row-build blocks at the smallest block size
   int32_t row_nblk = 0;        // blocks of the last row build
   int32_t *h_flags = nullptr;  // pinned: err, max row, pad, pad, total(2)
 
-  // timing
+  // timing: events around every rebuild and around every run of force
+  // launches between two rebuilds (or a rebuild and the end of a call)
   bool timing = false;
-  int32_t timing_every = 1;  // time one force launch in this many
-  int64_t timing_seq = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_force, ev_rebuild;
+  hipEvent_t run_e0 = nullptr;  // start of the open run of force launches
+  int32_t run_n = 0;            // launches in the open run
+  struct Run {
+    hipEvent_t e0, e1;
+    int32_t n;
+  };
+  std::vector<Run> ev_force;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_rebuild;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double force_ms = 0.0, rebuild_ms = 0.0;
@@ -221,11 +226,11 @@ hipEvent_t next_event(bmx_ctx *c) {
 
 // Collect finished event pairs into the running totals.
 bmx_status drain_timing(bmx_ctx *c) {
-  for (auto &pr : c->ev_force) {
+  for (auto &r : c->ev_force) {
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    HIPCHK(hipEventElapsedTime(&ms, r.e0, r.e1));
     c->force_ms += ms;
-    c->force_launches++;
+    c->force_launches += r.n;
   }
   for (auto &pr : c->ev_rebuild) {
     float ms = 0.f;
@@ -277,54 +282,79 @@ bmx_status make_box(const bmx_ctx *c, const bmx_geometry &gm, BoxGeom &b,
   return BMX_OK;
 }
 
+// Bins of every box for the current force and neighbor parameters.  The
+// boxes are formed first and committed only when all of them are valid, so
+// a rejected parameter change leaves the previous binning in place.
 bmx_status setup_boxes(bmx_ctx *c) {
   // Verlet radius: the largest r_lm any pair can have, plus the skin
   // (K.H:23-40).
   double rl = 0.0;
   if (c->any_yeast) rl = std::max(rl, 2.0 * c->rmax + c->fpar[0]);
   if (c->any_fungi) rl = std::max(rl, 1.5 * (2.0 * c->lmax) + c->fpar[0]);
-  c->verlet_radius = rl + c->nbp.skin;
+  const double vr = rl + c->nbp.skin;
+  double crit;
   if (c->nbp.mode == BMX_NBR_VERLET) {
-    if (!(c->verlet_radius > 0.0))
+    if (!(vr > 0.0))
       return fail(BMX_ERR_BAD_ARG, "interaction radius must be positive");
-    c->crit = c->verlet_radius * c->verlet_radius;
+    crit = vr * vr;
   } else {
-    c->crit = c->nbp.neighborhood;
+    crit = c->nbp.neighborhood;
   }
-  c->boxes.resize(c->ncol);
+  const double vr_old = c->verlet_radius;
+  c->verlet_radius = vr;  // make_box reads it
+  std::vector<BoxGeom> boxes((size_t)c->ncol);
   int64_t off = 0;
   for (int k = 0; k < c->ncol; k++) {
-    bmx_status st = make_box(c, c->colony_geom[k], c->boxes[k], (int32_t)off);
-    if (st != BMX_OK) return st;
-    off += (int64_t)c->boxes[k].nbin[0] * c->boxes[k].nbin[1] *
-           c->boxes[k].nbin[2];
-    if (off > (int64_t)1 << 30) return fail(BMX_ERR_BAD_ARG, "too many bins");
+    bmx_status st = make_box(c, c->colony_geom[k], boxes[k], (int32_t)off);
+    if (st == BMX_OK) {
+      off += (int64_t)boxes[k].nbin[0] * boxes[k].nbin[1] * boxes[k].nbin[2];
+      if (off > (int64_t)1 << 30) st = fail(BMX_ERR_BAD_ARG, "too many bins");
+    }
+    if (st != BMX_OK) {
+      c->verlet_radius = vr_old;
+      return st;
+    }
   }
-  c->nbins = (int32_t)off;
-  HIPCHK(c->d_boxes.alloc(c->ncol));
-  HIPCHK(hipMemcpyAsync(c->d_boxes.p, c->boxes.data(),
-                        sizeof(BoxGeom) * c->ncol, hipMemcpyHostToDevice,
-                        c->stream));
-  HIPCHK(c->d_count.alloc(c->nbins + 1));
-  c->count_clear = false;
-  HIPCHK(c->d_start.alloc(c->nbins + 1));
-  HIPCHK(c->d_scan.alloc(scan_scratch_elems(c->nbins) + 1));
+  // committed: the rows of the previous bins are stale from here on
   c->have_list = false;
+  c->crit = crit;
+  c->boxes.swap(boxes);
+  c->nbins = (int32_t)off;
+  c->count_clear = false;
   c->graph_step0 = -1;
+  hipError_t e = c->d_boxes.alloc(c->ncol);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->d_boxes.p, c->boxes.data(), sizeof(BoxGeom) * c->ncol,
+                       hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = c->d_count.alloc(c->nbins + 1);
+  if (e == hipSuccess) e = c->d_start.alloc(c->nbins + 1);
+  if (e == hipSuccess) e = c->d_scan.alloc(scan_scratch_elems(c->nbins) + 1);
+  if (e != hipSuccess) {
+    // the bin arrays no longer match the boxes: nothing may run on them
+    c->bound = false;
+    return fail(BMX_ERR_HIP, std::string("bin arrays: ") + hipGetErrorString(e));
+  }
   return BMX_OK;
 }
 
+// Row arrays for ncap particles and maxn entries per row.  A failure leaves
+// the context unbound (the rows would be smaller than the kernels assume).
 bmx_status ensure_rows(bmx_ctx *c) {
+  c->have_list = false;
   size_t need = (size_t)c->maxn * (size_t)c->ncap;
-  HIPCHK(c->d_nbr.alloc(need));
-  HIPCHK(c->d_ncnt.alloc(c->ncap));
   c->flat_tile = flat_tile(c->any_fungi ? 1 : 0);
   const int64_t nw = (c->ncap + c->flat_tile - 1) / c->flat_tile;
   const int64_t nflat = flat_entries(c->ncap, c->flat_tile, c->maxn);
-  HIPCHK(c->d_wtot.alloc(nw + 1));
-  HIPCHK(c->d_redo.alloc(nw + 1));
-  HIPCHK(c->d_cidx.alloc(nflat));
-  HIPCHK(c->d_cown.alloc(nflat));
+  hipError_t e = c->d_nbr.alloc(need);
+  if (e == hipSuccess) e = c->d_ncnt.alloc(c->ncap);
+  if (e == hipSuccess) e = c->d_wtot.alloc(nw + 1);
+  if (e == hipSuccess) e = c->d_redo.alloc(nw + 1);
+  if (e == hipSuccess) e = c->d_cidx.alloc(nflat);
+  if (e == hipSuccess) e = c->d_cown.alloc(nflat);
+  if (e != hipSuccess) {
+    c->bound = false;
+    return fail(BMX_ERR_HIP, std::string("row arrays: ") + hipGetErrorString(e));
+  }
   return BMX_OK;
 }
 
@@ -332,13 +362,16 @@ bmx_status ensure_rows(bmx_ctx *c) {
 // check reads every slot, so unused tails stay zero.
 bmx_status alloc_partials(bmx_ctx *c) {
   c->vmax_slots = std::max(1, c->nbp.rebuild_every);
-  c->vstride = std::max(1, row_blocks(c->ncap, 64));
-  HIPCHK(c->d_vpart.alloc((size_t)c->vmax_slots * c->vstride));
-  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
-                         (int64_t)c->vmax_slots * c->vstride, 0ull, c->stream));
-  HIPCHK(c->d_bstats.alloc(2 * (size_t)c->vstride));
-  HIPCHK(c->d_slotmax.alloc(c->vmax_slots));
-  HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, c->stream));
+  c->bstride = std::max(1, row_blocks(c->ncap, 64));
+  hipError_t e = c->d_vpart.alloc((size_t)c->vmax_slots * kVmaxWords);
+  if (e == hipSuccess)
+    e = launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
+                        (int64_t)c->vmax_slots * kVmaxWords, 0ull, c->stream);
+  if (e == hipSuccess) e = c->d_bstats.alloc(2 * (size_t)c->bstride);
+  if (e != hipSuccess) {
+    c->bound = false;
+    return fail(BMX_ERR_HIP, std::string("partials: ") + hipGetErrorString(e));
+  }
   return BMX_OK;
 }
 
@@ -346,9 +379,37 @@ FlatRows flat_rows(bmx_ctx *c) {
   return FlatRows{c->flat_tile, c->d_cidx.p, c->d_cown.p, c->d_wtot.p};
 }
 
-// Binning + gather into the other buffers + list build.
-bmx_status rebuild(bmx_ctx *c) {
+bool verlet_checked(const bmx_ctx *c) {
+  return c->nbp.mode == BMX_NBR_VERLET && c->nbp.rebuild_every > 1;
+}
+
+// Ends the open run of timed force launches (before a rebuild, at the end
+// of a call).
+bmx_status close_run(bmx_ctx *c) {
+  if (!c->run_e0) return BMX_OK;
+  hipEvent_t e1 = next_event(c);
+  if (!e1) return fail(BMX_ERR_HIP, "event allocation failed");
+  HIPCHK(hipEventRecord(e1, c->stream));
+  c->ev_force.push_back({c->run_e0, e1, c->run_n});
+  c->run_e0 = nullptr;
+  c->run_n = 0;
+  return BMX_OK;
+}
+
+// Binning + gather into the other buffers + list build.  With a Verlet list
+// in force, the displacements since that list was built are checked against
+// skin/2 first and their partials cleared, so every caller that rebuilds
+// (steps, exchange, fusion, InitBonds) starts the next period from zero.
+// check_skin = false is for passes that repeat checked arithmetic.
+bmx_status rebuild(bmx_ctx *c, bool check_skin = true) {
   hipStream_t s = c->stream;
+  if (c->timing) {
+    bmx_status st = close_run(c);
+    if (st != BMX_OK) return st;
+  }
+  if (check_skin && c->have_list && verlet_checked(c))
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 1,
+                             c->d_err.p, s));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = next_event(c);
@@ -391,33 +452,23 @@ bmx_status rebuild(bmx_ctx *c) {
   return BMX_OK;
 }
 
-bool verlet_checked(const bmx_ctx *c) {
-  return c->nbp.mode == BMX_NBR_VERLET && c->nbp.rebuild_every > 1;
-}
 
 bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   hipStream_t s = c->stream;
   bool need = !c->have_list ||
               (c->step - c->last_build) >= (int64_t)c->nbp.rebuild_every;
   if (need) {
-    if (c->have_list && verlet_checked(c))
-      HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                               0.5 * c->nbp.skin, 1, c->d_slotmax.p,
-                               c->d_err.p, s));
     bmx_status st = rebuild(c);
     if (st != BMX_OK) return st;
   }
   int slot = (int)(c->step - c->last_build);
   double *vm = verlet_checked(c)
-                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * kVmaxWords
                    : nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  const bool timed = c->timing && (c->timing_seq++ % c->timing_every) == 0;
-  if (timed) {
-    e0 = next_event(c);
-    e1 = next_event(c);
-    if (!e0 || !e1) return fail(BMX_ERR_HIP, "event allocation failed");
-    HIPCHK(hipEventRecord(e0, s));
+  if (c->timing && !c->run_e0) {
+    c->run_e0 = next_event(c);
+    if (!c->run_e0) return fail(BMX_ERR_HIP, "event allocation failed");
+    HIPCHK(hipEventRecord(c->run_e0, s));
   }
   ForceParams fp = force_params(c, dt);
   int multi = c->ncol > 1;
@@ -435,10 +486,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
         c->d_ncnt.p, c->ncap, c->n, c->maxn, c->d_boxes.p, c->boxes[0], multi,
         fp, fc, vm, c->d_err.p, shape(c), s));
   }
-  if (timed) {
-    HIPCHK(hipEventRecord(e1, s));
-    c->ev_force.push_back({e0, e1});
-  }
+  if (c->timing) c->run_n++;
   c->cd = 1 - c->cd;
   c->step++;
   if (write_fc) c->wrote_step = true;
@@ -451,7 +499,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
 bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
   ForceParams fp = force_params(c, dt);
   double *vm = (slot >= 0 && verlet_checked(c))
-                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * kVmaxWords
                    : nullptr;
   HIPCHK(launch_force_sphere_flat(
       c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
@@ -466,6 +514,10 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
 // Reads the error word and list statistics; clears the error word.
 bmx_status check_flags(bmx_ctx *c) {
   hipStream_t s = c->stream;
+  if (c->timing) {
+    bmx_status st = close_run(c);
+    if (st != BMX_OK) return st;
+  }
   if (c->row_nblk > 0)
     HIPCHK(launch_reduce_rowstats(c->d_bstats.p, c->row_nblk, c->d_stats.p,
                                   c->d_total.p, s));
@@ -528,9 +580,16 @@ bmx_status pack_records(bmx_ctx *c) {
   return BMX_OK;
 }
 
+// A bind replaces the previous one whatever its outcome: the context is
+// unbound from the first line, and only a bind that completes binds it
+// again, so no kernel can run on arrays sized for other particles.
 bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
                        int32_t ncol, const bmx_geometry *geoms,
                        const bmx_layout *layout, int32_t on_device) {
+  reset_graph(c);
+  c->bound = false;
+  c->have_list = false;
+  c->row_nblk = 0;
   if (!aos && cstart[ncol] > 0)
     return fail(BMX_ERR_BAD_ARG, "null record array");
   if (!c->have_fpar) return fail(BMX_ERR_STATE, "force params not set");
@@ -547,7 +606,12 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   for (int k = 0; k < ncol; k++)
     if (cstart[k + 1] < cstart[k])
       return fail(BMX_ERR_BAD_ARG, "colony offsets must be non-decreasing");
-  reset_graph(c);
+  for (int k = 0; k < ncol; k++)
+    for (int d = 0; d < 3; d++)
+      if (!(std::isfinite(geoms[k].prob_lo[d]) &&
+            std::isfinite(geoms[k].prob_hi[d]) &&
+            geoms[k].prob_hi[d] > geoms[k].prob_lo[d]))
+        return fail(BMX_ERR_BAD_ARG, "prob_hi must exceed prob_lo");
   hipStream_t s = c->stream;
   c->n = n;
   c->ncap = std::max<int64_t>(((n + 255) / 256) * 256, 256);
@@ -599,8 +663,6 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
       b.periodic[d] = geoms[k].periodic[d] ? 1 : 0;
       b.nbin[d] = 1;
       b.dxi[d] = 1.0;
-      if (!(b.L[d] > 0.0))
-        return fail(BMX_ERR_BAD_ARG, "prob_hi must exceed prob_lo");
     }
   }
   HIPCHK(c->d_boxes.alloc(ncol));
@@ -751,7 +813,6 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_bstats.release();
   c->d_total.release();
   c->d_vpart.release();
-  c->d_slotmax.release();
   c->d_ustats.release();
   c->d_chem.release();
   c->d_chem_inc.release();
@@ -773,13 +834,21 @@ bmx_status bmx_set_force_params(bmx_ctx *c, const double fpar[9]) {
   for (int k = 0; k < 9; k++)
     if (!std::isfinite(fpar[k]))
       return fail(BMX_ERR_BAD_ARG, "force parameters must be finite");
+  double old[9];
+  std::memcpy(old, c->fpar, sizeof old);
   std::memcpy(c->fpar, fpar, sizeof c->fpar);
-  c->have_fpar = true;
   reset_graph(c);
   if (c->bound) {
     (void)hipSetDevice(c->device);
-    return setup_boxes(c);  // interaction radius depends on W_A
+    // the interaction radius depends on W_A; a rejected change keeps the
+    // previous parameters and bins
+    bmx_status st = setup_boxes(c);
+    if (st != BMX_OK) {
+      std::memcpy(c->fpar, old, sizeof old);
+      return st;
+    }
   }
+  c->have_fpar = true;
   return BMX_OK;
 }
 
@@ -808,6 +877,8 @@ bmx_status bmx_set_neighbor_params(bmx_ctx *c, const bmx_neighbor_params *p) {
     return fail(BMX_ERR_BAD_ARG, "bad neighbor parameters");
   if (p->mode == BMX_NBR_STENCIL && p->rebuild_every != 1)
     return fail(BMX_ERR_BAD_ARG, "stencil lists are rebuilt every step");
+  const bmx_neighbor_params old = c->nbp;
+  const int32_t old_maxn = c->maxn;
   c->nbp = *p;
   if (c->nbp.bin_factor_pct == 0) c->nbp.bin_factor_pct = 100;
   c->maxn = p->max_neighbors > 0 ? p->max_neighbors : 96;
@@ -815,7 +886,13 @@ bmx_status bmx_set_neighbor_params(bmx_ctx *c, const bmx_neighbor_params *p) {
   if (c->bound) {
     (void)hipSetDevice(c->device);
     bmx_status st = setup_boxes(c);
-    if (st != BMX_OK) return st;
+    if (st != BMX_OK) {
+      if (c->bound) {  // rejected before any change: keep the old policy
+        c->nbp = old;
+        c->maxn = old_maxn;
+      }
+      return st;
+    }
     st = ensure_rows(c);
     if (st != BMX_OK) return st;
     st = alloc_partials(c);
@@ -910,10 +987,13 @@ bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
     st = enqueue_step(c, dt, k == nsteps - 1);
     if (st != BMX_OK) return st;
   }
+  if (c->timing) {  // the run ends before the closing skin check
+    st = close_run(c);
+    if (st != BMX_OK) return st;
+  }
   if (verlet_checked(c) && nsteps > 0 && c->have_list)
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                             0.5 * c->nbp.skin, 0, c->d_slotmax.p, c->d_err.p,
-                             c->stream));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 0,
+                             c->d_err.p, c->stream));
   return check_flags(c);
 }
 
@@ -943,9 +1023,8 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     for (int32_t k = 0; k < nsteps && cst == BMX_OK; k++)
       cst = enqueue_step(c, dt, k == nsteps - 1);
     if (cst == BMX_OK && verlet_checked(c))
-      (void)launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                              0.5 * c->nbp.skin, 0, c->d_slotmax.p,
-                              c->d_err.p, c->stream);
+      (void)launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin,
+                              0, c->d_err.p, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (cst != BMX_OK) {
@@ -1060,10 +1139,6 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
       HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n,
                              d_ckpt.p + (size_t)(t / C) * cap, s));
     if (t % K == 0) {
-      if (t > 0 && verlet_checked(c))
-        HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                                 0.5 * c->nbp.skin, 1, c->d_slotmax.p,
-                                 c->d_err.p, s));
       st = rebuild(c);
       if (st != BMX_OK) return st;
     }
@@ -1071,9 +1146,8 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
     if (st != BMX_OK) return st;
   }
   if (verlet_checked(c))
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
-                             s));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 1,
+                             c->d_err.p, s));
   HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n, d_final.p, s));
   HIPCHK(launch_adj_loss(c->pos[c->cd].p, c->oid[c->cs].p, d_target.p, n,
                          c->boxes[0], d_lam0.p, d_lpart.p, s));
@@ -1107,7 +1181,7 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
         if (t > 0)
           HIPCHK(hipMemcpyAsync(c->pos[c->cd].p, d_hpos.p + (size_t)t * cap,
                                 sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
-        st = rebuild(c);
+        st = rebuild(c, false);
         if (st != BMX_OK) return st;
         HIPCHK(hipMemcpyAsync(d_hpos.p + (size_t)t * cap, c->pos[c->cd].p,
                               sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
@@ -1322,10 +1396,13 @@ bmx_status bmx_get_bin_dims(bmx_ctx *c, int32_t colony, int32_t out3[3]) {
 
 bmx_status bmx_set_timing(bmx_ctx *c, int32_t enable) {
   if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
-  if (enable < 0) return fail(BMX_ERR_BAD_ARG, "negative timing period");
+  if (enable < 0) return fail(BMX_ERR_BAD_ARG, "negative timing switch");
+  if (!enable && c->run_e0) {
+    // an open run is dropped, not timed
+    c->run_e0 = nullptr;
+    c->run_n = 0;
+  }
   c->timing = enable != 0;
-  c->timing_every = enable > 0 ? enable : 1;
-  c->timing_seq = 0;
   return BMX_OK;
 }
 
@@ -1377,10 +1454,6 @@ extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) 
     c->have_chem = true;
   }
   // the rows are rebuilt at the current positions (xchng.cpp:68-77)
-  if (c->have_list && verlet_checked(c))
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
-                             s));
   st = rebuild(c);
   if (st != BMX_OK) return st;
   HIPCHK(launch_exchange(c->pos[c->cd].p, c->type[c->cs].p, c->topo[c->cd].p,
@@ -1459,8 +1532,7 @@ bmx_status snap_restore(bmx_ctx *c, const StateSnap &z) {
   c->last_build = -1;
   c->wrote_step = false;
   HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
-                         (int64_t)c->vmax_slots * c->vstride, 0ull, s));
-  HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, s));
+                         (int64_t)c->vmax_slots * kVmaxWords, 0ull, s));
   return BMX_OK;
 }
 
@@ -1939,13 +2011,17 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   const BoxGeom g0 = c->boxes[0];
   const double wa = c->fpar[0];
   const int64_t n = c->n;
-  DevBuf<int32_t> flag;
-  Releaser<int32_t> rf{flag};
+  DevBuf<int32_t> flag, tgt;
+  DevBuf<double> ttau;
+  Releaser<int32_t> rf{flag}, rt{tgt};
+  Releaser<double> rtt{ttau};
   HIPCHK(flag.alloc(1));
+  HIPCHK(tgt.alloc((size_t)n + 1));
+  HIPCHK(ttau.alloc((size_t)n + 1));
   HIPCHK(launch_fill_i32(flag.p, 1, 0, s));
   HIPCHK(launch_tip_fusion(c->d_aos.p, c->lay, n, c->pos[c->cd].p,
                            c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap, g0,
-                           wa, fus, seed, call, flag.p, s));
+                           wa, fus, seed, call, tgt.p, ttau.p, flag.p, s));
   int32_t any = 0;
   HIPCHK(hipMemcpyAsync(&any, flag.p, sizeof any, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

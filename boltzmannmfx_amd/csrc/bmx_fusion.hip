@@ -6,11 +6,11 @@
 // 2494-2508).
 //
 // Each pass is one thread per particle over the rows of the last rebuild,
-// reading and writing the device record array (original order).  A tip
-// writes only itself and reads only non-tips, which that pass leaves
-// alone; the interior pass records its decisions first and applies them
-// after, so every particle sees its neighbours as they stood when the pass
-// began; the cleanup pass writes bond lists and reads only ids, flags and
+// reading and writing the device record array (original order).  The tip
+// and interior passes record their decisions first and apply them after, so
+// every particle sees its neighbours as they stood when the pass began (a
+// tip that fuses turns interior, and would otherwise become a target of
+// another tip partway through the pass); the cleanup pass writes bond lists and reads only ids, flags and
 // deletion records.  Random numbers: bmx_uniform of the tip's (id, cpu),
 // one draw per candidate that passes the geometric tests, in row order.
 #include "bmx_kernels.h"
@@ -71,11 +71,13 @@ __device__ __forceinline__ void separation(const double4 &a, const double4 &b,
   d[2] = sep_axis(a.z, b.z, g.L[2], g.halfL[2], g.periodic[2]);
 }
 
-// checkTipFusion: true when tip i fuses into j (i's record updated)
-__device__ bool tip_fuses(const double rij[3], double *ir, int32_t *ii,
-                          const double *jr, const int32_t *jj,
-                          const FusionParams &P, uint64_t seed, uint32_t call,
-                          int32_t id, int32_t cpu, uint32_t &draw) {
+// checkTipFusion's tests: true when tip i fuses into j, with the contact
+// parameter along j in tau_out.  Reads both records, writes neither.
+__device__ bool tip_test(const double rij[3], const double *ir,
+                         const int32_t *ii, const double *jr, const int32_t *jj,
+                         const FusionParams &P, uint64_t seed, uint32_t call,
+                         int32_t id, int32_t cpu, uint32_t &draw,
+                         double &tau_out) {
   if (!(ii[rec::cell_type] == rec::FUNGI && ii[rec::position] == rec::TIP &&
         ii[rec::fuse_flag] == 0 && jj[rec::cell_type] == rec::FUNGI &&
         jj[rec::position] != rec::TIP))
@@ -114,36 +116,51 @@ __device__ bool tip_fuses(const double rij[3], double *ir, int32_t *ii,
   const double r = sqrt(rx * rx + ry * ry + rz * rz);
   if (!(r < P.max_sep)) return false;
   if (!(bmx_uniform(seed, call, id, cpu, draw++) < P.prob)) return false;
+  tau_out = tau_j;
+  return true;
+}
+
+// checkTipFusion's update of tip i after it fused into j at tau_j.
+__device__ void tip_apply(double *ir, int32_t *ii, int32_t j_id, int32_t j_cpu,
+                          double tau_j) {
   const int nb = ii[rec::n_bnds];
   const int site = ii[rec::site1] == 1 ? 2 : 1;
   ii[rec::fuse_flag] = 1;
-  ii[FUSE_ID] = jj[rec::id];
-  ii[FUSE_CPU] = jj[rec::cpu];
+  ii[FUSE_ID] = j_id;
+  ii[FUSE_CPU] = j_cpu;
   ii[rec::position] = rec::INTERIOR;
   ir[TAU_SPLIT] = tau_j;
   if (nb == 1 || nb == 2) {
     ii[rec::site1 + nb] = site;
-    ii[rec::seg1_id1 + nb] = jj[rec::id];
-    ii[rec::seg1_id2 + nb] = jj[rec::cpu];
+    ii[rec::seg1_id1 + nb] = j_id;
+    ii[rec::seg1_id2 + nb] = j_cpu;
   }
   ii[rec::n_bnds] = nb + 1;
   ir[rec::bond_scale] = 0.0;
   ii[rec::fuse_tip] = site;
-  return true;
 }
 
-__global__ void k_tip_fusion(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
-                             const double4 *__restrict__ pos,
+// Decisions of every tip against the records as they stand when the pass
+// begins (no record is written here): tgt[t] = original index of the
+// segment tip t fuses into (its first success in row order), or -1.  A tip
+// that fuses stops testing, as the reference's tip turns interior on its
+// first success and fails every later test.
+__global__ void k_tip_decide(const uint8_t *__restrict__ aos, AosLayout lay,
+                             int64_t n, const double4 *__restrict__ pos,
                              const int32_t *__restrict__ oid,
                              const int32_t *__restrict__ nbr,
                              const int32_t *__restrict__ ncnt, int64_t ncap,
                              BoxGeom g, double wa, FusionParams P,
                              uint64_t seed, uint32_t call,
+                             int32_t *__restrict__ tgt,
+                             double *__restrict__ ttau,
                              int32_t *__restrict__ fused) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  tgt[t] = -1;
   const int64_t o = oid[t];
-  RecP a = recp(aos, lay, o);
+  uint8_t *ab = const_cast<uint8_t *>(aos);
+  RecP a = recp(ab, lay, o);
   if (!(a.ip[rec::cell_type] == rec::FUNGI && a.ip[rec::position] == rec::TIP))
     return;
   const double4 pi = pos[t];
@@ -152,19 +169,38 @@ __global__ void k_tip_fusion(uint8_t *__restrict__ aos, AosLayout lay, int64_t n
   const int32_t id = *reinterpret_cast<const int32_t *>(base + lay.id);
   const int32_t cpu = *reinterpret_cast<const int32_t *>(base + lay.cpu);
   uint32_t draw = 0;
-  bool any = false;
   const int c = ncnt[t] & ((1 << 28) - 1);
   for (int k = 0; k < c; k++) {
     const int u = nbr[(int64_t)k * ncap + t];
-    RecP b = recp(aos, lay, oid[u]);
+    RecP b = recp(ab, lay, oid[u]);
     double d[3];
     separation(pi, pos[u], g, d);
     const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     const double rlm = pair_reach(a.rd, a.ip, b.rd, b.ip, wa);
-    if (r2 <= (rlm - 1.0e-15) * (rlm - 1.0e-15))
-      any |= tip_fuses(d, a.rd, a.ip, b.rd, b.ip, P, seed, call, id, cpu, draw);
+    double tau;
+    if (r2 <= (rlm - 1.0e-15) * (rlm - 1.0e-15) &&
+        tip_test(d, a.rd, a.ip, b.rd, b.ip, P, seed, call, id, cpu, draw, tau)) {
+      tgt[t] = oid[u];
+      ttau[t] = tau;
+      atomicOr(fused, 1);
+      return;
+    }
   }
-  if (any) atomicOr(fused, 1);
+}
+
+// Applies the decisions: every fused tip updates its own record.
+__global__ void k_tip_apply(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
+                            const int32_t *__restrict__ oid,
+                            const int32_t *__restrict__ tgt,
+                            const double *__restrict__ ttau) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t j = tgt[t];
+  if (j < 0) return;
+  RecP a = recp(aos, lay, oid[t]);
+  // the target's id words are not changed by this pass
+  RecP b = recp(aos, lay, j);
+  tip_apply(a.rd, a.ip, b.ip[rec::id], b.ip[rec::cpu], ttau[t]);
 }
 
 // checkInteriorFusion, recorded: cnt[o] candidates, the last one's tip
@@ -459,11 +495,15 @@ hipError_t launch_tip_fusion(uint8_t *aos, AosLayout lay, int64_t n,
                              const int32_t *nbr, const int32_t *ncnt,
                              int64_t ncap, const BoxGeom &g, double wa,
                              const double fus[4], uint64_t seed, uint32_t call,
-                             int32_t *fused, hipStream_t s) {
+                             int32_t *tgt, double *ttau, int32_t *fused,
+                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
   FusionParams P{fus[0], fus[1], fus[2], fus[3]};
-  hipLaunchKernelGGL(k_tip_fusion, dim3(grid_of(n)), dim3(256), 0, s, aos, lay, n,
-                     pos, oid, nbr, ncnt, ncap, g, wa, P, seed, call, fused);
+  hipLaunchKernelGGL(k_tip_decide, dim3(grid_of(n)), dim3(256), 0, s, aos, lay, n,
+                     pos, oid, nbr, ncnt, ncap, g, wa, P, seed, call, tgt, ttau,
+                     fused);
+  hipLaunchKernelGGL(k_tip_apply, dim3(grid_of(n)), dim3(256), 0, s, aos, lay, n,
+                     oid, tgt, ttau);
   return hipGetLastError();
 }
 

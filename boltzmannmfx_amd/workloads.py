@@ -69,16 +69,13 @@ def c1_small(seed: int = 21893) -> Colony:
 
 
 # Verlet policy of the large sphere workloads (the 1M colony and the
-# 256-colony ensemble): lists rebuilt every 40 steps with a skin of 8e-6 cm.
-# The largest displacement over 40 steps is 1.68e-6 cm (the whole 1M colony,
-# measured after 20 warm-up steps), against the skin check's bound
-# skin / 2 = 4e-6: a margin of 2.4; every call checks it and reports an
-# interval too long for the skin.  Lists hold 1.56 entries per sphere at
-# volume fraction 0.3.  Swept on the GPU: 1M colony 27.7 us per step here,
-# 27.6 at (1e-5, 60), 45.5 at (5e-6, 10); ensemble 2.9e10 particle-steps/s
-# here, 2.7e10 at (1e-5, 40), 1.9e10 at (5e-6, 10).
-C2_SKIN = 8.0e-6
-C2_REBUILD_EVERY = 40
+# 256-colony ensemble): lists rebuilt every 10 steps, the interval the
+# 1M-sphere workload is specified with, with a skin of 4e-6 cm.  Every call
+# checks the summed per-step displacement maxima against skin / 2 and reports
+# an interval too long for the skin; the largest displacement over 40 steps
+# of the 1M colony is 1.68e-6 cm, so 10 steps stay well inside 2e-6.
+C2_SKIN = 4.0e-6
+C2_REBUILD_EVERY = 10
 
 
 def c2_large(n: int = 1_000_000, seed: int = 7) -> Colony:

@@ -160,7 +160,9 @@ bmx_layout bmx_default_layout(void);
 /* Copies n records (host memory when aos_on_device == 0, device memory
  * otherwise) into the context and applies the periodic wrap that
  * Redistribute performs (src/des/bmx_pc_init.cpp:131).  The record array is
- * remembered for bmx_sync_to_aos. */
+ * remembered for bmx_sync_to_aos.  A bind replaces the previous one: when it
+ * fails, the context is left with no particles bound (BMX_ERR_STATE from
+ * every call that needs them until a bind succeeds). */
 bmx_status bmx_bind_particles_aos(bmx_ctx *ctx, const void *aos, int64_t n,
                                   const bmx_layout *layout,
                                   int32_t aos_on_device);
@@ -361,10 +363,13 @@ bmx_status bmx_get_bins(bmx_ctx *ctx, int32_t *out_n);
 bmx_status bmx_get_bin_dims(bmx_ctx *ctx, int32_t colony, int32_t out3[3]);
 
 /* --- measurement ------------------------------------------------------- */
-/* enable = 0 turns timing off.  enable = k > 0 brackets every rebuild and
- * one force launch in k (1: every launch) with HIP events on the context's
- * stream; force_ms / force_launches is then the mean over the timed
- * launches.  bmx_evolve_graph refuses to run while timing is on. */
+/* enable = 0 turns timing off; enable > 0 brackets, with HIP events on the
+ * context's stream, every rebuild and every run of force launches between
+ * two rebuilds (or a rebuild and the end of a bmx_evolve call).  Events sit
+ * only at the ends of runs, so force_ms / force_launches is the launches'
+ * mean duration without per-launch event overhead, and rebuild_ms plus
+ * force_ms covers the steps' device time.  bmx_evolve_graph refuses to run
+ * while timing is on. */
 bmx_status bmx_set_timing(bmx_ctx *ctx, int32_t enable);
 bmx_status bmx_get_timing(bmx_ctx *ctx, bmx_timing *out);
 bmx_status bmx_reset_timing(bmx_ctx *ctx);

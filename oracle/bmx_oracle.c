@@ -2551,15 +2551,18 @@ int64_t orc_fusion(orc_particle *p, int64_t n, int64_t capacity,
   g_pair_wa = fpar[ORC_F_WA];
   *fused = 0;
   int64_t result = n;
-  /* tip fusion: neighbours are non-tips, which this pass leaves alone */
+  /* tip fusion: every tip tests its neighbours as they stood when the pass
+   * began (a tip that fuses turns interior; a later tip must not see it as
+   * a target), so the neighbours are read from a snapshot */
   if (orc_build_neighbors(p, n, g, nbr_mode, nbr_radius, off, &idx)) {
     result = -1;
     goto done;
   }
+  memcpy(snap, p, sizeof(orc_particle) * (size_t)n);
   for (int64_t i = 0; i < n; i++) {
     orc_draws r = {seed, call, p[i].id, p[i].cpu, 0};
     for (int64_t k = off[i]; k < off[i + 1]; k++) {
-      const orc_particle *pj = &p[idx[k]];
+      const orc_particle *pj = &snap[idx[k]];
       double d[3];
       orc_pair_separation(p[i].pos, pj->pos, g, d);
       double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];

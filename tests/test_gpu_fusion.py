@@ -70,3 +70,40 @@ def test_no_fusion_changes_nothing(bmx, oracle, gpu):
     assert not fused and added == 0
     assert np.array_equal(grecs, before)
     ctx.close()
+
+
+@pytest.mark.parametrize("b_first", [False, True])
+def test_two_tips_match_oracle(bmx, oracle, gpu, b_first):
+    """Two tips in reach of each other and of one segment: decisions are taken
+    against the records as the pass began, on the GPU as in the oracle."""
+    from test_oracle_fusion import two_tip_scene
+    recs = two_tip_scene(oracle, b_first)
+    ctx, g = pair(bmx, oracle, recs)
+    grecs, fused, added = ctx.fusion(FUS, 5, 3, 10**6, 0)
+    out, ofused = oracle.fusion(recs, g, FPAR, oracle.NBR_PAIR, SKIN, FUS, 5, 3, 10**6, 0)
+    assert fused and ofused and added == 1 and len(out) == len(recs) + 1
+    assert np.array_equal(grecs, out)
+    ctx.close()
+
+
+def test_fusion_mid_period_keeps_skin_check_clean(bmx, oracle, gpu):
+    """Verlet lists every 10 steps: 9 steps, a fusion call that fuses nothing
+    (it rebuilds the rows), 1 step.  The displacement record restarts at that
+    rebuild, so the call succeeds, and the positions follow the oracle."""
+    recs = scene(oracle, gap=7e-4)
+    ctx = bmx.Context(0)
+    ctx.set_force_params(FPAR)
+    ctx.set_geometry(LO, HI, (0, 0, 0))
+    ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=10, skin=SKIN,
+                            max_neighbors=256)
+    ctx.bind(recs.copy())
+    g = oracle.make_geom(LO, HI, (0, 0, 0), ctx.bin_dims())
+    ctx.evolve(0.01, 9)
+    _, fused, added = ctx.fusion(FUS, 5, 3, 10**6, 0)
+    assert not fused and added == 0
+    ctx.evolve(0.01, 1)
+    orecs = recs.copy()
+    st = oracle.State(orecs, g, np.array(FPAR), oracle.NBR_PAIR, SKIN, 10)
+    st.evolve(0.01, 10)
+    np.testing.assert_allclose(ctx.positions(), orecs["pos"], rtol=0, atol=1e-16)
+    ctx.close()

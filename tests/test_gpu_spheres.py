@@ -264,3 +264,35 @@ def test_one_million_spheres_10_steps_bit_exact(bmx, oracle, gpu):
     og, ig = ctx.neighbors()
     oo, io = st.neighbors()
     assert np.array_equal(og, oo) and np.array_equal(ig, io)
+
+
+def test_failed_rebind_leaves_context_unbound(bmx, gpu, c1):
+    """A bind that fails (here: a record with a non-finite position, found
+    after the arrays were resized for it) leaves no particles bound: evolve
+    then reports STATE instead of running kernels over rows built for the
+    previous, smaller particle set.  A later good bind works again."""
+    ctx, _ = gpu_ctx(bmx, c1)
+    ctx.evolve(W.DT, 3)
+    big = np.concatenate([c1.recs, c1.recs, c1.recs]).astype(bmx.RECORD_DTYPE)
+    big["pos"][-1, 0] = np.nan
+    with pytest.raises(bmx.BMXError) as e:
+        ctx.bind(big)
+    assert e.value.code == 1                          # BAD_ARG
+    for call in (lambda: ctx.evolve(W.DT, 1), lambda: ctx.evolve_graph(W.DT, 2),
+                 lambda: ctx.fc()):
+        with pytest.raises(bmx.BMXError) as e:
+            call()
+        assert e.value.code == 7                      # STATE
+    # bad geometry fails before any state is read, and also unbinds
+    ctx2, _ = gpu_ctx(bmx, c1)
+    ctx2.evolve(W.DT, 1)
+    with pytest.raises(bmx.BMXError):
+        ctx2.bind_colonies(c1.recs.copy(), [0, len(c1.recs)],
+                           [bmx.make_geometry(c1.prob_hi, c1.prob_lo, c1.periodic)])
+    with pytest.raises(bmx.BMXError) as e:
+        ctx2.evolve(W.DT, 1)
+    assert e.value.code == 7
+    ctx2.close()
+    ctx.bind(c1.recs.copy())
+    ctx.evolve(W.DT, 1)
+    ctx.close()

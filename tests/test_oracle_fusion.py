@@ -137,3 +137,38 @@ def test_fusion_rate(oracle):
     out, fused = run(oracle, recs, fus=[0.4, 6e-4, 1e-3, 35e-4], seed=9)
     k = len(out) - len(recs)
     assert fused and abs(k - 0.4 * n) < 4.5 * math.sqrt(n * 0.4 * 0.6)
+
+
+def two_tip_scene(oracle, b_first=False):
+    """The scene above plus a second tip b (30) along +x whose free end sits
+    1e-5 from the middle of tip i's side, bonded at its site 1 to an interior
+    segment (31).  Every segment is within reach of every other.  At the
+    start of the pass i is a tip, so b has no target: only i fuses (into j).
+    A pass that let b see i after i turned interior would fuse b into i when
+    i comes first in record order; `b_first` puts b before i."""
+    X = (math.pi / 2, 0.0)
+    base = scene(oracle)
+    recs = np.zeros(7, dtype=oracle.RECORD_DTYPE)
+    recs[:5] = base
+    tip_len, blen = 20e-4, 20e-4
+    yb = -1e-5 - 0.5 * tip_len            # the middle of i along y
+    xb = -6e-4 - 1e-5 - 0.5 * blen        # b's +x end 1e-5 short of i's axis
+    seg(recs, 5, (xb, yb, 0.0), blen, *X, 30, 0)
+    seg(recs, 6, (xb - blen, yb, 0.0), blen, *X, 31, 3)
+    bond(recs, 5, 6, 1, 2)
+    if b_first:
+        recs = recs[[5, 6, 0, 1, 2, 3, 4]].copy()
+    return recs
+
+
+@pytest.mark.parametrize("b_first", [False, True])
+def test_tips_see_the_records_as_the_pass_began(oracle, b_first):
+    recs = two_tip_scene(oracle, b_first)
+    out, fused = run(oracle, recs)
+    assert fused and len(out) == 8           # j split once: only tip i fused
+    by_id = {int(r["idata"][33]): r for r in out}
+    i, b = by_id[20], by_id[30]
+    assert i["idata"][19] == 3 and i["idata"][20] == 1 and bonds(i) == {(21, 1), (10, 2)}
+    # b stays a tip bonded only to 31, with no fusion recorded
+    assert b["idata"][19] == 0 and b["idata"][20] == 0 and bonds(b) == {(31, 1)}
+    assert b["idata"][25] == 0 and b["rdata"][24] == 1.0
