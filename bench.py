@@ -398,6 +398,34 @@ def _hip_runtime():
         return C.CDLL("libamdhip64.so")
 
 
+def copy_probe(sizes_mib=(64, 1024), reps: int = 20) -> dict:
+    """Device-to-device copy rate of this GPU (read + write bytes per second,
+    wall clock over `reps` back-to-back copies): the HBM rate a streaming
+    kernel can reach here, and at 64 MiB what the 256 MB Infinity Cache adds
+    when the data fits in it."""
+    import ctypes as C
+    hip = _hip_runtime()
+    out = {}
+    for mib in sizes_mib:
+        nbytes = mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+        assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+        for _ in range(3):
+            assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        el = time.perf_counter() - t0
+        out[f"copy_{mib}MiB_GBs"] = 2.0 * nbytes * reps / el / 1e9
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return out
+
+
 def calibration_copy(nbytes: int) -> None:
     """One device-to-device copy of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written) for scaling the PMC byte counters."""
@@ -460,6 +488,7 @@ def main() -> int:
                                     cpu_steps=0 if args.no_cpu else 10)))
         return 0
 
+    probe = copy_probe()
     col = W.c2_large(args.n)
     if args.skin > 0:
         col.skin = args.skin
@@ -474,11 +503,17 @@ def main() -> int:
     ctx.set_launch_params(args.block, args.remap)
     recs = col.recs.copy()
     ctx.bind(recs)
-    # Warm-up: `warmup` steps, then on to the next rebuild (the first step
-    # after a bind rebuilds, then every K steps), so the timed window starts
-    # with a rebuild and holds ceil(steps / K) of them.
+    # Warm-up: at least `warmup` steps, in whole rebuild periods (the first
+    # step after a bind rebuilds, then every K steps), so the timed window
+    # starts with a rebuild and holds ceil(steps / K) of them.
     warm = -(-args.warmup // K) * K
     ctx.evolve(col.dt, warm)
+    # and whole periods more until the device has been busy for 0.3 s, so
+    # the window does not run at idle clocks
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < 0.3:
+        ctx.evolve(col.dt, K)
+        warm += K
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.evolve(col.dt, args.steps)  # synchronizes and checks skin/overflow/NaN
@@ -555,6 +590,7 @@ def main() -> int:
             "algorithmic_bytes_per_particle": algorithmic_bytes_per_particle(mean_row),
             "algorithmic_bytes_per_launch": alg_bytes,
             "survey_bytes_per_particle_step": survey_bytes_per_particle_step(mean_row, K),
+            "copy_probe": probe,
         },
     }
     ctx.close()

@@ -400,8 +400,11 @@ bmx_status close_run(bmx_ctx *c) {
 // in force, the displacements since that list was built are checked against
 // skin/2 first and their partials cleared, so every caller that rebuilds
 // (steps, exchange, fusion, InitBonds) starts the next period from zero.
-// check_skin = false is for passes that repeat checked arithmetic.
-bmx_status rebuild(bmx_ctx *c, bool check_skin = true) {
+// check_skin = false is for passes that repeat checked arithmetic.  The
+// force and adjoint passes read the flat rows only; cols = true also writes
+// the column-major rows of a sphere context, for the passes that walk rows
+// that way (segment contexts always have them).
+bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
   hipStream_t s = c->stream;
   if (c->timing) {
     bmx_status st = close_run(c);
@@ -429,9 +432,9 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true) {
   c->count_clear = true;
   HIPCHK(launch_bin_scatter(c->d_keytmp.p, c->d_rank.p, c->d_start.p, ss.oid,
                             c->n, c->d_permk.p, s));
-  HIPCHK(launch_bin_sort(c->d_start.p, c->nbins, c->d_permk.p, c->d_err.p, s));
-  HIPCHK(launch_gather(c->d_permk.p, c->n, ss, ds, sd, dd, c->d_keytmp.p,
-                       multi, c->any_fungi ? 1 : 0, s));
+  HIPCHK(launch_sort_gather(c->d_permk.p, c->d_start.p, c->n, ss, ds, sd, dd,
+                            c->d_keytmp.p, multi, c->any_fungi ? 1 : 0,
+                            c->d_err.p, s));
   c->cs = 1 - c->cs;
   c->cd = 1 - c->cd;
   StaticArrays st = static_arrays(c, c->cs);
@@ -441,7 +444,8 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true) {
                           c->n, c->nbp.mode == BMX_NBR_STENCIL ? 1 : 0,
                           c->crit, c->fpar[0], c->nbp.skin, c->d_nbr.p,
                           c->d_ncnt.p, c->ncap, c->maxn, c->d_err.p,
-                          c->d_bstats.p, flat_rows(c), shape(c), s));
+                          c->d_bstats.p, flat_rows(c), cols ? 1 : 0, shape(c),
+                          s));
   c->row_nblk = row_blocks(c->n, c->lp.block_size);
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, s));
@@ -1354,7 +1358,7 @@ bmx_status bmx_get_neighbors(bmx_ctx *c, int64_t *offsets, int32_t *indices,
     HIPCHK(out.alloc((size_t)std::max<int64_t>(*total, 1)));
     HIPCHK(hipMemcpyAsync(off.p, offsets, sizeof(int64_t) * (c->n + 1),
                           hipMemcpyHostToDevice, s));
-    HIPCHK(launch_export_nbr(c->d_nbr.p, c->d_ncnt.p, c->ncap,
+    HIPCHK(launch_export_nbr(flat_rows(c), c->maxn, c->d_ncnt.p,
                              c->oid[c->cs].p, c->n, off.p, out.p, s));
     if (*total)
       HIPCHK(hipMemcpyAsync(indices, out.p, sizeof(int32_t) * (*total),
@@ -1454,7 +1458,7 @@ extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) 
     c->have_chem = true;
   }
   // the rows are rebuilt at the current positions (xchng.cpp:68-77)
-  st = rebuild(c);
+  st = rebuild(c, true, true);
   if (st != BMX_OK) return st;
   HIPCHK(launch_exchange(c->pos[c->cd].p, c->type[c->cs].p, c->topo[c->cd].p,
                          c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
@@ -2004,7 +2008,7 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   // records and rows at the current positions (EvaluateTipFusion rebuilds)
   st = pack_records(c);
   if (st != BMX_OK) return st;
-  st = rebuild(c);
+  st = rebuild(c, true, true);
   if (st != BMX_OK) return st;
   st = check_flags(c);
   if (st != BMX_OK) return st;
@@ -2063,7 +2067,7 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   if (st != BMX_OK) return st;
   grown.release();
   // cleanup over rows rebuilt with the new segments
-  st = rebuild(c);
+  st = rebuild(c, true, true);
   if (st != BMX_OK) return st;
   st = check_flags(c);
   if (st != BMX_OK) return st;

@@ -41,7 +41,7 @@ hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            int64_t ncap, double mob, int wrote_step,
                            int any_fungi, hipStream_t s);
 
-// ---- binning: key/rank, scan, scatter, per-bin order, gather
+// ---- binning: key/rank, scan, scatter, per-bin order with the gather
 hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
                             const BoxGeom *boxes, BoxGeom g0, int multi,
                             int64_t n, int32_t *key, int32_t *rank,
@@ -54,17 +54,19 @@ hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
                               const int32_t *start, const int32_t *oid,
                               int64_t n, unsigned long long *permk,
                               hipStream_t s);
-hipError_t launch_bin_sort(const int32_t *start, int64_t nb,
-                           unsigned long long *permk, int32_t *err,
-                           hipStream_t s);
-hipError_t launch_gather(const unsigned long long *permk, int64_t n,
-                         StaticArrays src_s, StaticArrays dst_s,
-                         DynArrays src_d, DynArrays dst_d,
-                         const int32_t *key, int multi, int rods,
-                         hipStream_t s);
+// Orders each bin by original index and gathers the particles into that
+// order (src -> dst buffers); key: each old position's bin.
+hipError_t launch_sort_gather(const unsigned long long *permk,
+                              const int32_t *start, int64_t n,
+                              StaticArrays src_s, StaticArrays dst_s,
+                              DynArrays src_d, DynArrays dst_d,
+                              const int32_t *key, int multi, int rods,
+                              int32_t *err, hipStream_t s);
 
-// ---- neighbor list (column-major rows: nbr[k*ncap + i]; ncnt holds the row
-// length, bit 30 set for particles that need no periodic image test)
+// ---- neighbor list: flat rows (FlatRows) always; column-major rows
+// nbr[k*ncap + i] for segment contexts, and for sphere contexts only when
+// cols = 1 (the passes that walk rows that way: exchange, fusion); ncnt
+// holds the row length, bits 28..30 the axes that may need a periodic image
 // mode 0: r2 <= (reach_i + reach_j + wa + skin)^2 (reach = radius, or 1.5 *
 // c_length for segments); mode 1: r2 <= crit (the neighborhood value)
 hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
@@ -74,8 +76,8 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, FlatRows flat, LaunchShape ls,
-                            hipStream_t s);
+                            int32_t *bstats, FlatRows flat, int cols,
+                            LaunchShape ls, hipStream_t s);
 // Particles per tile of the flat row layout: 64 for spheres, 32 for
 // segment (and mixed) contexts.
 int flat_tile(int segments);
@@ -132,8 +134,8 @@ hipError_t launch_export_orient(const double4 *rod, const int32_t *oid,
                                 int64_t n, double *out, hipStream_t s);
 hipError_t launch_export_i32(const int32_t *v, const int32_t *oid, int64_t n,
                              int32_t *out, hipStream_t s);
-hipError_t launch_export_nbr(const int32_t *nbr, const int32_t *ncnt,
-                             int64_t ncap, const int32_t *oid, int64_t n,
+hipError_t launch_export_nbr(FlatRows flat, int32_t maxn, const int32_t *ncnt,
+                             const int32_t *oid, int64_t n,
                              const int64_t *row_off /* by original index */,
                              int32_t *out, hipStream_t s);
 hipError_t launch_export_counts(const int32_t *ncnt, const int32_t *oid,

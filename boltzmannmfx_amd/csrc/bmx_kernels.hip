@@ -374,45 +374,38 @@ __global__ void k_bin_scatter(const int32_t *__restrict__ key,
                (unsigned long long)(uint32_t)s;
 }
 
-// Orders each bin by original index (the high word of permk).  A bin of more
-// than kMaxBinSort particles is left unsorted and reported as an overflow
-// (its rows could not fit anyway), which bounds the work of every thread;
-// after a non-finite force nothing is sorted.
+// Orders each bin by original index and gathers the per-particle arrays
+// into that order in one pass.  Thread j takes slot j of the scattered
+// (bin-grouped, not yet ordered) entries (original index << 32 | old
+// position): its place in its bin is the number of the bin's entries below
+// it (the original indexes are distinct), and it copies its particle from
+// the old position to that place.  A bin of more than kMaxBinSort
+// particles is left in scatter order and reported as an overflow (its rows
+// could not fit anyway), which bounds the work of every thread; after a
+// non-finite force nothing is ordered.
 constexpr int64_t kMaxBinSort = 1024;
-__global__ void k_bin_sort(const int32_t *__restrict__ start, int64_t nb,
-                           unsigned long long *__restrict__ permk,
-                           int32_t *__restrict__ err) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  if (*err & ERRBIT_NAN) return;
-  int64_t a = start[b], e = start[b + 1];
-  if (e - a > kMaxBinSort) {
-    atomicOr(err, ERRBIT_OVERFLOW);
-    return;
+__global__ void k_sort_gather(const unsigned long long *__restrict__ permk,
+                              const int32_t *__restrict__ start, int64_t n,
+                              StaticArrays ss, StaticArrays ds, DynArrays sd,
+                              DynArrays dd, const int32_t *__restrict__ key,
+                              int multi, int rods, int32_t *__restrict__ err) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const unsigned long long e = permk[j];
+  const int64_t s = (int64_t)(uint32_t)(e & 0xffffffffull);
+  const int32_t b = key[s];
+  const int64_t a = start[b], z = start[b + 1];
+  int64_t t = j;
+  if (z - a > kMaxBinSort) {
+    if (j == a) atomicOr(err, ERRBIT_OVERFLOW);
+  } else if (!(*err & ERRBIT_NAN)) {
+    t = a;
+    for (int64_t i = a; i < z; i++) t += permk[i] < e;
   }
-  for (int64_t i = a + 1; i < e; i++) {
-    unsigned long long v = permk[i];
-    int64_t j = i - 1;
-    while (j >= a && permk[j] > v) {
-      permk[j + 1] = permk[j];
-      j--;
-    }
-    permk[j + 1] = v;
-  }
-}
-
-__global__ void k_gather(const unsigned long long *__restrict__ permk,
-                         int64_t n, StaticArrays ss, StaticArrays ds,
-                         DynArrays sd, DynArrays dd,
-                         const int32_t *__restrict__ key, int multi,
-                         int rods) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t s = (int64_t)(uint32_t)(permk[t] & 0xffffffffull);
   dd.pos[t] = sd.pos[s];
   ds.oid[t] = ss.oid[s];
   ds.type[t] = ss.type[s];
-  ds.key[t] = key[s];
+  ds.key[t] = b;
   if (multi) ds.col[t] = ss.col[s];
   if (rods) {
     ds.bscale[t] = ss.bscale[s];
@@ -487,38 +480,115 @@ __device__ __forceinline__ void scan_range(
 }
 
 
-// Candidate range q of a particle's bin stencil, in the canonical order
-// (oz, then oy, then ox): with nx = 1 (the particle's bin at least one bin
-// from both x faces) one range of three x bins per (oz, oy) row, with nx = 3
-// the x bins one at a time (periodic images, or clipped at the faces).  A
-// bin outside a non-periodic box gives an empty range.
-__device__ __forceinline__ int2 stencil_range(int q, int nx, int bx, int by,
-                                              int bz, const BoxGeom &g,
-                                              const int32_t *__restrict__ start) {
-  const int r = nx == 1 ? q : q / 3;
-  const int oz = r / 3 - 1, oy = r % 3 - 1;
-  int cz = bz + oz, cy = by + oy;
+// A particle's bin stencil as memory ranges of the sorted arrays, in the
+// canonical order (oz, then oy, then ox).  Each (oz, oy) row of bins is one
+// or two ranges: the x window [bx - 1, bx + 1] is contiguous in memory
+// (bins are x-fastest) unless it wraps a periodic x face, where it splits
+// into the bins before the face and the bins after it (piece 0 holds the
+// bins of ox = -1 first).  Rows outside a non-periodic box are empty.
+// Everything that does not change from range to range (the three z and
+// three y row offsets, the x pieces) is formed once per particle, so moving
+// to the next range costs a few selects and the two loads of its bounds.
+struct Stencil {
+  // (cz * nby) of oz = -1, 0, 1 and cy of oy = -1, 0, 1, or -1 outside the
+  // box (named fields, not arrays: a select between array elements becomes
+  // an indexed scratch access)
+  int z0, z1, z2, y0, y1, y2;
+  int xa0, xb0, xa1, xb1;  // x pieces [xa, xb) of every row
+  int two;     // 1 when a row has two pieces
+  int base;    // bin id of the box's first bin
+  int nbx;
+};
+
+__device__ __forceinline__ Stencil make_stencil(int bx, int by, int bz,
+                                                const BoxGeom &g) {
+  Stencil st;
   const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
-  bool ok = true;
-  if (g.periodic[2]) cz = (cz + nbz) % nbz;
-  else ok = cz >= 0 && cz < nbz;
-  if (g.periodic[1]) cy = (cy + nby) % nby;
-  else ok = ok && cy >= 0 && cy < nby;
-  int lo = bx - 1, hi = bx + 1;
-  if (nx != 1) {
-    int cx = bx + (q - 3 * r) - 1;
-    if (g.periodic[0]) cx = (cx + nbx) % nbx;
-    else ok = ok && cx >= 0 && cx < nbx;
-    lo = hi = cx;
+  auto zrow = [&](int cz) {
+    if (g.periodic[2]) cz = cz < 0 ? cz + nbz : (cz >= nbz ? cz - nbz : cz);
+    return (cz >= 0 && cz < nbz) ? cz * nby : -1;
+  };
+  auto ycol = [&](int cy) {
+    if (g.periodic[1]) cy = cy < 0 ? cy + nby : (cy >= nby ? cy - nby : cy);
+    return (cy >= 0 && cy < nby) ? cy : -1;
+  };
+  st.z0 = zrow(bz - 1);
+  st.z1 = zrow(bz);
+  st.z2 = zrow(bz + 1);
+  st.y0 = ycol(by - 1);
+  st.y1 = ycol(by);
+  st.y2 = ycol(by + 1);
+  const int lo = bx - 1, hi = bx + 2;  // [lo, hi) before any wrap
+  st.xa1 = st.xb1 = 0;
+  if (lo < 0) {
+    if (g.periodic[0]) {
+      st.xa0 = nbx - 1;
+      st.xb0 = nbx;
+      st.xb1 = hi < nbx ? hi : nbx;
+    } else {
+      st.xa0 = 0;
+      st.xb0 = hi < nbx ? hi : nbx;
+    }
+  } else if (hi > nbx) {
+    st.xa0 = lo;
+    st.xb0 = nbx;
+    if (g.periodic[0]) st.xb1 = hi - nbx;
+  } else {
+    st.xa0 = lo;
+    st.xb0 = hi;
   }
-  if (!ok) return make_int2(0, 0);
-  const int row = g.bin_offset + (cz * nby + cy) * nbx;
-  return make_int2(start[row + lo], start[row + hi + 1]);
+  st.two = st.xb1 > st.xa1;
+  st.base = g.bin_offset;
+  st.nbx = nbx;
+  return st;
+}
+
+// Memory range of range k of the stencil (k < 9 << two).  The field
+// choices are value selects written as arithmetic: a conditional between
+// two fields of one object can become a select of their addresses, which
+// puts the object in scratch memory.
+__device__ __forceinline__ int2 stencil_range(const Stencil st, int k,
+                                              const int32_t *__restrict__ start) {
+  const int r = st.two ? (k >> 1) : k;
+  const int piece = st.two ? (k & 1) : 0;
+  const int iz = r / 3, iy = r - 3 * iz;
+  const int zz = (iz == 0) * st.z0 + (iz == 1) * st.z1 + (iz == 2) * st.z2;
+  const int yy = (iy == 0) * st.y0 + (iy == 1) * st.y1 + (iy == 2) * st.y2;
+  if (zz < 0 || yy < 0) return make_int2(0, 0);
+  const int row = st.base + (zz + yy) * st.nbx;
+  const int xa = st.xa0 + piece * (st.xa1 - st.xa0);
+  const int xb = st.xb0 + piece * (st.xb1 - st.xb0);
+  return make_int2(start[row + xa], start[row + xb]);
+}
+
+// Where a sphere build puts a particle's row while it grows: the first
+// kStage entries in LDS (entry k at stage[k * stride], one word per thread
+// and entry, so the lanes of a wave write neighbouring banks), the rest in
+// the column-major row (entry k at col[k * ncap]).  With cols, every entry
+// also goes to the column-major row, for the passes that read rows that way
+// (exchange, fusion).  The flat rows are copied from here at the end.
+constexpr int kStage = 8;
+struct RowSink {
+  int32_t *stage;
+  int stride;
+  int32_t *col;
+  int64_t ncap;
+  int cols;
+};
+__device__ __forceinline__ void row_put(const RowSink &sk, int k, int32_t u) {
+  if (k < kStage) {
+    sk.stage[k * sk.stride] = u;
+    if (sk.cols) sk.col[(int64_t)k * sk.ncap] = u;
+  } else {
+    sk.col[(int64_t)k * sk.ncap] = u;
+  }
+}
+__device__ __forceinline__ int32_t row_get(const RowSink &sk, int k) {
+  return k < kStage ? sk.stage[k * sk.stride] : sk.col[(int64_t)k * sk.ncap];
 }
 
 // Scans a particle's stencil as one sequence of candidates and appends the
-// accepted ones to its row (out walks the column-major row, one stride of
-// ncap per entry).  Each lane moves on to its next range as soon as its
+// accepted ones to its row (through the row sink).  Each lane moves on to its
 // current one ends, so a wave takes as many iterations as its longest
 // stencil rather than the sum over ranges of the longest range; the next
 // range's bounds are loaded one range ahead.  Candidates visited are capped
@@ -529,15 +599,15 @@ __device__ __forceinline__ int2 stencil_range(int q, int nx, int bx, int by,
 // candidate's reach is its radius.
 template <int M>
 __device__ __forceinline__ void scan_stencil(
-    int nx, int bx, int by, int bz, const BoxGeom &g,
-    const int32_t *__restrict__ start, int64_t scan_cap, int t,
-    const double4 &p, const double4 *__restrict__ pos, int mode, double crit,
-    double was, int32_t *&out, int64_t ncap, int32_t maxn, int &cnt) {
-  const int nq = 9 * nx;
+    const Stencil sc, const BoxGeom &g, const int32_t *__restrict__ start,
+    int64_t scan_cap, int t, const double4 &p,
+    const double4 *__restrict__ pos, int mode, double crit, double was,
+    const RowSink &sk, int32_t maxn, int &cnt) {
+  const int nq = 9 << sc.two;
   int64_t scanned = 0;
   int q = 0;
-  const int2 first = stencil_range(0, nx, bx, by, bz, g, start);
-  int2 next = stencil_range(1, nx, bx, by, bz, g, start);
+  const int2 first = stencil_range(sc, 0, start);
+  int2 next = stencil_range(sc, 1, start);
   int u = first.x, e = first.y;
   scanned = e - u;
   if (scanned > scan_cap) {
@@ -555,7 +625,7 @@ __device__ __forceinline__ void scan_stencil(
         e = u;
       }
       scanned += len;
-      if (q + 1 < nq) next = stencil_range(q + 1, nx, bx, by, bz, g, start);
+      if (q + 1 < nq) next = stencil_range(sc, q + 1, start);
     }
     if (u >= e) break;
     const double4 o = pos[u];
@@ -571,10 +641,7 @@ __device__ __forceinline__ void scan_stencil(
       ok = r2 <= cut * cut;
     }
     if (ok && u != t) {
-      if (cnt < maxn) {
-        *out = u;
-        out += ncap;
-      }
+      if (cnt < maxn) row_put(sk, cnt, u);
       cnt++;
     }
     u++;
@@ -606,11 +673,16 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
     int64_t n, int mode, double crit, double wa, double skin,
     int32_t *__restrict__ nbr, int32_t *__restrict__ ncnt, int64_t ncap,
     int32_t maxn, int32_t *__restrict__ err, int32_t *__restrict__ bstats,
-    FlatRows flat, int remap) {
+    FlatRows flat, int cols, int remap) {
   const int64_t t = tile_of(remap) * blockDim.x + threadIdx.x;
   const int ti = (int)t;  // n < 2^31
   const double was = wa + skin;
   int32_t *out = nbr + t;
+  // sphere builds stage the first entries of each row in LDS (kStage words
+  // per thread, launched with that much dynamic LDS)
+  extern __shared__ int32_t srow[];
+  const RowSink sk{srow + threadIdx.x, (int)blockDim.x, nbr + t, ncap,
+                   (cols || !flat.cidx) ? 1 : 0};
   int cnt = 0;
   // Candidates visited per particle are capped (a pile of particles in a few
   // bins would otherwise make this quadratic); hitting the cap is reported
@@ -708,16 +780,16 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
           }
         }
       } else {
-        const int nx = (bx >= 1 && bx + 1 < nbx) ? 1 : 3;
+        const Stencil sc = make_stencil(bx, by, bz, g);
         if (wave_img == 0)
-          scan_stencil<0>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                          crit, was, out, ncap, maxn, cnt);
+          scan_stencil<0>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
+                          sk, maxn, cnt);
         else if (wave_img == 1)
-          scan_stencil<1>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                          crit, was, out, ncap, maxn, cnt);
+          scan_stencil<1>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
+                          sk, maxn, cnt);
         else
-          scan_stencil<7>(nx, bx, by, bz, g, start, scan_cap, ti, p, pos, mode,
-                          crit, was, out, ncap, maxn, cnt);
+          scan_stencil<7>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
+                          sk, maxn, cnt);
       }
       ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
       if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
@@ -747,7 +819,7 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
     if (place == 0 && t < n) flat.tot[t / tile] = tot;
     const int64_t b = (t / tile) * (int64_t)tile * maxn + (incl - kept);
     for (int k = 0; k < kept; k++) {
-      flat.cidx[b + k] = nbr[t + (int64_t)k * ncap];
+      flat.cidx[b + k] = RODS ? nbr[t + (int64_t)k * ncap] : row_get(sk, k);
       flat.cown[b + k] = (uint8_t)place;
     }
   }
@@ -1453,10 +1525,18 @@ __global__ __launch_bounds__(64) void k_skin_check(double *__restrict__ vmax,
                                                    int32_t *__restrict__ err) {
   const int l = threadIdx.x;
   double total = 0.0;
-  for (int sl = 0; sl < nslots; sl++) {
-    double *w = vmax + (int64_t)sl * kVmaxWords;
-    total += wave_max_f64(w[l]);
-    if (clear) w[l] = 0.0;
+  // four slots' loads in flight at a time; a missing slot adds 0.0, which
+  // leaves the sum's bits unchanged
+  for (int sl = 0; sl < nslots; sl += 4) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      v[k] = sl + k < nslots ? vmax[(int64_t)(sl + k) * kVmaxWords + l] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      total += wave_max_f64(v[k]);
+      if (clear && sl + k < nslots) vmax[(int64_t)(sl + k) * kVmaxWords + l] = 0.0;
+    }
   }
   if (l == 0 && total > limit) atomicOr(err, ERRBIT_SKIN);
 }
@@ -1505,16 +1585,21 @@ __global__ void k_export_counts(const int32_t *__restrict__ ncnt,
   if (t >= n) return;
   out[oid[t]] = row_len(ncnt[t]);
 }
-__global__ void k_export_nbr(const int32_t *__restrict__ nbr,
-                             const int32_t *__restrict__ ncnt, int64_t ncap,
+// Rows from the flat layout: particle t's row starts after the rows of the
+// particles before it in its tile.
+__global__ void k_export_nbr(FlatRows flat, int32_t maxn,
+                             const int32_t *__restrict__ ncnt,
                              const int32_t *__restrict__ oid, int64_t n,
                              const int64_t *__restrict__ row_off,
                              int32_t *__restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  const int64_t t0 = t / flat.tile * flat.tile;
+  int64_t b = t0 * maxn;
+  for (int64_t u = t0; u < t; u++) b += row_len(ncnt[u]);
   int64_t base = row_off[oid[t]];
   int c = row_len(ncnt[t]);
-  for (int k = 0; k < c; k++) out[base + k] = oid[nbr[(int64_t)k * ncap + t]];
+  for (int k = 0; k < c; k++) out[base + k] = oid[flat.cidx[b + k]];
 }
 
 }  // namespace
@@ -1593,21 +1678,16 @@ hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
                      key, rank, start, oid, n, permk);
   return hipGetLastError();
 }
-hipError_t launch_bin_sort(const int32_t *start, int64_t nb,
-                           unsigned long long *permk, int32_t *err,
-                           hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_sort, dim3(grid_for(nb, 256)), dim3(256), 0, s,
-                     start, nb, permk, err);
-  return hipGetLastError();
-}
-hipError_t launch_gather(const unsigned long long *permk, int64_t n,
-                         StaticArrays src_s, StaticArrays dst_s,
-                         DynArrays src_d, DynArrays dst_d, const int32_t *key,
-                         int multi, int rods, hipStream_t s) {
+hipError_t launch_sort_gather(const unsigned long long *permk,
+                              const int32_t *start, int64_t n,
+                              StaticArrays src_s, StaticArrays dst_s,
+                              DynArrays src_d, DynArrays dst_d,
+                              const int32_t *key, int multi, int rods,
+                              int32_t *err, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, permk,
-                     n, src_s, dst_s, src_d, dst_d, key, multi, rods);
+  hipLaunchKernelGGL(k_sort_gather, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     permk, start, n, src_s, dst_s, src_d, dst_d, key, multi,
+                     rods, err);
   return hipGetLastError();
 }
 
@@ -1618,15 +1698,16 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
                             int64_t n, int mode, double crit, double wa,
                             double skin, int32_t *nbr, int32_t *ncnt,
                             int64_t ncap, int32_t maxn, int32_t *err,
-                            int32_t *bstats, FlatRows flat, LaunchShape ls,
-                            hipStream_t s) {
+                            int32_t *bstats, FlatRows flat, int cols,
+                            LaunchShape ls, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int b = ls.block;
+  const size_t stage = rods ? 0 : sizeof(int32_t) * kStage * (size_t)b;
 #define BMX_BUILD(M, RD)                                                      \
-  hipLaunchKernelGGL((k_build_nbr<M, RD>), dim3(grid_for(n, b)), dim3(b), 0, s, \
-                     pos, col, type, rod, boxes, g0, start, key, n, mode,     \
-                     crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats, flat, \
-                     ls.remap)
+  hipLaunchKernelGGL((k_build_nbr<M, RD>), dim3(grid_for(n, b)), dim3(b),     \
+                     stage, s, pos, col, type, rod, boxes, g0, start, key, n, \
+                     mode, crit, wa, skin, nbr, ncnt, ncap, maxn, err, bstats, \
+                     flat, cols, ls.remap)
   if (multi && rods)
     BMX_BUILD(true, true);
   else if (multi)
@@ -1772,13 +1853,13 @@ hipError_t launch_export_counts(const int32_t *ncnt, const int32_t *oid,
                      ncnt, oid, n, out);
   return hipGetLastError();
 }
-hipError_t launch_export_nbr(const int32_t *nbr, const int32_t *ncnt,
-                             int64_t ncap, const int32_t *oid, int64_t n,
+hipError_t launch_export_nbr(FlatRows flat, int32_t maxn, const int32_t *ncnt,
+                             const int32_t *oid, int64_t n,
                              const int64_t *row_off, int32_t *out,
                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_export_nbr, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                     nbr, ncnt, ncap, oid, n, row_off, out);
+                     flat, maxn, ncnt, oid, n, row_off, out);
   return hipGetLastError();
 }
 
