@@ -58,6 +58,10 @@ FPAR_NAMES = (
 )
 
 NBR_VERLET, NBR_STENCIL = 0, 1
+# limits of bmx_evolve_aos_noalloc (include/bmx_mi355x.h)
+BMX_NOALLOC_MAX_PARTICLES = 32768
+BMX_NOALLOC_MAX_NEIGHBORS = 64
+BMX_NOALLOC_MAX_REBUILD_EVERY = 64
 DEPOSIT_ONE_TO_ONE, DEPOSIT_TRILINEAR = 0, 1
 
 STATUS = {
@@ -198,6 +202,9 @@ def load_library() -> C.CDLL:
         "bmx_write_checkpoint": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout),
                                            P(Geometry), C.c_int32, C.c_int64]),
         "bmx_autotune": (C.c_int, [vp, C.c_double, C.c_int32, C.c_char_p, P(Tuning)]),
+        "bmx_evolve_aos_noalloc": (C.c_int, [vp, C.c_int64, P(Layout), P(Geometry),
+                                             P(C.c_double), P(NeighborParams),
+                                             C.c_double, C.c_int32, vp]),
         "bmx_read_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64,
                                                P(C.c_int64), P(Layout)]),
         "bmx_write_particles_ascii": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout)]),
@@ -238,6 +245,24 @@ def _check(code: int) -> None:
     if code != 0:
         msg = load_library().bmx_last_error().decode(errors="replace")
         raise BMXError(code, msg)
+
+
+def evolve_aos_noalloc(aos_device: int, n: int, geometry: Geometry, fpar,
+                       dt: float, nsteps: int, mode=NBR_VERLET, rebuild_every=10,
+                       skin=0.0, max_neighbors=0, bin_factor_pct=100,
+                       neighborhood=1.0, stream: int = 0) -> None:
+    """bmx_evolve_aos_noalloc: bind + evolve + sync on n device records at
+    address aos_device, in place, with no device allocation."""
+    lib = load_library()
+    fp = np.ascontiguousarray(fpar, dtype=np.float64)
+    assert fp.shape == (9,)
+    p = NeighborParams(int(mode), int(rebuild_every), int(max_neighbors),
+                       int(bin_factor_pct), float(skin), float(neighborhood))
+    lay = lib.bmx_default_layout()
+    _check(lib.bmx_evolve_aos_noalloc(C.c_void_p(aos_device), int(n), C.byref(lay),
+                                      C.byref(geometry), _dptr(fp), C.byref(p),
+                                      float(dt), int(nsteps),
+                                      C.c_void_p(stream) if stream else None))
 
 
 def make_geometry(prob_lo, prob_hi, periodic, n_cell=(1, 1, 1)) -> Geometry:

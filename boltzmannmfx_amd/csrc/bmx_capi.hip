@@ -49,8 +49,12 @@ template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  // fixed: p is a slice of memory the buffer does not own (the static
+  // scratch of the allocation-free entry point); it never grows or frees
+  bool fixed = false;
   hipError_t alloc(size_t count) {
     if (count <= n && p) return hipSuccess;
+    if (fixed) return hipErrorOutOfMemory;
     release();
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     hipError_t e = hipMalloc((void **)&p, bytes);
@@ -58,6 +62,7 @@ struct DevBuf {
     return e;
   }
   void release() {
+    if (fixed) return;
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
@@ -83,7 +88,7 @@ struct bmx_ctx {
   int32_t ncol = 1;
   std::vector<bmx_geometry> colony_geom;
   std::vector<int64_t> colony_start;
-  std::vector<BoxGeom> boxes;
+  std::vector<BoxGeom> boxes, boxes_next;
   DevBuf<BoxGeom> d_boxes;
   DevBuf<int32_t> d_col_in;
   bool bound = false;
@@ -302,7 +307,8 @@ bmx_status setup_boxes(bmx_ctx *c) {
   }
   const double vr_old = c->verlet_radius;
   c->verlet_radius = vr;  // make_box reads it
-  std::vector<BoxGeom> boxes((size_t)c->ncol);
+  std::vector<BoxGeom> &boxes = c->boxes_next;
+  boxes.resize((size_t)c->ncol);
   int64_t off = 0;
   for (int k = 0; k < c->ncol; k++) {
     bmx_status st = make_box(c, c->colony_geom[k], boxes[k], (int32_t)off);
@@ -322,6 +328,12 @@ bmx_status setup_boxes(bmx_ctx *c) {
   c->nbins = (int32_t)off;
   c->count_clear = false;
   c->graph_step0 = -1;
+  if (c->d_count.fixed && (size_t)c->nbins + 1 > c->d_count.n) {
+    c->bound = false;
+    return fail(BMX_ERR_BAD_ARG, "the binning needs more bins (" +
+                                     std::to_string(c->nbins) +
+                                     ") than the fixed scratch holds");
+  }
   hipError_t e = c->d_boxes.alloc(c->ncol);
   if (e == hipSuccess)
     e = hipMemcpyAsync(c->d_boxes.p, c->boxes.data(), sizeof(BoxGeom) * c->ncol,
@@ -589,7 +601,8 @@ bmx_status pack_records(bmx_ctx *c) {
 // again, so no kernel can run on arrays sized for other particles.
 bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
                        int32_t ncol, const bmx_geometry *geoms,
-                       const bmx_layout *layout, int32_t on_device) {
+                       const bmx_layout *layout, int32_t on_device,
+                       bool in_place = false) {
   reset_graph(c);
   c->bound = false;
   c->have_list = false;
@@ -648,12 +661,18 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   size_t bytes = (size_t)n * (size_t)L.stride_bytes;
   c->aos_on_device = on_device != 0;
   c->user_aos = const_cast<void *>(aos);
-  HIPCHK(c->d_aos.alloc(bytes));
-  if (bytes)
-    HIPCHK(hipMemcpyAsync(c->d_aos.p, aos, bytes,
-                          on_device ? hipMemcpyDeviceToDevice
-                                    : hipMemcpyHostToDevice,
-                          s));
+  if (in_place) {
+    // the caller's device records are the context's record array
+    c->d_aos.p = static_cast<uint8_t *>(const_cast<void *>(aos));
+    c->d_aos.n = bytes;
+  } else {
+    HIPCHK(c->d_aos.alloc(bytes));
+    if (bytes)
+      HIPCHK(hipMemcpyAsync(c->d_aos.p, aos, bytes,
+                            on_device ? hipMemcpyDeviceToDevice
+                                      : hipMemcpyHostToDevice,
+                            s));
+  }
   // Boxes before unpack (wrap needs them); bins are set after the stats.
   c->boxes.resize(ncol);
   for (int k = 0; k < ncol; k++) {
@@ -2079,4 +2098,179 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   st = rebind_records(c, c->d_aos.p, c->n);
   c->user_aos = user;
   return st;
+}
+
+// ------------------------------------------------- allocation-free evolve
+// bmx_evolve_aos_noalloc: bind (device records, in place) + evolve + sync
+// for callers that cannot allocate device memory.  Every working array of
+// the call lives in `scratch`, a file-scope device array of fixed size that
+// each GPU holds from library load to process end.  In host code `scratch`
+// names a host-side stand-in, not device memory: the host uses only the
+// address hipGetSymbolAddress returns for the current device, kept per
+// device.  A per-device context whose buffers are fixed slices of that
+// memory runs the same binding, step and packing code as bmx_evolve; its
+// buffers never grow, so a call that would need more than a slice fails
+// with BMX_ERR_BAD_ARG instead of allocating.  All calls on one GPU share
+// its scratch and context: calls that overlap on one GPU race.
+namespace {
+
+constexpr int64_t kNaParticles = BMX_NOALLOC_MAX_PARTICLES;
+constexpr int32_t kNaNeighbors = BMX_NOALLOC_MAX_NEIGHBORS;
+constexpr int64_t kNaBins = BMX_NOALLOC_MAX_BINS;
+constexpr int32_t kNaRebuild = BMX_NOALLOC_MAX_REBUILD_EVERY;
+constexpr int kNaDevices = 64;
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of each slice of the scratch, in carve order.
+constexpr size_t kNaTiles = (size_t)(kNaParticles + 31) / 32 + 1;  // tiles of 32
+constexpr size_t kNaFlat = (size_t)kNaTiles * 32 * kNaNeighbors + 2 * 64;
+constexpr size_t kNaBytes =
+    2 * 3 * na_align(4 * kNaParticles)           // oid, type, key
+    + 2 * na_align(8 * kNaParticles)             // bscale
+    + 2 * 3 * na_align(32 * kNaParticles)        // pos, rod, trig
+    + 2 * na_align(sizeof(Topo) * kNaParticles)  // topo
+    + 2 * na_align(4 * kNaParticles)             // keytmp, rank
+    + na_align(8 * kNaParticles)                 // permk
+    + na_align(8 * 6 * kNaParticles)             // fc
+    + na_align(4 * kNaNeighbors * kNaParticles)  // nbr (column-major rows)
+    + na_align(4 * kNaParticles)                 // ncnt
+    + na_align(4 * kNaTiles) + na_align(kNaTiles)  // wtot, redo
+    + na_align(4 * kNaFlat) + na_align(kNaFlat)  // cidx, cown
+    + 2 * na_align(4 * (kNaBins + 1))            // count, start
+    + na_align(4 * ((kNaBins + 2047) / 2048 + 2))  // scan tile sums
+    + na_align(8 * 64 * kNaRebuild)              // displacement slots
+    + na_align(4 * 2 * (kNaParticles / 64 + 1))  // row-build block stats
+    + na_align(4) + na_align(16) + na_align(8) + na_align(64)  // err, stats, total, ustats
+    + na_align(sizeof(BoxGeom));
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+struct NaDevice {
+  bool ready = false;
+  uint8_t *ptr = nullptr;  // this device's copy of scratch
+  bmx_ctx ctx;
+  int32_t flags[16];       // host words the error/statistics reads land in
+};
+NaDevice g_na[kNaDevices];
+
+template <typename T>
+void na_slice(DevBuf<T> &b, uint8_t *&cur, size_t count) {
+  b.p = reinterpret_cast<T *>(cur);
+  b.n = count;
+  b.fixed = true;
+  cur += na_align(sizeof(T) * count);
+}
+
+// Once per device, with the device current: the scratch address and the
+// context's fixed buffers over it.
+bmx_status na_setup(NaDevice &d, int dev) {
+  uint8_t *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(BMX_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+  bmx_ctx &c = d.ctx;
+  c.device = dev;
+  uint8_t *cur = ptr;
+  const size_t N = (size_t)kNaParticles;
+  for (int k = 0; k < 2; k++) {
+    na_slice(c.oid[k], cur, N);
+    na_slice(c.type[k], cur, N);
+    na_slice(c.key[k], cur, N);
+    na_slice(c.bscale[k], cur, N);
+    na_slice(c.pos[k], cur, N);
+    na_slice(c.rod[k], cur, N);
+    na_slice(c.trig[k], cur, N);
+    na_slice(c.topo[k], cur, N);
+    c.col[k].fixed = true;  // one box: never used
+  }
+  na_slice(c.d_keytmp, cur, N);
+  na_slice(c.d_rank, cur, N);
+  na_slice(c.d_permk, cur, N);
+  na_slice(c.d_fc, cur, 6 * N);
+  na_slice(c.d_nbr, cur, (size_t)kNaNeighbors * N);
+  na_slice(c.d_ncnt, cur, N);
+  na_slice(c.d_wtot, cur, kNaTiles);
+  na_slice(c.d_redo, cur, kNaTiles);
+  na_slice(c.d_cidx, cur, kNaFlat);
+  na_slice(c.d_cown, cur, kNaFlat);
+  na_slice(c.d_count, cur, (size_t)kNaBins + 1);
+  na_slice(c.d_start, cur, (size_t)kNaBins + 1);
+  na_slice(c.d_scan, cur, (size_t)(kNaBins + 2047) / 2048 + 2);
+  na_slice(c.d_vpart, cur, (size_t)64 * kNaRebuild);
+  na_slice(c.d_bstats, cur, 2 * (N / 64 + 1));
+  na_slice(c.d_err, cur, 1);
+  na_slice(c.d_stats, cur, 4);
+  na_slice(c.d_total, cur, 1);
+  na_slice(c.d_ustats, cur, 8);
+  na_slice(c.d_boxes, cur, 1);
+  if ((size_t)(cur - ptr) > kNaBytes)
+    return fail(BMX_ERR_HIP, "scratch layout exceeds the scratch array");
+  // the record array is the caller's; a colony array is never needed
+  c.d_aos.fixed = true;
+  c.d_col_in.fixed = true;
+  c.h_flags = d.flags;
+  // host vectors reach their sizes here, not in a call
+  c.colony_geom.reserve(1);
+  c.colony_start.reserve(2);
+  c.boxes.reserve(1);
+  c.boxes_next.reserve(1);
+  d.ptr = ptr;
+  d.ready = true;
+  return BMX_OK;
+}
+
+}  // namespace
+
+extern "C" bmx_status bmx_evolve_aos_noalloc(void *aos, int64_t n,
+                                             const bmx_layout *layout,
+                                             const bmx_geometry *geom,
+                                             const double fpar[9],
+                                             const bmx_neighbor_params *nbp,
+                                             double dt, int32_t nsteps,
+                                             void *stream) {
+  if (!geom || !fpar || !nbp) return fail(BMX_ERR_BAD_ARG, "null argument");
+  if (n < 0 || n > kNaParticles)
+    return fail(BMX_ERR_BAD_ARG, "particle count outside 0.." +
+                                     std::to_string(kNaParticles));
+  if (n > 0 && !aos) return fail(BMX_ERR_BAD_ARG, "null record array");
+  if (nsteps < 0 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
+  const int32_t maxn = nbp->max_neighbors > 0 ? nbp->max_neighbors : kNaNeighbors;
+  if (maxn > kNaNeighbors)
+    return fail(BMX_ERR_BAD_ARG, "max_neighbors above " + std::to_string(kNaNeighbors));
+  if (nbp->rebuild_every > kNaRebuild)
+    return fail(BMX_ERR_BAD_ARG, "rebuild_every above " + std::to_string(kNaRebuild));
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaDevices)
+    return fail(BMX_ERR_NO_DEVICE, "no current HIP device");
+  NaDevice &d = g_na[dev];
+  if (!d.ready) {
+    bmx_status st = na_setup(d, dev);
+    if (st != BMX_OK) return st;
+  }
+  bmx_ctx *c = &d.ctx;
+  c->stream = static_cast<hipStream_t>(stream);
+  c->timing = false;
+  // The previous call's binding is void; the configuration calls then only
+  // record their parameters for the bind below, which sizes every buffer
+  // within its fixed slice (growth fails instead of allocating).
+  c->bound = false;
+  bmx_status st = bmx_set_force_params(c, fpar);
+  if (st != BMX_OK) return st;
+  bmx_neighbor_params p = *nbp;
+  p.max_neighbors = maxn;
+  st = bmx_set_neighbor_params(c, &p);
+  if (st != BMX_OK) return st;
+  st = bmx_set_geometry(c, geom);
+  if (st != BMX_OK) return st;
+  int64_t cs[2] = {0, n};
+  st = bind_common(c, aos, cs, 1, geom, layout, 1, true);
+  if (st != BMX_OK) return st;
+  st = bmx_evolve(c, dt, nsteps);
+  if (st != BMX_OK) return st;
+  st = pack_records(c);  // into the caller's records
+  if (st != BMX_OK) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return BMX_OK;
 }

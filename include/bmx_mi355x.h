@@ -343,6 +343,32 @@ bmx_status bmx_fusion(bmx_ctx *ctx, const bmx_fusion_params *params,
                       uint64_t seed, uint32_t call, int32_t next_id, int32_t cpu,
                       int32_t *fused, int64_t *n_new);
 
+/* --- allocation-free evolve ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one call does what bmx_bind_particles_aos (device
+ * records) + bmx_evolve(dt, nsteps) + bmx_sync_to_aos do on a context, on
+ * the caller's device records in place, with every working array in a
+ * static device scratch the library holds per GPU (about 40 MB, from load
+ * to process end).  It calls no allocating routine.  Limits: n <=
+ * BMX_NOALLOC_MAX_PARTICLES, max_neighbors <= BMX_NOALLOC_MAX_NEIGHBORS
+ * (0 = that), rebuild_every <= BMX_NOALLOC_MAX_REBUILD_EVERY, at most
+ * BMX_NOALLOC_MAX_BINS bins; beyond them BMX_ERR_BAD_ARG, before any GPU
+ * work.  One box (no colonies).  stream: a hipStream_t, or NULL for the
+ * null stream; the call synchronizes it before returning.  Calls share
+ * their GPU's scratch, which keeps the last call's contents: calls that
+ * overlap on one GPU race. */
+#define BMX_NOALLOC_MAX_PARTICLES 32768
+#define BMX_NOALLOC_MAX_NEIGHBORS 64
+#define BMX_NOALLOC_MAX_REBUILD_EVERY 64
+#define BMX_NOALLOC_MAX_BINS 524288
+bmx_status bmx_evolve_aos_noalloc(void *aos_device, int64_t n,
+                                  const bmx_layout *layout,
+                                  const bmx_geometry *geom,
+                                  const double fpar[9],
+                                  const bmx_neighbor_params *nbp, double dt,
+                                  int32_t nsteps, void *stream);
+
 bmx_status bmx_get_fc(bmx_ctx *ctx, double *out6n);
 /* positions, n*3 doubles (x,y,z per particle) */
 bmx_status bmx_get_positions(bmx_ctx *ctx, double *out3n);
