@@ -452,6 +452,8 @@ def main() -> int:
                     help="Verlet skin in cm (default: the workload's)")
     ap.add_argument("--rebuild-every", type=int, default=0,
                     help="steps between list builds (default: the workload's, 10)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of one graph replay")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--remap", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -512,26 +514,42 @@ def main() -> int:
     # the window does not run at idle clocks
     t_w = time.perf_counter()
     while time.perf_counter() - t_w < 0.3:
-        ctx.evolve(col.dt, K)
-        warm += K
+        ctx.evolve(col.dt, 10 * K)
+        warm += 10 * K
+    # The timed steps replay one captured HIP graph (the steps' launches,
+    # rebuilds included, submitted at once).  Its capture and a first replay
+    # are warm-up; a window of whole rebuild periods starts at the same list
+    # phase every time, so the timed call replays the captured graph.
+    graph = args.steps % K == 0 and not args.eager
+    if graph:
+        ctx.evolve_graph(col.dt, args.steps)
+        warm += args.steps
     ctx.synchronize()
+    rb0 = ctx.timing().rebuilds_total
     t0 = time.perf_counter()
-    ctx.evolve(col.dt, args.steps)  # synchronizes and checks skin/overflow/NaN
+    if graph:
+        ctx.evolve_graph(col.dt, args.steps)  # synchronizes, checks skin/overflow/NaN
+    else:
+        ctx.evolve(col.dt, args.steps)
     t1 = time.perf_counter()
+    rebuilds = ctx.timing().rebuilds_total - rb0
     secs = t1 - t0
     n = len(recs)
     value = n * args.steps / secs
     ms_per_step = secs * 1e3 / args.steps
 
-    # Kernel split: the same window again (it starts at a rebuild too) with
-    # HIP events on the library's stream around every rebuild and around
-    # every run of force launches between two rebuilds, so the force time per
-    # launch carries no per-launch event overhead and cannot exceed a step.
+    # Kernel split, in eager launches (events cannot sit inside a graph
+    # replay): HIP events on the library's stream around every rebuild and
+    # around every run of force launches between two rebuilds, so the force
+    # time per launch carries no per-launch event overhead and cannot exceed
+    # a step.  At least 40 rebuild periods, starting at a rebuild.
+    ev_steps = max(args.steps, 40 * K)
+    ev_steps = -(-ev_steps // K) * K
     ctx.set_timing(True)
     ctx.reset_timing()
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.evolve(col.dt, args.steps)
+    ctx.evolve(col.dt, ev_steps)
     secs_timed = time.perf_counter() - t0
     tm = ctx.timing()
     ctx.set_timing(False)
@@ -569,7 +587,9 @@ def main() -> int:
             "skin": col.skin,
             "mean_row": mean_row,
             "warmup_steps_run": warm,
-            "rebuilds": tm.rebuilds,
+            "launch": "one captured HIP graph per call" if graph else "eager launches",
+            "rebuilds": rebuilds,
+            "rebuilds_in_events_window": tm.rebuilds,
             "block": args.block,
             "xcd_remap": args.remap,
         },
@@ -585,8 +605,9 @@ def main() -> int:
             "kernel_launches_timed": tm.force_launches,
             "kernel_share_of_step": per_launch_ms / ms_per_step,
             "rebuild_avg_ms": rebuild_ms,
-            "events_window_ms_per_step": secs_timed * 1e3 / args.steps,
-            "device_ms_per_step": (tm.force_ms + tm.rebuild_ms) / args.steps,
+            "events_window_steps": ev_steps,
+            "events_window_ms_per_step": secs_timed * 1e3 / ev_steps,
+            "device_ms_per_step": (tm.force_ms + tm.rebuild_ms) / ev_steps,
             "algorithmic_bytes_per_particle": algorithmic_bytes_per_particle(mean_row),
             "algorithmic_bytes_per_launch": alg_bytes,
             "survey_bytes_per_particle_step": survey_bytes_per_particle_step(mean_row, K),

@@ -108,7 +108,8 @@ class Timing(C.Structure):
         ("force_ms", C.c_double), ("force_launches", C.c_int64),
         ("rebuild_ms", C.c_double), ("rebuilds", C.c_int64),
         ("neighbor_entries", C.c_int64), ("max_row", C.c_int32),
-        ("nbins", C.c_int32),
+        ("nbins", C.c_int32), ("rebuilds_total", C.c_int64),
+        ("steps_total", C.c_int64),
     ]
 
 

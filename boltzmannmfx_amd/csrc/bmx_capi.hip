@@ -141,9 +141,10 @@ struct bmx_ctx {
   bool wrote_step = false;
   DevBuf<int32_t> d_err, d_stats, d_bstats;
   DevBuf<unsigned long long> d_total, d_ustats;
-  DevBuf<double> d_vpart;      // [slot][kVmaxWords] max dt*|vel| per step
+  DevBuf<double> d_vpart;      // [slot][vstride] max dt*|vel| per wave/block
+  DevBuf<unsigned long long> d_slotmax;  // per-slot maxima (skin check)
   int32_t vmax_slots = 1;
-  int32_t bstride = 1;         // row-build blocks at the smallest block size
+  int32_t vstride = 1;         // waves of ncap particles (blocks of 64)
   int32_t row_nblk = 0;        // blocks of the last row build
   int32_t *h_flags = nullptr;  // pinned: err, max row, pad, pad, total(2)
 
@@ -164,6 +165,7 @@ struct bmx_ctx {
   int64_t force_launches = 0, rebuilds = 0;
   int64_t last_entries = 0;
   int32_t last_max_row = 0;
+  int64_t rebuilds_total = 0;  // list builds since bind (graph replays too)
 
   // captured step sequences
   hipGraphExec_t graph_exec = nullptr;
@@ -374,12 +376,15 @@ bmx_status ensure_rows(bmx_ctx *c) {
 // check reads every slot, so unused tails stay zero.
 bmx_status alloc_partials(bmx_ctx *c) {
   c->vmax_slots = std::max(1, c->nbp.rebuild_every);
-  c->bstride = std::max(1, row_blocks(c->ncap, 64));
-  hipError_t e = c->d_vpart.alloc((size_t)c->vmax_slots * kVmaxWords);
+  c->vstride = std::max(1, row_blocks(c->ncap, 64));
+  hipError_t e = c->d_vpart.alloc((size_t)c->vmax_slots * c->vstride);
   if (e == hipSuccess)
     e = launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
-                        (int64_t)c->vmax_slots * kVmaxWords, 0ull, c->stream);
-  if (e == hipSuccess) e = c->d_bstats.alloc(2 * (size_t)c->bstride);
+                        (int64_t)c->vmax_slots * c->vstride, 0ull, c->stream);
+  if (e == hipSuccess) e = c->d_bstats.alloc(2 * (size_t)c->vstride);
+  if (e == hipSuccess) e = c->d_slotmax.alloc(c->vmax_slots);
+  if (e == hipSuccess)
+    e = launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, c->stream);
   if (e != hipSuccess) {
     c->bound = false;
     return fail(BMX_ERR_HIP, std::string("partials: ") + hipGetErrorString(e));
@@ -423,8 +428,9 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
     if (st != BMX_OK) return st;
   }
   if (check_skin && c->have_list && verlet_checked(c))
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 1,
-                             c->d_err.p, s));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
+                             s));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = next_event(c);
@@ -465,6 +471,7 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
   }
   c->have_list = true;
   c->last_build = c->step;
+  c->rebuilds_total++;
   return BMX_OK;
 }
 
@@ -479,7 +486,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   }
   int slot = (int)(c->step - c->last_build);
   double *vm = verlet_checked(c)
-                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * kVmaxWords
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
                    : nullptr;
   if (c->timing && !c->run_e0) {
     c->run_e0 = next_event(c);
@@ -515,7 +522,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
 bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
   ForceParams fp = force_params(c, dt);
   double *vm = (slot >= 0 && verlet_checked(c))
-                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * kVmaxWords
+                   ? c->d_vpart.p + (int64_t)(slot % c->vmax_slots) * c->vstride
                    : nullptr;
   HIPCHK(launch_force_sphere_flat(
       c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
@@ -731,6 +738,7 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   HIPCHK(hipStreamSynchronize(s));
   c->step = 0;
   c->last_build = -1;
+  c->rebuilds_total = 0;
   c->have_list = false;
   c->wrote_step = false;
   c->have_chem = false;
@@ -836,6 +844,7 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_bstats.release();
   c->d_total.release();
   c->d_vpart.release();
+  c->d_slotmax.release();
   c->d_ustats.release();
   c->d_chem.release();
   c->d_chem_inc.release();
@@ -1015,8 +1024,9 @@ bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
     if (st != BMX_OK) return st;
   }
   if (verlet_checked(c) && nsteps > 0 && c->have_list)
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 0,
-                             c->d_err.p, c->stream));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 0, c->d_slotmax.p, c->d_err.p,
+                             c->stream));
   return check_flags(c);
 }
 
@@ -1046,8 +1056,9 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     for (int32_t k = 0; k < nsteps && cst == BMX_OK; k++)
       cst = enqueue_step(c, dt, k == nsteps - 1);
     if (cst == BMX_OK && verlet_checked(c))
-      (void)launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin,
-                              0, c->d_err.p, c->stream);
+      (void)launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                              0.5 * c->nbp.skin, 0, c->d_slotmax.p,
+                              c->d_err.p, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (cst != BMX_OK) {
@@ -1086,6 +1097,7 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
       c->cd = 1 - c->cd;
       c->have_list = true;
       c->last_build = c->step;
+      c->rebuilds_total++;
     }
     c->cd = 1 - c->cd;
     c->step++;
@@ -1169,8 +1181,9 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
     if (st != BMX_OK) return st;
   }
   if (verlet_checked(c))
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, 0.5 * c->nbp.skin, 1,
-                             c->d_err.p, s));
+    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
+                             s));
   HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n, d_final.p, s));
   HIPCHK(launch_adj_loss(c->pos[c->cd].p, c->oid[c->cs].p, d_target.p, n,
                          c->boxes[0], d_lam0.p, d_lpart.p, s));
@@ -1438,6 +1451,8 @@ bmx_status bmx_get_timing(bmx_ctx *c, bmx_timing *out) {
   out->neighbor_entries = c->last_entries;
   out->max_row = c->last_max_row;
   out->nbins = c->nbins;
+  out->rebuilds_total = c->rebuilds_total;
+  out->steps_total = c->step;
   return BMX_OK;
 }
 
@@ -1555,7 +1570,8 @@ bmx_status snap_restore(bmx_ctx *c, const StateSnap &z) {
   c->last_build = -1;
   c->wrote_step = false;
   HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
-                         (int64_t)c->vmax_slots * kVmaxWords, 0ull, s));
+                         (int64_t)c->vmax_slots * c->vstride, 0ull, s));
+  HIPCHK(launch_fill_u64(c->d_slotmax.p, c->vmax_slots, 0ull, s));
   return BMX_OK;
 }
 
@@ -2139,7 +2155,8 @@ constexpr size_t kNaBytes =
     + na_align(4 * kNaFlat) + na_align(kNaFlat)  // cidx, cown
     + 2 * na_align(4 * (kNaBins + 1))            // count, start
     + na_align(4 * ((kNaBins + 2047) / 2048 + 2))  // scan tile sums
-    + na_align(8 * 64 * kNaRebuild)              // displacement slots
+    + na_align(8 * (kNaParticles / 64 + 1) * kNaRebuild)  // displacement slots
+    + na_align(8 * kNaRebuild)                   // slot maxima
     + na_align(4 * 2 * (kNaParticles / 64 + 1))  // row-build block stats
     + na_align(4) + na_align(16) + na_align(8) + na_align(64)  // err, stats, total, ustats
     + na_align(sizeof(BoxGeom));
@@ -2197,7 +2214,8 @@ bmx_status na_setup(NaDevice &d, int dev) {
   na_slice(c.d_count, cur, (size_t)kNaBins + 1);
   na_slice(c.d_start, cur, (size_t)kNaBins + 1);
   na_slice(c.d_scan, cur, (size_t)(kNaBins + 2047) / 2048 + 2);
-  na_slice(c.d_vpart, cur, (size_t)64 * kNaRebuild);
+  na_slice(c.d_vpart, cur, (N / 64 + 1) * (size_t)kNaRebuild);
+  na_slice(c.d_slotmax, cur, (size_t)kNaRebuild);
   na_slice(c.d_bstats, cur, 2 * (N / 64 + 1));
   na_slice(c.d_err, cur, 1);
   na_slice(c.d_stats, cur, 4);

@@ -83,8 +83,8 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
 int flat_tile(int segments);
 // Sphere force + overdamped update, fused, over the flat row layout.  fc
 // (6*ncap, original order: fc[oid + c*ncap]) is written when non-null,
-// with fc_oid the original index of each sorted row; when vmax (a slot of
-// kVmaxWords doubles) is non-null, every wave folds its max dt*|vel| into it.
+// with fc_oid the original index of each sorted row; vmax[wave] receives
+// the wave's max dt*|vel| when non-null.
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
                                     const int32_t *wtot, const int32_t *ncnt,
@@ -103,7 +103,7 @@ hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
                                   hipStream_t s);
 
 // ---- segment (and mixed) force + overdamped update, fused: fc and vmax
-// as for launch_force_sphere_flat (vmax folded per block).
+// as for launch_force_sphere_flat, vmax per block.
 hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
                                 const int32_t *cidx, const uint8_t *cown,
                                 const int32_t *wtot, int8_t *redo_tile,
@@ -116,14 +116,16 @@ hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
                              const int32_t *ncnt, int64_t ncap, int64_t n,
                              const BoxGeom *boxes, BoxGeom g0, int multi,
                              ForceParams fp, int32_t *err, hipStream_t s);
-// Displacement record of the Verlet skin check: one slot of kVmaxWords
-// doubles per step since the last rebuild; force kernels fold each wave's
-// (or block's) largest dt*|vel| into their step's slot with atomic maxima.
-constexpr int kVmaxWords = 64;
-// sum over slots of the slot's largest word > limit -> err |= ERRBIT_SKIN;
-// clear = 1 zeroes the slots (at a rebuild).
-hipError_t launch_skin_check(double *vmax, int32_t nslots, double limit,
-                             int clear, int32_t *err, hipStream_t s);
+// Displacement record of the Verlet skin check: one slot of `stride`
+// doubles per step since the last rebuild; force kernels store each wave's
+// (or block's) largest dt*|vel| at its index in their step's slot.
+// sum over slots of max over the slot > limit -> err |= ERRBIT_SKIN;
+// clear = 1 zeroes the partials (at a rebuild).  slotmax: nslots zeroed
+// words of scratch (left zeroed).
+hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
+                             double limit, int clear,
+                             unsigned long long *slotmax, int32_t *err,
+                             hipStream_t s);
 
 // ---- exports to original order
 hipError_t launch_export_fc(const double *fc, int64_t ncap, int64_t n,

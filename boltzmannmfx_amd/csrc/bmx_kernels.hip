@@ -859,20 +859,11 @@ __global__ __launch_bounds__(1024) void k_reduce_rowstats(
 
 // ------------------------------------------------------- force + update ---
 
-// A displacement folded into a step's slot of kVmaxWords words: an atomic
-// max on word (k mod kVmaxWords), so concurrent writers spread over that many
-// addresses (non-negative doubles order as their bit patterns).
-__device__ __forceinline__ void vmax_fold(double d, double *slot, int64_t k) {
-  if (d > 0.0)
-    atomicMax(reinterpret_cast<unsigned long long *>(slot) + (k & (kVmaxWords - 1)),
-              dbits(d));
-}
-
-// Largest step displacement dt*|vel| of the block, one atomic per block.
-__device__ __forceinline__ void block_vmax(double v2, double dt, double *slot) {
+// Largest step displacement dt*|vel| of the block, one store per block.
+__device__ __forceinline__ void block_vmax(double v2, double dt, double *vpart) {
   __shared__ double shd[16];
   double m = block_max(v2, shd);
-  if (threadIdx.x == 0) vmax_fold(dt * sqrt(m), slot, blockIdx.x);
+  if (threadIdx.x == 0) vpart[blockIdx.x] = dt * sqrt(m);
 }
 
 // Separation of two spheres along the axes, with the image test of sep_m.
@@ -902,13 +893,15 @@ constexpr int kFlatWin = 2 * kWave;
 constexpr size_t kFlatWaveBytes =
     kWave * sizeof(double4) + 4 * kFlatWin * sizeof(double);
 
-// Largest step displacement dt*|vel| of the wave, one atomic per wave, with
-// no block barrier: waves that finish early do not wait for the block's
-// slowest.
-__device__ __forceinline__ void wave_vmax(double v2, double dt, double *slot,
+// Largest step displacement dt*|vel| of the wave, one store per wave at its
+// wave index (the partial arrays hold one slot per 64 particles), with no
+// block barrier: waves that finish early do not wait for the block's
+// slowest.  (One atomic maximum per wave into a few shared words instead
+// measured 2.4 times slower on the 1M-sphere step: the atomics serialize.)
+__device__ __forceinline__ void wave_vmax(double v2, double dt, double *vpart,
                                           int64_t t, int lane, int64_t n) {
   const double m = wave_max_f64(v2);
-  if (lane == 0 && t < n) vmax_fold(dt * sqrt(m), slot, t / kWave);
+  if (lane == 0 && t < n) vpart[t / kWave] = dt * sqrt(m);
 }
 
 
@@ -1440,7 +1433,8 @@ __global__ __launch_bounds__(256) void k_force_general_redo(
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   general_rows<MULTI>(t, g, in, st, nbr, ncnt, ncap, fp, acc, ti);
   const double v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
-  if (vmax) vmax_fold(fp.dt * sqrt(v2), vmax, t);
+  if (vmax)
+    atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(fp.dt * sqrt(v2)));
 }
 
 // InitBonds: per segment, scan the row in order and append a bond for the
@@ -1513,32 +1507,42 @@ __global__ void k_init_bonds(DynArrays dy, StaticArrays st,
   }
 }
 
-// Sum over the steps since the last rebuild of the largest displacement of
-// any particle bounds every particle's displacement; more than half the
-// skin means a pair may have entered interaction range unlisted.  One wave:
-// lane l holds word l of every slot; each slot's maximum is a wave maximum,
-// and the slot maxima are added in slot order.  clear = 1 zeroes the slots
-// (at a rebuild).
-__global__ __launch_bounds__(64) void k_skin_check(double *__restrict__ vmax,
-                                                   int32_t nslots, double limit,
-                                                   int clear,
-                                                   int32_t *__restrict__ err) {
-  const int l = threadIdx.x;
-  double total = 0.0;
-  // four slots' loads in flight at a time; a missing slot adds 0.0, which
-  // leaves the sum's bits unchanged
-  for (int sl = 0; sl < nslots; sl += 4) {
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      v[k] = sl + k < nslots ? vmax[(int64_t)(sl + k) * kVmaxWords + l] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      total += wave_max_f64(v[k]);
-      if (clear && sl + k < nslots) vmax[(int64_t)(sl + k) * kVmaxWords + l] = 0.0;
-    }
+// Sum over the steps since the last rebuild of the largest displacement in
+// any block bounds every particle's displacement; more than half the skin
+// means a pair may have entered interaction range unlisted.  Stage 1: each
+// block takes the per-slot max over a chunk of force blocks into slotmax
+// (one atomic per slot per block, values >= 0 compare as integers).
+// Block (x, y) takes slot y over a chunk of kSkinChunk force blocks.
+constexpr int kSkinChunk = 4096;
+__global__ __launch_bounds__(256) void k_skin_partial(
+    double *__restrict__ vpart, int32_t nslots, int32_t stride, int clear,
+    unsigned long long *__restrict__ slotmax) {
+  __shared__ double shd[4];
+  const int sl = blockIdx.y;
+  const int64_t b0 = (int64_t)blockIdx.x * kSkinChunk;
+  const int64_t b1 = b0 + kSkinChunk < stride ? b0 + kSkinChunk : stride;
+  double m = 0.0;
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    double v = vpart[(int64_t)sl * stride + b];
+    m = v > m ? v : m;
+    if (clear) vpart[(int64_t)sl * stride + b] = 0.0;
   }
-  if (l == 0 && total > limit) atomicOr(err, ERRBIT_SKIN);
+  m = block_max(m, shd);
+  if (threadIdx.x == 0 && m > 0.0) atomicMax(&slotmax[sl], dbits(m));
+}
+
+// Stage 2: sum the slot maxima in slot order, compare, reset the slot
+// maxima.
+__global__ void k_skin_final(unsigned long long *__restrict__ slotmax,
+                             int32_t nslots, double limit,
+                             int32_t *__restrict__ err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double total = 0.0;
+  for (int sl = 0; sl < nslots; sl++) {
+    total += __longlong_as_double((long long)slotmax[sl]);
+    slotmax[sl] = 0ull;
+  }
+  if (total > limit) atomicOr(err, ERRBIT_SKIN);
 }
 
 // ------------------------------------------------------------- exports ---
@@ -1811,10 +1815,16 @@ hipError_t launch_init_bonds(DynArrays dy, StaticArrays st, const int32_t *nbr,
   return hipGetLastError();
 }
 
-hipError_t launch_skin_check(double *vmax, int32_t nslots, double limit,
-                             int clear, int32_t *err, hipStream_t s) {
-  hipLaunchKernelGGL(k_skin_check, dim3(1), dim3(kWave), 0, s, vmax, nslots,
-                     limit, clear, err);
+hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
+                             double limit, int clear,
+                             unsigned long long *slotmax, int32_t *err,
+                             hipStream_t s) {
+  int chunks = (stride + kSkinChunk - 1) / kSkinChunk;
+  if (chunks < 1) chunks = 1;
+  hipLaunchKernelGGL(k_skin_partial, dim3(chunks, nslots > 0 ? nslots : 1),
+                     dim3(256), 0, s, vpart, nslots, stride, clear, slotmax);
+  hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(64), 0, s, slotmax, nslots,
+                     limit, err);
   return hipGetLastError();
 }
 

@@ -107,6 +107,8 @@ typedef struct {
   int64_t neighbor_entries; /* total list entries at the last rebuild */
   int32_t max_row;          /* longest row at the last rebuild */
   int32_t nbins;            /* bins at the last rebuild */
+  int64_t rebuilds_total;   /* list builds since bind, timed or not */
+  int64_t steps_total;      /* steps since bind */
 } bmx_timing;
 
 /* --- lifetime ---------------------------------------------------------- */
