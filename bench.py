@@ -388,6 +388,86 @@ def mesh_bench(bmx, W, n: int = 1_000_000, n_cell=(64, 64, 32), calls: int = 20,
     return out
 
 
+def dropin_bench(bmx, col, calls: int = 40) -> dict:
+    """The reference's cadence: bmx::Evolve calls EvolveParticles once per
+    step (src/timestepping/bmx_evolve.cpp:97), on particles that live in
+    device memory (AMReX GPU build).  Per call, three ways: (a) bind the
+    records again (copied), evolve 1 step, sync; (b) records bound once in
+    place, evolve 1 step and sync per call; (c) bound once in place, evolve
+    1 step per call, records synced every 10 calls; (d) as (c) with the
+    calls enqueued without waiting (bmx_evolve_async), completed every 10
+    calls.  Each from the state after a warm-up, timed over whole rebuild
+    periods."""
+    import ctypes as C
+    import numpy as np
+    hip = _hip_runtime()
+    recs = col.recs.copy()
+    nbytes = recs.nbytes
+    dev = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dev), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dev, recs.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
+    K = col.rebuild_every
+    n = len(recs)
+    out = {"workload": f"{n} spheres resident in device memory, {calls} calls of one "
+                       f"step each, Verlet list every {K} steps"}
+
+    def make():
+        ctx = bmx.Context(0)
+        ctx.set_force_params(col.fpar)
+        ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic)
+        ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=K, skin=col.skin,
+                                max_neighbors=64)
+        return ctx
+
+    try:
+        ctx = make()
+        for _ in range(3):                      # (a) warm-up
+            ctx.bind_device(dev.value, n)
+            ctx.evolve(col.dt, 1)
+            ctx.sync()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            ctx.bind_device(dev.value, n)
+            ctx.evolve(col.dt, 1)
+            ctx.sync()
+        out["rebind_evolve_sync_ms_per_call"] = (time.perf_counter() - t0) * 1e3 / calls
+        ctx.close()
+
+        ctx = make()
+        ctx.bind_device(dev.value, n, in_place=True)
+        for _ in range(2 * K):                  # (b) warm-up
+            ctx.evolve(col.dt, 1)
+            ctx.sync()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            ctx.evolve(col.dt, 1)
+            ctx.sync()
+        out["resident_evolve_sync_ms_per_call"] = (time.perf_counter() - t0) * 1e3 / calls
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in range(calls):                  # (c)
+            ctx.evolve(col.dt, 1)
+            if k % 10 == 9:
+                ctx.sync()
+        out["resident_evolve_ms_per_call_sync_every_10"] = \
+            (time.perf_counter() - t0) * 1e3 / calls
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in range(calls):                  # (d)
+            ctx.evolve_async(col.dt, 1)
+            if k % 10 == 9:
+                ctx.wait()
+                ctx.sync()
+        out["resident_async_ms_per_call_wait_sync_every_10"] = \
+            (time.perf_counter() - t0) * 1e3 / calls
+        ctx.close()
+    finally:
+        hip.hipFree(dev)
+    return out
+
+
 def _hip_runtime():
     """The HIP runtime the library links (/opt/rocm), not another copy a
     framework may have loaded under the same name."""
@@ -469,6 +549,8 @@ def main() -> int:
                     help="run only the 200,000-segment workload and print its "
                          "object (for profiling that kernel)")
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the one-step-per-call cadence legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -615,6 +697,9 @@ def main() -> int:
         },
     }
     ctx.close()
+    if not args.no_dropin:
+        out["dropin"] = dropin_bench(bmx, col)
+        print("# drop-in cadence done", file=sys.stderr, flush=True)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(bmx, W, args.ensemble_steps,
                                          seq=not args.no_sequential)

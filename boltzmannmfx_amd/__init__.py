@@ -58,6 +58,9 @@ FPAR_NAMES = (
 )
 
 NBR_VERLET, NBR_STENCIL = 0, 1
+AOS_HOST, AOS_DEVICE, AOS_IN_PLACE = 0, 1, 2
+FIELD_POSITIONS, FIELD_SIZES, FIELD_ORIENTATIONS, FIELD_BONDS, FIELD_SPECIES = \
+    1, 2, 4, 8, 16
 # limits of bmx_evolve_aos_noalloc (include/bmx_mi355x.h)
 BMX_NOALLOC_MAX_PARTICLES = 32768
 BMX_NOALLOC_MAX_NEIGHBORS = 64
@@ -109,7 +112,7 @@ class Timing(C.Structure):
         ("rebuild_ms", C.c_double), ("rebuilds", C.c_int64),
         ("neighbor_entries", C.c_int64), ("max_row", C.c_int32),
         ("nbins", C.c_int32), ("rebuilds_total", C.c_int64),
-        ("steps_total", C.c_int64),
+        ("steps_total", C.c_int64), ("guard_redone", C.c_int64),
     ]
 
 
@@ -203,6 +206,14 @@ def load_library() -> C.CDLL:
         "bmx_write_checkpoint": (C.c_int, [C.c_char_p, vp, C.c_int64, P(Layout),
                                            P(Geometry), C.c_int32, C.c_int64]),
         "bmx_autotune": (C.c_int, [vp, C.c_double, C.c_int32, C.c_char_p, P(Tuning)]),
+        "bmx_bind_tiles_aos": (C.c_int, [vp, P(vp), P(C.c_int64), C.c_int32,
+                                         P(Layout), C.c_int32]),
+        "bmx_refresh_from_aos": (C.c_int, [vp, vp, C.c_uint32]),
+        "bmx_evolve_async": (C.c_int, [vp, C.c_double, C.c_int32]),
+        "bmx_wait": (C.c_int, [vp]),
+        "bmx_query": (C.c_int, [vp, P(C.c_int32)]),
+        "bmx_set_stream": (C.c_int, [vp, vp]),
+        "bmx_set_guard": (C.c_int, [vp, C.c_int32]),
         "bmx_evolve_aos_noalloc": (C.c_int, [vp, C.c_int64, P(Layout), P(Geometry),
                                              P(C.c_double), P(NeighborParams),
                                              C.c_double, C.c_int32, vp]),
@@ -432,6 +443,55 @@ class Context:
         _check(self.lib.bmx_bind_particles_aos(
             self.h, recs.ctypes.data_as(C.c_void_p), len(recs), C.byref(lay), 0))
         self.n = len(recs)
+
+    def bind_device(self, address: int, n: int, in_place: bool = False) -> None:
+        """Binds n records in device memory at `address` (copied, or used in
+        place)."""
+        lay = self.lib.bmx_default_layout()
+        self._recs = None
+        _check(self.lib.bmx_bind_particles_aos(
+            self.h, C.c_void_p(address), int(n), C.byref(lay),
+            AOS_IN_PLACE if in_place else AOS_DEVICE))
+        self.n = int(n)
+
+    def bind_tiles(self, tiles: Sequence[np.ndarray]) -> None:
+        """Several host record arrays as one neighbour domain (the context's
+        geometry); sync() writes each back."""
+        for t in tiles:
+            assert t.dtype == RECORD_DTYPE and t.flags.c_contiguous
+        self._tiles = list(tiles)
+        self._recs = None
+        ptrs = (C.c_void_p * len(tiles))(*[t.ctypes.data for t in tiles])
+        counts = np.array([len(t) for t in tiles], dtype=np.int64)
+        lay = self.lib.bmx_default_layout()
+        _check(self.lib.bmx_bind_tiles_aos(
+            self.h, ptrs, counts.ctypes.data_as(C.POINTER(C.c_int64)), len(tiles),
+            C.byref(lay), 0))
+        self.n = int(counts.sum())
+
+    def refresh(self, fields: int, recs: Optional[np.ndarray] = None) -> None:
+        """bmx_refresh_from_aos: re-read the given BMX_FIELD_* bits."""
+        ptr = recs.ctypes.data_as(C.c_void_p) if recs is not None else None
+        _check(self.lib.bmx_refresh_from_aos(self.h, ptr, int(fields)))
+
+    def evolve_async(self, dt: float, nsteps: int) -> None:
+        _check(self.lib.bmx_evolve_async(self.h, float(dt), int(nsteps)))
+
+    def wait(self) -> None:
+        _check(self.lib.bmx_wait(self.h))
+
+    def query(self) -> bool:
+        d = C.c_int32()
+        _check(self.lib.bmx_query(self.h, C.byref(d)))
+        return bool(d.value)
+
+    def set_guard(self, on: bool = True) -> None:
+        """Guarded Verlet steps: periods that exceed the skin are taken again
+        with per-step lists instead of failing."""
+        _check(self.lib.bmx_set_guard(self.h, 1 if on else 0))
+
+    def set_stream(self, stream: int = 0) -> None:
+        _check(self.lib.bmx_set_stream(self.h, C.c_void_p(stream) if stream else None))
 
     def bind_colonies(self, recs: np.ndarray, colony_start: Sequence[int],
                       geoms: Sequence[Geometry]) -> None:

@@ -73,7 +73,11 @@ struct DevBuf {
 
 struct bmx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every call enqueues on
+  hipStream_t own_stream = nullptr;  // the context's own (stream when unset)
+  // bmx_evolve_async: steps in flight, done_ev recorded after them
+  bool pending = false;
+  hipEvent_t done_ev = nullptr;
 
   double fpar[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double mobility = 1.0;
@@ -103,10 +107,17 @@ struct bmx_ctx {
   int cs = 0;  // current static buffer
   int cd = 0;  // current dynamic buffer
 
-  // record array
+  // record array: recs is the device record array the kernels read and
+  // write, either the context's own copy (d_aos) or, bound in place, the
+  // caller's device records
   DevBuf<uint8_t> d_aos;
+  uint8_t *recs = nullptr;
+  bool in_place = false;
   void *user_aos = nullptr;
   bool aos_on_device = false;
+  // tiles bound as one domain: sync copies each tile's records back
+  std::vector<void *> tiles;
+  std::vector<int64_t> tile_start;
   AosLayout lay{};
 
   // binning and rows
@@ -146,7 +157,8 @@ struct bmx_ctx {
   int32_t vmax_slots = 1;
   int32_t vstride = 1;         // waves of ncap particles (blocks of 64)
   int32_t row_nblk = 0;        // blocks of the last row build
-  int32_t *h_flags = nullptr;  // pinned: err, max row, pad, pad, total(2)
+  int32_t *h_flags = nullptr;  // pinned: err, max row, total (2 words), ...
+  int32_t *h_flags_dev = nullptr;  // h_flags as the device addresses it
 
   // timing: events around every rebuild and around every run of force
   // launches between two rebuilds (or a rebuild and the end of a call)
@@ -166,6 +178,21 @@ struct bmx_ctx {
   int64_t last_entries = 0;
   int32_t last_max_row = 0;
   int64_t rebuilds_total = 0;  // list builds since bind (graph replays too)
+
+  // guarded Verlet steps: each rebuild period is checked as it ends and a
+  // period that moved a particle more than skin/2 is taken again with lists
+  // rebuilt every step (two snapshots: one period in flight while the one
+  // before it is checked)
+  bool guard = false;
+  bool guarding = false;  // inside a guarded evolve: rebuild() checks nothing
+  struct Snap {
+    DevBuf<double4> pos, rod, trig;
+    DevBuf<Topo> topo;
+    DevBuf<int32_t> oid, type, key, col;
+    DevBuf<double> bscale;
+    hipEvent_t done = nullptr;
+  } gsnap[2];
+  int64_t guard_redone = 0;  // periods taken again since bind
 
   // captured step sequences
   hipGraphExec_t graph_exec = nullptr;
@@ -427,7 +454,7 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
     bmx_status st = close_run(c);
     if (st != BMX_OK) return st;
   }
-  if (check_skin && c->have_list && verlet_checked(c))
+  if (check_skin && !c->guarding && c->have_list && verlet_checked(c))
     HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
                              0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
                              s));
@@ -535,11 +562,19 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
 }
 
 // Reads the error word and list statistics; clears the error word.
-bmx_status check_flags(bmx_ctx *c) {
+// Enqueues the read-back of the error word and list statistics (and clears
+// the error word) on the context's stream.
+bmx_status enqueue_flags(bmx_ctx *c) {
   hipStream_t s = c->stream;
   if (c->timing) {
     bmx_status st = close_run(c);
     if (st != BMX_OK) return st;
+  }
+  if (c->h_flags_dev) {
+    // one launch writes the words into host memory the device maps
+    HIPCHK(launch_flags_out(c->d_bstats.p, c->row_nblk, c->d_stats.p,
+                            c->d_total.p, c->d_err.p, c->h_flags_dev, s));
+    return BMX_OK;
   }
   if (c->row_nblk > 0)
     HIPCHK(launch_reduce_rowstats(c->d_bstats.p, c->row_nblk, c->d_stats.p,
@@ -551,12 +586,17 @@ bmx_status check_flags(bmx_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->h_flags + 2, c->d_total.p, sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s));
   HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
-  HIPCHK(hipStreamSynchronize(s));
+  return BMX_OK;
+}
+
+// Interprets the words enqueue_flags brought back, once they have landed.
+bmx_status read_flags(bmx_ctx *c) {
   if (c->timing) {
     bmx_status st = drain_timing(c);
     if (st != BMX_OK) return st;
   }
   int32_t err = c->h_flags[0];
+  c->h_flags[0] = 0;
   c->last_max_row = c->h_flags[1];
   unsigned long long tot;
   std::memcpy(&tot, c->h_flags + 2, sizeof(tot));
@@ -579,8 +619,29 @@ bmx_status check_flags(bmx_ctx *c) {
   return BMX_OK;
 }
 
-bmx_status require_ready(const bmx_ctx *c) {
+// Reads the error word and list statistics; clears the error word.
+bmx_status check_flags(bmx_ctx *c) {
+  bmx_status st = enqueue_flags(c);
+  if (st != BMX_OK) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return read_flags(c);
+}
+
+// Completes the steps a bmx_evolve_async call left in flight: waits for them
+// and returns their status.
+bmx_status settle(bmx_ctx *c) {
+  if (!c->pending) return BMX_OK;
+  c->pending = false;
+  HIPCHK(hipEventSynchronize(c->done_ev));
+  return read_flags(c);
+}
+
+// Every call that needs bound particles comes here first; steps still in
+// flight from bmx_evolve_async are completed, and their status returned.
+bmx_status require_ready(bmx_ctx *c) {
   if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  bmx_status st = settle(c);
+  if (st != BMX_OK) return st;
   if (!c->bound) return fail(BMX_ERR_STATE, "no particles bound");
   return BMX_OK;
 }
@@ -593,12 +654,12 @@ void reset_graph(bmx_ctx *c) {
 
 // Writes the state the force pass changes into the device record copy.
 bmx_status pack_records(bmx_ctx *c) {
-  HIPCHK(launch_aos_pack(c->d_aos.p, c->lay, c->n, static_arrays(c, c->cs),
+  HIPCHK(launch_aos_pack(c->recs, c->lay, c->n, static_arrays(c, c->cs),
                          dyn_arrays(c, c->cd), c->d_fc.p, c->ncap,
                          c->mobility, c->wrote_step ? 1 : 0,
                          c->any_fungi ? 1 : 0, c->stream));
   if (c->have_chem)
-    HIPCHK(launch_chem_pack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, c->d_chem.p,
                             c->d_chem_inc.p, c->stream));
   return BMX_OK;
 }
@@ -668,10 +729,12 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   size_t bytes = (size_t)n * (size_t)L.stride_bytes;
   c->aos_on_device = on_device != 0;
   c->user_aos = const_cast<void *>(aos);
+  c->tiles.clear();
+  c->tile_start.clear();
+  c->in_place = in_place;
   if (in_place) {
     // the caller's device records are the context's record array
-    c->d_aos.p = static_cast<uint8_t *>(const_cast<void *>(aos));
-    c->d_aos.n = bytes;
+    c->recs = static_cast<uint8_t *>(const_cast<void *>(aos));
   } else {
     HIPCHK(c->d_aos.alloc(bytes));
     if (bytes)
@@ -679,6 +742,7 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
                             on_device ? hipMemcpyDeviceToDevice
                                       : hipMemcpyHostToDevice,
                             s));
+    c->recs = c->d_aos.p;
   }
   // Boxes before unpack (wrap needs them); bins are set after the stats.
   c->boxes.resize(ncol);
@@ -714,7 +778,7 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
   StaticArrays st = static_arrays(c, 0);
   DynArrays dy = dyn_arrays(c, 0);
-  HIPCHK(launch_aos_unpack(c->d_aos.p, c->lay, n, dcol, c->d_boxes.p, st, dy,
+  HIPCHK(launch_aos_unpack(c->recs, c->lay, n, dcol, c->d_boxes.p, st, dy,
                            c->d_ustats.p, s));
   unsigned long long hs[8];
   HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
@@ -739,6 +803,7 @@ bmx_status bind_common(bmx_ctx *c, const void *aos, const int64_t *cstart,
   c->step = 0;
   c->last_build = -1;
   c->rebuilds_total = 0;
+  c->guard_redone = 0;
   c->have_list = false;
   c->wrote_step = false;
   c->have_chem = false;
@@ -791,7 +856,10 @@ bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
   c->nbp.bin_factor_pct = 100;
   c->nbp.skin = 0.0;
   c->nbp.neighborhood = 1.0;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  c->stream = c->own_stream;
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_flags, 64);
   if (e == hipSuccess) e = c->d_err.alloc(1) ;
   if (e == hipSuccess) e = c->d_stats.alloc(4);
@@ -802,6 +870,10 @@ bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
                                  hipGetErrorString(e));
   }
   std::memset(c->h_flags, 0, 64);
+  if (hipHostGetDevicePointer((void **)&c->h_flags_dev, c->h_flags, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    c->h_flags_dev = nullptr;  // read back by copies instead
+  }
   *out = c;
   return BMX_OK;
 }
@@ -810,6 +882,7 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   reset_graph(c);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   for (int k = 0; k < 2; k++) {
@@ -856,8 +929,21 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_mperm.release();
   c->d_mout.release();
   c->d_min.release();
+  for (auto &z : c->gsnap) {
+    z.pos.release();
+    z.rod.release();
+    z.trig.release();
+    z.topo.release();
+    z.oid.release();
+    z.type.release();
+    z.key.release();
+    z.col.release();
+    z.bscale.release();
+    if (z.done) (void)hipEventDestroy(z.done);
+  }
   if (c->h_flags) (void)hipHostFree(c->h_flags);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
@@ -949,9 +1035,66 @@ bmx_status bmx_bind_particles_aos(bmx_ctx *c, const void *aos, int64_t n,
                                   int32_t on_device) {
   if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
   if (!c->have_geom) return fail(BMX_ERR_STATE, "geometry not set");
+  if (on_device < BMX_AOS_HOST || on_device > BMX_AOS_IN_PLACE)
+    return fail(BMX_ERR_BAD_ARG, "aos_on_device must be 0, 1 or 2");
   (void)hipSetDevice(c->device);
+  if (on_device == BMX_AOS_IN_PLACE && n > 0) {
+    // in place: the records must be device memory of this context's device
+    hipPointerAttribute_t at;
+    if (!aos || hipPointerGetAttributes(&at, aos) != hipSuccess ||
+        at.type != hipMemoryTypeDevice || at.device != c->device) {
+      (void)hipGetLastError();
+      c->bound = false;
+      return fail(BMX_ERR_BAD_ARG,
+                  "records bound in place must be device memory of the "
+                  "context's device");
+    }
+  }
   int64_t cs[2] = {0, n};
-  return bind_common(c, aos, cs, 1, &c->geom, layout, on_device);
+  return bind_common(c, aos, cs, 1, &c->geom, layout, on_device ? 1 : 0,
+                     on_device == BMX_AOS_IN_PLACE);
+}
+
+bmx_status bmx_bind_tiles_aos(bmx_ctx *c, const void *const *tiles,
+                              const int64_t *counts, int32_t ntiles,
+                              const bmx_layout *layout, int32_t on_device) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  if (!c->have_geom) return fail(BMX_ERR_STATE, "geometry not set");
+  if (ntiles < 1 || !tiles || !counts || (on_device != 0 && on_device != 1))
+    return fail(BMX_ERR_BAD_ARG, "bad tile arguments");
+  bmx_layout L = layout ? *layout : bmx_default_layout();
+  if (L.stride_bytes <= 0) return fail(BMX_ERR_BAD_ARG, "invalid record layout");
+  (void)hipSetDevice(c->device);
+  std::vector<int64_t> start((size_t)ntiles + 1, 0);
+  for (int32_t k = 0; k < ntiles; k++) {
+    if (counts[k] < 0 || (counts[k] > 0 && !tiles[k]))
+      return fail(BMX_ERR_BAD_ARG, "bad tile count or pointer");
+    start[k + 1] = start[k] + counts[k];
+  }
+  const int64_t n = start[ntiles];
+  if (n > (int64_t)INT32_MAX / 2)
+    return fail(BMX_ERR_BAD_ARG, "particle count out of range");
+  // the tiles, one after another, form the context's record array
+  c->bound = false;
+  const size_t stride = (size_t)L.stride_bytes;
+  HIPCHK(c->d_aos.alloc((size_t)n * stride));
+  for (int32_t k = 0; k < ntiles; k++)
+    if (counts[k] > 0)
+      HIPCHK(hipMemcpyAsync(c->d_aos.p + (size_t)start[k] * stride, tiles[k],
+                            (size_t)counts[k] * stride,
+                            on_device ? hipMemcpyDeviceToDevice
+                                      : hipMemcpyHostToDevice,
+                            c->stream));
+  int64_t cs[2] = {0, n};
+  bmx_status st = bind_common(c, c->d_aos.p, cs, 1, &c->geom, &L, 1, true);
+  if (st != BMX_OK) return st;
+  // bound over the context's own copy, synced back tile by tile
+  c->in_place = false;
+  c->aos_on_device = on_device != 0;
+  c->user_aos = nullptr;
+  c->tiles.assign(const_cast<void **>(tiles), const_cast<void **>(tiles) + ntiles);
+  c->tile_start.swap(start);
+  return BMX_OK;
 }
 
 bmx_status bmx_bind_colonies_aos(bmx_ctx *c, const void *aos,
@@ -977,17 +1120,28 @@ bmx_status bmx_sync_to_aos(bmx_ctx *c, void *aos) {
   (void)hipSetDevice(c->device);
   st = pack_records(c);
   if (st != BMX_OK) return st;
-  void *dst = aos ? aos : c->user_aos;
-  if (!dst)
-    return fail(BMX_ERR_STATE,
-                "the particle count has changed since bind: pass a record "
-                "buffer with room for bmx_num_particles() records");
-  size_t bytes = (size_t)c->n * (size_t)c->lay.stride;
-  if (dst && bytes)
-    HIPCHK(hipMemcpyAsync(dst, c->d_aos.p, bytes,
-                          c->aos_on_device ? hipMemcpyDeviceToDevice
-                                           : hipMemcpyDeviceToHost,
-                          c->stream));
+  const hipMemcpyKind kind =
+      c->aos_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  const size_t stride = (size_t)c->lay.stride;
+  if (!aos && !c->tiles.empty()) {
+    // a level bound as tiles: each tile gets its own records back
+    for (size_t k = 0; k < c->tiles.size(); k++) {
+      const size_t b = (size_t)(c->tile_start[k + 1] - c->tile_start[k]) * stride;
+      if (b)
+        HIPCHK(hipMemcpyAsync(c->tiles[k], c->recs + (size_t)c->tile_start[k] * stride,
+                              b, kind, c->stream));
+    }
+  } else {
+    void *dst = aos ? aos : c->user_aos;
+    if (!dst)
+      return fail(BMX_ERR_STATE,
+                  "the particle count has changed since bind: pass a record "
+                  "buffer with room for bmx_num_particles() records");
+    size_t bytes = (size_t)c->n * stride;
+    // bound in place, the records were packed where they live
+    if (dst != c->recs && bytes)
+      HIPCHK(hipMemcpyAsync(dst, c->recs, bytes, kind, c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   return BMX_OK;
 }
@@ -1009,12 +1163,14 @@ bmx_status bmx_init_bonds(bmx_ctx *c) {
   return check_flags(c);
 }
 
-bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
-  bmx_status st = require_ready(c);
-  if (st != BMX_OK) return st;
-  if (nsteps < 0 || !std::isfinite(dt))
-    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
-  (void)hipSetDevice(c->device);
+}  // extern "C"
+namespace {
+// The steps of bmx_evolve, enqueued, with the closing skin check (unless
+// close_check is false: the rebuild points still check) and the read-back
+// of the error words; nothing waits for them.
+bmx_status enqueue_evolve(bmx_ctx *c, double dt, int32_t nsteps,
+                          bool close_check = true) {
+  bmx_status st;
   for (int32_t k = 0; k < nsteps; k++) {
     st = enqueue_step(c, dt, k == nsteps - 1);
     if (st != BMX_OK) return st;
@@ -1023,11 +1179,320 @@ bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
     st = close_run(c);
     if (st != BMX_OK) return st;
   }
-  if (verlet_checked(c) && nsteps > 0 && c->have_list)
+  if (close_check && verlet_checked(c) && nsteps > 0 && c->have_list)
     HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
                              0.5 * c->nbp.skin, 0, c->d_slotmax.p, c->d_err.p,
                              c->stream));
-  return check_flags(c);
+  return enqueue_flags(c);
+}
+}  // namespace
+extern "C" {
+
+}  // extern "C"
+namespace {
+
+// Host bookkeeping of a guarded period's start.
+struct GuardMark {
+  int slot;
+  int32_t steps;       // steps of the period in this call
+  int32_t done0;       // steps of the call before it
+  int64_t step, last_build;
+  int cs, cd;
+  bool have_list, wrote_step;
+};
+
+template <typename T>
+hipError_t dev_copy(T *dst, const T *src, int64_t n, hipStream_t s) {
+  return hipMemcpyAsync(dst, src, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, s);
+}
+
+// Snapshot of the state a period starts from: the current dynamic and
+// static arrays (the ones the next step reads).
+bmx_status guard_save(bmx_ctx *c, int slot) {
+  bmx_ctx::Snap &z = c->gsnap[slot];
+  const int64_t cap = c->ncap;
+  hipStream_t s = c->stream;
+  HIPCHK(z.pos.alloc(cap));
+  HIPCHK(z.oid.alloc(cap));
+  HIPCHK(z.type.alloc(cap));
+  HIPCHK(z.key.alloc(cap));
+  HIPCHK(dev_copy(z.pos.p, c->pos[c->cd].p, cap, s));
+  HIPCHK(dev_copy(z.oid.p, c->oid[c->cs].p, cap, s));
+  HIPCHK(dev_copy(z.type.p, c->type[c->cs].p, cap, s));
+  HIPCHK(dev_copy(z.key.p, c->key[c->cs].p, cap, s));
+  if (c->ncol > 1) {
+    HIPCHK(z.col.alloc(cap));
+    HIPCHK(dev_copy(z.col.p, c->col[c->cs].p, cap, s));
+  }
+  if (c->any_fungi) {
+    HIPCHK(z.rod.alloc(cap));
+    HIPCHK(z.trig.alloc(cap));
+    HIPCHK(z.topo.alloc(cap));
+    HIPCHK(z.bscale.alloc(cap));
+    HIPCHK(dev_copy(z.rod.p, c->rod[c->cd].p, cap, s));
+    HIPCHK(dev_copy(z.trig.p, c->trig[c->cd].p, cap, s));
+    HIPCHK(dev_copy(z.topo.p, c->topo[c->cd].p, cap, s));
+    HIPCHK(dev_copy(z.bscale.p, c->bscale[c->cs].p, cap, s));
+  }
+  return BMX_OK;
+}
+
+bmx_status guard_restore(bmx_ctx *c, const GuardMark &m) {
+  bmx_ctx::Snap &z = c->gsnap[m.slot];
+  const int64_t cap = c->ncap;
+  hipStream_t s = c->stream;
+  HIPCHK(dev_copy(c->pos[m.cd].p, z.pos.p, cap, s));
+  HIPCHK(dev_copy(c->oid[m.cs].p, z.oid.p, cap, s));
+  HIPCHK(dev_copy(c->type[m.cs].p, z.type.p, cap, s));
+  HIPCHK(dev_copy(c->key[m.cs].p, z.key.p, cap, s));
+  if (c->ncol > 1) HIPCHK(dev_copy(c->col[m.cs].p, z.col.p, cap, s));
+  if (c->any_fungi) {
+    HIPCHK(dev_copy(c->rod[m.cd].p, z.rod.p, cap, s));
+    HIPCHK(dev_copy(c->trig[m.cd].p, z.trig.p, cap, s));
+    HIPCHK(dev_copy(c->topo[m.cd].p, z.topo.p, cap, s));
+    HIPCHK(dev_copy(c->bscale[m.cs].p, z.bscale.p, cap, s));
+  }
+  c->step = m.step;
+  c->last_build = m.last_build;
+  c->cs = m.cs;
+  c->cd = m.cd;
+  c->have_list = m.have_list;
+  c->wrote_step = m.wrote_step;
+  return BMX_OK;
+}
+
+// bmx_evolve with the guard on: the call's steps in periods that end at
+// the Verlet list's rebuild points (or at the end of the call).  Each
+// period starts from a snapshot and ends with the skin check, its error
+// word copied to its own host word; the next period is enqueued before the
+// previous one's word is read, so the device does not wait on the host.  A
+// period whose displacement bound exceeded skin/2 is restored from its
+// snapshot and taken again with lists rebuilt before every step (a fresh
+// list never misses a pair); the period after it is discarded and
+// enqueued again.
+bmx_status evolve_guarded(bmx_ctx *c, double dt, int32_t nsteps) {
+  const int K = c->nbp.rebuild_every;
+  hipStream_t s = c->stream;
+  for (auto &z : c->gsnap)
+    if (!z.done) HIPCHK(hipEventCreateWithFlags(&z.done, hipEventDisableTiming));
+  GuardMark q[2];
+  int nq = 0, next_slot = 0;
+  int32_t done = 0;
+  bmx_status st = BMX_OK;
+  c->guarding = true;
+  auto finish = [&](bmx_status r) {
+    c->guarding = false;
+    return r;
+  };
+  while (done < nsteps || nq > 0) {
+    if (done < nsteps && nq < 2) {
+      // the period: up to the next rebuild point
+      const bool due = !c->have_list || (c->step - c->last_build) >= K;
+      const int32_t left = due ? K : (int32_t)(K - (c->step - c->last_build));
+      const int32_t len = std::min<int32_t>(left, nsteps - done);
+      GuardMark m{next_slot, len, done, c->step, c->last_build, c->cs, c->cd,
+                  c->have_list, c->wrote_step};
+      next_slot ^= 1;
+      st = guard_save(c, m.slot);
+      if (st != BMX_OK) return finish(st);
+      for (int32_t k = 0; k < len; k++) {
+        st = enqueue_step(c, dt, done + k == nsteps - 1);
+        if (st != BMX_OK) return finish(st);
+      }
+      // the period's bound, cleared when the period is complete
+      const bool ends = (c->step - c->last_build) >= K;
+      HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
+                               0.5 * c->nbp.skin, ends ? 1 : 0, c->d_slotmax.p,
+                               c->d_err.p, s));
+      HIPCHK(hipMemcpyAsync(c->h_flags + 8 + m.slot, c->d_err.p, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(launch_fill_i32(c->d_err.p, 1, 0, s));
+      HIPCHK(hipEventRecord(c->gsnap[m.slot].done, s));
+      q[nq++] = m;
+      done += len;
+      continue;
+    }
+    // the oldest period in flight
+    const GuardMark m = q[0];
+    HIPCHK(hipEventSynchronize(c->gsnap[m.slot].done));
+    const int32_t err = c->h_flags[8 + m.slot];
+    if (err & ERRBIT_SKIN) {
+      // drain the period after it, then take this one again
+      HIPCHK(hipStreamSynchronize(s));
+      st = guard_restore(c, m);
+      if (st != BMX_OK) return finish(st);
+      // nothing of the old period remains in the displacement record; each
+      // step taken again writes slot 0 of its own fresh period, so the last
+      // one's displacement starts the next period's record
+      HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(c->d_vpart.p),
+                             (int64_t)c->vmax_slots * c->vstride, 0ull, s));
+      for (int32_t k = 0; k < m.steps; k++) {
+        c->have_list = false;  // a fresh list before every step
+        st = enqueue_step(c, dt, m.done0 + k == nsteps - 1);
+        if (st != BMX_OK) return finish(st);
+      }
+      c->guard_redone++;
+      done = m.done0 + m.steps;
+      nq = 0;
+      continue;
+    }
+    if (err) {  // another error: reported by the flags below
+      HIPCHK(hipStreamSynchronize(s));
+      HIPCHK(launch_fill_i32(c->d_err.p, 1, err, s));
+      nq = 0;
+      break;
+    }
+    q[0] = q[1];
+    nq--;
+  }
+  c->guarding = false;
+  return enqueue_flags(c);
+}
+
+}  // namespace
+extern "C" {
+
+bmx_status bmx_set_guard(bmx_ctx *c, int32_t on) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  c->guard = on != 0;
+  reset_graph(c);
+  return BMX_OK;
+}
+
+bmx_status bmx_evolve(bmx_ctx *c, double dt, int32_t nsteps) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  if (nsteps < 0 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
+  (void)hipSetDevice(c->device);
+  if (c->guard && verlet_checked(c) && nsteps > 0) {
+    st = evolve_guarded(c, dt, nsteps);
+    if (st != BMX_OK) return st;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return read_flags(c);
+  }
+  st = enqueue_evolve(c, dt, nsteps);
+  if (st != BMX_OK) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return read_flags(c);
+}
+
+bmx_status bmx_evolve_async(bmx_ctx *c, double dt, int32_t nsteps) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  // asynchronous calls chain: the error words of all of them add up in the
+  // mapped host word until bmx_wait (with copied words, each waits for the
+  // one before)
+  bmx_status st = c->h_flags_dev ? BMX_OK : settle(c);
+  if (st != BMX_OK) return st;
+  if (!c->bound) return fail(BMX_ERR_STATE, "no particles bound");
+  if (nsteps < 0 || !std::isfinite(dt))
+    return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
+  if (c->guard && verlet_checked(c))
+    return fail(BMX_ERR_STATE, "guarded steps are taken by bmx_evolve");
+  (void)hipSetDevice(c->device);
+  st = enqueue_evolve(c, dt, nsteps, false);
+  if (st != BMX_OK) return st;
+  HIPCHK(hipEventRecord(c->done_ev, c->stream));
+  c->pending = true;
+  return BMX_OK;
+}
+
+bmx_status bmx_wait(bmx_ctx *c) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  (void)hipSetDevice(c->device);
+  return settle(c);
+}
+
+bmx_status bmx_query(bmx_ctx *c, int32_t *done) {
+  if (!c || !done) return fail(BMX_ERR_BAD_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  *done = 1;
+  if (!c->pending) return BMX_OK;
+  hipError_t e = hipEventQuery(c->done_ev);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return BMX_OK;
+  }
+  HIPCHK(e);
+  return BMX_OK;
+}
+
+bmx_status bmx_refresh_from_aos(bmx_ctx *c, const void *aos, uint32_t fields) {
+  bmx_status st = require_ready(c);
+  if (st != BMX_OK) return st;
+  const uint32_t all = kFieldPositions | kFieldSizes | kFieldOrientations |
+                       kFieldBonds | kFieldSpecies;
+  if (fields & ~all) return fail(BMX_ERR_BAD_ARG, "unknown field bits");
+  if (c->ncol != 1)
+    return fail(BMX_ERR_STATE, "refresh needs a single-colony context");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const size_t stride = (size_t)c->lay.stride;
+  const hipMemcpyKind kind =
+      c->aos_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  // the caller's records into the record array the kernels read
+  if (c->in_place) {
+    if (aos && aos != c->recs)
+      return fail(BMX_ERR_BAD_ARG, "records bound in place are re-read in place");
+  } else if (!aos && !c->tiles.empty()) {
+    for (size_t k = 0; k < c->tiles.size(); k++) {
+      const size_t b = (size_t)(c->tile_start[k + 1] - c->tile_start[k]) * stride;
+      if (b)
+        HIPCHK(hipMemcpyAsync(c->recs + (size_t)c->tile_start[k] * stride,
+                              c->tiles[k], b, kind, s));
+    }
+  } else {
+    const void *src = aos ? aos : c->user_aos;
+    if (!src)
+      return fail(BMX_ERR_STATE, "no record array to re-read: pass one");
+    if (c->n)
+      HIPCHK(hipMemcpyAsync(c->recs, src, (size_t)c->n * stride, kind, s));
+  }
+  HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
+  HIPCHK(launch_aos_refresh(c->recs, c->lay, c->n, c->d_boxes.p,
+                            static_arrays(c, c->cs), dyn_arrays(c, c->cd),
+                            fields, c->d_ustats.p, s));
+  unsigned long long hs[8];
+  HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hs[4] != 0) {
+    c->bound = false;
+    return fail(BMX_ERR_BAD_ARG,
+                "re-read records with negative radius, non-finite position, "
+                "or (segments) n_bnds outside 0..4 or c_length <= 0; "
+                "the context is unbound");
+  }
+  if (fields & kFieldSpecies) c->have_chem = false;  // reloaded at next use
+  if (fields & (kFieldPositions | kFieldSizes)) {
+    // the lists are stale: the next step rebuilds them, with bins for the
+    // new reach when sizes grew or shrank
+    double rmax, lmax;
+    std::memcpy(&rmax, &hs[0], sizeof rmax);
+    std::memcpy(&lmax, &hs[1], sizeof lmax);
+    c->have_list = false;
+    if (rmax != c->rmax || lmax != c->lmax) {
+      c->rmax = rmax;
+      c->lmax = lmax;
+      st = setup_boxes(c);
+      if (st != BMX_OK) {
+        c->bound = false;
+        return st;
+      }
+    }
+  }
+  reset_graph(c);
+  return BMX_OK;
+}
+
+bmx_status bmx_set_stream(bmx_ctx *c, void *stream) {
+  if (!c) return fail(BMX_ERR_BAD_ARG, "null context");
+  (void)hipSetDevice(c->device);
+  bmx_status st = settle(c);
+  if (st != BMX_OK) return st;
+  // work already enqueued on the old stream completes first
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  return BMX_OK;
 }
 
 bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
@@ -1037,6 +1502,8 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     return fail(BMX_ERR_BAD_ARG, "bad dt or nsteps");
   if (c->timing)
     return fail(BMX_ERR_STATE, "disable event timing for graph replay");
+  if (c->guard && verlet_checked(c))  // periods may be taken again: eager
+    return bmx_evolve(c, dt, nsteps);
   (void)hipSetDevice(c->device);
   // A captured sequence is valid for the same dt, step count, list phase and
   // buffer parity; rebuild points are part of the graph.
@@ -1354,7 +1821,7 @@ bmx_status bmx_get_int_fields(bmx_ctx *c, int32_t *out) {
   if (st != BMX_OK) return st;
   std::vector<uint8_t> host((size_t)c->n * c->lay.stride);
   if (!host.empty())
-    HIPCHK(hipMemcpyAsync(host.data(), c->d_aos.p, host.size(),
+    HIPCHK(hipMemcpyAsync(host.data(), c->recs, host.size(),
                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int64_t i = 0; i < c->n; i++)
@@ -1453,6 +1920,7 @@ bmx_status bmx_get_timing(bmx_ctx *c, bmx_timing *out) {
   out->nbins = c->nbins;
   out->rebuilds_total = c->rebuilds_total;
   out->steps_total = c->step;
+  out->guard_redone = c->guard_redone;
   return BMX_OK;
 }
 
@@ -1487,7 +1955,7 @@ extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) 
   if (!c->have_chem) {
     HIPCHK(c->d_chem.alloc((size_t)c->ncap));
     HIPCHK(c->d_chem_inc.alloc((size_t)c->ncap));
-    HIPCHK(launch_chem_unpack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+    HIPCHK(launch_chem_unpack(c->recs, c->lay, c->n, c->d_chem.p,
                               c->d_chem_inc.p, s));
     c->have_chem = true;
   }
@@ -1837,7 +2305,7 @@ bmx_status mesh_deposit(bmx_ctx *c, int scheme, int comp0, int ncomp, int mode,
     HIPCHK(c->d_mout.alloc((size_t)(nc * ncomp)));
     dst = c->d_mout.p;
   }
-  HIPCHK(launch_deposit(c->d_aos.p, c->lay, g, tri, c->d_mstart.p, c->d_mperm.p,
+  HIPCHK(launch_deposit(c->recs, c->lay, g, tri, c->d_mstart.p, c->d_mperm.p,
                         c->pos[c->cd].p, comp0, ncomp, mode, dst, c->stream));
   if (!on_device)
     HIPCHK(hipMemcpyAsync(out, dst, sizeof(double) * (size_t)(nc * ncomp),
@@ -1907,11 +2375,11 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
   // species are reloaded at the next exchange, and the rows are rebuilt at
   // the next step for the new reaches
   if (c->have_chem)
-    HIPCHK(launch_chem_pack(c->d_aos.p, c->lay, c->n, c->d_chem.p,
+    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, c->d_chem.p,
                             c->d_chem_inc.p, s));
   c->have_chem = false;
   HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
-  HIPCHK(launch_mesh_to_particle(c->d_aos.p, c->lay, c->n, g, c->d_mstart.p,
+  HIPCHK(launch_mesh_to_particle(c->recs, c->lay, c->n, g, c->d_mstart.p,
                                  c->oid[c->cs].p, c->type[c->cs].p,
                                  c->pos[c->cd].p, c->rod[c->cd].p, X, V,
                                  cnc_scheme, vf_scheme, chempar, dt, nloop,
@@ -1952,11 +2420,20 @@ bmx_status rebind_records(bmx_ctx *c, const uint8_t *src, int64_t n) {
   L.n_int = c->n_int;
   const bool was_on_device = c->aos_on_device;
   void *user = c->user_aos;
+  std::vector<void *> tiles;
+  std::vector<int64_t> tile_start;
+  tiles.swap(c->tiles);
+  tile_start.swap(c->tile_start);
   const bmx_geometry g0 = c->colony_geom[0];
   int64_t cs[2] = {0, n};
   bmx_status st = bind_common(c, copy.p, cs, 1, &g0, &L, 1);
   c->aos_on_device = was_on_device;
   c->user_aos = user;
+  if (!tile_start.empty() && tile_start.back() == n) {
+    // same particles: the tiles still hold them
+    c->tiles.swap(tiles);
+    c->tile_start.swap(tile_start);
+  }
   return st;
 }
 
@@ -1991,7 +2468,7 @@ extern "C" bmx_status bmx_split_particles(bmx_ctx *c, const bmx_split_params *sp
   HIPCHK(cnt.alloc((size_t)n + 1));
   HIPCHK(start.alloc((size_t)n + 1));
   HIPCHK(scratch.alloc((size_t)scan_scratch_elems(n) + 1));
-  HIPCHK(launch_split_count(c->d_aos.p, c->lay, n, p, seed, call, cnt.p, s));
+  HIPCHK(launch_split_count(c->recs, c->lay, n, p, seed, call, cnt.p, s));
   HIPCHK(launch_scan(cnt.p, start.p, n, scratch.p, s));
   int32_t total = 0;
   HIPCHK(hipMemcpyAsync(&total, start.p + n, sizeof total,
@@ -2005,7 +2482,7 @@ extern "C" bmx_status bmx_split_particles(bmx_ctx *c, const bmx_split_params *sp
     DevBuf<uint8_t> grown;
     Releaser<uint8_t> r4{grown};
     HIPCHK(grown.alloc((size_t)(n + total) * stride));
-    HIPCHK(hipMemcpyAsync(grown.p, c->d_aos.p, (size_t)n * stride,
+    HIPCHK(hipMemcpyAsync(grown.p, c->recs, (size_t)n * stride,
                           hipMemcpyDeviceToDevice, s));
     HIPCHK(launch_split_apply(grown.p, c->lay, n, start.p, p, seed, call,
                               next_id, cpu, s));
@@ -2058,7 +2535,7 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   HIPCHK(tgt.alloc((size_t)n + 1));
   HIPCHK(ttau.alloc((size_t)n + 1));
   HIPCHK(launch_fill_i32(flag.p, 1, 0, s));
-  HIPCHK(launch_tip_fusion(c->d_aos.p, c->lay, n, c->pos[c->cd].p,
+  HIPCHK(launch_tip_fusion(c->recs, c->lay, n, c->pos[c->cd].p,
                            c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap, g0,
                            wa, fus, seed, call, tgt.p, ttau.p, flag.p, s));
   int32_t any = 0;
@@ -2079,7 +2556,7 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   HIPCHK(scratch.alloc((size_t)scan_scratch_elems(n) + 1));
   HIPCHK(dec.alloc((size_t)n));
   HIPCHK(dtau.alloc((size_t)n));
-  HIPCHK(launch_interior_fusion(c->d_aos.p, c->lay, n, c->pos[c->cd].p,
+  HIPCHK(launch_interior_fusion(c->recs, c->lay, n, c->pos[c->cd].p,
                                 c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
                                 g0, wa, cnt.p, dec.p, dtau.p, one.p, s));
   HIPCHK(launch_scan(one.p, start.p, n, scratch.p, s));
@@ -2092,7 +2569,7 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   DevBuf<uint8_t> grown;
   Releaser<uint8_t> r7{grown};
   HIPCHK(grown.alloc((size_t)(n + total) * stride));
-  HIPCHK(hipMemcpyAsync(grown.p, c->d_aos.p, (size_t)n * stride,
+  HIPCHK(hipMemcpyAsync(grown.p, c->recs, (size_t)n * stride,
                         hipMemcpyDeviceToDevice, s));
   HIPCHK(launch_fusion_split(grown.p, c->lay, n, cnt.p, start.p, next_id, cpu, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2106,12 +2583,12 @@ extern "C" bmx_status bmx_fusion(bmx_ctx *c, const bmx_fusion_params *fp,
   if (st != BMX_OK) return st;
   st = check_flags(c);
   if (st != BMX_OK) return st;
-  HIPCHK(launch_fusion_cleanup(c->d_aos.p, c->lay, c->n, c->pos[c->cd].p,
+  HIPCHK(launch_fusion_cleanup(c->recs, c->lay, c->n, c->pos[c->cd].p,
                                c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
                                c->boxes[0], fus[2], 1.5 * fus[3], s));
   HIPCHK(hipStreamSynchronize(s));
   void *user = c->user_aos;
-  st = rebind_records(c, c->d_aos.p, c->n);
+  st = rebind_records(c, c->recs, c->n);
   c->user_aos = user;
   return st;
 }
@@ -2225,7 +2702,7 @@ bmx_status na_setup(NaDevice &d, int dev) {
   if ((size_t)(cur - ptr) > kNaBytes)
     return fail(BMX_ERR_HIP, "scratch layout exceeds the scratch array");
   // the record array is the caller's; a colony array is never needed
-  c.d_aos.fixed = true;
+  c.d_aos.fixed = true;  // never used: the records are bound in place
   c.d_col_in.fixed = true;
   c.h_flags = d.flags;
   // host vectors reach their sizes here, not in a call

@@ -35,6 +35,18 @@ hipError_t launch_aos_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
                              const int32_t *col, const BoxGeom *boxes,
                              StaticArrays st, DynArrays dy,
                              unsigned long long *stats, hipStream_t s);
+// Fields bmx_refresh_from_aos re-reads (the BMX_FIELD_* values of the ABI).
+enum : uint32_t {
+  kFieldPositions = 1, kFieldSizes = 2, kFieldOrientations = 4,
+  kFieldBonds = 8, kFieldSpecies = 16
+};
+// AoS -> the current sorted arrays, selected fields only, in place of the
+// values there (stats as launch_aos_unpack: [0] max radius, [1] max
+// c_length, [4] bad records)
+hipError_t launch_aos_refresh(const uint8_t *aos, AosLayout lay, int64_t n,
+                              const BoxGeom *boxes, StaticArrays st,
+                              DynArrays dy, uint32_t fields,
+                              unsigned long long *stats, hipStream_t s);
 // sorted SoA -> AoS at original positions (fields the force pass writes)
 hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy, const double *fc,
@@ -97,6 +109,12 @@ hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     hipStream_t s);
 // blocks of a row build / force launch over n particles (partial arrays)
 int32_t row_blocks(int64_t n, int block);
+// The end of a call in one launch: row statistics (nblk > 0), then err,
+// longest row and entry total (low, high word) into host_out (host memory
+// mapped for the device), err cleared.
+hipError_t launch_flags_out(const int32_t *bstats, int32_t nblk, int32_t *stats,
+                            unsigned long long *total, int32_t *err,
+                            int32_t *host_out, hipStream_t s);
 // stats[0] = longest row, *total = entries, from the per-block partials
 hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
                                   int32_t *stats, unsigned long long *total,

@@ -173,6 +173,89 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
   }
 }
 
+// Re-reads the selected fields of every bound particle from its record
+// into the current sorted arrays (slot t holds original index st.oid[t]),
+// the inverse of k_aos_pack for those fields; the sorted order, the static
+// arrays and everything not selected are kept.  stats as in k_aos_unpack
+// (maxima over the particles after the update, bad records).
+__global__ void k_aos_refresh(const uint8_t *__restrict__ aos, AosLayout lay,
+                              int64_t n, const BoxGeom *__restrict__ boxes,
+                              StaticArrays st, DynArrays dy, uint32_t fields,
+                              unsigned long long *stats) {
+  __shared__ double shd[16];
+  __shared__ unsigned long long shu[16];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rmax = 0.0, lmax = 0.0;
+  unsigned long long nbad = 0;
+  if (t < n) {
+    const int64_t o = st.oid[t];
+    const uint8_t *r = aos + o * (int64_t)lay.stride;
+    const double *pos = reinterpret_cast<const double *>(r + lay.pos);
+    const double *rd = reinterpret_cast<const double *>(r + lay.rdata);
+    const int32_t *id = reinterpret_cast<const int32_t *>(r + lay.idata);
+    const BoxGeom &g = boxes[st.col ? st.col[t] : 0];
+    double4 p = dy.pos[t];
+    if (fields & kFieldPositions) {
+      p.x = wrap_axis(pos[0], g.plo[0], g.phi[0], g.L[0], g.periodic[0]);
+      p.y = wrap_axis(pos[1], g.plo[1], g.phi[1], g.L[1], g.periodic[1]);
+      p.z = wrap_axis(pos[2], g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
+    }
+    if (fields & kFieldSizes) p.w = rd[rec::radius];
+    dy.pos[t] = p;
+    bool bad = !(p.w >= 0.0) || !(p.x == p.x) || !(p.y == p.y) || !(p.z == p.z);
+    if (p.w >= 0.0) rmax = p.w;
+    const int ty = st.type[t];
+    if (dy.rod) {
+      double4 rr = dy.rod[t];
+      if (fields & kFieldSizes) {
+        rr.x = rd[rec::c_length];
+        rr.w = rd[rec::area];
+        st.bscale[t] = rd[rec::bond_scale];
+      }
+      if (fields & kFieldOrientations) {
+        rr.y = rd[rec::theta];
+        rr.z = rd[rec::phi];
+        dy.trig[t] = make_double4(bmx_cos(rr.y), bmx_sin(rr.y), bmx_cos(rr.z),
+                                  bmx_sin(rr.z));
+      }
+      dy.rod[t] = rr;
+      if (fields & kFieldBonds) {
+        Topo tp;
+        tp.n_bnds = id[rec::n_bnds];
+        for (int b = 0; b < 4; b++) {
+          tp.seg_id[b] = id[rec::seg1_id1 + b];
+          tp.seg_cpu[b] = id[rec::seg1_id2 + b];
+          tp.site[b] = id[rec::site1 + b];
+        }
+        tp.position = id[rec::position];
+        tp.fuse_tip = id[rec::fuse_tip];
+        tp.fix_site = id[rec::fix_site];
+        tp.self_id = id[rec::id];
+        tp.self_cpu = id[rec::cpu];
+        tp.dltd = 0;
+        tp.pad = 0;
+        dy.topo[t] = tp;
+      }
+      if (ty == rec::FUNGI) {
+        if (rr.x >= 0.0) lmax = rr.x;
+        const int nb = dy.topo[t].n_bnds;
+        if (nb < 0 || nb > 4 || !(rr.x > 0.0)) bad = true;
+      }
+    }
+    if (bad) nbad = 1;
+  }
+  rmax = block_max(rmax, shd);
+  __syncthreads();
+  lmax = block_max(lmax, shd);
+  __syncthreads();
+  nbad = block_sum(nbad, shu);
+  if (threadIdx.x == 0) {
+    if (rmax > 0.0) atomicMax(&stats[0], dbits(rmax));
+    if (lmax > 0.0) atomicMax(&stats[1], dbits(lmax));
+    if (nbad) atomicAdd(&stats[4], nbad);
+  }
+}
+
 __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy,
                            const double *__restrict__ fc, int64_t ncap,
@@ -1633,6 +1716,16 @@ hipError_t launch_aos_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
                      aos, lay, n, col, boxes, st, dy, stats);
   return hipGetLastError();
 }
+hipError_t launch_aos_refresh(const uint8_t *aos, AosLayout lay, int64_t n,
+                              const BoxGeom *boxes, StaticArrays st,
+                              DynArrays dy, uint32_t fields,
+                              unsigned long long *stats, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_refresh, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     aos, lay, n, boxes, st, dy, fields, stats);
+  return hipGetLastError();
+}
+
 hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy, const double *fc,
                            int64_t ncap, double mob, int wrote_step,
@@ -1725,6 +1818,51 @@ hipError_t launch_build_nbr(const double4 *pos, const int32_t *col,
 }
 
 int32_t row_blocks(int64_t n, int block) { return grid_for(n, block); }
+
+// The tail of a call in one launch: the row statistics of the last build
+// (nblk > 0), then the error word (OR-ed in: calls enqueued back to back
+// add up until the host reads and clears it), the longest row and the entry
+// total written straight into host memory mapped for the device (out: 4
+// words), and the device's error word cleared.
+__global__ __launch_bounds__(1024) void k_flags_out(
+    const int32_t *__restrict__ bstats, int32_t nblk,
+    int32_t *__restrict__ stats, unsigned long long *__restrict__ total,
+    int32_t *__restrict__ err, int32_t *__restrict__ out) {
+  __shared__ int32_t shi[16];
+  __shared__ unsigned long long shu[16];
+  if (nblk > 0) {
+    int m = 0;
+    unsigned long long sm = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+      int v = bstats[2 * b];
+      m = v > m ? v : m;
+      sm += (unsigned long long)bstats[2 * b + 1];
+    }
+    m = block_max(m, shi);
+    __syncthreads();
+    sm = block_sum(sm, shu);
+    if (threadIdx.x == 0) {
+      stats[0] = m;
+      *total = sm;
+    }
+  }
+  if (threadIdx.x == 0) {
+    const unsigned long long tot = *total;
+    out[0] |= *err;  // the host clears the word when it reads it
+    out[1] = stats[0];
+    out[2] = (int32_t)(uint32_t)(tot & 0xffffffffull);
+    out[3] = (int32_t)(uint32_t)(tot >> 32);
+    *err = 0;
+  }
+}
+
+hipError_t launch_flags_out(const int32_t *bstats, int32_t nblk, int32_t *stats,
+                            unsigned long long *total, int32_t *err,
+                            int32_t *host_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_flags_out, dim3(1), dim3(1024), 0, s, bstats, nblk,
+                     stats, total, err, host_out);
+  return hipGetLastError();
+}
 
 hipError_t launch_reduce_rowstats(const int32_t *bstats, int32_t nblk,
                                   int32_t *stats, unsigned long long *total,

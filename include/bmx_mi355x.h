@@ -13,7 +13,8 @@
  *
  * Plain C types only.  Every call returns a bmx_status; bmx_last_error()
  * gives a message for the calling thread's last failure.  A context owns one
- * HIP stream and all device buffers; calls on one context must not overlap.
+ * HIP stream (or uses the caller's, bmx_set_stream) and all device buffers;
+ * calls on one context must not overlap.
  */
 #ifndef BMX_MI355X_H
 #define BMX_MI355X_H
@@ -109,6 +110,7 @@ typedef struct {
   int32_t nbins;            /* bins at the last rebuild */
   int64_t rebuilds_total;   /* list builds since bind, timed or not */
   int64_t steps_total;      /* steps since bind */
+  int64_t guard_redone;     /* guarded periods taken again since bind */
 } bmx_timing;
 
 /* --- lifetime ---------------------------------------------------------- */
@@ -159,15 +161,36 @@ bmx_status bmx_autotune(bmx_ctx *ctx, double dt, int32_t trial_steps,
 bmx_layout bmx_default_layout(void);
 
 /* --- particles --------------------------------------------------------- */
-/* Copies n records (host memory when aos_on_device == 0, device memory
- * otherwise) into the context and applies the periodic wrap that
- * Redistribute performs (src/des/bmx_pc_init.cpp:131).  The record array is
- * remembered for bmx_sync_to_aos.  A bind replaces the previous one: when it
- * fails, the context is left with no particles bound (BMX_ERR_STATE from
- * every call that needs them until a bind succeeds). */
+/* Where bound records live: copied from host memory, copied from device
+ * memory, or device memory used in place (the kernels read and write the
+ * caller's records directly: no copy at bind or sync; the caller keeps the
+ * array alive, and its count and order unchanged, until the next bind). */
+enum { BMX_AOS_HOST = 0, BMX_AOS_DEVICE = 1, BMX_AOS_IN_PLACE = 2 };
+/* Binds n records (aos_on_device: a BMX_AOS_* value) and applies the
+ * periodic wrap that Redistribute performs (src/des/bmx_pc_init.cpp:131).
+ * The record array is remembered for bmx_sync_to_aos.  A bind replaces the
+ * previous one: when it fails, the context is left with no particles bound
+ * (BMX_ERR_STATE from every call that needs them until a bind succeeds).
+ * The lists, the sorted order and the step count start afresh; between
+ * binds the context keeps them, so a caller that binds once and then calls
+ * bmx_evolve per step (the reference's EvolveParticles cadence,
+ * src/timestepping/bmx_evolve.cpp:97) keeps its Verlet lists across calls. */
 bmx_status bmx_bind_particles_aos(bmx_ctx *ctx, const void *aos, int64_t n,
                                   const bmx_layout *layout,
                                   int32_t aos_on_device);
+/* A level held as several tiles (AMReX: one particle tile per box and tile,
+ * ParIter over the level, src/des/bmx_pc_interaction.cpp:114), bound as ONE
+ * neighbour domain with the context's geometry (the level's
+ * Geom(lev).ProbLo/ProbHi and periodicity): pairs across tile and box
+ * boundaries are found as AMReX finds them through its ghost copies
+ * (fillNeighbors, :100), and periodic images apply only at the domain's
+ * periodic faces.  Tile k holds counts[k] records (host memory, or device
+ * memory when aos_on_device == 1); particle indices run through the tiles
+ * in order.  bmx_sync_to_aos(ctx, NULL) writes each tile's records back to
+ * it. */
+bmx_status bmx_bind_tiles_aos(bmx_ctx *ctx, const void *const *tiles,
+                              const int64_t *counts, int32_t ntiles,
+                              const bmx_layout *layout, int32_t aos_on_device);
 /* Several independent periodic boxes ("colonies") in one context: particles
  * [colony_start[c], colony_start[c+1]) live in geoms[c].  Each colony's
  * neighbor search, bins and wrap are its own. */
@@ -178,8 +201,26 @@ bmx_status bmx_bind_colonies_aos(bmx_ctx *ctx, const void *aos,
                                  int32_t aos_on_device);
 /* Writes positions, velocities, angular velocities, orientation angles and
  * the integer fields the force pass changes back into 'aos' (host or device,
- * as bound; NULL = the bound array). */
+ * as bound; NULL = the bound array, or each bound tile; records bound in
+ * place are written where they are). */
 bmx_status bmx_sync_to_aos(bmx_ctx *ctx, void *aos);
+/* Fields bmx_refresh_from_aos re-reads. */
+enum {
+  BMX_FIELD_POSITIONS = 1,    /* pos[3] */
+  BMX_FIELD_SIZES = 2,        /* radius; segments: c_length, area, bond_scale */
+  BMX_FIELD_ORIENTATIONS = 4, /* segments: theta, phi */
+  BMX_FIELD_BONDS = 8,        /* segments: n_bnds, bond ids/cpus/sites,
+                                 position, fuse_tip, fix_site */
+  BMX_FIELD_SPECIES = 16      /* rdata[25..30] (exchange state) */
+};
+/* The caller changed the given fields of the bound particles (same count,
+ * same order): re-read just those, keeping the sorted order and, unless
+ * positions or sizes changed, the neighbour lists.  aos = NULL re-reads the
+ * bound array (or tiles); records bound in place are re-read in place.
+ * Positions or sizes make the next step rebuild the lists (with new bins
+ * when the largest reach changed).  A record that fails the bind's checks
+ * unbinds the context (BMX_ERR_BAD_ARG). */
+bmx_status bmx_refresh_from_aos(bmx_ctx *ctx, const void *aos, uint32_t fields);
 int64_t bmx_num_particles(const bmx_ctx *ctx);
 
 /* --- the hot path ------------------------------------------------------ */
@@ -193,6 +234,34 @@ bmx_status bmx_evolve(bmx_ctx *ctx, double dt, int32_t nsteps);
 /* Same steps, replayed from one captured HIP graph (captured on first use
  * for this (dt, nsteps) and re-used while the configuration is unchanged). */
 bmx_status bmx_evolve_graph(bmx_ctx *ctx, double dt, int32_t nsteps);
+/* Guarded Verlet steps (on = 1; off by default): bmx_evolve checks every
+ * rebuild period as it ends, and a period in which the summed per-step
+ * displacement bound exceeded skin/2 (a pair may have come into range
+ * unlisted) is restored from a snapshot taken at its start and taken again
+ * with lists rebuilt before every step, so BMX_ERR_SKIN is never returned
+ * and every force uses a list that holds all interacting pairs, as the
+ * reference's per-step rebuild (src/des/bmx_pc_interaction.cpp:97-103)
+ * does.  The next period is enqueued before the previous one's check is
+ * read, so the device does not wait on it; the snapshot is one copy of the
+ * particle arrays per period.  With the guard on, bmx_evolve_graph runs
+ * these eager guarded steps.  bmx_timing.guard_redone counts the periods
+ * taken again. */
+bmx_status bmx_set_guard(bmx_ctx *ctx, int32_t on);
+/* The steps of bmx_evolve, enqueued on the context's stream without waiting
+ * for them; asynchronous calls may follow one another, the device running
+ * them back to back.  bmx_wait completes them and returns their combined
+ * status (list overflow, skin, non-finite force); any other call on the
+ * context does so first too, so an error of asynchronous steps is reported
+ * by bmx_wait or by the next call.  The skin check of asynchronous steps
+ * runs at the rebuild points (not also at the end of each call).  Not with
+ * the guard on.  bmx_query sets *done to 1 once they have finished. */
+bmx_status bmx_evolve_async(bmx_ctx *ctx, double dt, int32_t nsteps);
+bmx_status bmx_wait(bmx_ctx *ctx);
+bmx_status bmx_query(bmx_ctx *ctx, int32_t *done);
+/* Enqueue the context's work on the caller's stream (a hipStream_t; NULL =
+ * the context's own stream, the default).  Work already enqueued completes
+ * first.  Keep the stream alive while the context uses it. */
+bmx_status bmx_set_stream(bmx_ctx *ctx, void *stream);
 
 /* --- gradients (no counterpart in the reference) ------------------------ */
 typedef struct {

@@ -18,8 +18,12 @@ from test_gpu_decks import parse_deck
 
 pytestmark = pytest.mark.gpu
 
+# (checkpoint, deck); the 23,522-bacteria monolayer of
+# exec/nlev_large_test/dbg/chk300000 lies on level 2 (the refined slab
+# around the growth surface), whose 128 x 128 x 64 mesh sets its bins
 CASES = [("rst_test_chk20000", "rst_test/input_real_nlev"),
-         ("fungi_chk357900", "fungi/input_fungi")]
+         ("fungi_chk357900", "fungi/input_fungi"),
+         ("nlev_large_chk300000", "nlev_large_test/input_real_nlev")]
 
 
 def deck_setup(path):
@@ -52,7 +56,13 @@ def deck_setup(path):
 def test_restart_continues_like_the_oracle(bmx, oracle, gpu, ckpt, deck):
     lo, hi, per, n_cell, fpar, hood, xpar, dt = deck_setup(
         os.path.join(GOLDEN, "ref_decks", deck))
-    recs = bmx.read_checkpoint(os.path.join(GOLDEN, "ref_checkpoints", ckpt))
+    path = os.path.join(GOLDEN, "ref_checkpoints", ckpt)
+    # the particles' level (the checkpoint's finest) sets the mesh cells
+    lev = bmx.checkpoint_info(path)["finest_level"]
+    kv = parse_deck(os.path.join(GOLDEN, "ref_decks", deck))
+    n_cell = [int(kv["amr.n_cell"][d]) * 2 ** lev for d in range(3)]
+    recs = bmx.read_checkpoint(path)
+    print(f"restart {ckpt}: {len(recs)} particles, level {lev}, mesh {n_cell}", flush=True)
     ctx = bmx.Context(0)
     ctx.set_force_params(fpar)
     ctx.set_geometry(lo, hi, per, n_cell)

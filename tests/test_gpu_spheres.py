@@ -296,3 +296,37 @@ def test_failed_rebind_leaves_context_unbound(bmx, gpu, c1):
     ctx.bind(c1.recs.copy())
     ctx.evolve(W.DT, 1)
     ctx.close()
+
+
+def test_guard_takes_violating_periods_again(bmx, gpu):
+    """A skin of 1e-9 cm is far too small for lists kept 10 steps: unguarded,
+    the call fails with BMX_ERR_SKIN; guarded, every such period is taken
+    again with a fresh list before each step, so the run advances and equals
+    a run that rebuilds every step (the same lists: bit for bit) and the
+    reference's stencil list every step (same pairs, another order: 1e-12)."""
+    col = W.c1_small()
+    col.skin = 1e-9
+    plain, _ = gpu_ctx(bmx, col, K=10)
+    with pytest.raises(bmx.BMXError) as e:
+        plain.evolve(W.DT, 10)
+    assert e.value.code == 6
+    plain.close()
+    g, _ = gpu_ctx(bmx, col, K=10)
+    g.set_guard(True)
+    g.evolve(W.DT, 7)
+    g.evolve(W.DT, 23)                 # a period spanning two calls
+    t = g.timing()
+    assert t.guard_redone >= 3 and t.steps_total == 30
+    every, _ = gpu_ctx(bmx, col, K=1)
+    every.evolve(W.DT, 30)
+    assert np.array_equal(g.positions(), every.positions())
+    L = col.prob_hi - col.prob_lo
+    n_cell = tuple(int(v) for v in np.floor(L / 2.0e-4))
+    sten, _ = gpu_ctx(bmx, col, mode=1, K=1, n_cell=n_cell, neighborhood=1.0)
+    sten.evolve(W.DT, 30)
+    np.testing.assert_allclose(g.positions(), sten.positions(), rtol=0,
+                               atol=1e-12 * L.max())
+    moved = np.abs(g.positions() - col.recs["pos"]).max()
+    assert moved > 1e-9                # the skin was exceeded many times over
+    for c in (g, every, sten):
+        c.close()
