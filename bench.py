@@ -264,12 +264,24 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
     pmc_path = os.path.join(ROOT, "profiles", "force_general_200k_pmc.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as f:
-            flop = json.load(f).get("fp64_flop_per_launch")
+            prof = json.load(f)
+        flop = prof.get("fp64_flop_per_launch")
         if flop:
+            # algorithmic FP64 operations counted on the oracle over this
+            # workload (the kernel's formulation; tests/test_oracle_flops.py)
             ach = flop / (force_ms * 1e-3) / 1e12
+            ref_flop = prof.get("fp64_flop_per_launch_reference_formulation")
+            cnt = prof.get("counters_per_launch", {})
             roof = {"bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFS,
                     "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFS,
-                    "flop_per_launch": flop}
+                    "traffic": None,
+                    "flop_per_launch": flop,
+                    "interacting_pairs_per_launch": prof.get("interacting_pairs_per_launch"),
+                    "reference_formulation_flop_per_launch": ref_flop,
+                    "reference_formulation_equivalent_tflops":
+                        ref_flop / (force_ms * 1e-3) / 1e12 if ref_flop else None,
+                    "valu_utilization_pct": cnt.get("VALUUtilization"),
+                    "valu_busy_pct": cnt.get("VALUBusy")}
     return {
         "workload": f"{n} hyphal segments (2,000 roots), mesh bins {col.n_cell}, "
                     f"reference stencil list every step, {steps} steps",
