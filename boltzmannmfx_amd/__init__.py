@@ -242,9 +242,58 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+class _DLDevice(C.Structure):
+    _fields_ = [("device_type", C.c_int32), ("device_id", C.c_int32)]
+
+
+class _DLDataType(C.Structure):
+    _fields_ = [("code", C.c_uint8), ("bits", C.c_uint8), ("lanes", C.c_uint16)]
+
+
+class _DLTensor(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("device", _DLDevice), ("ndim", C.c_int32),
+                ("dtype", _DLDataType), ("shape", C.POINTER(C.c_int64)),
+                ("strides", C.POINTER(C.c_int64)), ("byte_offset", C.c_uint64)]
+
+
+_KDL_ROCM = 10   # DLDeviceType kDLROCM
+_KDL_FLOAT = 2   # DLDataTypeCode kDLFloat
+
+
+def _dlpack_view(a, device_type: int = _KDL_ROCM):
+    """(address, element count) of a contiguous float64 array through the
+    DLPack protocol, None for an object without it or on another device
+    (ROCm, unless `device_type` says otherwise).
+    The capsule is read, not consumed: its destructor hands the tensor back
+    to its producer, and the address stays valid while `a` lives."""
+    dev = getattr(a, "__dlpack_device__", None)
+    if dev is None or not hasattr(a, "__dlpack__"):
+        return None
+    if int(dev()[0]) != device_type:
+        return None
+    cap = a.__dlpack__()
+    get = C.pythonapi.PyCapsule_GetPointer
+    get.restype = C.c_void_p
+    get.argtypes = [C.py_object, C.c_char_p]
+    t = _DLTensor.from_address(get(cap, b"dltensor"))  # DLManagedTensor starts with it
+    assert t.dtype.code == _KDL_FLOAT and t.dtype.bits == 64 and t.dtype.lanes == 1, \
+        "device meshes are float64"
+    shape = [t.shape[i] for i in range(t.ndim)]
+    if t.strides:
+        want = 1
+        for i in reversed(range(t.ndim)):
+            assert shape[i] == 1 or t.strides[i] == want, "device meshes must be contiguous"
+            want *= shape[i]
+    return int(t.data) + int(t.byte_offset), int(np.prod(shape, dtype=np.int64))
+
+
 def _device_view(a):
-    """(address, element count) of a contiguous float64 device array that
-    exposes __cuda_array_interface__, else None."""
+    """(address, element count) of a contiguous float64 device array, taken
+    through DLPack (`__dlpack__` on the ROCm device) or
+    `__cuda_array_interface__`; None for a host array."""
+    dl = _dlpack_view(a)
+    if dl is not None:
+        return dl
     cai = getattr(a, "__cuda_array_interface__", None)
     if cai is None:
         return None
@@ -329,8 +378,9 @@ class Context:
         self._n_cell = tuple(int(v) for v in n_cell)
 
     # particle <-> mesh.  Host meshes are numpy arrays [comp, k, j, i];
-    # device meshes are any float64 array on the context's device exposing
-    # __cuda_array_interface__ (a torch tensor, for one), passed by address.
+    # device meshes are any contiguous float64 array on the context's device
+    # that speaks DLPack or __cuda_array_interface__ (a torch tensor, for
+    # one), passed by address.
     def _mesh_arg(self, a, count: int):
         dv = _device_view(a)
         if dv is not None:

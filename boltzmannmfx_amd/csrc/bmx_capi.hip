@@ -825,18 +825,6 @@ void bmx_io_set_error(const char *msg) { g_last_error = msg ? msg : ""; }
 
 const char *bmx_last_error(void) { return g_last_error.c_str(); }
 
-bmx_layout bmx_default_layout(void) {
-  bmx_layout L;
-  L.stride_bytes = 448;
-  L.pos_offset = 0;
-  L.rdata_offset = 24;
-  L.id_offset = 296;
-  L.cpu_offset = 300;
-  L.idata_offset = 304;
-  L.n_real = 34;
-  L.n_int = 35;
-  return L;
-}
 
 bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
   if (!out) return fail(BMX_ERR_BAD_ARG, "null output pointer");

@@ -93,6 +93,19 @@ void finish_record(Rec &r, int64_t i, int32_t pid) {
 
 extern "C" {
 
+bmx_layout bmx_default_layout(void) {
+  bmx_layout L;
+  L.stride_bytes = 448;
+  L.pos_offset = 0;
+  L.rdata_offset = 24;
+  L.id_offset = 296;
+  L.cpu_offset = 300;
+  L.idata_offset = 304;
+  L.n_real = 34;
+  L.n_int = 35;
+  return L;
+}
+
 bmx_status bmx_read_particles_ascii(const char *path, void *records,
                                     int64_t capacity, int64_t *n_out,
                                     const bmx_layout *layout) {

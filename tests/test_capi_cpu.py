@@ -231,3 +231,19 @@ def test_checkpoint_errors(bmx, tmp_path):
     # an older layout without a map
     with pytest.raises(bmx.BMXError):
         bmx.read_checkpoint(os.path.join(CKPT, "fungi_chk357900"), real_map=list(range(34)))
+
+
+def test_dlpack_view_reads_address_and_size(bmx):
+    """The DLPack reader that device meshes go through, on host arrays
+    (DLPack device type 1): address, element count, dtype and contiguity."""
+    a = np.arange(24, dtype=np.float64).reshape(2, 3, 4)
+    assert bmx._dlpack_view(a) is None  # a host array is not a device mesh
+    addr, count = bmx._dlpack_view(a, device_type=1)
+    assert addr == a.ctypes.data and count == 24
+    sub = a[1]
+    addr, count = bmx._dlpack_view(sub, device_type=1)
+    assert addr == a.ctypes.data + 12 * 8 and count == 12
+    with pytest.raises(AssertionError):
+        bmx._dlpack_view(a[:, :, ::2], device_type=1)
+    with pytest.raises(AssertionError):
+        bmx._dlpack_view(a.astype(np.float32), device_type=1)
