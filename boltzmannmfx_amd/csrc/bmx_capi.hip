@@ -454,10 +454,12 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
     bmx_status st = close_run(c);
     if (st != BMX_OK) return st;
   }
+  // the displacement check of the period this rebuild ends rides on the
+  // bin count and the scan
+  SkinCheck skc{};
   if (check_skin && !c->guarding && c->have_list && verlet_checked(c))
-    HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
-                             0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
-                             s));
+    skc = SkinCheck{c->d_vpart.p, c->vmax_slots, c->vstride, c->d_slotmax.p,
+                    0.5 * c->nbp.skin, c->d_err.p};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = next_event(c);
@@ -472,8 +474,8 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
   // the scan leaves the counts at zero; fill only a fresh array
   if (!c->count_clear) HIPCHK(launch_fill_i32(c->d_count.p, c->nbins + 1, 0, s));
   HIPCHK(launch_bin_count(sd.pos, ss.col, c->d_boxes.p, g0, multi, c->n,
-                          c->d_keytmp.p, c->d_rank.p, c->d_count.p, s));
-  HIPCHK(launch_scan(c->d_count.p, c->d_start.p, c->nbins, c->d_scan.p, s));
+                          c->d_keytmp.p, c->d_rank.p, c->d_count.p, s, skc));
+  HIPCHK(launch_scan(c->d_count.p, c->d_start.p, c->nbins, c->d_scan.p, s, skc));
   c->count_clear = true;
   HIPCHK(launch_bin_scatter(c->d_keytmp.p, c->d_rank.p, c->d_start.p, ss.oid,
                             c->n, c->d_permk.p, s));

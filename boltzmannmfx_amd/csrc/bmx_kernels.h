@@ -53,15 +53,30 @@ hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            int64_t ncap, double mob, int wrote_step,
                            int any_fungi, hipStream_t s);
 
+// The displacement check of a Verlet period folded into the first two
+// binning kernels of the rebuild that ends it: the first blocks of the bin
+// count reduce the per-step partials (as launch_skin_check with clear = 1,
+// into slotmax), and the first block of the scan sums the slot maxima,
+// compares them with limit and resets them.  vpart = nullptr: no check.
+struct SkinCheck {
+  double *vpart;
+  int32_t nslots, stride;
+  unsigned long long *slotmax;
+  double limit;
+  int32_t *err;
+};
+
 // ---- binning: key/rank, scan, scatter, per-bin order with the gather
 hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
                             const BoxGeom *boxes, BoxGeom g0, int multi,
                             int64_t n, int32_t *key, int32_t *rank,
-                            int32_t *count, hipStream_t s);
+                            int32_t *count, hipStream_t s,
+                            SkinCheck sk = SkinCheck{});
 // exclusive scan of count[0..nb) into start[0..nb]; scratch >= scan_scratch(nb)
 int64_t scan_scratch_elems(int64_t nb);
 hipError_t launch_scan(int32_t *count, int32_t *start, int64_t nb,
-                       int32_t *scratch, hipStream_t s);
+                       int32_t *scratch, hipStream_t s,
+                       SkinCheck sk = SkinCheck{});
 hipError_t launch_bin_scatter(const int32_t *key, const int32_t *rank,
                               const int32_t *start, const int32_t *oid,
                               int64_t n, unsigned long long *permk,

@@ -302,13 +302,15 @@ __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
 
 // ------------------------------------------------------------- binning ---
 
+// blocks of the bin count that take part in a folded displacement check
+constexpr int kSkinBlocks = 64;
 template <bool MULTI>
 __global__ void k_bin_count(const double4 *__restrict__ pos,
                             const int32_t *__restrict__ col,
                             const BoxGeom *__restrict__ boxes, BoxGeom g0,
                             int64_t n, int32_t *__restrict__ key,
                             int32_t *__restrict__ rank,
-                            int32_t *__restrict__ count) {
+                            int32_t *__restrict__ count, SkinCheck sk) {
   // The input is in the order of the previous rebuild, so neighboring lanes
   // mostly share a bin: each run of equal bins in the wave takes its slots
   // with one atomic (the head's) and numbers its lanes from there.  Only the
@@ -335,6 +337,28 @@ __global__ void k_bin_count(const double4 *__restrict__ pos,
   if (head) base = atomicAdd(&count[b], stop - lane);
   base = __shfl(base, first, kWave);
   if (active) rank[s] = base + (lane - first);
+  // the first kSkinBlocks blocks also reduce the displacement partials of
+  // the period this rebuild ends, one share of every slot each, and clear
+  // them (block-uniform branch: the block reductions synchronise)
+  const int nsb = gridDim.x < kSkinBlocks ? (int)gridDim.x : kSkinBlocks;
+  if (sk.vpart && (int)blockIdx.x < nsb) {
+    __shared__ double shd[256 / kWave];  // blocks of 256 (launch_bin_count)
+    const int64_t chunk = (sk.stride + nsb - 1) / nsb;
+    const int64_t c0 = blockIdx.x * chunk;
+    const int64_t c1 = c0 + chunk < sk.stride ? c0 + chunk : sk.stride;
+    for (int sl = 0; sl < sk.nslots; sl++) {
+      double *v = sk.vpart + (int64_t)sl * sk.stride;
+      double m = 0.0;
+      for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+        const double x = v[i];
+        m = x > m ? x : m;
+        v[i] = 0.0;
+      }
+      m = block_max(m, shd);
+      if (threadIdx.x == 0 && m > 0.0) atomicMax(&sk.slotmax[sl], dbits(m));
+      __syncthreads();
+    }
+  }
 }
 
 // Exclusive scan: 256 threads x 8 items per block, wave prefix by shuffles.
@@ -368,8 +392,18 @@ __device__ __forceinline__ T block_excl_scan(T s, T *wsum, T &total) {
 // leaves the count array cleared for the next binning.
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(
     int32_t *__restrict__ in, int32_t *__restrict__ out,
-    int32_t *__restrict__ bsum, int64_t n) {
+    int32_t *__restrict__ bsum, int64_t n, SkinCheck sk) {
   __shared__ int32_t wsum[kScanThreads / kWave];
+  if (sk.vpart && blockIdx.x == 0 && threadIdx.x == 0) {
+    // the folded displacement check: the slot maxima the bin count left,
+    // summed in slot order (as k_skin_final)
+    double total = 0.0;
+    for (int sl = 0; sl < sk.nslots; sl++) {
+      total += __longlong_as_double((long long)sk.slotmax[sl]);
+      sk.slotmax[sl] = 0ull;
+    }
+    if (total > sk.limit) atomicOr(sk.err, ERRBIT_SKIN);
+  }
   int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   // a thread's 8 counts are 32 aligned bytes (the arrays are allocation
   // bases): two 16-byte loads and stores, element by element at the tail
@@ -1739,14 +1773,14 @@ hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
 hipError_t launch_bin_count(const double4 *pos, const int32_t *col,
                             const BoxGeom *boxes, BoxGeom g0, int multi,
                             int64_t n, int32_t *key, int32_t *rank,
-                            int32_t *count, hipStream_t s) {
+                            int32_t *count, hipStream_t s, SkinCheck sk) {
   if (n <= 0) return hipSuccess;
   if (multi)
     hipLaunchKernelGGL(k_bin_count<true>, dim3(grid_for(n, 256)), dim3(256), 0,
-                       s, pos, col, boxes, g0, n, key, rank, count);
+                       s, pos, col, boxes, g0, n, key, rank, count, sk);
   else
     hipLaunchKernelGGL(k_bin_count<false>, dim3(grid_for(n, 256)), dim3(256),
-                       0, s, pos, col, boxes, g0, n, key, rank, count);
+                       0, s, pos, col, boxes, g0, n, key, rank, count, sk);
   return hipGetLastError();
 }
 
@@ -1755,12 +1789,12 @@ int64_t scan_scratch_elems(int64_t nb) {
 }
 
 hipError_t launch_scan(int32_t *count, int32_t *start, int64_t nb,
-                       int32_t *scratch, hipStream_t s) {
+                       int32_t *scratch, hipStream_t s, SkinCheck sk) {
   int64_t nt = (nb + kScanTile - 1) / kScanTile;
   if (nt < 1) nt = 1;
   if (nt > 1024 * 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nt), dim3(kScanThreads), 0,
-                     s, count, start, scratch, nb);
+                     s, count, start, scratch, nb, sk);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(kScanThreads), 0, s,
                      start, scratch, nt, nb);
   return hipGetLastError();

@@ -41,6 +41,40 @@ def gpu(bmx):
     return True
 
 
+class _DLTensorView:
+    """DLPack producer over a HipArray (device type kDLROCM), for checking
+    the binding's DLPack path with the same HIP runtime the library uses (a
+    framework in the same process may bring its own runtime)."""
+
+    def __init__(self, arr):
+        import ctypes as C
+        from boltzmannmfx_amd import _DLTensor
+        self._arr = arr
+        self._shape = (C.c_int64 * len(arr.shape))(*arr.shape)
+        t = _DLTensor()
+        t.data = arr.ptr.value
+        t.device.device_type = 10
+        t.device.device_id = 0
+        t.ndim = len(arr.shape)
+        t.dtype.code, t.dtype.bits, t.dtype.lanes = 2, 64, 1
+        t.shape = C.cast(self._shape, C.POINTER(C.c_int64))
+        t.strides = None
+        t.byte_offset = 0
+        # DLManagedTensor: the DLTensor, then manager_ctx and deleter (unset)
+        self._managed = (C.c_byte * (C.sizeof(_DLTensor) + 2 * C.sizeof(C.c_void_p)))()
+        C.memmove(self._managed, C.byref(t), C.sizeof(_DLTensor))
+
+    def __dlpack_device__(self):
+        return (10, 0)
+
+    def __dlpack__(self, stream=None):
+        import ctypes as C
+        new = C.pythonapi.PyCapsule_New
+        new.restype = C.py_object
+        new.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+        return new(C.addressof(self._managed), b"dltensor", None)
+
+
 class HipArray:
     """float64 device memory (hipMalloc through ctypes) exposing
     __cuda_array_interface__, for passing device meshes to the C ABI without
@@ -68,6 +102,10 @@ class HipArray:
         self.nbytes = 8 * int(np.prod(self.shape))
         self.ptr = C.c_void_p()
         assert self._lib().hipMalloc(C.byref(self.ptr), C.c_size_t(self.nbytes)) == 0
+
+    def dlpack(self):
+        """This array seen only through the DLPack protocol."""
+        return _DLTensorView(self)
 
     @classmethod
     def upload(cls, a):

@@ -171,27 +171,29 @@ def test_mesh_to_particle_from_device_memory(bmx, oracle, gpu):
 
 
 def test_device_meshes_through_dlpack(bmx, oracle, gpu):
-    """torch tensors on the GPU are taken through DLPack (kDLROCM)."""
-    import torch
+    """Device meshes handed over through DLPack (kDLROCM) only: the binding
+    reads address, size and dtype from the capsule."""
     col = spheres(seed=10)
     ctx, grecs, g = setup(bmx, oracle, col)
     ctx.sync()
     orecs = grecs.copy()
     X, vf = mesh_inputs(col, seed=11)
-    dX = torch.from_numpy(X).to("cuda:0")
-    dvf = torch.from_numpy(vf).to("cuda:0")
+    dX, dvf = HipArray.upload(X), HipArray.upload(vf)
     nx, ny, nz = col.n_cell
-    out = torch.full((nz, ny, nx), -1.0, dtype=torch.float64, device="cuda:0")
-    torch.cuda.synchronize()
-    for t in (dX, dvf, out):
-        assert bmx._dlpack_view(t) == (t.data_ptr(), t.numel())
-    ctx.volume_fraction(1, out=out)
+    out = HipArray((nz, ny, nx))
+    views = [a.dlpack() for a in (dX, dvf, out)]
+    for a, v in zip((dX, dvf, out), views):
+        assert not hasattr(v, "__cuda_array_interface__")
+        assert bmx._device_view(v) == (a.ptr.value, int(np.prod(a.shape)))
+    ctx.volume_fraction(1, out=views[2])
     ovf, _ = oracle.volume_fraction(grecs, g, 1)
-    ctx.txfr_particle(CHEMPAR, dX, dvf, 0.25, 4, 1, 1)
+    ctx.txfr_particle(CHEMPAR, views[0], views[1], 0.25, 4, 1, 1)
     ctx.sync()
-    assert np.array_equal(out.cpu().numpy(), ovf)
+    assert np.array_equal(out.download(), ovf)
     assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 1) == 0
     assert np.array_equal(grecs["rdata"], orecs["rdata"])
+    for a in (dX, dvf, out):
+        a.free()
     ctx.close()
 
 
