@@ -561,6 +561,12 @@ def main() -> int:
                     help="run only the 200,000-segment workload and print its "
                          "object (for profiling that kernel)")
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the 100,000-sphere adjoint against its forward "
+                         "and print its object")
+    ap.add_argument("--mesh-only", action="store_true",
+                    help="run only the particle <-> mesh calls on the 1,000,000 "
+                         "spheres and print their object (for profiling)")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the one-step-per-call cadence legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -582,6 +588,12 @@ def main() -> int:
     if args.rods_only:
         print(json.dumps(rods_bench(bmx, W, steps=args.rods_steps,
                                     cpu_steps=0 if args.no_cpu else 10)))
+        return 0
+    if args.adjoint_only:
+        print(json.dumps(adjoint_bench(bmx, W)))
+        return 0
+    if args.mesh_only:
+        print(json.dumps(mesh_bench(bmx, W, n=args.n, cpu=not args.no_cpu)))
         return 0
 
     probe = copy_probe()

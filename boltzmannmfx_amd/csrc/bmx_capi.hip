@@ -137,7 +137,27 @@ struct bmx_ctx {
   DevBuf<int8_t> d_redo;  // segment tiles left to the row loop this step
   // species A, B, C and volume, and their exchange increments, in original
   // order (loaded from the records at the first exchange)
-  DevBuf<double4> d_chem, d_chem_inc;
+  // Adjoint working memory, kept across calls: checkpoints, loss target and
+  // partials, multipliers, and two history sets (positions of every step of
+  // a segment, and the rows, row counts and order of each of its rebuilds)
+  // so that one segment is recomputed while the previous one is reversed.
+  struct AdjHist {
+    DevBuf<double4> pos;
+    DevBuf<int32_t> idx, tot, cnt, oid;
+    DevBuf<uint8_t> own;
+  };
+  struct AdjBufs {
+    DevBuf<double> target, gr, acc, lpart, out;
+    DevBuf<double4> ckpt, lam0, lam1, lamo, fin;
+    AdjHist h[2];
+    hipStream_t rev = nullptr;  // the reverse steps (default priority)
+    hipEvent_t rdone[2] = {nullptr, nullptr}, adone[2] = {nullptr, nullptr};
+    hipEvent_t join = nullptr;
+  } adj;
+  // chemistry state (chem:: layout, original order), loaded from the
+  // records at the first exchange or mesh -> particle call after a bind and
+  // written back when the records are packed
+  DevBuf<double> d_chemst;
   bool have_chem = false;
   // particle <-> mesh coupling: bins over the level mesh, staged fields
   DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan;
@@ -157,7 +177,10 @@ struct bmx_ctx {
   int32_t vmax_slots = 1;
   int32_t vstride = 1;         // waves of ncap particles (blocks of 64)
   int32_t row_nblk = 0;        // blocks of the last row build
-  int32_t *h_flags = nullptr;  // pinned: err, max row, total (2 words), ...
+  // pinned: err, max row, total (2 words), ..., guard periods' errors
+  // (8, 9), the chemistry's size maxima (16..19), the adjoint forward's
+  // error word (20)
+  int32_t *h_flags = nullptr;
   int32_t *h_flags_dev = nullptr;  // h_flags as the device addresses it
 
   // timing: events around every rebuild and around every run of force
@@ -655,14 +678,28 @@ void reset_graph(bmx_ctx *c) {
 }
 
 // Writes the state the force pass changes into the device record copy.
+// The layout holds the amounts taken from the fluid (rdata[first_data + 6..8]).
+int chem_rel(const bmx_ctx *c) { return c->n_real >= rec::first_data + 9 ? 1 : 0; }
+
+// The chemistry state from the records, unless it is loaded.
+bmx_status load_chem(bmx_ctx *c) {
+  if (c->have_chem) return BMX_OK;
+  HIPCHK(c->d_chemst.alloc((size_t)c->ncap * chem::stride));
+  HIPCHK(launch_chem_unpack(c->recs, c->lay, c->n, chem_rel(c), c->d_chemst.p,
+                            c->stream));
+  c->have_chem = true;
+  return BMX_OK;
+}
+
 bmx_status pack_records(bmx_ctx *c) {
   HIPCHK(launch_aos_pack(c->recs, c->lay, c->n, static_arrays(c, c->cs),
                          dyn_arrays(c, c->cd), c->d_fc.p, c->ncap,
                          c->mobility, c->wrote_step ? 1 : 0,
                          c->any_fungi ? 1 : 0, c->stream));
   if (c->have_chem)
-    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, c->d_chem.p,
-                            c->d_chem_inc.p, c->stream));
+    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, chem_rel(c), c->d_chemst.p,
+                            c->oid[c->cs].p, c->type[c->cs].p, c->pos[c->cd].p,
+                            c->rod[c->cd].p, c->stream));
   return BMX_OK;
 }
 
@@ -850,7 +887,7 @@ bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
   c->stream = c->own_stream;
   if (e == hipSuccess)
     e = hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_flags, 64);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_flags, 128);
   if (e == hipSuccess) e = c->d_err.alloc(1) ;
   if (e == hipSuccess) e = c->d_stats.alloc(4);
   if (e == hipSuccess) e = c->d_total.alloc(1);
@@ -909,8 +946,7 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_vpart.release();
   c->d_slotmax.release();
   c->d_ustats.release();
-  c->d_chem.release();
-  c->d_chem_inc.release();
+  c->d_chemst.release();
   c->d_mkey.release();
   c->d_mrank.release();
   c->d_mcount.release();
@@ -930,6 +966,23 @@ void bmx_ctx_destroy(bmx_ctx *c) {
     z.col.release();
     z.bscale.release();
     if (z.done) (void)hipEventDestroy(z.done);
+  }
+  {
+    bmx_ctx::AdjBufs &a = c->adj;
+    if (a.rev) (void)hipStreamSynchronize(a.rev);
+    for (DevBuf<double> *b : {&a.target, &a.gr, &a.acc, &a.lpart, &a.out}) b->release();
+    for (DevBuf<double4> *b : {&a.ckpt, &a.lam0, &a.lam1, &a.lamo, &a.fin}) b->release();
+    for (auto &h : a.h) {
+      h.pos.release();
+      h.idx.release();
+      h.tot.release();
+      h.cnt.release();
+      h.oid.release();
+      h.own.release();
+    }
+    for (hipEvent_t e : {a.rdone[0], a.rdone[1], a.adone[0], a.adone[1], a.join})
+      if (e) (void)hipEventDestroy(e);
+    if (a.rev) (void)hipStreamDestroy(a.rev);
   }
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->done_ev) (void)hipEventDestroy(c->done_ev);
@@ -1452,7 +1505,8 @@ bmx_status bmx_refresh_from_aos(bmx_ctx *c, const void *aos, uint32_t fields) {
                 "or (segments) n_bnds outside 0..4 or c_length <= 0; "
                 "the context is unbound");
   }
-  if (fields & kFieldSpecies) c->have_chem = false;  // reloaded at next use
+  // the chemistry state holds species and sizes: reloaded at next use
+  if (fields & (kFieldSpecies | kFieldSizes)) c->have_chem = false;
   if (fields & (kFieldPositions | kFieldSizes)) {
     // the lists are stale: the next step rebuilds them, with bins for the
     // new reach when sizes grew or shrank
@@ -1588,32 +1642,42 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
   // rows of each rebuild of a segment, in the flat layout of the force pass
   const size_t rowbuf = (size_t)flat_entries(cap, c->flat_tile, c->maxn);
   const size_t tilebuf = (size_t)((cap + c->flat_tile - 1) / c->flat_tile + 1);
-  DevBuf<double> d_target, d_gr, d_acc, d_lpart, d_out;
-  DevBuf<double4> d_ckpt, d_hpos, d_lam0, d_lam1, d_lamo, d_final;
-  DevBuf<int32_t> d_hidx, d_htot, d_hcnt, d_hoid;
-  DevBuf<uint8_t> d_hown;
-  HIPCHK(d_target.alloc(3 * (size_t)std::max<int64_t>(n, 1)));
-  HIPCHK(d_gr.alloc((size_t)std::max<int64_t>(n, 1)));
-  HIPCHK(d_ckpt.alloc((size_t)S * cap));
-  HIPCHK(d_hpos.alloc((size_t)C * cap));
-  HIPCHK(d_hidx.alloc((size_t)R * rowbuf));
-  HIPCHK(d_hown.alloc((size_t)R * rowbuf));
-  HIPCHK(d_htot.alloc((size_t)R * tilebuf));
-  HIPCHK(d_hcnt.alloc((size_t)R * cap));
-  HIPCHK(d_hoid.alloc((size_t)R * cap));
-  HIPCHK(d_lam0.alloc(cap));
-  HIPCHK(d_lam1.alloc(cap));
-  HIPCHK(d_lamo.alloc(cap));
-  HIPCHK(d_final.alloc(cap));
+  bmx_ctx::AdjBufs &A = c->adj;
+  if (!A.rev) {
+    HIPCHK(hipStreamCreateWithFlags(&A.rev, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&A.rdone[0], &A.rdone[1], &A.adone[0], &A.adone[1], &A.join})
+      HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
+  hipStream_t s2 = A.rev;
+  // work enqueued on the context stream before this call comes first
+  HIPCHK(hipEventRecord(A.join, s));
+  HIPCHK(hipStreamWaitEvent(s2, A.join, 0));
+  HIPCHK(A.target.alloc(3 * (size_t)std::max<int64_t>(n, 1)));
+  HIPCHK(A.gr.alloc((size_t)std::max<int64_t>(n, 1)));
+  HIPCHK(A.ckpt.alloc((size_t)S * cap));
+  for (auto &h : A.h) {
+    HIPCHK(h.pos.alloc((size_t)C * cap));
+    HIPCHK(h.idx.alloc((size_t)R * rowbuf));
+    HIPCHK(h.own.alloc((size_t)R * rowbuf));
+    HIPCHK(h.tot.alloc((size_t)R * tilebuf));
+    HIPCHK(h.cnt.alloc((size_t)R * cap));
+    HIPCHK(h.oid.alloc((size_t)R * cap));
+  }
+  HIPCHK(A.lam0.alloc(cap));
+  HIPCHK(A.lam1.alloc(cap));
+  HIPCHK(A.lamo.alloc(cap));
+  HIPCHK(A.fin.alloc(cap));
   const int32_t nblk = adj_blocks(n);
-  HIPCHK(d_acc.alloc(3 * (size_t)nblk));
-  HIPCHK(d_lpart.alloc((size_t)nblk));
-  HIPCHK(d_out.alloc(4));
+  HIPCHK(A.acc.alloc(3 * (size_t)nblk));
+  HIPCHK(A.lpart.alloc((size_t)nblk));
+  HIPCHK(A.out.alloc(4));
+  double *d_gr = A.gr.p, *d_acc = A.acc.p, *d_lpart = A.lpart.p, *d_out = A.out.p;
+  double4 *d_ckpt = A.ckpt.p, *d_lamo = A.lamo.p, *d_final = A.fin.p;
   if (n)
-    HIPCHK(hipMemcpyAsync(d_target.p, target, sizeof(double) * 3 * n,
+    HIPCHK(hipMemcpyAsync(A.target.p, target, sizeof(double) * 3 * n,
                           hipMemcpyHostToDevice, s));
-  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_gr.p), n, 0ull, s));
-  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc.p),
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_gr), n, 0ull, s));
+  HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long *>(d_acc),
                          3 * (int64_t)nblk, 0ull, s));
   hipEvent_t e0, e1, e2;
   HIPCHK(hipEventCreate(&e0));
@@ -1629,7 +1693,7 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
   for (int t = 0; t < nsteps; t++) {
     if (t % C == 0)
       HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n,
-                             d_ckpt.p + (size_t)(t / C) * cap, s));
+                             d_ckpt + (size_t)(t / C) * cap, s));
     if (t % K == 0) {
       st = rebuild(c);
       if (st != BMX_OK) return st;
@@ -1641,17 +1705,17 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
     HIPCHK(launch_skin_check(c->d_vpart.p, c->vmax_slots, c->vstride,
                              0.5 * c->nbp.skin, 1, c->d_slotmax.p, c->d_err.p,
                              s));
-  HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n, d_final.p, s));
-  HIPCHK(launch_adj_loss(c->pos[c->cd].p, c->oid[c->cs].p, d_target.p, n,
-                         c->boxes[0], d_lam0.p, d_lpart.p, s));
-  HIPCHK(launch_to_orig4(d_lam0.p, c->oid[c->cs].p, n, d_lamo.p, s));
+  HIPCHK(launch_to_orig4(c->pos[c->cd].p, c->oid[c->cs].p, n, d_final, s));
+  HIPCHK(launch_adj_loss(c->pos[c->cd].p, c->oid[c->cs].p, A.target.p, n,
+                         c->boxes[0], A.lam0.p, d_lpart, s));
+  HIPCHK(launch_to_orig4(A.lam0.p, c->oid[c->cs].p, n, d_lamo, s));
   HIPCHK(hipEventRecord(e1, s));
   {
     // a stale list (skin exceeded) or a non-finite force ends the call here
-    int32_t ferr = 0;
-    HIPCHK(hipMemcpyAsync(&ferr, c->d_err.p, sizeof ferr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(c->h_flags + 20, c->d_err.p, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (ferr) {
+    if (c->h_flags[20]) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       (void)hipEventDestroy(e2);
@@ -1659,12 +1723,18 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
       return check_flags(c);
     }
   }
-  // ---- reverse, one segment at a time
-  double4 *lam_in = d_lam0.p, *lam_out = d_lam1.p;
+  // ---- reverse.  Segment sg is recomputed on the context stream into
+  // history set sg % 2 while the reverse steps of segment sg + 1 run on the
+  // second stream from the other set; a set is written again only after
+  // the reverse steps that read it (two segments later) are done.
+  double4 *lam_in = A.lam0.p, *lam_out = A.lam1.p;
   for (int sg = S - 1; sg >= 0; sg--) {
     const int t0 = sg * C;
     const int len = std::min(C, nsteps - t0);
-    HIPCHK(launch_from_orig4(d_ckpt.p + (size_t)sg * cap, c->oid[c->cs].p, n,
+    const int hs = sg & 1;
+    bmx_ctx::AdjHist &H = A.h[hs];
+    if (sg + 2 <= S - 1) HIPCHK(hipStreamWaitEvent(s, A.adone[hs], 0));
+    HIPCHK(launch_from_orig4(d_ckpt + (size_t)sg * cap, c->oid[c->cs].p, n,
                              c->pos[c->cd].p, s));
     c->have_list = false;
     for (int t = 0; t < len; t++) {
@@ -1672,62 +1742,68 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
       // context's buffers carry the state only through a rebuild.
       if (t % K == 0) {
         if (t > 0)
-          HIPCHK(hipMemcpyAsync(c->pos[c->cd].p, d_hpos.p + (size_t)t * cap,
+          HIPCHK(hipMemcpyAsync(c->pos[c->cd].p, H.pos.p + (size_t)t * cap,
                                 sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
         st = rebuild(c, false);
         if (st != BMX_OK) return st;
-        HIPCHK(hipMemcpyAsync(d_hpos.p + (size_t)t * cap, c->pos[c->cd].p,
+        HIPCHK(hipMemcpyAsync(H.pos.p + (size_t)t * cap, c->pos[c->cd].p,
                               sizeof(double4) * cap, hipMemcpyDeviceToDevice, s));
         const int r = t / K;
-        HIPCHK(hipMemcpyAsync(d_hidx.p + (size_t)r * rowbuf, c->d_cidx.p,
+        HIPCHK(hipMemcpyAsync(H.idx.p + (size_t)r * rowbuf, c->d_cidx.p,
                               sizeof(int32_t) * rowbuf, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_hown.p + (size_t)r * rowbuf, c->d_cown.p, rowbuf,
+        HIPCHK(hipMemcpyAsync(H.own.p + (size_t)r * rowbuf, c->d_cown.p, rowbuf,
                               hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_htot.p + (size_t)r * tilebuf, c->d_wtot.p,
+        HIPCHK(hipMemcpyAsync(H.tot.p + (size_t)r * tilebuf, c->d_wtot.p,
                               sizeof(int32_t) * tilebuf, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_hcnt.p + (size_t)r * cap, c->d_ncnt.p,
+        HIPCHK(hipMemcpyAsync(H.cnt.p + (size_t)r * cap, c->d_ncnt.p,
                               sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_hoid.p + (size_t)r * cap, c->oid[c->cs].p,
+        HIPCHK(hipMemcpyAsync(H.oid.p + (size_t)r * cap, c->oid[c->cs].p,
                               sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
       }
       if (t + 1 < len)  // the last state of the segment is not needed
         HIPCHK(launch_force_sphere_flat(
-            d_hpos.p + (size_t)t * cap, d_hpos.p + (size_t)(t + 1) * cap,
+            H.pos.p + (size_t)t * cap, H.pos.p + (size_t)(t + 1) * cap,
             c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n,
             c->maxn, nullptr, c->d_boxes.p, c->boxes[0], 0, fp, nullptr,
             nullptr, nullptr, c->d_err.p, shape(c), s));
     }
+    HIPCHK(hipEventRecord(A.rdone[hs], s));
+    HIPCHK(hipStreamWaitEvent(s2, A.rdone[hs], 0));
     // lambda of the segment's last step, in that step's order
     int r = (len - 1) / K;
-    HIPCHK(launch_from_orig4(d_lamo.p, d_hoid.p + (size_t)r * cap, n, lam_in, s));
+    HIPCHK(launch_from_orig4(d_lamo, H.oid.p + (size_t)r * cap, n, lam_in, s2));
     for (int t = len - 1; t >= 0; t--) {
       r = t / K;
-      const int32_t *oid = d_hoid.p + (size_t)r * cap;
-      HIPCHK(launch_adj_step(d_hpos.p + (size_t)t * cap, d_hidx.p + (size_t)r * rowbuf,
-                             d_hown.p + (size_t)r * rowbuf,
-                             d_htot.p + (size_t)r * tilebuf,
-                             d_hcnt.p + (size_t)r * cap, c->maxn, oid, n,
-                             c->boxes[0], fp, lam_in, lam_out, d_gr.p, d_acc.p, s));
+      const int32_t *oid = H.oid.p + (size_t)r * cap;
+      HIPCHK(launch_adj_step(H.pos.p + (size_t)t * cap, H.idx.p + (size_t)r * rowbuf,
+                             H.own.p + (size_t)r * rowbuf,
+                             H.tot.p + (size_t)r * tilebuf,
+                             H.cnt.p + (size_t)r * cap, c->maxn, oid, n,
+                             c->boxes[0], fp, lam_in, lam_out, d_gr, d_acc, s2));
       std::swap(lam_in, lam_out);
       if (t % K == 0 && t > 0) {  // back across a rebuild: reorder lambda
-        HIPCHK(launch_to_orig4(lam_in, oid, n, d_lamo.p, s));
-        HIPCHK(launch_from_orig4(d_lamo.p, d_hoid.p + (size_t)(r - 1) * cap, n,
-                                 lam_in, s));
+        HIPCHK(launch_to_orig4(lam_in, oid, n, d_lamo, s2));
+        HIPCHK(launch_from_orig4(d_lamo, H.oid.p + (size_t)(r - 1) * cap, n,
+                                 lam_in, s2));
       }
     }
-    HIPCHK(launch_to_orig4(lam_in, d_hoid.p, n, d_lamo.p, s));
+    HIPCHK(launch_to_orig4(lam_in, H.oid.p, n, d_lamo, s2));
+    HIPCHK(hipEventRecord(A.adone[hs], s2));
   }
-  HIPCHK(launch_sum_partials(d_acc.p, nblk, 3, d_out.p, s));
-  HIPCHK(launch_sum_partials(d_lpart.p, nblk, 1, d_out.p + 3, s));
+  // the reverse steps end before the sums
+  HIPCHK(hipEventRecord(A.join, s2));
+  HIPCHK(hipStreamWaitEvent(s, A.join, 0));
+  HIPCHK(launch_sum_partials(d_acc, nblk, 3, d_out, s));
+  HIPCHK(launch_sum_partials(d_lpart, nblk, 1, d_out + 3, s));
   HIPCHK(hipEventRecord(e2, s));
   // leave the context in the final forward state
-  HIPCHK(launch_from_orig4(d_final.p, c->oid[c->cs].p, n, c->pos[c->cd].p, s));
+  HIPCHK(launch_from_orig4(d_final, c->oid[c->cs].p, n, c->pos[c->cd].p, s));
   c->have_list = false;
   c->step = step0 + nsteps;
   double out[4];
-  HIPCHK(hipMemcpyAsync(out, d_out.p, sizeof out, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(out, d_out, sizeof out, hipMemcpyDeviceToHost, s));
   if (n)
-    HIPCHK(hipMemcpyAsync(grad_r, d_gr.p, sizeof(double) * n,
+    HIPCHK(hipMemcpyAsync(grad_r, d_gr, sizeof(double) * n,
                           hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   float fms = 0.f, bms = 0.f;
@@ -1942,19 +2018,14 @@ extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) 
                 "the record layout has no species fields (rdata[25..30])");
   (void)hipSetDevice(c->device);
   hipStream_t s = c->stream;
-  if (!c->have_chem) {
-    HIPCHK(c->d_chem.alloc((size_t)c->ncap));
-    HIPCHK(c->d_chem_inc.alloc((size_t)c->ncap));
-    HIPCHK(launch_chem_unpack(c->recs, c->lay, c->n, c->d_chem.p,
-                              c->d_chem_inc.p, s));
-    c->have_chem = true;
-  }
+  st = load_chem(c);
+  if (st != BMX_OK) return st;
   // the rows are rebuilt at the current positions (xchng.cpp:68-77)
   st = rebuild(c, true, true);
   if (st != BMX_OK) return st;
   HIPCHK(launch_exchange(c->pos[c->cd].p, c->type[c->cs].p, c->topo[c->cd].p,
                          c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
-                         c->n, c->d_chem.p, c->d_chem_inc.p, xpar, dt, s));
+                         c->n, c->d_chemst.p, xpar, dt, s));
   return check_flags(c);
 }
 
@@ -2295,8 +2366,19 @@ bmx_status mesh_deposit(bmx_ctx *c, int scheme, int comp0, int ncomp, int mode,
     HIPCHK(c->d_mout.alloc((size_t)(nc * ncomp)));
     dst = c->d_mout.p;
   }
-  HIPCHK(launch_deposit(c->recs, c->lay, g, tri, c->d_mstart.p, c->d_mperm.p,
-                        c->pos[c->cd].p, comp0, ncomp, mode, dst, c->stream));
+  // the volume and the amounts taken from the fluid are read from the
+  // chemistry state while it holds them
+  const uint8_t *src = c->recs;
+  AosLayout lay = c->lay;
+  int comp = comp0;
+  if (c->have_chem && (comp0 == rec::vol || comp0 == rec::first_data + 6)) {
+    src = reinterpret_cast<const uint8_t *>(c->d_chemst.p);
+    lay.stride = chem::stride * (int32_t)sizeof(double);
+    lay.rdata = 0;
+    comp = comp0 == rec::vol ? chem::vol : chem::rel0;
+  }
+  HIPCHK(launch_deposit(src, lay, g, tri, c->d_mstart.p, c->d_mperm.p,
+                        c->pos[c->cd].p, comp, ncomp, mode, dst, c->stream));
   if (!on_device)
     HIPCHK(hipMemcpyAsync(out, dst, sizeof(double) * (size_t)(nc * ncomp),
                           hipMemcpyDeviceToHost, c->stream));
@@ -2360,27 +2442,26 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
     X = c->d_min.p;
     V = c->d_min.p + 3 * ng;
   }
-  // the species of the last exchange go back into the records first; the
-  // chemistry updates the records, the sorted sizes and their maxima, the
-  // species are reloaded at the next exchange, and the rows are rebuilt at
-  // the next step for the new reaches
-  if (c->have_chem)
-    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, c->d_chem.p,
-                            c->d_chem_inc.p, s));
-  c->have_chem = false;
+  // the chemistry updates the chemistry state (species, volume, area,
+  // rates, gradient, amounts taken), the sorted sizes and their maxima; the
+  // records get it when they are packed, and the rows are rebuilt at the
+  // next step for the new reaches
+  st = load_chem(c);
+  if (st != BMX_OK) return st;
   HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
-  HIPCHK(launch_mesh_to_particle(c->recs, c->lay, c->n, g, c->d_mstart.p,
+  HIPCHK(launch_mesh_to_particle(c->d_chemst.p, c->n, g, c->d_mstart.p,
                                  c->oid[c->cs].p, c->type[c->cs].p,
+                                 c->any_fungi ? c->topo[c->cd].p : nullptr,
                                  c->pos[c->cd].p, c->rod[c->cd].p, X, V,
                                  cnc_scheme, vf_scheme, chempar, dt, nloop,
                                  c->d_ustats.p, c->d_err.p, s));
-  unsigned long long hs[8];
-  HIPCHK(hipMemcpyAsync(hs, c->d_ustats.p, sizeof hs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->h_flags + 16, c->d_ustats.p, 2 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s));
   st = mesh_flags(c);  // synchronizes the stream
   if (st == BMX_ERR_HIP) return st;
   // the sorted sizes have changed either way: new bins, rows rebuilt
-  std::memcpy(&c->rmax, &hs[0], sizeof(double));
-  std::memcpy(&c->lmax, &hs[1], sizeof(double));
+  std::memcpy(&c->rmax, c->h_flags + 16, sizeof(double));
+  std::memcpy(&c->lmax, c->h_flags + 18, sizeof(double));
   reset_graph(c);
   bmx_status sb = setup_boxes(c);
   return st != BMX_OK ? st : sb;

@@ -18,31 +18,56 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;  // M_PI of <math.h>
 
 __global__ void k_chem_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
-                              int64_t n, double4 *__restrict__ chem,
-                              double4 *__restrict__ inc) {
+                              int64_t n, int has_rel, double *__restrict__ state) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double *rd =
       reinterpret_cast<const double *>(aos + i * (int64_t)lay.stride + lay.rdata);
-  chem[i] = make_double4(rd[rec::first_data], rd[rec::first_data + 1],
-                         rd[rec::first_data + 2], rd[rec::vol]);
-  inc[i] = make_double4(rd[rec::first_data + 3], rd[rec::first_data + 4],
-                        rd[rec::first_data + 5], 0.0);
+  double4 *q = reinterpret_cast<double4 *>(state + i * chem::stride);
+  q[0] = make_double4(rd[rec::first_data], rd[rec::first_data + 1],
+                      rd[rec::first_data + 2], rd[rec::vol]);
+  q[1] = make_double4(rd[rec::first_data + 3], rd[rec::first_data + 4],
+                      rd[rec::first_data + 5], rd[rec::area]);
+  q[2] = make_double4(rd[rec::dadt], rd[rec::dvdt], rd[rec::gx], rd[rec::gy]);
+  q[3] = has_rel ? make_double4(rd[rec::gz], rd[rec::first_data + 6],
+                                rd[rec::first_data + 7], rd[rec::first_data + 8])
+                 : make_double4(rd[rec::gz], 0.0, 0.0, 0.0);
 }
 
+// Walks the sorted arrays, so each record also gets its radius and, for a
+// segment, its length from where the mechanics keep them.
 __global__ void k_chem_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
-                            const double4 *__restrict__ chem,
-                            const double4 *__restrict__ inc) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double *rd = reinterpret_cast<double *>(aos + i * (int64_t)lay.stride + lay.rdata);
-  const double4 c = chem[i], d = inc[i];
-  rd[rec::first_data] = c.x;
-  rd[rec::first_data + 1] = c.y;
-  rd[rec::first_data + 2] = c.z;
-  rd[rec::first_data + 3] = d.x;
-  rd[rec::first_data + 4] = d.y;
-  rd[rec::first_data + 5] = d.z;
+                            int has_rel, const double *__restrict__ state,
+                            const int32_t *__restrict__ oid,
+                            const int32_t *__restrict__ type,
+                            const double4 *__restrict__ pos,
+                            const double4 *__restrict__ rod) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t o = oid[t];
+  double *rd = reinterpret_cast<double *>(aos + o * (int64_t)lay.stride + lay.rdata);
+  const double4 *q = reinterpret_cast<const double4 *>(state + o * chem::stride);
+  const double4 a = q[0], b = q[1], c = q[2], d = q[3];
+  rd[rec::radius] = pos[t].w;
+  if (type[t] == rec::FUNGI) rd[rec::c_length] = rod[t].x;
+  rd[rec::area] = b.w;
+  rd[rec::vol] = a.w;
+  rd[rec::gx] = c.z;
+  rd[rec::gy] = c.w;
+  rd[rec::gz] = d.x;
+  rd[rec::dadt] = c.x;
+  rd[rec::dvdt] = c.y;
+  rd[rec::first_data] = a.x;
+  rd[rec::first_data + 1] = a.y;
+  rd[rec::first_data + 2] = a.z;
+  rd[rec::first_data + 3] = b.x;
+  rd[rec::first_data + 4] = b.y;
+  rd[rec::first_data + 5] = b.z;
+  if (has_rel) {
+    rd[rec::first_data + 6] = d.y;
+    rd[rec::first_data + 7] = d.z;
+    rd[rec::first_data + 8] = d.w;
+  }
 }
 
 // The flux of one species from i to j (evaluateExchange, bmx_chem_K.H:
@@ -63,14 +88,14 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
                                 const int32_t *__restrict__ oid,
                                 const int32_t *__restrict__ nbr,
                                 const int32_t *__restrict__ ncnt, int64_t ncap,
-                                int64_t n, const double4 *__restrict__ chem,
-                                double4 *__restrict__ inc, double mtA,
-                                double mtB, double mtC, double dt) {
+                                int64_t n, double *__restrict__ state,
+                                double mtA, double mtB, double mtC, double dt) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || type[t] != rec::FUNGI) return;
   const Topo ti = topo[t];
   const double r1 = pos[t].w;
-  const double4 c1 = chem[oid[t]];
+  const int64_t o = oid[t];
+  const double4 c1 = *reinterpret_cast<const double4 *>(state + o * chem::stride);
   double ia = 0.0, ib = 0.0, ic = 0.0;  // initExchange
   const int c = ncnt[t] & ((1 << 28) - 1);  // bits 28..30: image axes
   const int32_t *rp = nbr + t;
@@ -85,7 +110,8 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
       if (tj.seg_id[bj & 3] == ti.self_id && tj.seg_cpu[bj & 3] == ti.self_cpu) break;
     if (!(bi < ti.n_bnds && bj < tj.n_bnds)) continue;
     const double r2 = pos[u].w;
-    const double4 c2 = chem[oid[u]];
+    const double4 c2 =
+        *reinterpret_cast<const double4 *>(state + (int64_t)oid[u] * chem::stride);
     const double area1 = kPi * r1 * r1, area2 = kPi * r2 * r2;
     const double area = 0.5 * (area1 + area2);
     const double dA = species_flux(c1.x, c2.x, area, mtA, dt, c1.w, c2.w, ti.n_bnds, tj.n_bnds);
@@ -95,23 +121,22 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
     ib -= dB / c1.w;
     ic -= dC / c1.w;
   }
-  inc[oid[t]] = make_double4(ia, ib, ic, 0.0);
+  double *inc = state + o * chem::stride;
+  inc[chem::dA] = ia;
+  inc[chem::dB] = ib;
+  inc[chem::dC] = ic;
 }
 
 // applyIncrements (bmx_chem_K.H:2092-2112)
 __global__ void k_exchange_apply(const int32_t *__restrict__ type,
                                  const int32_t *__restrict__ oid, int64_t n,
-                                 double4 *__restrict__ chem,
-                                 const double4 *__restrict__ inc) {
+                                 double *__restrict__ state) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || type[t] != rec::FUNGI) return;
-  const int64_t o = oid[t];
-  double4 c = chem[o];
-  const double4 d = inc[o];
-  c.x += d.x;
-  c.y += d.y;
-  c.z += d.z;
-  chem[o] = c;
+  double *q = state + (int64_t)oid[t] * chem::stride;
+  q[chem::A] += q[chem::dA];
+  q[chem::B] += q[chem::dB];
+  q[chem::C] += q[chem::dC];
 }
 
 int grid256(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
@@ -119,34 +144,35 @@ int grid256(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)
 }  // namespace
 
 hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
-                              double4 *chem, double4 *inc, hipStream_t s) {
+                              int has_rel, double *state, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_chem_unpack, dim3(grid256(n)), dim3(256), 0, s, aos, lay,
-                     n, chem, inc);
+                     n, has_rel, state);
   return hipGetLastError();
 }
 
 hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
-                            const double4 *chem, const double4 *inc,
+                            int has_rel, const double *state,
+                            const int32_t *oid, const int32_t *type,
+                            const double4 *pos, const double4 *rod,
                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_chem_pack, dim3(grid256(n)), dim3(256), 0, s, aos, lay,
-                     n, chem, inc);
+                     n, has_rel, state, oid, type, pos, rod);
   return hipGetLastError();
 }
 
 hipError_t launch_exchange(const double4 *pos, const int32_t *type,
                            const Topo *topo, const int32_t *oid,
                            const int32_t *nbr, const int32_t *ncnt,
-                           int64_t ncap, int64_t n, double4 *chem,
-                           double4 *inc, const double xpar[3], double dt,
-                           hipStream_t s) {
+                           int64_t ncap, int64_t n, double *state,
+                           const double xpar[3], double dt, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_exchange_rows, dim3(grid256(n)), dim3(256), 0, s, pos,
-                     type, topo, oid, nbr, ncnt, ncap, n, chem, inc, xpar[0],
+                     type, topo, oid, nbr, ncnt, ncap, n, state, xpar[0],
                      xpar[1], xpar[2], dt);
   hipLaunchKernelGGL(k_exchange_apply, dim3(grid256(n)), dim3(256), 0, s, type,
-                     oid, n, chem, inc);
+                     oid, n, state);
   return hipGetLastError();
 }
 

@@ -196,18 +196,35 @@ hipError_t launch_from_orig4(const double4 *in, const int32_t *oid, int64_t n,
 hipError_t launch_sum_partials(const double *part, int64_t nblk, int stride,
                                double *out, hipStream_t s);
 
-// ParticleExchange (bmx_exchange.hip)
+// The device chemistry state: per particle, in original (record) order, the
+// 16 doubles of one 128-byte line that ParticleExchange and the mesh ->
+// particle chemistry read and write, so neither touches the records until
+// they are packed (at sync, and before a pass that works on the records).
+namespace chem {
+enum : int {
+  A = 0, B = 1, C = 2, vol = 3,          // species, cell volume
+  dA = 4, dB = 5, dC = 6, area = 7,      // species increments, area
+  dadt = 8, dvdt = 9, gx = 10, gy = 11,  // growth rates, species-A gradient
+  gz = 12, rel0 = 13, rel1 = 14, rel2 = 15,  // amounts taken from the fluid
+  stride = 16
+};
+}
+// records -> state; has_rel: the layout holds rdata[first_data + 6..8]
 hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
-                              double4 *chem, double4 *inc, hipStream_t s);
+                              int has_rel, double *state, hipStream_t s);
+// state -> records, with each particle's radius (pos.w) and, for segments,
+// its length (rod.x) from the sorted arrays
 hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
-                            const double4 *chem, const double4 *inc,
+                            int has_rel, const double *state,
+                            const int32_t *oid, const int32_t *type,
+                            const double4 *pos, const double4 *rod,
                             hipStream_t s);
+// ParticleExchange (bmx_exchange.hip) on the chemistry state
 hipError_t launch_exchange(const double4 *pos, const int32_t *type,
                            const Topo *topo, const int32_t *oid,
                            const int32_t *nbr, const int32_t *ncnt,
-                           int64_t ncap, int64_t n, double4 *chem,
-                           double4 *inc, const double xpar[3], double dt,
-                           hipStream_t s);
+                           int64_t ncap, int64_t n, double *state,
+                           const double xpar[3], double dt, hipStream_t s);
 
 // ---- particle <-> mesh (bmx_mesh.hip), over the sorted arrays
 // Bins the particles by mesh cell, or by trilinear base cell (tri);
@@ -225,14 +242,16 @@ hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
                           const unsigned long long *perm, const double4 *pos,
                           int comp0, int ncomp, int mode, double *out,
                           hipStream_t s);
-// interpolation + chemistry per particle; records updated in place, new
-// sizes into pos.w / rod.x / rod.w; stats[0], stats[1]: max radius and
+// interpolation + chemistry per particle on the chemistry state (original
+// order); segments' bond count and tip flag from topo (sorted, current);
+// new sizes into pos.w / rod.x / rod.w; stats[0], stats[1]: max radius and
 // segment length (bits)
-hipError_t launch_mesh_to_particle(uint8_t *aos, AosLayout lay, int64_t n,
+hipError_t launch_mesh_to_particle(double *state, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
                                    const int32_t *oid, const int32_t *type,
-                                   double4 *pos, double4 *rod, const double *X,
-                                   const double *vf, int cnc_tri, int vf_tri,
+                                   const Topo *topo, double4 *pos, double4 *rod,
+                                   const double *X, const double *vf,
+                                   int cnc_tri, int vf_tri,
                                    const double chempar[10], double dt,
                                    int nloop, unsigned long long *stats,
                                    int32_t *err, hipStream_t s);

@@ -523,13 +523,13 @@ __device__ __forceinline__ double block_max_nonneg(double v, double *sh) {
 // the chemistry, the record fields written back, and the new radius, length
 // and area into the sorted arrays with their maxima (one atomic per block).
 __global__ __launch_bounds__(256) void k_mesh_to_particle(
-    uint8_t *__restrict__ aos, AosLayout lay, int64_t n, MeshGeom g,
+    double *__restrict__ state, int64_t n, MeshGeom g,
     const int32_t *__restrict__ start, const int32_t *__restrict__ oid,
-    const int32_t *__restrict__ type, double4 *__restrict__ pos,
-    double4 *__restrict__ rod, const double *__restrict__ X,
-    const double *__restrict__ vf, int cnc_tri, int vf_tri, ChemParams P,
-    double dt, int nloop, unsigned long long *__restrict__ stats,
-    int32_t *__restrict__ err) {
+    const int32_t *__restrict__ type, const Topo *__restrict__ topo,
+    double4 *__restrict__ pos, double4 *__restrict__ rod,
+    const double *__restrict__ X, const double *__restrict__ vf, int cnc_tri,
+    int vf_tri, ChemParams P, double dt, int nloop,
+    unsigned long long *__restrict__ stats, int32_t *__restrict__ err) {
   __shared__ double sh[2][4];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rmax = 0.0, lmax = 0.0;
@@ -541,9 +541,16 @@ __global__ __launch_bounds__(256) void k_mesh_to_particle(
     const double x[3] = {p.x, p.y, p.z};
     int c[3];
     if (mesh_cell(x, g, c)) {  // else reported by the binning pass
-      uint8_t *r = aos + (int64_t)oid[t] * lay.stride;
-      double *rd = reinterpret_cast<double *>(r + lay.rdata);
-      const int32_t *id = reinterpret_cast<const int32_t *>(r + lay.idata);
+      // the particle's line of the chemistry state, all four parts at once
+      double4 *q = reinterpret_cast<double4 *>(state + (int64_t)oid[t] * chem::stride);
+      const double4 q0 = q[0], q1 = q[1], q2 = q[2];
+      bool tip = false;
+      int nb = 0;
+      if (ty == rec::FUNGI) {
+        const Topo tp = topo[t];
+        tip = tp.position == rec::TIP;
+        nb = tp.n_bnds;
+      }
       const int64_t ng = (int64_t)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
       const int64_t cell = ((int64_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
       const int64_t gc = ghost_lin(g, c[0], c[1], c[2]);
@@ -554,29 +561,19 @@ __global__ __launch_bounds__(256) void k_mesh_to_particle(
       const double np = (double)(start[cell + 1] - start[cell]);
       // gradient of species A at the cell, centred differences
       const int64_t sy = g.n[0] + 2, sz = sy * (g.n[1] + 2);
-      rd[rec::gx] = (X[gc + 1] - X[gc - 1]) * 0.5 * g.dxi[0];
-      rd[rec::gy] = (X[gc + sy] - X[gc - sy]) * 0.5 * g.dxi[1];
-      rd[rec::gz] = (X[gc + sz] - X[gc - sz]) * 0.5 * g.dxi[2];
-      Sizes z{rd[rec::radius], rd[rec::c_length], rd[rec::area],
-              rd[rec::vol],    rd[rec::dadt],     rd[rec::dvdt]};
-      double cs[3] = {rd[rec::first_data], rd[rec::first_data + 1],
-                      rd[rec::first_data + 2]};
+      const double gx = (X[gc + 1] - X[gc - 1]) * 0.5 * g.dxi[0];
+      const double gy = (X[gc + sy] - X[gc - sy]) * 0.5 * g.dxi[1];
+      const double gz = (X[gc + sz] - X[gc - sz]) * 0.5 * g.dxi[2];
+      Sizes z{p.w, rr.x, q1.w, q0.w, q2.x, q2.y};
+      double cs[3] = {q0.x, q0.y, q0.z};
       double rel[3];
-      e = chem_update(g.grid_vol * v, np, z, cs, rel, fm, ty,
-                      id[rec::position] == rec::TIP, id[rec::n_bnds], dt, nloop, P);
-      rd[rec::radius] = z.radius;
-      rd[rec::c_length] = z.c_length;
-      rd[rec::area] = z.area;
-      rd[rec::vol] = z.vol;
-      rd[rec::dadt] = z.dadt;
-      rd[rec::dvdt] = z.dvdt;
-      for (int k = 0; k < 3; k++) {
-        rd[rec::first_data + k] = cs[k];
-        rd[rec::first_data + 3 + k] = cs[k];
-        rd[rec::first_data + 6 + k] = rel[k];
-      }
+      e = chem_update(g.grid_vol * v, np, z, cs, rel, fm, ty, tip, nb, dt, nloop, P);
+      q[0] = make_double4(cs[0], cs[1], cs[2], z.vol);
+      q[1] = make_double4(cs[0], cs[1], cs[2], z.area);
+      q[2] = make_double4(z.dadt, z.dvdt, gx, gy);
+      q[3] = make_double4(gz, rel[0], rel[1], rel[2]);
       p.w = z.radius;
-      rr.x = z.c_length;
+      if (ty == rec::FUNGI) rr.x = z.c_length;
       rr.w = z.area;
       pos[t].w = z.radius;
       rod[t] = rr;
@@ -642,20 +639,21 @@ hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
   return hipGetLastError();
 }
 
-hipError_t launch_mesh_to_particle(uint8_t *aos, AosLayout lay, int64_t n,
+hipError_t launch_mesh_to_particle(double *state, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
                                    const int32_t *oid, const int32_t *type,
-                                   double4 *pos, double4 *rod, const double *X,
-                                   const double *vf, int cnc_tri, int vf_tri,
+                                   const Topo *topo, double4 *pos, double4 *rod,
+                                   const double *X, const double *vf,
+                                   int cnc_tri, int vf_tri,
                                    const double chempar[10], double dt,
                                    int nloop, unsigned long long *stats,
                                    int32_t *err, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   ChemParams P{chempar[0], chempar[1], chempar[2], chempar[3], chempar[4],
                chempar[5], chempar[6], chempar[7], chempar[8], chempar[9]};
-  hipLaunchKernelGGL(k_mesh_to_particle, dim3(grid_of(n)), dim3(256), 0, s, aos,
-                     lay, n, g, start, oid, type, pos, rod, X, vf, cnc_tri,
-                     vf_tri, P, dt, nloop, stats, err);
+  hipLaunchKernelGGL(k_mesh_to_particle, dim3(grid_of(n)), dim3(256), 0, s,
+                     state, n, g, start, oid, type, topo, pos, rod, X, vf,
+                     cnc_tri, vf_tri, P, dt, nloop, stats, err);
   return hipGetLastError();
 }
 

@@ -154,6 +154,46 @@ def test_mesh_to_particle_matches_oracle(bmx, oracle, gpu, kind, cnc, vfs):
     ctx.close()
 
 
+@pytest.mark.parametrize("kind", ["spheres", "hyphae"])
+def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
+    """Steps, mesh -> particle, the deposits that read what it wrote, and
+    mesh -> particle again, with no sync in between: each call sees the
+    device chemistry state and the bond state the steps left."""
+    col = spheres(seed=12) if kind == "spheres" else hyphae(seed=19)
+    mode = 0 if kind == "spheres" else 1
+    ctx, grecs, g = setup(bmx, oracle, col, mode)
+    orecs = col.recs.copy()
+    if kind == "hyphae":
+        ctx.init_bonds()
+        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL,
+                          1.1 * (2 * 2e-4 + 1e-2), 1)
+        st.init_bonds()
+    else:
+        st = oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
+                                                  ctx.bin_dims()),
+                          col.fpar, oracle.NBR_PAIR, col.skin, 10)
+    ctx.evolve(col.dt, 3)
+    st.evolve(col.dt, 3)
+    X, vf = mesh_inputs(col, seed=13)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 1, 1)
+    rhs = ctx.txfr_fluid(1)
+    vfn = ctx.volume_fraction(0)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 0, 0)
+    ctx.sync()
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 1) == 0
+    orhs, _ = oracle.deposit(orecs, g, 1, 31, 3)
+    ovf, _ = oracle.volume_fraction(orecs, g, 0)
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 0, 0) == 0
+    assert np.abs(rhs).max() > 0
+    assert np.array_equal(rhs, orhs)
+    assert np.array_equal(vfn, ovf)
+    assert np.array_equal(grecs["rdata"], orecs["rdata"])
+    assert np.array_equal(grecs["idata"], orecs["idata"])
+    assert np.array_equal(grecs["pos"], orecs["pos"])
+    st.close()
+    ctx.close()
+
+
 def test_mesh_to_particle_from_device_memory(bmx, oracle, gpu):
     col = spheres(seed=8)
     ctx, grecs, g = setup(bmx, oracle, col)
