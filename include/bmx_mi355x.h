@@ -202,7 +202,10 @@ bmx_status bmx_bind_colonies_aos(bmx_ctx *ctx, const void *aos,
 /* Writes positions, velocities, angular velocities, orientation angles and
  * the integer fields the force pass changes back into 'aos' (host or device,
  * as bound; NULL = the bound array, or each bound tile; records bound in
- * place are written where they are). */
+ * place are written where they are), and, once bmx_exchange or
+ * bmx_calc_txfr_particle has run since the bind, the chemistry fields they
+ * keep on the device: radius, c_length, area, vol, gx..gz, dadt, dvdt and
+ * rdata[first_data .. first_data + 8]. */
 bmx_status bmx_sync_to_aos(bmx_ctx *ctx, void *aos);
 /* Fields bmx_refresh_from_aos re-reads. */
 enum {
@@ -211,15 +214,17 @@ enum {
   BMX_FIELD_ORIENTATIONS = 4, /* segments: theta, phi */
   BMX_FIELD_BONDS = 8,        /* segments: n_bnds, bond ids/cpus/sites,
                                  position, fuse_tip, fix_site */
-  BMX_FIELD_SPECIES = 16      /* rdata[25..30] (exchange state) */
+  BMX_FIELD_SPECIES = 16      /* rdata[25..33] (exchange and chemistry) */
 };
 /* The caller changed the given fields of the bound particles (same count,
  * same order): re-read just those, keeping the sorted order and, unless
  * positions or sizes changed, the neighbour lists.  aos = NULL re-reads the
  * bound array (or tiles); records bound in place are re-read in place.
  * Positions or sizes make the next step rebuild the lists (with new bins
- * when the largest reach changed).  A record that fails the bind's checks
- * unbinds the context (BMX_ERR_BAD_ARG). */
+ * when the largest reach changed); sizes or species make the next exchange
+ * or chemistry call reload the device chemistry fields from the records
+ * (so sync before editing records, as for every field).  A record that
+ * fails the bind's checks unbinds the context (BMX_ERR_BAD_ARG). */
 bmx_status bmx_refresh_from_aos(bmx_ctx *ctx, const void *aos, uint32_t fields);
 int64_t bmx_num_particles(const bmx_ctx *ctx);
 
