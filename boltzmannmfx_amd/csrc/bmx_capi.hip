@@ -154,10 +154,12 @@ struct bmx_ctx {
     hipEvent_t rdone[2] = {nullptr, nullptr}, adone[2] = {nullptr, nullptr};
     hipEvent_t join = nullptr;
   } adj;
-  // chemistry state (chem:: layout, original order), loaded from the
-  // records at the first exchange or mesh -> particle call after a bind and
-  // written back when the records are packed
-  DevBuf<double> d_chemst;
+  // chemistry state (chem:: layout, sorted order: d_chemst[cc] is current,
+  // the rebuild gathers it into the other buffer), loaded from the records
+  // at the first exchange or mesh -> particle call after a bind and written
+  // back when the records are packed
+  DevBuf<double> d_chemst[2];
+  int cc = 0;
   bool have_chem = false;
   // particle <-> mesh coupling: bins over the level mesh, staged fields
   DevBuf<int32_t> d_mkey, d_mrank, d_mcount, d_mstart, d_mscan;
@@ -502,11 +504,14 @@ bmx_status rebuild(bmx_ctx *c, bool check_skin = true, bool cols = false) {
   c->count_clear = true;
   HIPCHK(launch_bin_scatter(c->d_keytmp.p, c->d_rank.p, c->d_start.p, ss.oid,
                             c->n, c->d_permk.p, s));
+  const double *chem_src = c->have_chem ? c->d_chemst[c->cc].p : nullptr;
+  double *chem_dst = c->have_chem ? c->d_chemst[1 - c->cc].p : nullptr;
   HIPCHK(launch_sort_gather(c->d_permk.p, c->d_start.p, c->n, ss, ds, sd, dd,
                             c->d_keytmp.p, multi, c->any_fungi ? 1 : 0,
-                            c->d_err.p, s));
+                            c->d_err.p, s, chem_src, chem_dst));
   c->cs = 1 - c->cs;
   c->cd = 1 - c->cd;
+  if (c->have_chem) c->cc = 1 - c->cc;
   StaticArrays st = static_arrays(c, c->cs);
   DynArrays dy = dyn_arrays(c, c->cd);
   HIPCHK(launch_build_nbr(dy.pos, st.col, st.type, dy.rod, c->d_boxes.p, g0,
@@ -681,13 +686,30 @@ void reset_graph(bmx_ctx *c) {
 // The layout holds the amounts taken from the fluid (rdata[first_data + 6..8]).
 int chem_rel(const bmx_ctx *c) { return c->n_real >= rec::first_data + 9 ? 1 : 0; }
 
-// The chemistry state from the records, unless it is loaded.
+// The chemistry state from the records, unless it is loaded.  A captured
+// step sequence does not move it, so the graph is captured again.
 bmx_status load_chem(bmx_ctx *c) {
   if (c->have_chem) return BMX_OK;
-  HIPCHK(c->d_chemst.alloc((size_t)c->ncap * chem::stride));
-  HIPCHK(launch_chem_unpack(c->recs, c->lay, c->n, chem_rel(c), c->d_chemst.p,
-                            c->stream));
+  for (auto &b : c->d_chemst) HIPCHK(b.alloc((size_t)c->ncap * chem::stride));
+  c->cc = 0;
+  HIPCHK(launch_chem_unpack(c->recs, c->lay, c->n, chem_rel(c), c->oid[c->cs].p,
+                            c->d_chemst[0].p, c->stream));
   c->have_chem = true;
+  reset_graph(c);
+  return BMX_OK;
+}
+
+// Writes the chemistry state back into the records and drops it (reloaded
+// at the next use), for passes that keep their own copies of the sorted
+// arrays (the guard's snapshots, the autotuner's trials, the adjoint).
+bmx_status drop_chem(bmx_ctx *c) {
+  if (!c->have_chem) return BMX_OK;
+  HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, chem_rel(c),
+                          c->d_chemst[c->cc].p, c->oid[c->cs].p,
+                          c->type[c->cs].p, c->pos[c->cd].p, c->rod[c->cd].p,
+                          c->stream));
+  c->have_chem = false;
+  reset_graph(c);
   return BMX_OK;
 }
 
@@ -697,9 +719,10 @@ bmx_status pack_records(bmx_ctx *c) {
                          c->mobility, c->wrote_step ? 1 : 0,
                          c->any_fungi ? 1 : 0, c->stream));
   if (c->have_chem)
-    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, chem_rel(c), c->d_chemst.p,
-                            c->oid[c->cs].p, c->type[c->cs].p, c->pos[c->cd].p,
-                            c->rod[c->cd].p, c->stream));
+    HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, chem_rel(c),
+                            c->d_chemst[c->cc].p, c->oid[c->cs].p,
+                            c->type[c->cs].p, c->pos[c->cd].p, c->rod[c->cd].p,
+                            c->stream));
   return BMX_OK;
 }
 
@@ -946,7 +969,7 @@ void bmx_ctx_destroy(bmx_ctx *c) {
   c->d_vpart.release();
   c->d_slotmax.release();
   c->d_ustats.release();
-  c->d_chemst.release();
+  for (auto &b : c->d_chemst) b.release();
   c->d_mkey.release();
   c->d_mrank.release();
   c->d_mcount.release();
@@ -1316,6 +1339,11 @@ bmx_status guard_restore(bmx_ctx *c, const GuardMark &m) {
 bmx_status evolve_guarded(bmx_ctx *c, double dt, int32_t nsteps) {
   const int K = c->nbp.rebuild_every;
   hipStream_t s = c->stream;
+  {
+    // the snapshots hold the particle arrays only
+    bmx_status sc = drop_chem(c);
+    if (sc != BMX_OK) return sc;
+  }
   for (auto &z : c->gsnap)
     if (!z.done) HIPCHK(hipEventCreateWithFlags(&z.done, hipEventDisableTiming));
   GuardMark q[2];
@@ -1606,6 +1634,7 @@ bmx_status bmx_evolve_graph(bmx_ctx *c, double dt, int32_t nsteps) {
     if (need) {
       c->cs = 1 - c->cs;
       c->cd = 1 - c->cd;
+      if (c->have_chem) c->cc = 1 - c->cc;
       c->have_list = true;
       c->last_build = c->step;
       c->rebuilds_total++;
@@ -1634,6 +1663,8 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
                 "of rebuild_every)");
   (void)hipSetDevice(c->device);
   reset_graph(c);
+  st = drop_chem(c);  // the checkpoints hold positions only
+  if (st != BMX_OK) return st;
   hipStream_t s = c->stream;
   const int64_t n = c->n, cap = c->ncap;
   const int C = ckpt_every;
@@ -2024,8 +2055,8 @@ extern "C" bmx_status bmx_exchange(bmx_ctx *c, const double xpar[3], double dt) 
   st = rebuild(c, true, true);
   if (st != BMX_OK) return st;
   HIPCHK(launch_exchange(c->pos[c->cd].p, c->type[c->cs].p, c->topo[c->cd].p,
-                         c->oid[c->cs].p, c->d_nbr.p, c->d_ncnt.p, c->ncap,
-                         c->n, c->d_chemst.p, xpar, dt, s));
+                         c->d_nbr.p, c->d_ncnt.p, c->ncap, c->n,
+                         c->d_chemst[c->cc].p, xpar, dt, s));
   return check_flags(c);
 }
 
@@ -2185,6 +2216,8 @@ extern "C" bmx_status bmx_autotune(bmx_ctx *c, double dt, int32_t trial_steps,
   c->timing = false;
   const int K = c->nbp.rebuild_every;
   const int steps = std::max(K, ((trial_steps + K - 1) / K) * K);
+  st = drop_chem(c);  // the trials' snapshot holds the particle arrays only
+  if (st != BMX_OK) return st;
   StateSnap z;
   st = snap_take(c, z);
   if (st != BMX_OK) return st;
@@ -2370,15 +2403,17 @@ bmx_status mesh_deposit(bmx_ctx *c, int scheme, int comp0, int ncomp, int mode,
   // chemistry state while it holds them
   const uint8_t *src = c->recs;
   AosLayout lay = c->lay;
-  int comp = comp0;
+  int comp = comp0, by_sorted = 0;
   if (c->have_chem && (comp0 == rec::vol || comp0 == rec::first_data + 6)) {
-    src = reinterpret_cast<const uint8_t *>(c->d_chemst.p);
+    src = reinterpret_cast<const uint8_t *>(c->d_chemst[c->cc].p);
     lay.stride = chem::stride * (int32_t)sizeof(double);
     lay.rdata = 0;
     comp = comp0 == rec::vol ? chem::vol : chem::rel0;
+    by_sorted = 1;
   }
   HIPCHK(launch_deposit(src, lay, g, tri, c->d_mstart.p, c->d_mperm.p,
-                        c->pos[c->cd].p, comp, ncomp, mode, dst, c->stream));
+                        c->pos[c->cd].p, comp, ncomp, mode, dst, c->stream,
+                        by_sorted));
   if (!on_device)
     HIPCHK(hipMemcpyAsync(out, dst, sizeof(double) * (size_t)(nc * ncomp),
                           hipMemcpyDeviceToHost, c->stream));
@@ -2449,8 +2484,8 @@ extern "C" bmx_status bmx_calc_txfr_particle(bmx_ctx *c, const double chempar[10
   st = load_chem(c);
   if (st != BMX_OK) return st;
   HIPCHK(launch_fill_u64(c->d_ustats.p, 8, 0ull, s));
-  HIPCHK(launch_mesh_to_particle(c->d_chemst.p, c->n, g, c->d_mstart.p,
-                                 c->oid[c->cs].p, c->type[c->cs].p,
+  HIPCHK(launch_mesh_to_particle(c->d_chemst[c->cc].p, c->n, g, c->d_mstart.p,
+                                 c->type[c->cs].p,
                                  c->any_fungi ? c->topo[c->cd].p : nullptr,
                                  c->pos[c->cd].p, c->rod[c->cd].p, X, V,
                                  cnc_scheme, vf_scheme, chempar, dt, nloop,

@@ -18,12 +18,14 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;  // M_PI of <math.h>
 
 __global__ void k_chem_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
-                              int64_t n, int has_rel, double *__restrict__ state) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double *rd =
-      reinterpret_cast<const double *>(aos + i * (int64_t)lay.stride + lay.rdata);
-  double4 *q = reinterpret_cast<double4 *>(state + i * chem::stride);
+                              int64_t n, int has_rel,
+                              const int32_t *__restrict__ oid,
+                              double *__restrict__ state) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double *rd = reinterpret_cast<const double *>(
+      aos + (int64_t)oid[t] * lay.stride + lay.rdata);
+  double4 *q = reinterpret_cast<double4 *>(state + t * chem::stride);
   q[0] = make_double4(rd[rec::first_data], rd[rec::first_data + 1],
                       rd[rec::first_data + 2], rd[rec::vol]);
   q[1] = make_double4(rd[rec::first_data + 3], rd[rec::first_data + 4],
@@ -46,7 +48,7 @@ __global__ void k_chem_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
   if (t >= n) return;
   const int64_t o = oid[t];
   double *rd = reinterpret_cast<double *>(aos + o * (int64_t)lay.stride + lay.rdata);
-  const double4 *q = reinterpret_cast<const double4 *>(state + o * chem::stride);
+  const double4 *q = reinterpret_cast<const double4 *>(state + t * chem::stride);
   const double4 a = q[0], b = q[1], c = q[2], d = q[3];
   rd[rec::radius] = pos[t].w;
   if (type[t] == rec::FUNGI) rd[rec::c_length] = rod[t].x;
@@ -85,7 +87,6 @@ __device__ __forceinline__ double species_flux(double c1, double c2, double area
 __global__ void k_exchange_rows(const double4 *__restrict__ pos,
                                 const int32_t *__restrict__ type,
                                 const Topo *__restrict__ topo,
-                                const int32_t *__restrict__ oid,
                                 const int32_t *__restrict__ nbr,
                                 const int32_t *__restrict__ ncnt, int64_t ncap,
                                 int64_t n, double *__restrict__ state,
@@ -94,8 +95,7 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
   if (t >= n || type[t] != rec::FUNGI) return;
   const Topo ti = topo[t];
   const double r1 = pos[t].w;
-  const int64_t o = oid[t];
-  const double4 c1 = *reinterpret_cast<const double4 *>(state + o * chem::stride);
+  const double4 c1 = *reinterpret_cast<const double4 *>(state + t * chem::stride);
   double ia = 0.0, ib = 0.0, ic = 0.0;  // initExchange
   const int c = ncnt[t] & ((1 << 28) - 1);  // bits 28..30: image axes
   const int32_t *rp = nbr + t;
@@ -110,8 +110,7 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
       if (tj.seg_id[bj & 3] == ti.self_id && tj.seg_cpu[bj & 3] == ti.self_cpu) break;
     if (!(bi < ti.n_bnds && bj < tj.n_bnds)) continue;
     const double r2 = pos[u].w;
-    const double4 c2 =
-        *reinterpret_cast<const double4 *>(state + (int64_t)oid[u] * chem::stride);
+    const double4 c2 = *reinterpret_cast<const double4 *>(state + (int64_t)u * chem::stride);
     const double area1 = kPi * r1 * r1, area2 = kPi * r2 * r2;
     const double area = 0.5 * (area1 + area2);
     const double dA = species_flux(c1.x, c2.x, area, mtA, dt, c1.w, c2.w, ti.n_bnds, tj.n_bnds);
@@ -121,19 +120,18 @@ __global__ void k_exchange_rows(const double4 *__restrict__ pos,
     ib -= dB / c1.w;
     ic -= dC / c1.w;
   }
-  double *inc = state + o * chem::stride;
+  double *inc = state + t * chem::stride;
   inc[chem::dA] = ia;
   inc[chem::dB] = ib;
   inc[chem::dC] = ic;
 }
 
 // applyIncrements (bmx_chem_K.H:2092-2112)
-__global__ void k_exchange_apply(const int32_t *__restrict__ type,
-                                 const int32_t *__restrict__ oid, int64_t n,
+__global__ void k_exchange_apply(const int32_t *__restrict__ type, int64_t n,
                                  double *__restrict__ state) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n || type[t] != rec::FUNGI) return;
-  double *q = state + (int64_t)oid[t] * chem::stride;
+  double *q = state + t * chem::stride;
   q[chem::A] += q[chem::dA];
   q[chem::B] += q[chem::dB];
   q[chem::C] += q[chem::dC];
@@ -144,10 +142,11 @@ int grid256(int64_t n) { return (int)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)
 }  // namespace
 
 hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
-                              int has_rel, double *state, hipStream_t s) {
+                              int has_rel, const int32_t *oid, double *state,
+                              hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_chem_unpack, dim3(grid256(n)), dim3(256), 0, s, aos, lay,
-                     n, has_rel, state);
+                     n, has_rel, oid, state);
   return hipGetLastError();
 }
 
@@ -163,16 +162,16 @@ hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
 }
 
 hipError_t launch_exchange(const double4 *pos, const int32_t *type,
-                           const Topo *topo, const int32_t *oid,
+                           const Topo *topo,
                            const int32_t *nbr, const int32_t *ncnt,
                            int64_t ncap, int64_t n, double *state,
                            const double xpar[3], double dt, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_exchange_rows, dim3(grid256(n)), dim3(256), 0, s, pos,
-                     type, topo, oid, nbr, ncnt, ncap, n, state, xpar[0],
+                     type, topo, nbr, ncnt, ncap, n, state, xpar[0],
                      xpar[1], xpar[2], dt);
   hipLaunchKernelGGL(k_exchange_apply, dim3(grid256(n)), dim3(256), 0, s, type,
-                     oid, n, state);
+                     n, state);
   return hipGetLastError();
 }
 

@@ -88,7 +88,9 @@ hipError_t launch_sort_gather(const unsigned long long *permk,
                               StaticArrays src_s, StaticArrays dst_s,
                               DynArrays src_d, DynArrays dst_d,
                               const int32_t *key, int multi, int rods,
-                              int32_t *err, hipStream_t s);
+                              int32_t *err, hipStream_t s,
+                              const double *chem_src = nullptr,
+                              double *chem_dst = nullptr);
 
 // ---- neighbor list: flat rows (FlatRows) always; column-major rows
 // nbr[k*ncap + i] for segment contexts, and for sphere contexts only when
@@ -196,10 +198,11 @@ hipError_t launch_from_orig4(const double4 *in, const int32_t *oid, int64_t n,
 hipError_t launch_sum_partials(const double *part, int64_t nblk, int stride,
                                double *out, hipStream_t s);
 
-// The device chemistry state: per particle, in original (record) order, the
-// 16 doubles of one 128-byte line that ParticleExchange and the mesh ->
-// particle chemistry read and write, so neither touches the records until
-// they are packed (at sync, and before a pass that works on the records).
+// The device chemistry state: per particle, in sorted order (moved with the
+// other sorted arrays at every rebuild), the 16 doubles of one 128-byte line
+// that ParticleExchange, the mesh -> particle chemistry and the deposits read
+// and write, so none of them touches the records until they are packed (at
+// sync, and before a pass that works on the records).
 namespace chem {
 enum : int {
   A = 0, B = 1, C = 2, vol = 3,          // species, cell volume
@@ -209,9 +212,11 @@ enum : int {
   stride = 16
 };
 }
-// records -> state; has_rel: the layout holds rdata[first_data + 6..8]
+// records -> state (sorted position t from record oid[t]); has_rel: the
+// layout holds rdata[first_data + 6..8]
 hipError_t launch_chem_unpack(const uint8_t *aos, AosLayout lay, int64_t n,
-                              int has_rel, double *state, hipStream_t s);
+                              int has_rel, const int32_t *oid, double *state,
+                              hipStream_t s);
 // state -> records, with each particle's radius (pos.w) and, for segments,
 // its length (rod.x) from the sorted arrays
 hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
@@ -221,7 +226,7 @@ hipError_t launch_chem_pack(uint8_t *aos, AosLayout lay, int64_t n,
                             hipStream_t s);
 // ParticleExchange (bmx_exchange.hip) on the chemistry state
 hipError_t launch_exchange(const double4 *pos, const int32_t *type,
-                           const Topo *topo, const int32_t *oid,
+                           const Topo *topo,
                            const int32_t *nbr, const int32_t *ncnt,
                            int64_t ncap, int64_t n, double *state,
                            const double xpar[3], double dt, hipStream_t s);
@@ -236,19 +241,21 @@ hipError_t launch_mesh_bin(const double4 *pos, const int32_t *oid, int64_t n,
                            int32_t *scan_scratch, unsigned long long *perm,
                            int sorted, int32_t *err, hipStream_t s);
 // mode 0: sums of rdata[comp0..comp0+ncomp); 1: volume fraction of the
-// deposited volume; 2: particles per cell (one-to-one bins)
+// deposited volume; 2: particles per cell (one-to-one bins).  The source
+// records are indexed by original index, or by sorted index with by_sorted
+// (the chemistry state seen through a layout of stride 128 bytes)
 hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
                           int tri, const int32_t *start,
                           const unsigned long long *perm, const double4 *pos,
                           int comp0, int ncomp, int mode, double *out,
-                          hipStream_t s);
-// interpolation + chemistry per particle on the chemistry state (original
-// order); segments' bond count and tip flag from topo (sorted, current);
+                          hipStream_t s, int by_sorted = 0);
+// interpolation + chemistry per particle on the chemistry state; segments'
+// bond count and tip flag from topo (current);
 // new sizes into pos.w / rod.x / rod.w; stats[0], stats[1]: max radius and
 // segment length (bits)
 hipError_t launch_mesh_to_particle(double *state, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
-                                   const int32_t *oid, const int32_t *type,
+                                   const int32_t *type,
                                    const Topo *topo, double4 *pos, double4 *rod,
                                    const double *X, const double *vf,
                                    int cnc_tri, int vf_tri,

@@ -505,7 +505,9 @@ __global__ void k_sort_gather(const unsigned long long *__restrict__ permk,
                               const int32_t *__restrict__ start, int64_t n,
                               StaticArrays ss, StaticArrays ds, DynArrays sd,
                               DynArrays dd, const int32_t *__restrict__ key,
-                              int multi, int rods, int32_t *__restrict__ err) {
+                              int multi, int rods, int32_t *__restrict__ err,
+                              const double *__restrict__ chem_src,
+                              double *__restrict__ chem_dst) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const unsigned long long e = permk[j];
@@ -529,6 +531,15 @@ __global__ void k_sort_gather(const unsigned long long *__restrict__ permk,
     dd.rod[t] = sd.rod[s];
     dd.trig[t] = sd.trig[s];
     dd.topo[t] = sd.topo[s];
+  }
+  if (chem_src) {  // the chemistry state's line (chem::stride doubles)
+    const double4 *a = reinterpret_cast<const double4 *>(chem_src + s * chem::stride);
+    double4 *b = reinterpret_cast<double4 *>(chem_dst + t * chem::stride);
+    const double4 x0 = a[0], x1 = a[1], x2 = a[2], x3 = a[3];
+    b[0] = x0;
+    b[1] = x1;
+    b[2] = x2;
+    b[3] = x3;
   }
 }
 
@@ -1814,11 +1825,12 @@ hipError_t launch_sort_gather(const unsigned long long *permk,
                               StaticArrays src_s, StaticArrays dst_s,
                               DynArrays src_d, DynArrays dst_d,
                               const int32_t *key, int multi, int rods,
-                              int32_t *err, hipStream_t s) {
+                              int32_t *err, hipStream_t s,
+                              const double *chem_src, double *chem_dst) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_gather, dim3(grid_for(n, 256)), dim3(256), 0, s,
                      permk, start, n, src_s, dst_s, src_d, dst_d, key, multi,
-                     rods, err);
+                     rods, err, chem_src, chem_dst);
   return hipGetLastError();
 }
 

@@ -187,7 +187,7 @@ __global__ void k_deposit_one(const uint8_t *__restrict__ aos, AosLayout lay,
                               MeshGeom g, const int32_t *__restrict__ start,
                               const unsigned long long *__restrict__ perm,
                               int comp0, int ncomp, int mode,
-                              double *__restrict__ out) {
+                              double *__restrict__ out, int by_sorted) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
@@ -198,8 +198,10 @@ __global__ void k_deposit_one(const uint8_t *__restrict__ aos, AosLayout lay,
   }
   double sum[3] = {0.0, 0.0, 0.0};
   for (int32_t q = a; q < e; q++) {
+    const unsigned long long e = perm[q];
+    const int64_t src = by_sorted ? (int64_t)(uint32_t)e : (int64_t)(e >> 32);
     const double *rd = reinterpret_cast<const double *>(
-        aos + (int64_t)(perm[q] >> 32) * lay.stride + lay.rdata) + comp0;
+        aos + src * lay.stride + lay.rdata) + comp0;
     for (int m = 0; m < ncomp; m++) sum[m] += rd[m];
   }
   for (int m = 0; m < ncomp; m++) out[m * nc + c] = finish(sum[m], mode, g.inv_cell_vol);
@@ -232,7 +234,8 @@ __global__ void k_deposit_tri(const uint8_t *__restrict__ aos, AosLayout lay,
                               MeshGeom g, const int32_t *__restrict__ start,
                               const unsigned long long *__restrict__ perm,
                               const double4 *__restrict__ pos, int comp0,
-                              int ncomp, int mode, double *__restrict__ out) {
+                              int ncomp, int mode, double *__restrict__ out,
+                              int by_sorted) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
@@ -270,8 +273,8 @@ __global__ void k_deposit_tri(const uint8_t *__restrict__ aos, AosLayout lay,
     hv[best] = head[best] < end[best] ? perm[head[best]] : ~0ull;
     const int ii = best & 1, jj = (best >> 1) & 1, kk = best >> 2;
     const double w = tri_weight(pos[bt], g, ii, jj, kk);
-    const double *rd =
-        reinterpret_cast<const double *>(aos + bo * lay.stride + lay.rdata);
+    const double *rd = reinterpret_cast<const double *>(
+        aos + (by_sorted ? bt : bo) * lay.stride + lay.rdata);
     for (int m = 0; m < ncomp; m++) sum[m] += w * rd[comp0 + m];
   }
   for (int m = 0; m < ncomp; m++) out[m * nc + c] = finish(sum[m], mode, g.inv_cell_vol);
@@ -524,7 +527,7 @@ __device__ __forceinline__ double block_max_nonneg(double v, double *sh) {
 // and area into the sorted arrays with their maxima (one atomic per block).
 __global__ __launch_bounds__(256) void k_mesh_to_particle(
     double *__restrict__ state, int64_t n, MeshGeom g,
-    const int32_t *__restrict__ start, const int32_t *__restrict__ oid,
+    const int32_t *__restrict__ start,
     const int32_t *__restrict__ type, const Topo *__restrict__ topo,
     double4 *__restrict__ pos, double4 *__restrict__ rod,
     const double *__restrict__ X, const double *__restrict__ vf, int cnc_tri,
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(256) void k_mesh_to_particle(
     int c[3];
     if (mesh_cell(x, g, c)) {  // else reported by the binning pass
       // the particle's line of the chemistry state, all four parts at once
-      double4 *q = reinterpret_cast<double4 *>(state + (int64_t)oid[t] * chem::stride);
+      double4 *q = reinterpret_cast<double4 *>(state + t * chem::stride);
       const double4 q0 = q[0], q1 = q[1], q2 = q[2];
       bool tip = false;
       int nb = 0;
@@ -628,20 +631,21 @@ hipError_t launch_deposit(const uint8_t *aos, AosLayout lay, const MeshGeom &g,
                           int tri, const int32_t *start,
                           const unsigned long long *perm, const double4 *pos,
                           int comp0, int ncomp, int mode, double *out,
-                          hipStream_t s) {
+                          hipStream_t s, int by_sorted) {
   const int64_t nc = (int64_t)g.n[0] * g.n[1] * g.n[2];
   if (tri)
     hipLaunchKernelGGL(k_deposit_tri, dim3(grid_of(nc)), dim3(256), 0, s, aos,
-                       lay, g, start, perm, pos, comp0, ncomp, mode, out);
+                       lay, g, start, perm, pos, comp0, ncomp, mode, out,
+                       by_sorted);
   else
     hipLaunchKernelGGL(k_deposit_one, dim3(grid_of(nc)), dim3(256), 0, s, aos,
-                       lay, g, start, perm, comp0, ncomp, mode, out);
+                       lay, g, start, perm, comp0, ncomp, mode, out, by_sorted);
   return hipGetLastError();
 }
 
 hipError_t launch_mesh_to_particle(double *state, int64_t n,
                                    const MeshGeom &g, const int32_t *start,
-                                   const int32_t *oid, const int32_t *type,
+                                   const int32_t *type,
                                    const Topo *topo, double4 *pos, double4 *rod,
                                    const double *X, const double *vf,
                                    int cnc_tri, int vf_tri,
@@ -652,7 +656,7 @@ hipError_t launch_mesh_to_particle(double *state, int64_t n,
   ChemParams P{chempar[0], chempar[1], chempar[2], chempar[3], chempar[4],
                chempar[5], chempar[6], chempar[7], chempar[8], chempar[9]};
   hipLaunchKernelGGL(k_mesh_to_particle, dim3(grid_of(n)), dim3(256), 0, s,
-                     state, n, g, start, oid, type, topo, pos, rod, X, vf,
+                     state, n, g, start, type, topo, pos, rod, X, vf,
                      cnc_tri, vf_tri, P, dt, nloop, stats, err);
   return hipGetLastError();
 }

@@ -1,14 +1,22 @@
-// flop_count.cpp -- FP64 operation counts of one step of the segment path,
-// measured on the oracle's restatement of the reference's arithmetic.
+// flop_count.cpp -- FP64 operation counts of one step of the segment path
+// and of one mesh -> particle chemistry call, measured on the oracle's
+// restatement of the reference's arithmetic.
 //
 // TEST INFRASTRUCTURE: built and run by tests/test_oracle_flops.py.  The
 // oracle's source is compiled with every `double` replaced by a type that
 // counts its +, -, *, / and sqrt (fma counts 2), so the counts follow the
-// branches each pair actually takes.  Input: n particle records (the
-// Particle<34,35> layout, 448 bytes) in a file, and the level geometry,
-// force parameters and stencil neighborhood of the run.  Output (one JSON
-// object): candidates and interacting pairs of the step's lists, and the
-// operations of each part of the step.
+// branches each particle and pair actually takes.
+//
+//   flop_count records n plo[3] phi[3] periodic[3] n_cell[3] fpar[9] hood
+//     one segment step in the stencil policy: candidates and interacting
+//     pairs of the step's lists, and the operations of each part of it
+//   flop_count mesh records n plo[3] phi[3] periodic[3] n_cell[3] X vf
+//              chempar[10] dt nloop cnc_scheme vf_scheme
+//     one bmx_calc_txfr_particle over the records (X: 3 ghosted species
+//     meshes, vf: the ghosted volume fraction, files of doubles)
+//
+// Records are Particle<34,35> (448 bytes) in a file.  Output: one JSON
+// object.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -62,7 +70,63 @@ inline int isnan(Flop a) { return std::isnan(a.v); }
 #include "../../oracle/bmx_oracle.c"
 #undef double
 
+static Flop *read_doubles(const char *path, size_t count) {
+  Flop *a = (Flop *)std::malloc(sizeof(Flop) * count);
+  FILE *f = std::fopen(path, "rb");
+  if (!a || !f) return nullptr;
+  for (size_t i = 0; i < count; i++) {
+    double v;
+    if (std::fread(&v, sizeof v, 1, f) != 1) return nullptr;
+    a[i] = Flop(v);
+  }
+  std::fclose(f);
+  return a;
+}
+
+static int mesh_mode(int argc, char **argv) {
+  if (argc != 32) {
+    std::fprintf(stderr, "usage: flop_count mesh records n plo[3] phi[3] periodic[3] "
+                         "n_cell[3] X vf chempar[10] dt nloop cnc vfs\n");
+    return 2;
+  }
+  const long long n = std::atoll(argv[3]);
+  orc_particle *p = (orc_particle *)std::malloc(sizeof(orc_particle) * (size_t)n);
+  FILE *f = std::fopen(argv[2], "rb");
+  if (!f || !p || std::fread(p, sizeof(orc_particle), (size_t)n, f) != (size_t)n) return 3;
+  std::fclose(f);
+  orc_geom g;
+  int a = 4;
+  for (int d = 0; d < 3; d++) g.plo[d] = std::atof(argv[a++]);
+  for (int d = 0; d < 3; d++) g.phi[d] = std::atof(argv[a++]);
+  for (int d = 0; d < 3; d++) g.periodic[d] = std::atoi(argv[a++]);
+  for (int d = 0; d < 3; d++) g.nbin[d] = std::atoi(argv[a++]);
+  const size_t ng = (size_t)(g.nbin[0] + 2) * (g.nbin[1] + 2) * (g.nbin[2] + 2);
+  Flop *X = read_doubles(argv[a++], 3 * ng);
+  Flop *vf = read_doubles(argv[a++], ng);
+  if (!X || !vf) return 3;
+  Flop chempar[10];
+  for (int k = 0; k < 10; k++) chempar[k] = std::atof(argv[a++]);
+  const Flop dt = std::atof(argv[a++]);
+  const int nloop = std::atoi(argv[a++]);
+  const int cnc = std::atoi(argv[a++]);
+  const int vfs = std::atoi(argv[a++]);
+  Flop::n = 0;
+  const int err = orc_mesh_to_particle(p, n, &g, X, vf, chempar, dt, nloop, cnc, vfs);
+  const long long f_call = Flop::n;
+  Flop::n = 0;
+  Flop c = bmx_pow(Flop(2.3e-13), Flop(1.0) / Flop(3.0));
+  (void)c;
+  const long long f_pow = Flop::n - 1;
+  std::printf("{\"particles\": %lld, \"error_bits\": %d, \"flop_mesh_to_particle\": %lld, "
+              "\"flop_pow\": %lld}\n", n, err, f_call, f_pow);
+  std::free(p);
+  std::free(X);
+  std::free(vf);
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 1 && std::strcmp(argv[1], "mesh") == 0) return mesh_mode(argc, argv);
   if (argc != 25) {
     std::fprintf(stderr,
                  "usage: flop_count records n plo[3] phi[3] periodic[3] "

@@ -156,40 +156,59 @@ def test_mesh_to_particle_matches_oracle(bmx, oracle, gpu, kind, cnc, vfs):
 
 @pytest.mark.parametrize("kind", ["spheres", "hyphae"])
 def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
-    """Steps, mesh -> particle, the deposits that read what it wrote, and
-    mesh -> particle again, with no sync in between: each call sees the
+    """Steps, exchange (segments), mesh -> particle, the deposits that read
+    what it wrote, mesh -> particle again, more steps (the lists rebuilt
+    for the new sizes, the device chemistry moved with the new order), and
+    mesh -> particle once more, with no sync in between: each call sees the
     device chemistry state and the bond state the steps left."""
     col = spheres(seed=12) if kind == "spheres" else hyphae(seed=19)
     mode = 0 if kind == "spheres" else 1
     ctx, grecs, g = setup(bmx, oracle, col, mode)
     orecs = col.recs.copy()
+    hood = 1.1 * (2 * 2e-4 + 1e-2)
+    xpar = [2e-5 * 50, 2e-5 * 70, 2e-5 * 30]
+
+    def oracle_state():
+        if kind == "hyphae":
+            return oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, hood, 1)
+        return oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
+                                                    ctx.bin_dims()),
+                            col.fpar, oracle.NBR_PAIR, col.skin, 10)
+    st = oracle_state()
     if kind == "hyphae":
         ctx.init_bonds()
-        st = oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL,
-                          1.1 * (2 * 2e-4 + 1e-2), 1)
         st.init_bonds()
-    else:
-        st = oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
-                                                  ctx.bin_dims()),
-                          col.fpar, oracle.NBR_PAIR, col.skin, 10)
     ctx.evolve(col.dt, 3)
     st.evolve(col.dt, 3)
+    if kind == "hyphae":
+        ctx.exchange(xpar, 1.0)
+        st.exchange(xpar, 1.0)
     X, vf = mesh_inputs(col, seed=13)
     ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 1, 1)
     rhs = ctx.txfr_fluid(1)
     vfn = ctx.volume_fraction(0)
     ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 0, 0)
+    ctx.evolve(col.dt, 3)
+    if kind == "hyphae":
+        ctx.exchange(xpar, 1.0)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 1, 0)
     ctx.sync()
     assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 1) == 0
     orhs, _ = oracle.deposit(orecs, g, 1, 31, 3)
     ovf, _ = oracle.volume_fraction(orecs, g, 0)
     assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 0, 0) == 0
+    st.close()
+    st = oracle_state()             # the grown sizes: rows rebuilt, as on the GPU
+    st.evolve(col.dt, 3)
+    if kind == "hyphae":
+        st.exchange(xpar, 1.0)
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 0) == 0
     assert np.abs(rhs).max() > 0
     assert np.array_equal(rhs, orhs)
     assert np.array_equal(vfn, ovf)
+    assert np.array_equal(grecs["pos"], orecs["pos"])
     assert np.array_equal(grecs["rdata"], orecs["rdata"])
     assert np.array_equal(grecs["idata"], orecs["idata"])
-    assert np.array_equal(grecs["pos"], orecs["pos"])
     st.close()
     ctx.close()
 

@@ -1,7 +1,10 @@
-"""Algorithmic FP64 operation counts of the segment step, counted on the
-oracle (tests/flops/flop_count.cpp compiles it with a counting double) over
-the 200,000-segment workload after InitBonds, and checked against the
-figures the segment roofline uses (profiles/force_general_200k_pmc.json).
+"""Algorithmic FP64 operation counts, counted on the oracle
+(tests/flops/flop_count.cpp compiles it with a counting double) and checked
+against the figures the rooflines use:
+  * the segment step over the 200,000-segment workload after InitBonds
+    (profiles/force_general_200k_pmc.json);
+  * one mesh -> particle chemistry call over the 1,000,000-sphere mesh
+    workload (profiles/mesh_to_particle_1M_flops.json).
 
 Two formulations of the same arithmetic:
   * the reference's (K.H:277-993 per pair: the sines and cosines of both
@@ -23,7 +26,7 @@ from boltzmannmfx_amd import workloads as W
 from conftest import ROOT
 
 
-def counts(tmp_path):
+def build_counter(tmp_path):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.fail("g++ is needed to build the operation counter")
@@ -31,6 +34,11 @@ def counts(tmp_path):
     src = os.path.join(ROOT, "tests", "flops", "flop_count.cpp")
     subprocess.run([gxx, "-std=c++17", "-O2", "-ffp-contract=off", "-fpermissive", "-w",
                     "-o", exe, src], check=True)
+    return exe
+
+
+def counts(tmp_path):
+    exe = build_counter(tmp_path)
     col = W.c3_hyphae()
     recs = str(tmp_path / "recs.bin")
     col.recs.tofile(recs)
@@ -66,3 +74,47 @@ def test_segment_step_operation_counts(tmp_path):
     assert prof["interacting_pairs_per_launch"] == c["pairs"]
     assert prof["fp64_flop_per_launch_reference_formulation"] == reference
     assert prof["fp64_flop_per_launch"] == kernel
+
+
+# The fungi deck's chemistry rates (exec/fungi/input_fungi): k1 k2 k3 kr1 kr2
+# kr3 kg kv, max_seg_radius, max_seg_length
+MESH_CHEMPAR = [2.0e-5, 4.0, 2.0e-5, 2.0e-5, 6.0e-4, 2.0e-5, 4.0, 4.0e4, 2.5e-4, 35.0e-4]
+
+
+def test_mesh_to_particle_operation_counts(tmp_path):
+    """The 1,000,000-sphere mesh workload as the bench builds it (volumes,
+    areas and species set from the radii, the 64 x 64 x 32 mesh's species
+    and volume fraction drawn with the same seed), one one-to-one call with
+    4 substeps."""
+    exe = build_counter(tmp_path)
+    n, n_cell = 1_000_000, (64, 64, 32)
+    col = W.c2_large(n)
+    r = col.recs["rdata"]
+    rng = np.random.default_rng(11)
+    r[:, 5] = 4.0 / 3.0 * np.pi * r[:, 0] ** 3
+    r[:, 4] = 4.0 * np.pi * r[:, 0] ** 2
+    r[:, 25:28] = rng.uniform(0.0, 2e-5, (n, 3))
+    nx, ny, nz = n_cell
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    X = np.empty((3, ng))
+    X[0] = rng.uniform(1.0e-5, 3.0e-5, ng)
+    X[1] = rng.uniform(0.0, 1.0e-7, ng)
+    X[2] = rng.uniform(1.0e-6, 3.0e-6, ng)
+    vf = rng.uniform(0.2, 1.0, ng)
+    recs, fx, fv = (str(tmp_path / f) for f in ("recs.bin", "X.bin", "vf.bin"))
+    col.recs.tofile(recs)
+    X.tofile(fx)
+    vf.tofile(fv)
+    args = [exe, "mesh", recs, str(n)]
+    args += [repr(float(v)) for v in col.prob_lo] + [repr(float(v)) for v in col.prob_hi]
+    args += [str(v) for v in col.periodic] + [str(v) for v in n_cell] + [fx, fv]
+    args += [repr(v) for v in MESH_CHEMPAR] + ["0.001", "4", "0", "0"]
+    c = json.loads(subprocess.run(args, check=True, capture_output=True, text=True).stdout)
+    assert c["error_bits"] == 0
+    # the correctly rounded pow of the chemistry, four calls per sphere
+    assert 300 < c["flop_pow"] < 1000
+    assert c["flop_mesh_to_particle"] > 4 * c["flop_pow"] * n
+    with open(os.path.join(ROOT, "profiles", "mesh_to_particle_1M_flops.json")) as f:
+        prof = json.load(f)
+    assert prof["fp64_flop_per_launch"] == c["flop_mesh_to_particle"]
+    assert prof["flop_pow"] == c["flop_pow"]
