@@ -168,13 +168,13 @@ def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
     hood = 1.1 * (2 * 2e-4 + 1e-2)
     xpar = [2e-5 * 50, 2e-5 * 70, 2e-5 * 30]
 
-    def oracle_state():
+    def oracle_state(bins):
         if kind == "hyphae":
             return oracle.State(orecs, g, col.fpar, oracle.NBR_STENCIL, hood, 1)
         return oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
-                                                    ctx.bin_dims()),
+                                                    bins),
                             col.fpar, oracle.NBR_PAIR, col.skin, 10)
-    st = oracle_state()
+    st = oracle_state(ctx.bin_dims())
     if kind == "hyphae":
         ctx.init_bonds()
         st.init_bonds()
@@ -189,6 +189,7 @@ def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
     vfn = ctx.volume_fraction(0)
     ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 0, 0)
     ctx.evolve(col.dt, 3)
+    bins = ctx.bin_dims()           # of these steps (the next growth changes them)
     if kind == "hyphae":
         ctx.exchange(xpar, 1.0)
     ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 1, 0)
@@ -198,7 +199,7 @@ def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
     ovf, _ = oracle.volume_fraction(orecs, g, 0)
     assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 0, 0) == 0
     st.close()
-    st = oracle_state()             # the grown sizes: rows rebuilt, as on the GPU
+    st = oracle_state(bins)         # the grown sizes: rows rebuilt, as on the GPU
     st.evolve(col.dt, 3)
     if kind == "hyphae":
         st.exchange(xpar, 1.0)
@@ -209,6 +210,37 @@ def test_chemistry_sequence_without_sync(bmx, oracle, gpu, kind):
     assert np.array_equal(grecs["pos"], orecs["pos"])
     assert np.array_equal(grecs["rdata"], orecs["rdata"])
     assert np.array_equal(grecs["idata"], orecs["idata"])
+    st.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("tune,guard", [(False, False), (False, True), (True, False)])
+def test_chemistry_state_across_guarded_steps_and_autotune(bmx, oracle, gpu, tune, guard):
+    """The passes that snapshot the sorted arrays (guarded steps, the
+    autotuner's trials) write the device chemistry back first: mesh ->
+    particle, autotune, guarded steps, mesh -> particle, then one sync,
+    against the oracle."""
+    col = spheres(seed=21)
+    ctx, grecs, g = setup(bmx, oracle, col)
+    orecs = col.recs.copy()
+    X, vf = mesh_inputs(col, seed=22)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 0, 0)
+    if tune:
+        ctx.autotune(col.dt, trial_steps=10)
+    ctx.set_guard(guard)
+    ctx.evolve(col.dt, 12)
+    bins = ctx.bin_dims()           # of these steps (the next growth changes them)
+    ctx.txfr_particle(CHEMPAR, X, vf, 0.25, 4, 1, 1)
+    ctx.sync()
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 0, 0) == 0
+    st = oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic, bins),
+                      col.fpar, oracle.NBR_PAIR, col.skin, 10)
+    st.evolve(col.dt, 12)
+    assert oracle.mesh_to_particle(orecs, g, X, vf, CHEMPAR, 0.25, 4, 1, 1) == 0
+    assert np.array_equal(grecs["pos"], orecs["pos"])
+    bad = np.where((grecs["rdata"] != orecs["rdata"]).any(axis=0))[0]
+    rows = np.where((grecs["rdata"] != orecs["rdata"]).any(axis=1))[0]
+    assert bad.size == 0, f"rdata columns {bad.tolist()} differ in {rows.size} records"
     st.close()
     ctx.close()
 
