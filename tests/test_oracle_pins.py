@@ -259,6 +259,52 @@ def test_min_distance_special_cases(oracle):
     assert (ti, tj) == (1.0, 0.0)
 
 
+# Known answers in every branch of evaluateMinimumDistance (K.H:51-221), each
+# worked by hand for segment i = (0,0,0)-(4,0,0) and a skew segment j whose
+# coordinates are dyadic, so every quantity is exact in binary.  Take
+# e = i1 - j1, Bi = -(e.ri), Bj = e.rj, Aii = 16, Aij = -ri.rj:
+#  A  j = (0.5,-6,1)-(1.5,-4,1):  unconstrained (0.875, 3): tau_j out; the
+#     projections of j's ends, 0.125 and 0.375, both inside; their
+#     separations 37 > 17 -> (0.375, 1)
+#  B  j reversed: unconstrained (0.875, -2); projections 0.375, 0.125;
+#     separations 17 < 37 -> (0.375, 0)
+#  C  j = (0.5,6,1)-(-0.5,8,1): unconstrained (0.875, -3); only the first
+#     projection, 0.125, inside -> (0.125, 0)
+#  D  j reversed: unconstrained (0.875, 4); only the second, 0.125 -> (0.125, 1)
+#  E  j = (-0.5,6,1)-(-1.75,8,1): unconstrained (0.8125, -3); projections
+#     -0.125 and -0.4375 both outside -> the end pairs, nearest (i1, j1):
+#     37.25 against 68.0625, 57.25, 98.0625 -> (0, 0)
+# Swapping the segments takes the mirror branch (tau_i outside, tau_j
+# inside) and must give the mirrored answer.
+MIN_DISTANCE_CASES = [
+    ((0.5, -6, 1, 1.5, -4, 1), (0.375, 1.0)),
+    ((1.5, -4, 1, 0.5, -6, 1), (0.375, 0.0)),
+    ((0.5, 6, 1, -0.5, 8, 1), (0.125, 0.0)),
+    ((-0.5, 8, 1, 0.5, 6, 1), (0.125, 1.0)),
+    ((-0.5, 6, 1, -1.75, 8, 1), (0.0, 0.0)),
+]
+
+
+@pytest.mark.parametrize("sj,expected", MIN_DISTANCE_CASES)
+def test_min_distance_branch_known_answers(oracle, sj, expected):
+    si = (0, 0, 0, 4, 0, 0)
+    assert oracle.min_distance(si, sj) == expected
+    assert oracle.min_distance(sj, si) == expected[::-1]
+
+
+def test_min_distance_end_pair_ties(oracle):
+    """The end-pair search keeps the first of equal separations (strict <,
+    K.H:189-219).  j = (2,-1,3)-(6,-0.5,7) passes the axis of i outside
+    both segments, so both parameters are out of range; its first end is
+    equidistant (14) from both ends of i, the other pairs are farther
+    (85.25, 53.25): (i1, j1) wins the tie in either order of i's ends and
+    in the swapped call."""
+    sj = (2, -1, 3, 6, -0.5, 7)
+    assert oracle.min_distance((0, 0, 0, 4, 0, 0), sj) == (0.0, 0.0)
+    assert oracle.min_distance((4, 0, 0, 0, 0, 0), sj) == (0.0, 0.0)
+    assert oracle.min_distance(sj, (0, 0, 0, 4, 0, 0)) == (0.0, 0.0)
+
+
 # ------------------------------------------------------------- rod pairs --
 def rp_coeffs(area, L, visc):
     """Two-site mobility of an area-equivalent sphere pair (model1.tex:624-843):
