@@ -2,6 +2,7 @@
 // reader.
 #include "bmx_container.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -81,6 +82,62 @@ bmx_geometry Deck::finest_geometry() const {
     g.n_cell[d] = nc;
   }
   return g;
+}
+
+void Deck::finest_region(double lo[3], double hi[3]) const {
+  const int levels = integer_or("amr.max_level", 0);
+  const int ratio = integer_or("amr.ref_ratio", 2);
+  const int buf = integer_or("amr.n_error_buf", 1);
+  const int bf = std::max(1, integer_or("amr.blocking_factor", 8));
+  bool tag = false;
+  if (has("bmx.tag_region")) {
+    const std::string v = str("bmx.tag_region");
+    tag = v == "true" || v == "1" || v == "True";
+  }
+  int64_t klo[3], khi[3], nc[3];
+  double plo[3], phi[3];
+  for (int d = 0; d < 3; d++) {
+    plo[d] = real("geometry.prob_lo", d);
+    phi[d] = real("geometry.prob_hi", d);
+    nc[d] = integer("amr.n_cell", d);
+    klo[d] = 0;
+    khi[d] = nc[d] - 1;
+  }
+  for (int l = 0; l < levels && tag; l++) {
+    // tagged cells of level l, with ErrorEst's own test: the centre
+    // (k + 0.5) dx (from 0, as bmx_tagging.cpp forms it) in [tlo, thi]
+    int64_t a[3], b[3];
+    bool any = true;
+    for (int d = 0; d < 3; d++) {
+      const double dx = (phi[d] - plo[d]) / (double)nc[d];
+      const double tlo = real("bmx.tag_region_lo", d), thi = real("bmx.tag_region_hi", d);
+      a[d] = khi[d] + 1;
+      b[d] = klo[d] - 1;
+      for (int64_t k = klo[d]; k <= khi[d]; k++) {
+        const double c = ((double)k + 0.5) * dx;
+        if (c >= tlo && c <= thi) {
+          a[d] = std::min(a[d], k);
+          b[d] = std::max(b[d], k);
+        }
+      }
+      any = any && a[d] <= b[d];
+    }
+    if (!any) break;  // nothing tagged: the finest level is this one
+    for (int d = 0; d < 3; d++) {
+      // grown by the error buffer within level l, refined, aligned outward
+      // to the blocking factor, within the domain
+      const int64_t g0 = std::max(a[d] - buf, klo[d]);
+      const int64_t g1 = std::min(b[d] + buf, khi[d]);
+      nc[d] *= ratio;
+      klo[d] = std::max<int64_t>((g0 * ratio) / bf * bf, 0);
+      khi[d] = std::min<int64_t>(((g1 + 1) * ratio + bf - 1) / bf * bf - 1, nc[d] - 1);
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    const double dx = (phi[d] - plo[d]) / (double)nc[d];
+    lo[d] = plo[d] + (double)klo[d] * dx;
+    hi[d] = plo[d] + (double)(khi[d] + 1) * dx;
+  }
 }
 
 // BMXCellInteraction::setParams (src/chemistry/bmx_cell_interaction.cpp:54-93)
@@ -308,6 +365,22 @@ const std::vector<uint8_t> &BMXParticleContainer::Records() {
     dirty_ = false;
   }
   return host_;
+}
+
+int64_t BMXParticleContainer::CountOutside(const double lo[3], const double hi[3]) {
+  const std::vector<uint8_t> &r = Records();
+  const bmx_layout L = bmx_default_layout();
+  int64_t out = 0;
+  for (int64_t i = 0; i < n_; i++) {
+    double x[3];
+    std::memcpy(x, r.data() + i * L.stride_bytes + L.pos_offset, sizeof x);
+    for (int d = 0; d < 3; d++)
+      if (!(x[d] >= lo[d] && x[d] < hi[d])) {
+        out++;
+        break;
+      }
+  }
+  return out;
 }
 
 void BMXParticleContainer::WriteAsciiFileForInit(const std::string &file) {

@@ -50,6 +50,14 @@ struct Deck {
 
   // derived
   bmx_geometry finest_geometry() const;  // level mesh of amr.max_level
+  // The region the finest level covers when refinement is tagged by the
+  // deck's bmx.tag_region box (bmx::ErrorEst, src/setup/bmx_tagging.cpp):
+  // at each level the cells whose centres lie in the box, grown by
+  // amr.n_error_buf, refined by amr.ref_ratio and aligned outward to
+  // amr.blocking_factor, within the level below.  A model of AMReX's grid
+  // generation for one tagged box; the whole domain without tagging or
+  // with amr.max_level = 0.
+  void finest_region(double lo[3], double hi[3]) const;
   void force_params(double fpar[9]) const;
   double neighborhood() const;           // DEM::neighborhood
   // chem_species.mass_transfer_A/B/C (src/chemistry/bmx_chem.cpp:100-105)
@@ -118,6 +126,8 @@ class BMXParticleContainer {
   int64_t NumberOfParticles() const { return n_; }
   // Host copy of the records, brought up to date.
   const std::vector<uint8_t> &Records();
+  // Particles whose position lies outside [lo, hi).
+  int64_t CountOutside(const double lo[3], const double hi[3]);
   bmx_ctx *context() { return ctx_; }
 
  private:

@@ -39,6 +39,21 @@ int main(int argc, char **argv) {
     pc.InitParticlesAscii(input);
     std::printf("Read %lld particles from %s\n", (long long)pc.NumberOfParticles(),
                 input.c_str());
+    // Every particle runs on the finest level here; the reference keeps the
+    // particles outside the refined region on coarser levels, with their
+    // own lists (src/des/bmx_pc_interaction.cpp:22-28).  Say so when any lie
+    // there.
+    double rlo[3], rhi[3];
+    deck.finest_region(rlo, rhi);
+    auto outside = [&](const char *when) {
+      const int64_t k = pc.CountOutside(rlo, rhi);
+      if (k > 0)
+        std::printf("Warning: %lld particles %s lie outside the finest level's region "
+                    "[%g %g %g] - [%g %g %g]; this driver keeps them on the finest "
+                    "level\n", (long long)k, when, rlo[0], rlo[1], rlo[2], rhi[0],
+                    rhi[1], rhi[2]);
+    };
+    outside("at the start");
     double time = 0.0;
     for (int step = 0; step < max_step; step++) {
       auto t0 = std::chrono::steady_clock::now();
@@ -54,6 +69,7 @@ int main(int argc, char **argv) {
                   (long long)pc.NumberOfParticles(),
                   std::chrono::duration<double>(t1 - t0).count());
     }
+    outside("at the end");
     if (argc > 3) pc.WriteAsciiFileForInit(argv[3]);
   } catch (const bmx::Error &e) {
     std::fprintf(stderr, "bmx_run: %s (status %d)\n", e.what(), (int)e.code());

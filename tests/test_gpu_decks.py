@@ -46,6 +46,9 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
                          timeout=120)
     assert res.returncode == 0, res.stderr
     assert res.stdout.count("Time per step") == steps
+    # the deck's particles stay inside its refined region (the driver warns
+    # otherwise: the reference would move them to a coarser level)
+    assert "Warning" not in res.stdout
 
     kv = parse_deck(path)
     def f(k, i=0, default=None):
