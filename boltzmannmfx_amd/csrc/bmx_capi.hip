@@ -170,7 +170,7 @@ struct bmx_ctx {
   int64_t step = 0, last_build = -1;
 
   // outputs and flags
-  DevBuf<double> d_fc;
+  DevBuf<double> d_fc;  // last step's fc: original order, 6 per particle
   bool wrote_step = false;
   DevBuf<int32_t> d_err, d_stats, d_bstats;
   DevBuf<unsigned long long> d_total, d_ustats;

@@ -14,6 +14,8 @@
 // does (src/des/bmx_pc_interaction.cpp:170-279), then applies the overdamped
 // update (src/des/bmx_pc_interaction.cpp:311-449) in the same launch,
 // writing the next state into the other half of a ping-pong pair.
+#include <type_traits>
+
 #include "bmx_device.h"
 #include "bmx_kernels.h"
 
@@ -256,6 +258,19 @@ __global__ void k_aos_refresh(const uint8_t *__restrict__ aos, AosLayout lay,
   }
 }
 
+// fc of the last step is kept in original particle order, six doubles per
+// particle (force, then torque) in one 48-byte run: a step that writes it
+// stores three 16-byte pieces per particle where the component-major
+// layout of the export would scatter six 8-byte words.
+__device__ __forceinline__ void store_fc(double *__restrict__ fc, int64_t o,
+                                         double f0, double f1, double f2,
+                                         double f3, double f4, double f5) {
+  double2 *q = reinterpret_cast<double2 *>(fc + o * 6);
+  q[0] = make_double2(f0, f1);
+  q[1] = make_double2(f2, f3);
+  q[2] = make_double2(f4, f5);
+}
+
 __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy,
                            const double *__restrict__ fc, int64_t ncap,
@@ -272,12 +287,13 @@ __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
   pos[1] = p.y;
   pos[2] = p.z;
   if (wrote_step) {
-    rd[rec::velx] = mob * fc[o];  // fc is in original order
-    rd[rec::vely] = mob * fc[o + ncap];
-    rd[rec::velz] = mob * fc[o + 2 * ncap];
-    rd[rec::wx] = fc[o + 3 * ncap];
-    rd[rec::wy] = fc[o + 4 * ncap];
-    rd[rec::wz] = fc[o + 5 * ncap];
+    const double *f = fc + o * 6;  // original order, six per particle
+    rd[rec::velx] = mob * f[0];
+    rd[rec::vely] = mob * f[1];
+    rd[rec::velz] = mob * f[2];
+    rd[rec::wx] = f[3];
+    rd[rec::wy] = f[4];
+    rd[rec::wz] = f[5];
     id[rec::fuse_flag] = 0;
     id[rec::split_flag] = 0;
     id[rec::new_flag] = 0;
@@ -614,18 +630,18 @@ __device__ __forceinline__ void scan_range(
 // (bins are x-fastest) unless it wraps a periodic x face, where it splits
 // into the bins before the face and the bins after it (piece 0 holds the
 // bins of ox = -1 first).  Rows outside a non-periodic box are empty.
-// Everything that does not change from range to range (the three z and
-// three y row offsets, the x pieces) is formed once per particle, so moving
-// to the next range costs a few selects and the two loads of its bounds.
+// Everything that does not change from range to range is formed once per
+// particle, with the row offsets already multiplied out, so that moving to
+// the next range costs a few full-rate selects and adds and the two loads
+// of its bounds (no integer multiply, which runs at a quarter of the rate).
 struct Stencil {
-  // (cz * nby) of oz = -1, 0, 1 and cy of oy = -1, 0, 1, or -1 outside the
-  // box (named fields, not arrays: a select between array elements becomes
-  // an indexed scratch access)
+  // bin id of the first bin of z row cz (base + cz * nby * nbx) for
+  // oz = -1, 0, 1, and cy * nbx for oy = -1, 0, 1; -1 outside the box
+  // (named fields, not arrays: a select between array elements becomes an
+  // indexed scratch access)
   int z0, z1, z2, y0, y1, y2;
   int xa0, xb0, xa1, xb1;  // x pieces [xa, xb) of every row
-  int two;     // 1 when a row has two pieces
-  int base;    // bin id of the box's first bin
-  int nbx;
+  int two;                 // 1 when a row has two pieces
 };
 
 __device__ __forceinline__ Stencil make_stencil(int bx, int by, int bz,
@@ -634,11 +650,11 @@ __device__ __forceinline__ Stencil make_stencil(int bx, int by, int bz,
   const int nbx = g.nbin[0], nby = g.nbin[1], nbz = g.nbin[2];
   auto zrow = [&](int cz) {
     if (g.periodic[2]) cz = cz < 0 ? cz + nbz : (cz >= nbz ? cz - nbz : cz);
-    return (cz >= 0 && cz < nbz) ? cz * nby : -1;
+    return (cz >= 0 && cz < nbz) ? g.bin_offset + cz * nby * nbx : -1;
   };
   auto ycol = [&](int cy) {
     if (g.periodic[1]) cy = cy < 0 ? cy + nby : (cy >= nby ? cy - nby : cy);
-    return (cy >= 0 && cy < nby) ? cy : -1;
+    return (cy >= 0 && cy < nby) ? cy * nbx : -1;
   };
   st.z0 = zrow(bz - 1);
   st.z1 = zrow(bz);
@@ -666,26 +682,27 @@ __device__ __forceinline__ Stencil make_stencil(int bx, int by, int bz,
     st.xb0 = hi;
   }
   st.two = st.xb1 > st.xa1;
-  st.base = g.bin_offset;
-  st.nbx = nbx;
   return st;
 }
 
-// Memory range of range k of the stencil (k < 9 << two).  The field
-// choices are value selects written as arithmetic: a conditional between
-// two fields of one object can become a select of their addresses, which
-// puts the object in scratch memory.
+// Memory range of range k of the stencil (k < 9 << two): row r = k >> two
+// (oz = r / 3, oy = r % 3), piece k & two.  Fields are read into locals
+// before they are selected, so that the selects are between values.
 __device__ __forceinline__ int2 stencil_range(const Stencil st, int k,
                                               const int32_t *__restrict__ start) {
-  const int r = st.two ? (k >> 1) : k;
-  const int piece = st.two ? (k & 1) : 0;
-  const int iz = r / 3, iy = r - 3 * iz;
-  const int zz = (iz == 0) * st.z0 + (iz == 1) * st.z1 + (iz == 2) * st.z2;
-  const int yy = (iy == 0) * st.y0 + (iy == 1) * st.y1 + (iy == 2) * st.y2;
-  if (zz < 0 || yy < 0) return make_int2(0, 0);
-  const int row = st.base + (zz + yy) * st.nbx;
-  const int xa = st.xa0 + piece * (st.xa1 - st.xa0);
-  const int xb = st.xb0 + piece * (st.xb1 - st.xb0);
+  const int z0 = st.z0, z1 = st.z1, z2 = st.z2;
+  const int y0 = st.y0, y1 = st.y1, y2 = st.y2;
+  const int r = k >> st.two, piece = k & st.two;
+  // r / 3 as 11 r / 32 (exact for r < 9), with 24-bit multiplies, which
+  // run at the full rate
+  const int iz = (int)(__umul24((unsigned)r, 11u) >> 5);
+  const int iy = r - (int)__umul24((unsigned)iz, 3u);
+  const int zz = iz == 0 ? z0 : (iz == 1 ? z1 : z2);
+  const int yy = iy == 0 ? y0 : (iy == 1 ? y1 : y2);
+  if ((zz | yy) < 0) return make_int2(0, 0);
+  const int xa0 = st.xa0, xa1 = st.xa1, xb0 = st.xb0, xb1 = st.xb1;
+  const int row = zz + yy;
+  const int xa = piece ? xa1 : xa0, xb = piece ? xb1 : xb0;
   return make_int2(start[row + xa], start[row + xb]);
 }
 
@@ -723,36 +740,44 @@ __device__ __forceinline__ int32_t row_get(const RowSink &sk, int k) {
 // at scan_cap per particle (a range that would pass it is dropped and the
 // row reported as overflowing).  Bit a of M asks for the periodic image
 // test along axis a; an axis left out is one along which no candidate of
-// the wave can need an image.  was = W_A + skin.  Sphere builds: a
-// candidate's reach is its radius.
-template <int M>
+// the wave can need an image.  CRIT: the stencil policy's test of the
+// squared distance against crit; otherwise the Verlet test against the
+// reaches plus was = W_A + skin (a sphere's reach is its radius).
+template <int M, bool CRIT>
 __device__ __forceinline__ void scan_stencil(
     const Stencil sc, const BoxGeom &g, const int32_t *__restrict__ start,
     int64_t scan_cap, int t, const double4 &p,
-    const double4 *__restrict__ pos, int mode, double crit, double was,
+    const double4 *__restrict__ pos, double crit, double was,
     const RowSink &sk, int32_t maxn, int &cnt) {
   const int nq = 9 << sc.two;
-  int64_t scanned = 0;
+  // 32-bit count of the candidates taken; once a range would pass the cap,
+  // it and every later range are dropped (scanned sticks above the cap)
+  const int cap = scan_cap < (1 << 30) ? (int)scan_cap : (1 << 30);
+  int scanned = 0;
   int q = 0;
   const int2 first = stencil_range(sc, 0, start);
   int2 next = stencil_range(sc, 1, start);
   int u = first.x, e = first.y;
-  scanned = e - u;
-  if (scanned > scan_cap) {
+  if (e - u > cap) {
     cnt = maxn + 1;
     e = u;
+    scanned = cap + 1;
+  } else {
+    scanned = e - u;
   }
   for (;;) {
     while (u >= e && q + 1 < nq) {
       q++;
       u = next.x;
       e = next.y;
-      const int64_t len = e - u;
-      if (scanned + len > scan_cap) {
+      const int len = e - u;
+      if (len > cap - scanned) {
         cnt = maxn + 1;  // reported as overflow
         e = u;
+        scanned = cap + 1;
+      } else {
+        scanned += len;
       }
-      scanned += len;
       if (q + 1 < nq) next = stencil_range(sc, q + 1, start);
     }
     if (u >= e) break;
@@ -762,7 +787,7 @@ __device__ __forceinline__ void scan_stencil(
     const double d2 = sep_m<M, 2>(p.z, o.z, g);
     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     bool ok;
-    if (mode == 1) {
+    if (CRIT) {
       ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
     } else {
       const double cut = p.w + o.w + was;
@@ -909,15 +934,22 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
         }
       } else {
         const Stencil sc = make_stencil(bx, by, bz, g);
-        if (wave_img == 0)
-          scan_stencil<0>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
-                          sk, maxn, cnt);
-        else if (wave_img == 1)
-          scan_stencil<1>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
-                          sk, maxn, cnt);
+        auto run = [&](auto crit_tag) {
+          constexpr bool C = decltype(crit_tag)::value;
+          if (wave_img == 0)
+            scan_stencil<0, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                               maxn, cnt);
+          else if (wave_img == 1)
+            scan_stencil<1, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                               maxn, cnt);
+          else
+            scan_stencil<7, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                               maxn, cnt);
+        };
+        if (mode == 1)
+          run(std::true_type{});
         else
-          scan_stencil<7>(sc, g, start, scan_cap, ti, p, pos, mode, crit, was,
-                          sk, maxn, cnt);
+          run(std::false_type{});
       }
       ncnt[t] = (cnt < maxn ? cnt : maxn) | (force_img << kImgShift);
       if (cnt > maxn) atomicOr(err, ERRBIT_OVERFLOW);
@@ -1217,13 +1249,7 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
     z = wrap_axis(z, g.plo[2], g.phi[2], g.L[2], g.periodic[2]);
     pos_out[t] = make_double4(x, y, z, p.w);
     if (fc) {  // original order, so a later rebuild does not reorder it
-      const int64_t o = fc_oid[t];
-      fc[o] = a0;
-      fc[o + ncap] = a1;
-      fc[o + 2 * ncap] = a2;
-      fc[o + 3 * ncap] = 0.0;
-      fc[o + 4 * ncap] = 0.0;
-      fc[o + 5 * ncap] = 0.0;
+      store_fc(fc, fc_oid[t], a0, a1, a2, 0.0, 0.0, 0.0);
     }
     if (!(isfinite(a0) && isfinite(a1) && isfinite(a2))) atomicOr(err, ERRBIT_NAN);
     v2 = vx * vx + vy * vy + vz * vz;
@@ -1334,9 +1360,7 @@ __device__ double general_finish(int64_t t, const BoxGeom &g,
   }
   out.topo[t] = ti;
   if (fc) {  // original order
-    const int64_t o = st.oid[t];
-#pragma unroll
-    for (int k = 0; k < 6; k++) fc[o + k * ncap] = acc[k];
+    store_fc(fc, st.oid[t], acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]);
   }
   bool fin = true;
 #pragma unroll
@@ -1675,12 +1699,13 @@ __global__ void k_skin_final(unsigned long long *__restrict__ slotmax,
 
 // ------------------------------------------------------------- exports ---
 
-// fc is kept in original order with row stride ncap
+// fc from its kept layout (original order, six per particle) to the
+// reference's component-major fc[i + c * n]
 __global__ void k_export_fc(const double *__restrict__ fc, int64_t ncap,
                             int64_t n, double *__restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int c = 0; c < 6; c++) out[c * n + i] = fc[c * ncap + i];
+  for (int c = 0; c < 6; c++) out[c * n + i] = fc[i * 6 + c];
 }
 __global__ void k_export_pos(const double4 *__restrict__ pos,
                              const int32_t *__restrict__ oid, int64_t n,

@@ -1746,6 +1746,13 @@ void orc_interp(const double *x, const double *f, const orc_geom *g, int scheme,
 }
 
 double orc_pow(double x, double y) { return bmx_pow(x, y); }
+/* bmx_pow's general double-double path alone, and its cube-root path (0
+ * where that path leaves the call to the general one): the tests check that
+ * the two paths give the same bits. */
+double orc_pow_dd(double x, double y) {
+  return bmx_exp_dd(bmx_dd_mul_d(bmx_log_dd(x), y));
+}
+double orc_pow_third(double x, int k) { return bmx_pow_third(x, k); }
 
 /* ---- division and branching (split_particles) -----------------------------
  * src/des/bmx_split_particles.cpp:16-166 with checkSplit, setNewCell,

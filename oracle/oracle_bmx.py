@@ -100,6 +100,8 @@ def lib() -> C.CDLL:
             "orc_mesh_to_particle": (C.c_int, [vp, I64, P(Geom), D, D, D, C.c_double,
                                                C.c_int, C.c_int, C.c_int]),
             "orc_pow": (C.c_double, [C.c_double, C.c_double]),
+            "orc_pow_dd": (C.c_double, [C.c_double, C.c_double]),
+            "orc_pow_third": (C.c_double, [C.c_double, C.c_int]),
             "orc_interp": (None, [D, D, P(Geom), C.c_int, C.c_int, D]),
             "orc_split_particles": (I64, [vp, I64, I64, P(Geom), D, C.c_uint64,
                                           C.c_uint32, C.c_int32, C.c_int32]),
@@ -382,6 +384,17 @@ def interp(x, field, geom: Geom, scheme: int):
 
 def pow_(x: float, y: float) -> float:
     return lib().orc_pow(float(x), float(y))
+
+
+def pow_dd(x: float, y: float) -> float:
+    """bmx_pow's general double-double path alone."""
+    return lib().orc_pow_dd(float(x), float(y))
+
+
+def pow_third(x: float, k: int) -> float:
+    """bmx_pow's cube-root path for y = 1/3 (k = 1) or 2/3 (k = 2); 0 where
+    it leaves the call to the general path."""
+    return lib().orc_pow_third(float(x), int(k))
 
 
 def split_particles(recs, geom: Geom, sp, seed: int, call: int, next_id: int, cpu: int):

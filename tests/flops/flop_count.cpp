@@ -115,10 +115,15 @@ static int mesh_mode(int argc, char **argv) {
   const long long f_call = Flop::n;
   Flop::n = 0;
   Flop c = bmx_pow(Flop(2.3e-13), Flop(1.0) / Flop(3.0));
-  (void)c;
   const long long f_pow = Flop::n - 1;
+  // the general double-double path that the cube-root path falls back to
+  Flop::n = 0;
+  c = c + bmx_exp_dd(bmx_dd_mul_d(bmx_log_dd(Flop(2.3e-13)), Flop(1.0) / Flop(3.0)));
+  (void)c;
+  const long long f_pow_general = Flop::n - 2;
   std::printf("{\"particles\": %lld, \"error_bits\": %d, \"flop_mesh_to_particle\": %lld, "
-              "\"flop_pow\": %lld}\n", n, err, f_call, f_pow);
+              "\"flop_pow\": %lld, \"flop_pow_general\": %lld}\n", n, err, f_call, f_pow,
+              f_pow_general);
   std::free(p);
   std::free(X);
   std::free(vf);

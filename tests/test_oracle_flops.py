@@ -111,10 +111,14 @@ def test_mesh_to_particle_operation_counts(tmp_path):
     args += [repr(v) for v in MESH_CHEMPAR] + ["0.001", "4", "0", "0"]
     c = json.loads(subprocess.run(args, check=True, capture_output=True, text=True).stdout)
     assert c["error_bits"] == 0
-    # the correctly rounded pow of the chemistry, four calls per sphere
-    assert 300 < c["flop_pow"] < 1000
+    # the pow of the chemistry, four calls per sphere: y = 1/3 and 2/3 take
+    # the cube-root path, several times cheaper than the general one
+    assert 40 < c["flop_pow"] < 150
+    assert 300 < c["flop_pow_general"] < 1000
+    assert c["flop_pow_general"] > 4 * c["flop_pow"]
     assert c["flop_mesh_to_particle"] > 4 * c["flop_pow"] * n
     with open(os.path.join(ROOT, "profiles", "mesh_to_particle_1M_flops.json")) as f:
         prof = json.load(f)
     assert prof["fp64_flop_per_launch"] == c["flop_mesh_to_particle"]
     assert prof["flop_pow"] == c["flop_pow"]
+    assert prof["flop_pow_general"] == c["flop_pow_general"]

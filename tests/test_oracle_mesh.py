@@ -70,6 +70,30 @@ def test_pow_is_correctly_rounded_on_a_sample(oracle):
         assert abs(Decimal(m) - exact) <= Decimal(math.ulp(m)) / 2, (x, y)
 
 
+def test_pow_cube_root_path_gives_the_general_paths_bits(oracle):
+    """pow(x, 1/3) and pow(x, 2/3) go through a cube root in double-double
+    with a rounding test and fall back to the general double-double path
+    when the test cannot decide: on 300,000 arguments over the whole normal
+    range, on cell-sized volumes and on exact cubes, whatever it returns is
+    that path's value, and it falls back rarely."""
+    rng = np.random.default_rng(17)
+    wide = 2.0 ** rng.uniform(-999, 999, 100000)
+    vols = 10.0 ** rng.uniform(-14, -6, 100000) * rng.uniform(1, 2, 100000)
+    t = np.round(rng.uniform(1, 2, 100000) * 2 ** 16) / 2 ** 16 * 2.0 ** rng.integers(-300, 300, 100000)
+    cubes = t * t * t                                  # exact: 51-bit products
+    fallbacks = 0
+    for xs in (wide, vols, cubes):
+        for k, y in ((1, 1 / 3), (2, 2 / 3)):
+            for v in xs:
+                f = oracle.pow_third(v, k)
+                if f == 0.0:
+                    fallbacks += 1
+                    continue
+                assert f == oracle.pow_dd(v, y), (v, k)
+                assert oracle.pow_(v, y) == f
+    assert fallbacks < 60                              # of 600,000 calls
+
+
 def test_pow_exact_cases(oracle):
     rng = np.random.default_rng(6)
     x = 10.0 ** rng.uniform(-20, 5, 2000)
