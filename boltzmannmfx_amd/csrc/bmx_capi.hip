@@ -919,7 +919,7 @@ bmx_status bmx_ctx_create(bmx_ctx **out, int32_t device) {
     return fail(BMX_ERR_HIP, std::string("context setup: ") +
                                  hipGetErrorString(e));
   }
-  std::memset(c->h_flags, 0, 64);
+  std::memset(c->h_flags, 0, 128);
   if (hipHostGetDevicePointer((void **)&c->h_flags_dev, c->h_flags, 0) != hipSuccess) {
     (void)hipGetLastError();
     c->h_flags_dev = nullptr;  // read back by copies instead
