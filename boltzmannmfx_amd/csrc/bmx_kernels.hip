@@ -320,6 +320,7 @@ __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
 
 // blocks of the bin count that take part in a folded displacement check
 constexpr int kSkinBlocks = 64;
+constexpr int kSkinGroup = 16;  // slots reduced together
 template <bool MULTI>
 __global__ void k_bin_count(const double4 *__restrict__ pos,
                             const int32_t *__restrict__ col,
@@ -355,23 +356,49 @@ __global__ void k_bin_count(const double4 *__restrict__ pos,
   if (active) rank[s] = base + (lane - first);
   // the first kSkinBlocks blocks also reduce the displacement partials of
   // the period this rebuild ends, one share of every slot each, and clear
-  // them (block-uniform branch: the block reductions synchronise)
+  // them (block-uniform branch: the block reductions synchronise).  Up to
+  // kSkinGroup slots at a time: every load of the group is issued before
+  // any reduction, and the group's block maxima share one barrier.
   const int nsb = gridDim.x < kSkinBlocks ? (int)gridDim.x : kSkinBlocks;
   if (sk.vpart && (int)blockIdx.x < nsb) {
-    __shared__ double shd[256 / kWave];  // blocks of 256 (launch_bin_count)
+    // blocks of 256 (launch_bin_count): four waves
+    __shared__ double shd[kSkinGroup][256 / kWave];
     const int64_t chunk = (sk.stride + nsb - 1) / nsb;
     const int64_t c0 = blockIdx.x * chunk;
     const int64_t c1 = c0 + chunk < sk.stride ? c0 + chunk : sk.stride;
-    for (int sl = 0; sl < sk.nslots; sl++) {
-      double *v = sk.vpart + (int64_t)sl * sk.stride;
-      double m = 0.0;
+    const int wid = threadIdx.x / kWave;
+    for (int g = 0; g < sk.nslots; g += kSkinGroup) {
+      double m[kSkinGroup];
+#pragma unroll
+      for (int j = 0; j < kSkinGroup; j++) m[j] = 0.0;
       for (int64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-        const double x = v[i];
-        m = x > m ? x : m;
-        v[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < kSkinGroup; j++) {
+          if (g + j < sk.nslots) {
+            double *v = sk.vpart + (int64_t)(g + j) * sk.stride + i;
+            const double x = *v;
+            m[j] = x > m[j] ? x : m[j];
+            *v = 0.0;
+          }
+        }
       }
-      m = block_max(m, shd);
-      if (threadIdx.x == 0 && m > 0.0) atomicMax(&sk.slotmax[sl], dbits(m));
+#pragma unroll
+      for (int j = 0; j < kSkinGroup; j++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const double w = __shfl_xor(m[j], o, kWave);
+          m[j] = w > m[j] ? w : m[j];
+        }
+        if (lane == 0) shd[j][wid] = m[j];
+      }
+      __syncthreads();
+      const int j = threadIdx.x;
+      if (j < kSkinGroup && g + j < sk.nslots) {
+        double mm = shd[j][0];
+        for (int w = 1; w < (int)(blockDim.x / kWave); w++)
+          mm = shd[j][w] > mm ? shd[j][w] : mm;
+        if (mm > 0.0) atomicMax(&sk.slotmax[g + j], dbits(mm));
+      }
       __syncthreads();
     }
   }
