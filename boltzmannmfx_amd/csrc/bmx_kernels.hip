@@ -761,11 +761,13 @@ __device__ __forceinline__ int32_t row_get(const RowSink &sk, int k) {
 
 // Scans a particle's stencil as one sequence of candidates and appends the
 // accepted ones to its row (through the row sink).  Each lane moves on to its
-// current one ends, so a wave takes as many iterations as its longest
-// stencil rather than the sum over ranges of the longest range; the next
-// range's bounds are loaded one range ahead.  Candidates visited are capped
-// at scan_cap per particle (a range that would pass it is dropped and the
-// row reported as overflowing).  Bit a of M asks for the periodic image
+// next range as soon as its current one ends, so a wave takes as many
+// iterations as its longest stencil rather than the sum over ranges of the
+// longest range; the next range's bounds are loaded one range ahead.  An
+// iteration tests two candidates of the range (three or four spill at 64
+// VGPRs).  Candidates visited are capped at scan_cap per particle (a range
+// that would pass it is dropped and the row reported as overflowing).  Bit
+// a of M asks for the periodic image
 // test along axis a; an axis left out is one along which no candidate of
 // the wave can need an image.  CRIT: the stencil policy's test of the
 // squared distance against crit; otherwise the Verlet test against the
@@ -808,23 +810,46 @@ __device__ __forceinline__ void scan_stencil(
       if (q + 1 < nq) next = stencil_range(sc, q + 1, start);
     }
     if (u >= e) break;
+    // two candidates of the range per iteration (the second only when the
+    // range holds it), appended in order
+    const bool two = u + 1 < e;
+    const int u1 = two ? u + 1 : u;
     const double4 o = pos[u];
-    const double d0 = sep_m<M, 0>(p.x, o.x, g);
-    const double d1 = sep_m<M, 1>(p.y, o.y, g);
-    const double d2 = sep_m<M, 2>(p.z, o.z, g);
-    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    bool ok;
-    if (CRIT) {
-      ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
-    } else {
-      const double cut = p.w + o.w + was;
-      ok = r2 <= cut * cut;
+    const double4 o1 = pos[u1];
+    bool ok, ok1;
+    {
+      const double d0 = sep_m<M, 0>(p.x, o.x, g);
+      const double d1 = sep_m<M, 1>(p.y, o.y, g);
+      const double d2 = sep_m<M, 2>(p.z, o.z, g);
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (CRIT) {
+        ok = r2 <= crit;  // BMXCheckPair (src/des/bmx_pc.H:76)
+      } else {
+        const double cut = p.w + o.w + was;
+        ok = r2 <= cut * cut;
+      }
+    }
+    {
+      const double d0 = sep_m<M, 0>(p.x, o1.x, g);
+      const double d1 = sep_m<M, 1>(p.y, o1.y, g);
+      const double d2 = sep_m<M, 2>(p.z, o1.z, g);
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (CRIT) {
+        ok1 = r2 <= crit;
+      } else {
+        const double cut = p.w + o1.w + was;
+        ok1 = r2 <= cut * cut;
+      }
     }
     if (ok && u != t) {
       if (cnt < maxn) row_put(sk, cnt, u);
       cnt++;
     }
-    u++;
+    if (two && ok1 && u1 != t) {
+      if (cnt < maxn) row_put(sk, cnt, u1);
+      cnt++;
+    }
+    u += two ? 2 : 1;
   }
 }
 
