@@ -84,7 +84,7 @@ bmx_geometry Deck::finest_geometry() const {
   return g;
 }
 
-void Deck::finest_region(double lo[3], double hi[3]) const {
+std::vector<std::array<double, 6>> Deck::level_regions() const {
   const int levels = integer_or("amr.max_level", 0);
   const int ratio = integer_or("amr.ref_ratio", 2);
   const int buf = integer_or("amr.n_error_buf", 1);
@@ -103,6 +103,18 @@ void Deck::finest_region(double lo[3], double hi[3]) const {
     klo[d] = 0;
     khi[d] = nc[d] - 1;
   }
+  auto box = [&]() {
+    std::array<double, 6> r;
+    for (int d = 0; d < 3; d++) {
+      const double dx = (phi[d] - plo[d]) / (double)nc[d];
+      r[d] = plo[d] + (double)klo[d] * dx;
+      r[3 + d] = plo[d] + (double)(khi[d] + 1) * dx;
+    }
+    return r;
+  };
+  std::vector<std::array<double, 6>> out{box()};
+  // without tagging every level covers the domain: the finest is the mesh of
+  // amr.max_level over the whole domain (one level here)
   for (int l = 0; l < levels && tag; l++) {
     // tagged cells of level l, with ErrorEst's own test: the centre
     // (k + 0.5) dx (from 0, as bmx_tagging.cpp forms it) in [tlo, thi]
@@ -132,12 +144,37 @@ void Deck::finest_region(double lo[3], double hi[3]) const {
       klo[d] = std::max<int64_t>((g0 * ratio) / bf * bf, 0);
       khi[d] = std::min<int64_t>(((g1 + 1) * ratio + bf - 1) / bf * bf - 1, nc[d] - 1);
     }
+    out.push_back(box());
   }
+  return out;
+}
+
+void Deck::finest_region(double lo[3], double hi[3]) const {
+  const std::array<double, 6> r = level_regions().back();
   for (int d = 0; d < 3; d++) {
-    const double dx = (phi[d] - plo[d]) / (double)nc[d];
-    lo[d] = plo[d] + (double)klo[d] * dx;
-    hi[d] = plo[d] + (double)(khi[d] + 1) * dx;
+    lo[d] = r[d];
+    hi[d] = r[3 + d];
   }
+}
+
+bmx_geometry Deck::level_geometry(int level) const {
+  const std::vector<std::array<double, 6>> regs = level_regions();
+  if (level < 0 || level >= (int)regs.size())
+    throw Error(BMX_ERR_BAD_ARG, "no such level: " + std::to_string(level));
+  const int ratio = integer_or("amr.ref_ratio", 2);
+  bmx_geometry g;
+  for (int d = 0; d < 3; d++) {
+    const double plo = real("geometry.prob_lo", d), phi = real("geometry.prob_hi", d);
+    int64_t nc = integer("amr.n_cell", d);
+    for (int l = 0; l < level; l++) nc *= ratio;
+    const double dx = (phi - plo) / (double)nc;
+    g.prob_lo[d] = regs[level][d];
+    g.prob_hi[d] = regs[level][3 + d];
+    g.n_cell[d] = (int32_t)std::llround((g.prob_hi[d] - g.prob_lo[d]) / dx);
+    // a level wraps across a periodic face only where it spans the domain
+    g.periodic[d] = integer_or("geometry.is_periodic", 0, d) && (int64_t)g.n_cell[d] == nc;
+  }
+  return g;
 }
 
 // BMXCellInteraction::setParams (src/chemistry/bmx_cell_interaction.cpp:54-93)
@@ -212,6 +249,16 @@ static std::vector<double> fpar_of(const Deck &deck) {
 
 BMXParticleContainer::BMXParticleContainer(const Deck &deck, int device)
     : BMXParticleContainer(deck.finest_geometry(), fpar_of(deck).data(),
+                           stencil_params(deck), device) {
+  deck.exchange_params(xpar_);
+  deck.init_conc(conc_);
+  have_sp_ = deck.split_params(sp_);
+  deck.fusion_params(fp_);
+  seed_ = (uint64_t)deck.integer_or("bmx.split_seed", 0);
+}
+
+BMXParticleContainer::BMXParticleContainer(const Deck &deck, int level, int device)
+    : BMXParticleContainer(deck.level_geometry(level), fpar_of(deck).data(),
                            stencil_params(deck), device) {
   deck.exchange_params(xpar_);
   deck.init_conc(conc_);
@@ -387,6 +434,13 @@ void BMXParticleContainer::WriteAsciiFileForInit(const std::string &file) {
   const std::vector<uint8_t> &r = Records();
   bmx_layout L = bmx_default_layout();
   check(bmx_write_particles_ascii(file.c_str(), r.data(), n_, &L));
+}
+
+void BMXParticleContainer::WriteAscii(const std::string &file, const void *records,
+                                      int64_t n) {
+  bmx_layout L = bmx_default_layout();
+  const bmx_status st = bmx_write_particles_ascii(file.c_str(), records, n, &L);
+  if (st != BMX_OK) throw Error(st, bmx_last_error());
 }
 
 }  // namespace bmx

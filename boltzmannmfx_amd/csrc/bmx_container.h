@@ -18,6 +18,7 @@
 // holds one level on one device.
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <map>
 #include <stdexcept>
@@ -58,6 +59,13 @@ struct Deck {
   // generation for one tagged box; the whole domain without tagging or
   // with amr.max_level = 0.
   void finest_region(double lo[3], double hi[3]) const;
+  // Every level's box under the same model, coarsest first, each as
+  // {lo[3], hi[3]}: level 0 is the domain; the list ends at the finest
+  // level that tagging creates.
+  std::vector<std::array<double, 6>> level_regions() const;
+  // The mesh of one level over its box: the level's cells, periodic along
+  // an axis where the domain is and the box spans it.
+  bmx_geometry level_geometry(int level) const;
   void force_params(double fpar[9]) const;
   double neighborhood() const;           // DEM::neighborhood
   // chem_species.mass_transfer_A/B/C (src/chemistry/bmx_chem.cpp:100-105)
@@ -85,6 +93,9 @@ class BMXParticleContainer {
   // Container configured from a deck: the level mesh of amr.max_level, the
   // cell_force parameters and the reference's per-step stencil list.
   explicit BMXParticleContainer(const Deck &deck, int device = 0);
+  // Container for one AMR level of a deck: that level's mesh over its box
+  // (Deck::level_geometry), with the deck's parameters.
+  BMXParticleContainer(const Deck &deck, int level, int device);
   ~BMXParticleContainer();
   BMXParticleContainer(const BMXParticleContainer &) = delete;
   BMXParticleContainer &operator=(const BMXParticleContainer &) = delete;
@@ -117,6 +128,8 @@ class BMXParticleContainer {
   // n steps without copying back in between.
   void EvolveSteps(double dt, int nsteps);
   void WriteAsciiFileForInit(const std::string &file);
+  // The same format for records held by the caller (default layout).
+  static void WriteAscii(const std::string &file, const void *records, int64_t n);
   // The particle part of a BMX checkpoint (pc->Checkpoint(name, "particles"),
   // src/io/bmx_chk.cpp:156-159; pc->Restart, src/io/bmx_restart.cpp:104-107):
   // dir is the "particles" directory, written at this container's level.
@@ -124,6 +137,17 @@ class BMXParticleContainer {
   void Restart(const std::string &dir);
 
   int64_t NumberOfParticles() const { return n_; }
+  // NextID: the id the next new particle gets (one past the largest id
+  // bound, advanced by splits and fusions); containers of the levels of one
+  // run share one sequence through these.
+  int64_t NextID() const { return next_id_; }
+  void SetNextID(int64_t id) { next_id_ = id; }
+  // The call numbers of the next split_particles and Fusion draws (each
+  // counts its own calls from 0); the levels of one run number them by step.
+  void SetDrawCalls(uint32_t split_call, uint32_t fusion_call) {
+    split_calls_ = split_call;
+    fusion_calls_ = fusion_call;
+  }
   // Host copy of the records, brought up to date.
   const std::vector<uint8_t> &Records();
   // Particles whose position lies outside [lo, hi).

@@ -37,9 +37,14 @@ def region(bmx, tmp_path_factory):
                     "-L", libdir, "-l:" + os.path.basename(bmx.LIB_PATH),
                     "-Wl,-rpath," + libdir], check=True)
 
-    def run(deck):
-        out = subprocess.run([exe, os.path.join(GOLDEN, "ref_decks", deck)],
-                             check=True, capture_output=True, text=True).stdout.split()
+    def run(deck, levels=False):
+        args = [exe, os.path.join(GOLDEN, "ref_decks", deck)] + (["levels"] if levels else [])
+        out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+        if levels:
+            rows = [line.split() for line in out.strip().splitlines()]
+            return [([float(v) for v in r[:6]], [int(v) for v in r[6:9]],
+                     [int(v) for v in r[9:]]) for r in rows]
+        out = out.split()
         return [float(v) for v in out[:6]], [int(v) for v in out[6:]]
     return run
 
@@ -54,3 +59,32 @@ def test_nlev_deck_region(region):
     box, n_cell = region("nlev_large_test/input_real_nlev")
     assert n_cell == [64, 64, 32]
     assert box == pytest.approx([0.0, 0.0, 0.0036, 0.0128, 0.0128, 0.006], abs=1e-15)
+
+
+def test_nlev_deck_levels(region):
+    """The two levels the driver runs particles on when some lie outside the
+    refined slab: level 0 is the domain on amr.n_cell (periodic in x and y);
+    level 1 is the slab on the refined cells, periodic in x and y because it
+    spans the domain there, not in z."""
+    levels = region("nlev_large_test/input_real_nlev", levels=True)
+    assert len(levels) == 2
+    box0, n0, per0 = levels[0]
+    assert box0 == pytest.approx([0.0, 0.0, 0.0, 0.0128, 0.0128, 0.0064], abs=1e-15)
+    assert n0 == [32, 32, 16] and per0 == [1, 1, 0]
+    box1, n1, per1 = levels[1]
+    assert box1 == pytest.approx([0.0, 0.0, 0.0036, 0.0128, 0.0128, 0.006], abs=1e-15)
+    # (0.006 - 0.0036) / 2e-4 = 12 refined cells in z
+    assert n1 == [64, 64, 12] and per1 == [1, 1, 0]
+    # the last level is the finest region
+    fin, _ = region("nlev_large_test/input_real_nlev")
+    assert fin == pytest.approx(box1, abs=1e-15)
+
+
+def test_fungi_deck_levels(region):
+    levels = region("fungi/input_fungi", levels=True)
+    fin, n_fin = region("fungi/input_fungi")
+    assert levels[0][0][:3] == [0.0, 0.0, 0.0]
+    assert levels[-1][0] == pytest.approx(fin, abs=1e-15)
+    # the finest level's cells are those of the finest mesh
+    dz = (fin[5] - fin[2]) / levels[-1][1][2]
+    assert dz == pytest.approx(levels[0][0][5] / n_fin[2], rel=1e-12)

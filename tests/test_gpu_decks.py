@@ -8,7 +8,8 @@ EvolveParticles, split_particles, ParticleExchange and the fusion steps each
 step, and writes the particles
 back in the input format (which holds no species: those are compared by
 test_gpu_exchange.py).  The same run on the oracle must give the same
-bits.
+bits.  A deck whose particles lie on two AMR levels runs level by level,
+each level on its own mesh over its box; the oracle repeats that placement.
 """
 import math
 import os
@@ -36,21 +37,12 @@ def parse_deck(path):
     return kv
 
 
-@pytest.mark.parametrize("deck,steps", [("fungi/input_fungi", 40),
-                                        ("wall_test/input_fungi", 40),
-                                        ("nlev_large_test/input_real_nlev", 40)])
-def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, steps):
-    path = os.path.join(DECKS, deck)
-    out = str(tmp_path / "out.dat")
-    res = subprocess.run([RUN, path, str(steps), out], capture_output=True, text=True,
-                         timeout=120)
-    assert res.returncode == 0, res.stderr
-    assert res.stdout.count("Time per step") == steps
-    # the deck's particles stay inside its refined region (the driver warns
-    # otherwise: the reference would move them to a coarser level)
-    assert "Warning" not in res.stdout
-
+def deck_params(bmx, path, particles=None):
+    """The values bmx_run takes from a deck, as the oracle needs them, and
+    its particles (the deck's file, or `particles`) with the species at
+    fluid.init_conc_species."""
     kv = parse_deck(path)
+
     def f(k, i=0, default=None):
         if k not in kv and default is not None:
             return default
@@ -68,8 +60,8 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
             f("cell_force.viscous_drag")]
     vol = f("chem_species.max_vol", default=0.0)
     hood = 1.1 * (2 * (3 * vol / (4 * math.pi)) ** (1 / 3) + f("cell_force.neighbor_width"))
-    recs = bmx.read_particles_ascii(os.path.join(os.path.dirname(path),
-                                                 kv["particles.input_file"][0]))
+    recs = bmx.read_particles_ascii(particles or os.path.join(
+        os.path.dirname(path), kv["particles.input_file"][0]))
     # species start at fluid.init_conc_species (bmx_pc_init.cpp:87-91)
     conc = [float(v) for v in kv.get("fluid.init_conc_species", [])][:3]
     recs["rdata"][:, 25:25 + len(conc)] = conc
@@ -86,6 +78,27 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
            f("chem_species.max_fusion_separation", default=5.0e-4),
            f("chem_species.bond_scaling_increment", default=0.001),
            f("chem_species.max_seg_length", default=0.0035)]
+    return {"kv": kv, "f": f, "lo": lo, "hi": hi, "per": per, "n_cell": n_cell, "fpar": fpar,
+            "hood": hood, "xpar": xpar, "sp": sp, "fus": fus, "recs": recs}
+
+
+@pytest.mark.parametrize("deck,steps", [("fungi/input_fungi", 40),
+                                        ("wall_test/input_fungi", 40),
+                                        ("nlev_large_test/input_real_nlev", 40)])
+def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, steps):
+    path = os.path.join(DECKS, deck)
+    out = str(tmp_path / "out.dat")
+    res = subprocess.run([RUN, path, str(steps), out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.count("Time per step") == steps
+    # the deck's particles stay inside its refined region: one level
+    assert "Levels:" not in res.stdout
+
+    P = deck_params(bmx, path)
+    lo, hi, per, n_cell = P["lo"], P["hi"], P["per"], P["n_cell"]
+    fpar, hood, xpar, sp, fus, recs = P["fpar"], P["hood"], P["xpar"], P["sp"], P["fus"], P["recs"]
+    f = P["f"]
     g = oracle.make_geom(lo, hi, per, n_cell)
     st = oracle.State(recs, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
     st.init_bonds()
@@ -107,3 +120,107 @@ def test_reference_deck_runs_match_oracle(bmx, oracle, gpu, tmp_path, deck, step
     assert len(got) == len(recs)
     assert np.array_equal(got["pos"], recs["pos"])
     assert np.array_equal(got["rdata"][:, :23], recs["rdata"][:, :23])
+
+
+def test_particles_on_two_levels_run_level_by_level(bmx, oracle, gpu, tmp_path):
+    """The nlev deck with 18 bacteria on both of its levels: rows of
+    overlapping cells in the refined slab (level 1, z in [0.0036, 0.006)) and
+    below it (level 0), a pair across the periodic x face on each level, and
+    a pair 1.2e-4 apart across the slab's lower face, one on each level, in
+    the attractive band of the cubic law, which interacts on neither level
+    (each level's list holds that level's particles,
+    src/des/bmx_pc_interaction.cpp:22-28).  bmx_run places every particle on
+    the finest level whose box holds it, runs each level on its own mesh and
+    places them again after each step; the oracle repeats the placement, on
+    the level meshes pinned by test_deck_region.py, and must give the same
+    bits."""
+    src = os.path.join(DECKS, "nlev_large_test", "input_real_nlev")
+    seed = bmx.read_particles_ascii(os.path.join(DECKS, "nlev_large_test",
+                                                 "abc_real_input.dat"))
+    pos = []
+    for k in range(6):
+        pos.append((0.0060 + 1.2e-4 * k, 0.0064, 0.00482))   # level 1 row
+    pos += [(0.0128 - 0.6e-4, 0.0030, 0.0050), (0.6e-4, 0.0030, 0.0050)]
+    for k in range(6):
+        pos.append((0.0060 + 1.2e-4 * k, 0.0064, 0.0020))    # level 0 row
+    pos += [(0.0128 - 0.6e-4, 0.0090, 0.0015), (0.6e-4, 0.0090, 0.0015)]
+    pos += [(0.0030, 0.0030, 0.00366), (0.0030, 0.0030, 0.00354)]  # across the face
+    recs = np.repeat(seed, len(pos))
+    recs["pos"] = np.array(pos)
+    parts = str(tmp_path / "two_levels.dat")
+    bmx.write_particles_ascii(parts, recs)
+    text = open(src).read().replace('"abc_real_input.dat"', '"two_levels.dat"')
+    deck = str(tmp_path / "input_two_levels")
+    open(deck, "w").write(text)
+    steps = 4
+    out = str(tmp_path / "out.dat")
+    res = subprocess.run([RUN, deck, str(steps), out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "Levels: 2" in res.stdout
+    assert res.stdout.count("Time per step") == steps
+
+    P = deck_params(bmx, deck, particles=parts)
+    fpar, hood, xpar, sp, fus = P["fpar"], P["hood"], P["xpar"], P["sp"], P["fus"]
+    dt = P["f"]("bmx.fixed_dt")
+    # the level meshes (test_deck_region.py::test_nlev_deck_levels)
+    geoms = [oracle.make_geom((0, 0, 0), (0.0128, 0.0128, 0.0064), (1, 1, 0), (32, 32, 16)),
+             oracle.make_geom((0, 0, 0.0036), (0.0128, 0.0128, 0.006), (1, 1, 0), (64, 64, 12))]
+    slab = ((0.0, 0.0, 0.0036), (0.0128, 0.0128, 0.006))
+
+    def place(r):
+        r = r[np.lexsort((r["id"], r["cpu"]))]
+        p = r["pos"]
+        inside = np.all((p >= slab[0]) & (p < slab[1]), axis=1)
+        return [r[~inside].copy(), r[inside].copy()]
+
+    def cat(arrays):
+        # np.concatenate would repack the padded record dtype
+        out = np.empty(sum(len(a) for a in arrays), dtype=P["recs"].dtype)
+        k = 0
+        for a in arrays:
+            out[k:k + len(a)] = a
+            k += len(a)
+        return out
+
+    levels = place(P["recs"])
+    assert [len(v) for v in levels] == [9, 9]
+    next_id = int(P["recs"]["id"].max()) + 1
+    for step in range(steps):
+        done = []
+        for lv, r in enumerate(levels):
+            if len(r) == 0:
+                continue
+            g = geoms[lv]
+            st = oracle.State(r, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
+            if step == 0:
+                st.init_bonds()
+            st.evolve(dt, 1)
+            grown = oracle.split_particles(r, g, sp, 0, step, next_id, 0)
+            if len(grown) > len(r):
+                st.close()
+                next_id += len(grown) - len(r)
+                r = grown
+                st = oracle.State(r, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
+            st.exchange(xpar, dt)
+            st.close()
+            _, fused = oracle.fusion(r, g, fpar, oracle.NBR_STENCIL, hood, fus,
+                                     0 ^ 0x66757369, step, next_id, 0)
+            assert not fused
+            done.append(r)
+        levels = place(cat(done))
+    want = cat(levels)
+    want = want[np.lexsort((want["id"], want["cpu"]))]
+    got = bmx.read_particles_ascii(out)
+    assert len(got) == len(want)
+    assert np.array_equal(got["pos"], want["pos"])
+    assert np.array_equal(got["rdata"][:, :23], want["rdata"][:, :23])
+    # kept on the finest mesh over the whole domain, the pair across the slab
+    # face interacts and the result differs: the placement is what is tested
+    one = P["recs"].copy()
+    g = oracle.make_geom(P["lo"], P["hi"], P["per"], P["n_cell"])
+    st = oracle.State(one, g, np.array(fpar), oracle.NBR_STENCIL, hood, 1)
+    st.init_bonds()
+    st.evolve(dt, steps)
+    st.close()
+    assert not np.array_equal(one["pos"], want["pos"])
