@@ -764,15 +764,16 @@ __device__ __forceinline__ int32_t row_get(const RowSink &sk, int k) {
 // next range as soon as its current one ends, so a wave takes as many
 // iterations as its longest stencil rather than the sum over ranges of the
 // longest range; the next range's bounds are loaded one range ahead.  An
-// iteration tests two candidates of the range (three or four spill at 64
-// VGPRs).  Candidates visited are capped at scan_cap per particle (a range
+// iteration tests NC candidates of the range: two for one box (three or
+// four spill at 64 VGPRs), one for several boxes (two spill there).
+// Candidates visited are capped at scan_cap per particle (a range
 // that would pass it is dropped and the row reported as overflowing).  Bit
 // a of M asks for the periodic image
 // test along axis a; an axis left out is one along which no candidate of
 // the wave can need an image.  CRIT: the stencil policy's test of the
 // squared distance against crit; otherwise the Verlet test against the
 // reaches plus was = W_A + skin (a sphere's reach is its radius).
-template <int M, bool CRIT>
+template <int M, bool CRIT, int NC>
 __device__ __forceinline__ void scan_stencil(
     const Stencil sc, const BoxGeom &g, const int32_t *__restrict__ start,
     int64_t scan_cap, int t, const double4 &p,
@@ -810,13 +811,12 @@ __device__ __forceinline__ void scan_stencil(
       if (q + 1 < nq) next = stencil_range(sc, q + 1, start);
     }
     if (u >= e) break;
-    // two candidates of the range per iteration (the second only when the
-    // range holds it), appended in order
-    const bool two = u + 1 < e;
+    // NC = 2: two candidates of the range per iteration (the second only
+    // when the range holds it), appended in order
+    const bool two = NC == 2 && u + 1 < e;
     const int u1 = two ? u + 1 : u;
     const double4 o = pos[u];
-    const double4 o1 = pos[u1];
-    bool ok, ok1;
+    bool ok, ok1 = false;
     {
       const double d0 = sep_m<M, 0>(p.x, o.x, g);
       const double d1 = sep_m<M, 1>(p.y, o.y, g);
@@ -829,7 +829,8 @@ __device__ __forceinline__ void scan_stencil(
         ok = r2 <= cut * cut;
       }
     }
-    {
+    if (NC == 2) {
+      const double4 o1 = pos[u1];
       const double d0 = sep_m<M, 0>(p.x, o1.x, g);
       const double d1 = sep_m<M, 1>(p.y, o1.y, g);
       const double d2 = sep_m<M, 2>(p.z, o1.z, g);
@@ -988,15 +989,16 @@ __global__ __launch_bounds__(512, 8) void k_build_nbr(
         const Stencil sc = make_stencil(bx, by, bz, g);
         auto run = [&](auto crit_tag) {
           constexpr bool C = decltype(crit_tag)::value;
+          constexpr int NC = MULTI ? 1 : 2;
           if (wave_img == 0)
-            scan_stencil<0, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
-                               maxn, cnt);
+            scan_stencil<0, C, NC>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                                   maxn, cnt);
           else if (wave_img == 1)
-            scan_stencil<1, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
-                               maxn, cnt);
+            scan_stencil<1, C, NC>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                                   maxn, cnt);
           else
-            scan_stencil<7, C>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
-                               maxn, cnt);
+            scan_stencil<7, C, NC>(sc, g, start, scan_cap, ti, p, pos, crit, was, sk,
+                                   maxn, cnt);
         };
         if (mode == 1)
           run(std::true_type{});
