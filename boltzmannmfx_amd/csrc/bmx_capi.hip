@@ -556,7 +556,7 @@ bmx_status enqueue_step(bmx_ctx *c, double dt, bool write_fc) {
   if (!c->any_fungi) {
     HIPCHK(launch_force_sphere_flat(
         c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
-        c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n, c->maxn,
+        c->d_wtot.p, c->d_ncnt.p, c->n, c->maxn,
         multi ? c->col[c->cs].p : nullptr, c->d_boxes.p, c->boxes[0], multi,
         fp, fc, c->oid[c->cs].p, vm, c->d_err.p, shape(c), s));
   } else {
@@ -583,7 +583,7 @@ bmx_status enqueue_force_only(bmx_ctx *c, double dt, int slot) {
                    : nullptr;
   HIPCHK(launch_force_sphere_flat(
       c->pos[c->cd].p, c->pos[1 - c->cd].p, c->d_cidx.p, c->d_cown.p,
-      c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n, c->maxn, nullptr, c->d_boxes.p,
+      c->d_wtot.p, c->d_ncnt.p, c->n, c->maxn, nullptr, c->d_boxes.p,
       c->boxes[0], 0, fp, nullptr, nullptr, vm, c->d_err.p, shape(c),
       c->stream));
   c->cd = 1 - c->cd;
@@ -715,9 +715,8 @@ bmx_status drop_chem(bmx_ctx *c) {
 
 bmx_status pack_records(bmx_ctx *c) {
   HIPCHK(launch_aos_pack(c->recs, c->lay, c->n, static_arrays(c, c->cs),
-                         dyn_arrays(c, c->cd), c->d_fc.p, c->ncap,
-                         c->mobility, c->wrote_step ? 1 : 0,
-                         c->any_fungi ? 1 : 0, c->stream));
+                         dyn_arrays(c, c->cd), c->d_fc.p, c->mobility,
+                         c->wrote_step ? 1 : 0, c->any_fungi ? 1 : 0, c->stream));
   if (c->have_chem)
     HIPCHK(launch_chem_pack(c->recs, c->lay, c->n, chem_rel(c),
                             c->d_chemst[c->cc].p, c->oid[c->cs].p,
@@ -1794,7 +1793,7 @@ bmx_status bmx_adjoint(bmx_ctx *c, double dt, int32_t nsteps,
       if (t + 1 < len)  // the last state of the segment is not needed
         HIPCHK(launch_force_sphere_flat(
             H.pos.p + (size_t)t * cap, H.pos.p + (size_t)(t + 1) * cap,
-            c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_ncnt.p, c->ncap, c->n,
+            c->d_cidx.p, c->d_cown.p, c->d_wtot.p, c->d_ncnt.p, c->n,
             c->maxn, nullptr, c->d_boxes.p, c->boxes[0], 0, fp, nullptr,
             nullptr, nullptr, c->d_err.p, shape(c), s));
     }
@@ -1866,7 +1865,7 @@ bmx_status bmx_get_fc(bmx_ctx *c, double *out) {
   (void)hipSetDevice(c->device);
   DevBuf<double> tmp;
   HIPCHK(tmp.alloc(6 * (size_t)std::max<int64_t>(c->n, 1)));
-  HIPCHK(launch_export_fc(c->d_fc.p, c->ncap, c->n, tmp.p, c->stream));
+  HIPCHK(launch_export_fc(c->d_fc.p, c->n, tmp.p, c->stream));
   if (c->n)
     HIPCHK(hipMemcpyAsync(out, tmp.p, sizeof(double) * 6 * c->n,
                           hipMemcpyDeviceToHost, c->stream));

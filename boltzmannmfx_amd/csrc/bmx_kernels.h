@@ -50,8 +50,8 @@ hipError_t launch_aos_refresh(const uint8_t *aos, AosLayout lay, int64_t n,
 // sorted SoA -> AoS at original positions (fields the force pass writes)
 hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy, const double *fc,
-                           int64_t ncap, double mob, int wrote_step,
-                           int any_fungi, hipStream_t s);
+                           double mob, int wrote_step, int any_fungi,
+                           hipStream_t s);
 
 // The displacement check of a Verlet period folded into the first two
 // binning kernels of the rebuild that ends it: the first blocks of the bin
@@ -117,7 +117,7 @@ int flat_tile(int segments);
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
                                     const int32_t *wtot, const int32_t *ncnt,
-                                    int64_t ncap, int64_t n, int32_t maxn,
+                                    int64_t n, int32_t maxn,
                                     const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,
@@ -163,8 +163,8 @@ hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
                              hipStream_t s);
 
 // ---- exports to original order
-hipError_t launch_export_fc(const double *fc, int64_t ncap, int64_t n,
-                            double *out, hipStream_t s);
+hipError_t launch_export_fc(const double *fc, int64_t n, double *out,
+                            hipStream_t s);
 hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,
                              double *out, hipStream_t s);
 hipError_t launch_export_orient(const double4 *rod, const int32_t *oid,

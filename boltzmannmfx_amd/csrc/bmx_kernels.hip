@@ -39,20 +39,6 @@ __device__ __forceinline__ int64_t tile_of(int remap) {
   return x * per + (x < extra ? x : extra) + r;
 }
 
-__device__ __forceinline__ int wave_max_i32(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    int w = __shfl_xor(v, o, kWave);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-__device__ __forceinline__ long long wave_sum_i64(long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
-}
-
 // Block-wide max / sum through LDS (sh: one slot per wave); every thread
 // must call it; the result is valid in thread 0.
 template <typename T>
@@ -273,8 +259,8 @@ __device__ __forceinline__ void store_fc(double *__restrict__ fc, int64_t o,
 
 __global__ void k_aos_pack(uint8_t *__restrict__ aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy,
-                           const double *__restrict__ fc, int64_t ncap,
-                           double mob, int wrote_step, int any_fungi) {
+                           const double *__restrict__ fc, double mob,
+                           int wrote_step, int any_fungi) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int64_t o = st.oid[t];
@@ -1209,7 +1195,7 @@ __global__ __launch_bounds__(512, 6) void k_force_sphere_flat(
     const double4 *__restrict__ pos_in, double4 *__restrict__ pos_out,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
     const int32_t *__restrict__ wtot, const int32_t *__restrict__ ncnt,
-    int64_t ncap, int64_t n, int32_t maxn, const int32_t *__restrict__ col,
+    int64_t n, int32_t maxn, const int32_t *__restrict__ col,
     const BoxGeom *__restrict__ boxes, BoxGeom g0, ForceParams fp,
     double *__restrict__ fc, const int32_t *__restrict__ fc_oid,
     double *__restrict__ vmax, int32_t *__restrict__ err, int remap) {
@@ -1370,7 +1356,7 @@ __device__ double general_finish(int64_t t, const BoxGeom &g,
                                  const DynArrays &in, const DynArrays &out,
                                  const StaticArrays &st, const ForceParams &fp,
                                  double acc[6], Topo &ti,
-                                 double *__restrict__ fc, int64_t ncap,
+                                 double *__restrict__ fc,
                                  int32_t *__restrict__ err) {
   const int ty = st.type[t];
   const double4 p = in.pos[t];
@@ -1477,7 +1463,7 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
     DynArrays in, DynArrays out, StaticArrays st,
     const int32_t *__restrict__ cidx, const uint8_t *__restrict__ cown,
     const int32_t *__restrict__ wtot, int8_t *__restrict__ redo_tile,
-    const int32_t *__restrict__ ncnt, int64_t ncap, int64_t n, int32_t maxn,
+    const int32_t *__restrict__ ncnt, int64_t n, int32_t maxn,
     const BoxGeom *__restrict__ boxes, BoxGeom g0,
     ForceParams fp, double *__restrict__ fc, double *__restrict__ vmax,
     int32_t *__restrict__ err, int remap) {
@@ -1615,7 +1601,7 @@ __global__ __launch_bounds__(256, 4) void k_force_general_flat(
       ti.dltd = 1;  // K.H:364-369
       if (promoted) ti.position = rec::INTERIOR;
     }
-    v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
+    v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, err);
   }
   if (vmax) block_vmax(v2, fp.dt, vmax);
 }
@@ -1638,7 +1624,7 @@ __global__ __launch_bounds__(256) void k_force_general_redo(
   Topo ti = in.topo[t];
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   general_rows<MULTI>(t, g, in, st, nbr, ncnt, ncap, fp, acc, ti);
-  const double v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, ncap, err);
+  const double v2 = general_finish(t, g, in, out, st, fp, acc, ti, fc, err);
   if (vmax)
     atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(fp.dt * sqrt(v2)));
 }
@@ -1721,7 +1707,7 @@ __global__ void k_init_bonds(DynArrays dy, StaticArrays st,
 // Block (x, y) takes slot y over a chunk of kSkinChunk force blocks.
 constexpr int kSkinChunk = 4096;
 __global__ __launch_bounds__(256) void k_skin_partial(
-    double *__restrict__ vpart, int32_t nslots, int32_t stride, int clear,
+    double *__restrict__ vpart, int32_t stride, int clear,
     unsigned long long *__restrict__ slotmax) {
   __shared__ double shd[4];
   const int sl = blockIdx.y;
@@ -1755,8 +1741,8 @@ __global__ void k_skin_final(unsigned long long *__restrict__ slotmax,
 
 // fc from its kept layout (original order, six per particle) to the
 // reference's component-major fc[i + c * n]
-__global__ void k_export_fc(const double *__restrict__ fc, int64_t ncap,
-                            int64_t n, double *__restrict__ out) {
+__global__ void k_export_fc(const double *__restrict__ fc, int64_t n,
+                            double *__restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int c = 0; c < 6; c++) out[c * n + i] = fc[i * 6 + c];
@@ -1852,11 +1838,11 @@ hipError_t launch_aos_refresh(const uint8_t *aos, AosLayout lay, int64_t n,
 
 hipError_t launch_aos_pack(uint8_t *aos, AosLayout lay, int64_t n,
                            StaticArrays st, DynArrays dy, const double *fc,
-                           int64_t ncap, double mob, int wrote_step,
-                           int any_fungi, hipStream_t s) {
+                           double mob, int wrote_step, int any_fungi,
+                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_aos_pack, dim3(grid_for(n, 256)), dim3(256), 0, s, aos,
-                     lay, n, st, dy, fc, ncap, mob, wrote_step, any_fungi);
+                     lay, n, st, dy, fc, mob, wrote_step, any_fungi);
   return hipGetLastError();
 }
 
@@ -2004,7 +1990,7 @@ size_t flat_lds_bytes(int block, int multi) {
 hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
                                     const int32_t *cidx, const uint8_t *cown,
                                     const int32_t *wtot, const int32_t *ncnt,
-                                    int64_t ncap, int64_t n, int32_t maxn,
+                                    int64_t n, int32_t maxn,
                                     const int32_t *col,
                                     const BoxGeom *boxes, BoxGeom g0, int multi,
                                     ForceParams fp, double *fc,
@@ -2016,14 +2002,12 @@ hipError_t launch_force_sphere_flat(const double4 *pos_in, double4 *pos_out,
   const size_t lds = flat_lds_bytes(b, multi);
   if (multi)
     hipLaunchKernelGGL(k_force_sphere_flat<true>, dim3(grid_for(n, b)), dim3(b),
-                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, ncap, n,
-                       maxn, col, boxes, g0, fp, fc, fc_oid, vmax, err,
-                       ls.remap);
+                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, n, maxn,
+                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
   else
     hipLaunchKernelGGL(k_force_sphere_flat<false>, dim3(grid_for(n, b)), dim3(b),
-                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, ncap, n,
-                       maxn, col, boxes, g0, fp, fc, fc_oid, vmax, err,
-                       ls.remap);
+                       lds, s, pos_in, pos_out, cidx, cown, wtot, ncnt, n, maxn,
+                       col, boxes, g0, fp, fc, fc_oid, vmax, err, ls.remap);
   return hipGetLastError();
 }
 
@@ -2042,15 +2026,15 @@ hipError_t launch_force_general(DynArrays in, DynArrays out, StaticArrays st,
   const size_t lds = (size_t)(b / kWave) * kGenWaveBytes;
   if (multi) {
     hipLaunchKernelGGL(k_force_general_flat<true>, dim3(grid), dim3(b), lds, s,
-                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, ncap, n,
-                       maxn, boxes, g0, fp, fc, vmax, err, ls.remap);
+                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, n, maxn,
+                       boxes, g0, fp, fc, vmax, err, ls.remap);
     hipLaunchKernelGGL(k_force_general_redo<true>, dim3(grid_for(n, 256)),
                        dim3(256), 0, s, in, out, st, redo_tile, nbr, ncnt,
                        ncap, n, boxes, g0, fp, fc, vmax, err);
   } else {
     hipLaunchKernelGGL(k_force_general_flat<false>, dim3(grid), dim3(b), lds, s,
-                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, ncap, n,
-                       maxn, boxes, g0, fp, fc, vmax, err, ls.remap);
+                       in, out, st, cidx, cown, wtot, redo_tile, ncnt, n, maxn,
+                       boxes, g0, fp, fc, vmax, err, ls.remap);
     hipLaunchKernelGGL(k_force_general_redo<false>, dim3(grid_for(n, 256)),
                        dim3(256), 0, s, in, out, st, redo_tile, nbr, ncnt,
                        ncap, n, boxes, g0, fp, fc, vmax, err);
@@ -2085,17 +2069,17 @@ hipError_t launch_skin_check(double *vpart, int32_t nslots, int32_t stride,
   int chunks = (stride + kSkinChunk - 1) / kSkinChunk;
   if (chunks < 1) chunks = 1;
   hipLaunchKernelGGL(k_skin_partial, dim3(chunks, nslots > 0 ? nslots : 1),
-                     dim3(256), 0, s, vpart, nslots, stride, clear, slotmax);
+                     dim3(256), 0, s, vpart, stride, clear, slotmax);
   hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(64), 0, s, slotmax, nslots,
                      limit, err);
   return hipGetLastError();
 }
 
-hipError_t launch_export_fc(const double *fc, int64_t ncap, int64_t n,
+hipError_t launch_export_fc(const double *fc, int64_t n,
                             double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_export_fc, dim3(grid_for(n, 256)), dim3(256), 0, s, fc,
-                     ncap, n, out);
+  hipLaunchKernelGGL(k_export_fc, dim3(grid_for(n, 256)), dim3(256), 0, s, fc, n,
+                     out);
   return hipGetLastError();
 }
 hipError_t launch_export_pos(const double4 *pos, const int32_t *oid, int64_t n,
