@@ -251,11 +251,13 @@ def test_errors_are_reported(bmx, gpu, c1):
     assert ei.value.code == 1
 
 
-def test_one_million_spheres_10_steps_bit_exact(bmx, oracle, gpu):
+@pytest.mark.parametrize("block", [256, 64])
+def test_one_million_spheres_10_steps_bit_exact(bmx, oracle, gpu, block):
     """The 1,000,000-sphere workload at volume fraction 0.3: 10 steps (one
-    Verlet rebuild) on the GPU and on the CPU oracle."""
+    Verlet rebuild) on the GPU and on the CPU oracle, with 256-thread blocks
+    and with the one-wave blocks the benchmark launches."""
     col = W.c2_large()
-    ctx, _ = gpu_ctx(bmx, col)
+    ctx, _ = gpu_ctx(bmx, col, block=block)
     ctx.evolve(W.DT, 10)
     st, orecs = oracle_state(oracle, col, ctx.bin_dims())
     fo = st.evolve(W.DT, 10)
