@@ -43,9 +43,9 @@ def algorithmic_bytes_per_particle(mean_row: float) -> float:
     return 32.0 + 4.0 + 4.0 * mean_row + 32.0
 
 
-def survey_bytes_per_particle_step(mean_row: float, K: int) -> float:
+def spec_bytes_per_particle_step(mean_row: float, K: int) -> float:
     """Compulsory bytes per particle-step of a sphere step with rebuilds every
-    K steps, as the workload's accounting counts them: step 84 + 4(L+1)
+    K steps, as the workload's specification counts them: step 84 + 4(L+1)
     (x, y, z, r and type read; x, y, z and velocity written; row offset and
     L indices read), rebuild (112 + 4(L+1)) / K."""
     return 84.0 + 4.0 * (mean_row + 1.0) + (112.0 + 4.0 * (mean_row + 1.0)) / K
@@ -726,7 +726,7 @@ def main() -> int:
             "device_ms_per_step": (tm.force_ms + tm.rebuild_ms) / ev_steps,
             "algorithmic_bytes_per_particle": algorithmic_bytes_per_particle(mean_row),
             "algorithmic_bytes_per_launch": alg_bytes,
-            "survey_bytes_per_particle_step": survey_bytes_per_particle_step(mean_row, K),
+            "spec_bytes_per_particle_step": spec_bytes_per_particle_step(mean_row, K),
             "copy_probe": probe,
         },
     }
