@@ -14,6 +14,22 @@ namespace bmx {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Word-wide maxima and flags that every block of a launch updates at its
+// end.  The word is read first and the atomic taken only when it would
+// change it, so the blocks do not queue on one address (most blocks find
+// the maximum already there).  A stale read only costs an atomic.
+__device__ __forceinline__ void max_u64_to(unsigned long long *w,
+                                           unsigned long long v) {
+  if (v > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(w, v);
+}
+__device__ __forceinline__ void or_u64_to(unsigned long long *w,
+                                          unsigned long long bits) {
+  if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bits) !=
+      bits)
+    atomicOr(w, bits);
+}
+
 // Image of coordinate xj nearest to xi along one axis, then xi - image
 // (the separation the reference forms against a periodic ghost copy,
 // src/des/bmx_pc_interaction.cpp:191-193).

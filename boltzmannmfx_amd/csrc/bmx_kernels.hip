@@ -153,10 +153,10 @@ __global__ void k_aos_unpack(const uint8_t *__restrict__ aos, AosLayout lay,
   __syncthreads();
   unsigned long long ff = block_max(flags & 2ull, shu);
   if (threadIdx.x == 0) {
-    if (rmax > 0.0) atomicMax(&stats[0], dbits(rmax));
-    if (lmax > 0.0) atomicMax(&stats[1], dbits(lmax));
-    if (fy) atomicOr(&stats[2], 1ull);
-    if (ff) atomicOr(&stats[3], 1ull);
+    if (rmax > 0.0) max_u64_to(&stats[0], dbits(rmax));
+    if (lmax > 0.0) max_u64_to(&stats[1], dbits(lmax));
+    if (fy) or_u64_to(&stats[2], 1ull);
+    if (ff) or_u64_to(&stats[3], 1ull);
     if (nbad) atomicAdd(&stats[4], nbad);
   }
 }
@@ -238,8 +238,8 @@ __global__ void k_aos_refresh(const uint8_t *__restrict__ aos, AosLayout lay,
   __syncthreads();
   nbad = block_sum(nbad, shu);
   if (threadIdx.x == 0) {
-    if (rmax > 0.0) atomicMax(&stats[0], dbits(rmax));
-    if (lmax > 0.0) atomicMax(&stats[1], dbits(lmax));
+    if (rmax > 0.0) max_u64_to(&stats[0], dbits(rmax));
+    if (lmax > 0.0) max_u64_to(&stats[1], dbits(lmax));
     if (nbad) atomicAdd(&stats[4], nbad);
   }
 }

@@ -16,6 +16,7 @@
 // thread per mesh cell sums the contributions of its source bins in that
 // order.  Every cell therefore gets the same sum, in the same order, as a
 // serial loop over the particles -- no floating-point atomics.
+#include "bmx_device.h"
 #include "bmx_kernels.h"
 #include "bmx_math.h"
 
@@ -589,9 +590,9 @@ __global__ __launch_bounds__(256) void k_mesh_to_particle(
   rmax = block_max_nonneg(rmax, sh[0]);
   lmax = block_max_nonneg(lmax, sh[1]);
   if (e) atomicOr(err, e);
-  if (threadIdx.x == 0) {
-    atomicMax(&stats[0], (unsigned long long)__double_as_longlong(rmax));
-    atomicMax(&stats[1], (unsigned long long)__double_as_longlong(lmax));
+  if (threadIdx.x == 0) {  // non-negative doubles order as their bits
+    max_u64_to(&stats[0], (unsigned long long)__double_as_longlong(rmax));
+    max_u64_to(&stats[1], (unsigned long long)__double_as_longlong(lmax));
   }
 }
 
@@ -655,7 +656,11 @@ hipError_t launch_mesh_to_particle(double *state, int64_t n,
   if (n <= 0) return hipSuccess;
   ChemParams P{chempar[0], chempar[1], chempar[2], chempar[3], chempar[4],
                chempar[5], chempar[6], chempar[7], chempar[8], chempar[9]};
-  hipLaunchKernelGGL(k_mesh_to_particle, dim3(grid_of(n)), dim3(256), 0, s,
+  // one wave per block: each wave runs a long dependent chain per particle
+  // at three waves per SIMD, and one-wave blocks let the dispatcher refill
+  // each SIMD wave by wave
+  const unsigned nblk = (unsigned)((n + 63) / 64);
+  hipLaunchKernelGGL(k_mesh_to_particle, dim3(nblk), dim3(64), 0, s,
                      state, n, g, start, type, topo, pos, rod, X, vf,
                      cnc_tri, vf_tri, P, dt, nloop, stats, err);
   return hipGetLastError();
