@@ -525,8 +525,10 @@ __device__ __forceinline__ double block_max_nonneg(double v, double *sh) {
 
 // One thread per particle in sorted order: interpolation at its position,
 // the chemistry, the record fields written back, and the new radius, length
-// and area into the sorted arrays with their maxima (one atomic per block).
-__global__ __launch_bounds__(256) void k_mesh_to_particle(
+// and area into the sorted arrays with their maxima (at most one atomic per
+// block and word).  Bounds of 4 waves per SIMD: 128 VGPRs, no spill (135
+// and 3 waves unbounded).
+__global__ __launch_bounds__(256, 4) void k_mesh_to_particle(
     double *__restrict__ state, int64_t n, MeshGeom g,
     const int32_t *__restrict__ start,
     const int32_t *__restrict__ type, const Topo *__restrict__ topo,
