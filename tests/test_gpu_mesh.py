@@ -140,6 +140,7 @@ def test_mesh_to_particle_matches_oracle(bmx, oracle, gpu, kind, cnc, vfs):
     assert np.abs(grecs["rdata"][:, 5] - col.recs["rdata"][:, 5]).max() > 0  # grew
     assert np.array_equal(grecs["rdata"], orecs["rdata"])
     assert np.array_equal(grecs["idata"], orecs["idata"])
+    grown = orecs.copy()
     # the mechanics go on with the new sizes
     if kind == "spheres":
         st = oracle.State(orecs, oracle.make_geom(col.prob_lo, col.prob_hi, col.periodic,
@@ -151,6 +152,16 @@ def test_mesh_to_particle_matches_oracle(bmx, oracle, gpu, kind, cnc, vfs):
     ctx.evolve(col.dt, 5)
     st.evolve(col.dt, 5)
     assert np.array_equal(ctx.positions(), orecs["pos"])
+    if kind == "spheres":
+        # The rows of the rebuild after the growth hold every pair within the
+        # grown reaches: the bins are sized from the largest radius the call
+        # reports, and one too small would leave pairs out of the stencil.
+        og, ig = ctx.neighbors()
+        ob, ib = oracle.build_neighbors(grown, g, oracle.NBR_PAIR, col.skin, brute=True,
+                                        W_A=col.fpar[0])
+        assert ob[-1] > len(grown)
+        for i in range(len(grown)):
+            assert sorted(ig[og[i]:og[i + 1]]) == list(ib[ob[i]:ob[i + 1]])
     ctx.close()
 
 
