@@ -221,7 +221,7 @@ def adjoint_bench(bmx, W, n: int = 100_000, steps: int = 200):
     }
 
 
-def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
+def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10, block: int = 64):
     """200,000 hyphal segments (2,000 roots, branch probability 0.05 per
     segment at 30-60 degrees, 30e-4 cm long, radius 2e-4 cm, bond springs
     k = 1e-2) on the growth surface, in the reference's policy: bins are the
@@ -234,6 +234,7 @@ def rods_bench(bmx, W, steps: int = 500, cpu_steps: int = 10):
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
     ctx.set_neighbor_params(mode=bmx.NBR_STENCIL, rebuild_every=1, neighborhood=hood,
                             max_neighbors=128)
+    ctx.set_launch_params(block, 1)
     ctx.bind(col.recs.copy())
     ctx.init_bonds()
     ctx.evolve(col.dt, 10)
@@ -594,7 +595,8 @@ def main() -> int:
         calibration_copy(args.calibrate_copy << 20)
     if args.rods_only:
         print(json.dumps(rods_bench(bmx, W, steps=args.rods_steps,
-                                    cpu_steps=0 if args.no_cpu else 10)))
+                                    cpu_steps=0 if args.no_cpu else 10,
+                                    block=args.block)))
         return 0
     if args.adjoint_only:
         print(json.dumps(adjoint_bench(bmx, W)))
@@ -744,7 +746,7 @@ def main() -> int:
         print("# adjoint done", file=sys.stderr, flush=True)
     if not args.no_rods:
         out["rods"] = rods_bench(bmx, W, steps=args.rods_steps,
-                                 cpu_steps=0 if args.no_cpu else 10)
+                                 cpu_steps=0 if args.no_cpu else 10, block=args.block)
         print("# rods done", file=sys.stderr, flush=True)
     if not args.no_mesh:
         out["mesh"] = mesh_bench(bmx, W, cpu=not args.no_cpu)

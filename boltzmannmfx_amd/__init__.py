@@ -481,7 +481,7 @@ class Context:
                            int(bin_factor_pct), float(skin), float(neighborhood))
         _check(self.lib.bmx_set_neighbor_params(self.h, C.byref(p)))
 
-    def set_launch_params(self, block_size=256, xcd_remap=1) -> None:
+    def set_launch_params(self, block_size=64, xcd_remap=1) -> None:
         p = LaunchParams(int(block_size), int(xcd_remap))
         _check(self.lib.bmx_set_launch_params(self.h, C.byref(p)))
 

@@ -85,7 +85,7 @@ struct bmx_ctx {
   bmx_geometry geom{};
   bool have_geom = false;
   bmx_neighbor_params nbp{};
-  bmx_launch_params lp{256, 1};
+  bmx_launch_params lp{64, 1};  // one wave per block: the fastest shape measured
 
   // particles
   int64_t n = 0, ncap = 0;

@@ -93,7 +93,8 @@ typedef struct {
                              (src/chemistry/bmx_chem.cpp:124) */
 } bmx_neighbor_params;
 
-/* Launch shape of the pair kernels (the autotuner's search space). */
+/* Launch shape of the pair kernels (the autotuner's search space).
+   A new context uses {64, 1}. */
 typedef struct {
   int32_t block_size;     /* threads per block: 64, 128, 256, 512 */
   int32_t xcd_remap;      /* 1: give each of the 8 XCDs a contiguous range
