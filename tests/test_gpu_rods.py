@@ -23,8 +23,10 @@ def small_hyphae(n=3000, nroots=30, seed=11):
     return W.c3_hyphae(n=n, nroots=nroots, seed=seed)
 
 
-def make_pair(bmx, oracle, col, mode=0, K=10, hood=1.0, maxn=0):
+def make_pair(bmx, oracle, col, mode=0, K=10, hood=1.0, maxn=0, block=None):
     ctx = bmx.Context(0)
+    if block is not None:
+        ctx.set_launch_params(block, 1)
     ctx.set_force_params(col.fpar)
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
     ctx.set_neighbor_params(mode=mode, rebuild_every=K, skin=col.skin,
@@ -61,12 +63,14 @@ def close_traj(ctx, orecs, box, rel=1e-10):
     assert dphi.max() <= rel * 2 * math.pi, dphi.max()
 
 
+@pytest.mark.parametrize("block", [64, 256])  # one-wave blocks are the default
 @pytest.mark.parametrize("mode", [0, 1])
-def test_hyphae_init_bonds_and_steps(bmx, oracle, gpu, mode):
+def test_hyphae_init_bonds_and_steps(bmx, oracle, gpu, mode, block):
     col = small_hyphae()
     hood = 1.1 * (2 * 5.47e-4 + 1e-2)  # fungi deck: 1.1 (2 r(max_vol) + width)
     ctx, grecs, st, orecs = make_pair(bmx, oracle, col, mode=mode,
-                                      K=10 if mode == 0 else 1, hood=hood, maxn=256)
+                                      K=10 if mode == 0 else 1, hood=hood, maxn=256,
+                                      block=block)
     ctx.init_bonds()
     assert st.init_bonds() == 0
     gi = ctx.int_fields()
@@ -117,7 +121,8 @@ def test_mixed_spheres_and_segments(bmx, oracle, gpu):
     assert np.array_equal(ctx.int_fields(), orecs["idata"])
 
 
-def test_bond_repair_and_promotion(bmx, oracle, gpu):
+@pytest.mark.parametrize("block", [64, 256])
+def test_bond_repair_and_promotion(bmx, oracle, gpu, block):
     """One-sided bond tables (a bond dropped from one end, as division and
     fusion leave them) and second segments: the force pass repairs the
     tables and promotes the segments as the row loop does (on the GPU a
@@ -139,7 +144,7 @@ def test_bond_repair_and_promotion(bmx, oracle, gpu):
     col.recs = recs.copy()
     before = col.recs["idata"].copy()
     ctx, grecs, st, orecs = make_pair(bmx, oracle, col, mode=1, K=1, hood=hood,
-                                      maxn=256)
+                                      maxn=256, block=block)
     for _ in range(3):
         ctx.evolve(col.dt, 1)
         fo = st.evolve(col.dt, 1)
