@@ -27,8 +27,10 @@ def mat(n=3000, nroots=30, seed=17):
     return col
 
 
-def pair(bmx, oracle, col, mode, K):
+def pair(bmx, oracle, col, mode, K, block=None):
     ctx = bmx.Context(0)
+    if block is not None:
+        ctx.set_launch_params(block, 1)
     ctx.set_force_params(col.fpar)
     ctx.set_geometry(col.prob_lo, col.prob_hi, col.periodic, col.n_cell)
     ctx.set_neighbor_params(mode=mode, rebuild_every=K, skin=col.skin,
@@ -45,10 +47,11 @@ def pair(bmx, oracle, col, mode, K):
     return ctx, grecs, st, orecs
 
 
+@pytest.mark.parametrize("block", [64, 256])  # one-wave blocks are the default
 @pytest.mark.parametrize("mode,K", [(1, 1), (0, 10)])
-def test_exchange_matches_oracle(bmx, oracle, gpu, mode, K):
+def test_exchange_matches_oracle(bmx, oracle, gpu, mode, K, block):
     col = mat()
-    ctx, grecs, st, orecs = pair(bmx, oracle, col, mode, K)
+    ctx, grecs, st, orecs = pair(bmx, oracle, col, mode, K, block)
     ctx.init_bonds()
     assert st.init_bonds() == 0
     for steps in (4, 3):

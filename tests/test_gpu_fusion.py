@@ -17,8 +17,10 @@ SKIN = 1e-3
 LO, HI = (-0.06, -0.06, -0.06), (0.06, 0.06, 0.06)
 
 
-def pair(bmx, oracle, recs):
+def pair(bmx, oracle, recs, block=None):
     ctx = bmx.Context(0)
+    if block is not None:
+        ctx.set_launch_params(block, 1)
     ctx.set_force_params(FPAR)
     ctx.set_geometry(LO, HI, (0, 0, 0))
     ctx.set_neighbor_params(mode=bmx.NBR_VERLET, rebuild_every=1, skin=SKIN,
@@ -43,10 +45,11 @@ def spread(oracle, n):
     return recs
 
 
+@pytest.mark.parametrize("block", [64, 256])  # one-wave blocks are the default
 @pytest.mark.parametrize("n,prob", [(1, 1.0), (400, 0.4)])
-def test_fusion_matches_oracle(bmx, oracle, gpu, n, prob):
+def test_fusion_matches_oracle(bmx, oracle, gpu, n, prob, block):
     recs = scene(oracle) if n == 1 else spread(oracle, n)
-    ctx, g = pair(bmx, oracle, recs)
+    ctx, g = pair(bmx, oracle, recs, block)
     fus = [prob] + FUS[1:]
     nid = 10**6
     grecs, fused, added = ctx.fusion(fus, 5, 3, nid, 0)
